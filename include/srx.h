@@ -1,0 +1,316 @@
+/*
+ * srx.h -- C ABI of the Salvia rasterization pipeline on MI355X (gfx950).
+ *
+ * Every entry point replaces one method of the abstract C++ class
+ * salvia::core::renderer (salvia/include/salvia/core/renderer.h:42-131) or of a
+ * resource class it hands out, with the same argument order and meaning.
+ * Objects are opaque, reference-counted handles (the reference uses
+ * std::shared_ptr for every object, EFLIB_DECLARE_CLASS_SHARED_PTR).
+ * Descriptor structs are plain C with the reference's field order.
+ *
+ * Two libraries export this ABI:
+ *   libsrx_hip.so  -- the product: HIP kernels on one MI355X.
+ *   libsrx_cpu.so  -- the CPU restatement used as test oracle (oracle/).
+ *
+ * Threading contract (renderer.h, async_renderer.cpp:17-110): one API thread per
+ * renderer; commands are queued in submission order on the renderer's stream and
+ * srx_flush() waits for all of them.
+ */
+#ifndef SRX_H_
+#define SRX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SRX_ABI_VERSION 1
+
+/* ---- result codes: numbering of salvia::result (salvia/common/constants.h:9) ---- */
+enum srx_result {
+  SRX_OK = 0,
+  SRX_FAILED = 1,
+  SRX_OUT_OF_MEMORY = 2,
+  SRX_INVALID_PARAMETER = 3,
+  SRX_E_HIP = 16,         /* device error; srx_last_error() carries the hipError_t */
+  SRX_E_UNSUPPORTED = 17  /* valid in the reference but not implemented here */
+};
+
+/* async_status (constants.h:178) */
+enum srx_async_status { SRX_ASYNC_ERROR = 0, SRX_ASYNC_TIMEOUT = 1, SRX_ASYNC_READY = 2 };
+
+/* pixel_format codes (salvia/common/colors_convertors.h:40-47) */
+enum srx_pixel_format {
+  SRX_PF_RGBA32F = 0,
+  SRX_PF_RGB32F = 1,
+  SRX_PF_BGRA8 = 2,
+  SRX_PF_RGBA8 = 3,
+  SRX_PF_R32F = 4,
+  SRX_PF_RG32F = 5, /* the only depth/stencil format: depth fp32 in .r, stencil u32 in .g */
+  SRX_PF_R32I = 6
+};
+
+/* salvia::format subset used by vertex and index data (salvia/common/format.h) */
+enum srx_format {
+  SRX_FMT_R32G32B32A32_FLOAT = 2,
+  SRX_FMT_R32G32B32A32_SINT = 4,
+  SRX_FMT_R32G32B32_FLOAT = 6,
+  SRX_FMT_R32G32_FLOAT = 16,
+  SRX_FMT_R32_FLOAT = 41,
+  SRX_FMT_R32_UINT = 42,
+  SRX_FMT_R16_UINT = 57
+};
+
+/* primitive_topology (constants.h:48-60) */
+enum srx_topology {
+  SRX_TOPOLOGY_LINE_LIST = 0,
+  SRX_TOPOLOGY_LINE_STRIP = 1,
+  SRX_TOPOLOGY_TRIANGLE_LIST = 2,
+  SRX_TOPOLOGY_TRIANGLE_FAN = 3,
+  SRX_TOPOLOGY_TRIANGLE_STRIP = 4
+};
+
+enum srx_cull_mode { SRX_CULL_NONE = 0, SRX_CULL_FRONT = 1, SRX_CULL_BACK = 2 };
+enum srx_fill_mode { SRX_FILL_WIREFRAME = 0, SRX_FILL_SOLID = 1 };
+enum srx_address_mode { SRX_ADDRESS_WRAP = 0, SRX_ADDRESS_MIRROR = 1, SRX_ADDRESS_CLAMP = 2, SRX_ADDRESS_BORDER = 3 };
+enum srx_filter { SRX_FILTER_POINT = 0, SRX_FILTER_LINEAR = 1, SRX_FILTER_ANISOTROPIC = 2 };
+enum srx_mip_quality { SRX_MIP_LO = 0, SRX_MIP_MI = 1, SRX_MIP_HI = 2 };
+enum srx_compare {
+  SRX_CMP_NEVER = 0, SRX_CMP_LESS = 1, SRX_CMP_EQUAL = 2, SRX_CMP_LESS_EQUAL = 3,
+  SRX_CMP_GREATER = 4, SRX_CMP_NOT_EQUAL = 5, SRX_CMP_GREATER_EQUAL = 6, SRX_CMP_ALWAYS = 7
+};
+enum srx_stencil_op {
+  SRX_SOP_KEEP = 1, SRX_SOP_ZERO = 2, SRX_SOP_REPLACE = 3, SRX_SOP_INCR_SAT = 4,
+  SRX_SOP_DECR_SAT = 5, SRX_SOP_INVERT = 6, SRX_SOP_INCR_WRAP = 7, SRX_SOP_DECR_WRAP = 8
+};
+enum srx_clear_flag { SRX_CLEAR_DEPTH = 1, SRX_CLEAR_STENCIL = 2 };
+enum srx_map_mode { SRX_MAP_READ = 1, SRX_MAP_WRITE = 2, SRX_MAP_READ_WRITE = 3, SRX_MAP_WRITE_DISCARD = 4 };
+
+/* async_object_ids (constants.h:166-175) */
+enum srx_query_id { SRX_QUERY_PIPELINE_STATISTICS = 3, SRX_QUERY_INTERNAL_STATISTICS = 5, SRX_QUERY_PIPELINE_PROFILES = 6 };
+
+/* vs_output attribute modifiers (shader_regs.h:55-61) */
+enum srx_attr_modifier {
+  SRX_AM_LINEAR = 1, SRX_AM_CENTROID = 2, SRX_AM_NOINTERPOLATION = 4, SRX_AM_NOPERSPECTIVE = 8, SRX_AM_SAMPLE = 16
+};
+
+/* renderer_capacity.h:7-15 */
+#define SRX_MAX_VS_INPUT_ATTRS 8
+#define SRX_MAX_VS_OUTPUT_ATTRS 5 /* shader.cpp:45-77 initialises only ops 0..5 */
+#define SRX_MAX_INPUT_SLOTS 32
+#define SRX_MAX_SAMPLE_COUNT 16
+#define SRX_MAX_RENDER_TARGET_SIZE 8192
+#define SRX_MAX_SAMPLER_SLOTS 4
+
+/* ---- descriptor structs (POD, reference field order) ---- */
+
+/* viewport (salvia/core/viewport.h:5-12) */
+typedef struct srx_viewport { float x, y, w, h, minz, maxz; } srx_viewport;
+
+/* raster_desc (salvia/core/raster_state.h:17-40) */
+typedef struct srx_raster_desc {
+  int32_t fill_mode;   /* srx_fill_mode, default SOLID */
+  int32_t cull_mode;   /* srx_cull_mode, default BACK */
+  int32_t front_ccw;   /* default 0 */
+  int32_t depth_bias;
+  float depth_bias_clamp;
+  float slope_scaled_depth_bias;
+  int32_t depth_clip_enable;
+  int32_t scissor_enable;
+  int32_t multisample_enable;
+  int32_t anti_aliased_line_enable;
+} srx_raster_desc;
+
+/* depth_stencil_op_desc (salvia/core/framebuffer.h:20-31) */
+typedef struct srx_depth_stencil_op_desc {
+  int32_t stencil_fail_op;
+  int32_t stencil_depth_fail_op;
+  int32_t stencil_pass_op;
+  int32_t stencil_func;
+} srx_depth_stencil_op_desc;
+
+/* depth_stencil_desc (salvia/core/framebuffer.h:33-50) */
+typedef struct srx_depth_stencil_desc {
+  int32_t depth_enable;     /* default 1 */
+  int32_t depth_write_mask; /* default 1 */
+  int32_t depth_func;       /* default LESS */
+  int32_t stencil_enable;   /* default 0 */
+  uint32_t stencil_read_mask;  /* default 0xFF */
+  uint32_t stencil_write_mask; /* default 0xFF */
+  srx_depth_stencil_op_desc front_face;
+  srx_depth_stencil_op_desc back_face;
+} srx_depth_stencil_desc;
+
+/* sampler_desc (salvia/resource/sampler.h:16-45) */
+typedef struct srx_sampler_desc {
+  int32_t min_filter, mag_filter, mip_filter; /* default POINT */
+  int32_t mip_qual;                           /* default MI */
+  int32_t addr_mode_u, addr_mode_v, addr_mode_w; /* default WRAP */
+  float mip_lod_bias;
+  uint32_t max_anisotropy;
+  int32_t comparison_func;
+  float border_color[4];
+  float min_lod, max_lod;
+} srx_sampler_desc;
+
+/* input_element_desc (salvia/resource/input_layout.h:20-52) */
+typedef struct srx_input_element_desc {
+  const char* semantic_name; /* POSITION, NORMAL, TEXCOORD, COLOR ... (case-insensitive) */
+  uint32_t semantic_index;
+  uint32_t data_format;      /* srx_format */
+  uint32_t input_slot;
+  uint32_t aligned_byte_offset;
+  uint32_t slot_class;       /* 0 = per vertex */
+  uint32_t instance_data_step_rate;
+} srx_input_element_desc;
+
+/* pipeline_statistics + internal_statistics (salvia/core/async_object.h:90-175) */
+typedef struct srx_pipeline_statistics {
+  uint64_t ia_vertices, ia_primitives, vs_invocations, gs_invocations, gs_primitives;
+  uint64_t cinvocations, cprimitives, ps_invocations, hs_invocations, ds_invocations, cs_invocations;
+} srx_pipeline_statistics;
+typedef struct srx_internal_statistics { uint64_t backend_input_pixels; } srx_internal_statistics;
+
+/* ---- built-in shaders ----
+ * The reference runs shaders as host C++ virtuals (cpp_vertex_shader::shader_prog,
+ * shader.h:159-215) that device code cannot call.  The sample shaders are provided as
+ * device functions selected by id; their constant blobs are listed next to each id.
+ */
+enum srx_vertex_shader_id {
+  /* in: POSITION0 (r0), COLOR0 (r1).  out: pos = r0; attr0 = r1. consts: none */
+  SRX_VS_PASSTHROUGH_COLOR = 1,
+  /* in: POSITION0, NORMAL0, TEXCOORD0.  out: pos = mul(float4(P,1), wvp); attr0 = N; attr1 = UV.
+     consts: float wvp[16] (row-major, row-vector convention of eflib::transform). */
+  SRX_VS_WVP_NORMAL_UV = 2,
+  /* in: POSITION0, COLOR0.  out: pos = mul(P, wvp); attr0 = COLOR0. consts: float wvp[16] */
+  SRX_VS_WVP_COLOR = 3
+};
+enum srx_pixel_shader_id {
+  /* color0 = attr0 (Gouraud). consts: none */
+  SRX_PS_GOURAUD = 1,
+  /* n = normalize3(attr0.xyz); k = clamp(dot(n, L), 0, 1); t = tex2D(s0, attr1.xy) with the
+     SASL per-row/per-column derivatives and sample_2d_grad (sampler_api.cpp:13-18);
+     color0 = (t.r*k, t.g*k, t.b*k, 1).  consts: float L[4] (xyz used, already normalised) */
+  SRX_PS_LAMBERT_TEX = 2,
+  /* color0 = tex2d(s0, attr0) with the cpp_pixel_shader coarse derivatives and the cached
+     per-quad LOD (cpp_pixel_shader.cpp:22-31); alpha forced to 1 (AnisotropicFilter.cpp:146-152).
+     consts: none */
+  SRX_PS_TEX_CPP = 3,
+  /* color0 = constant. consts: float c[4] */
+  SRX_PS_CONSTANT = 4
+};
+enum srx_blend_shader_id {
+  SRX_BLEND_OFF = 0,      /* ts_blend_off: target = src (TextureAndBlending.cpp:182-193) */
+  SRX_BLEND_ON = 1,       /* ts_blend_on: target = lerp(target, src, src.a) (:168-180) */
+  SRX_BLEND_DISABLED = 2  /* color_disabled_bs: no colour write (StencilMirror.cpp:156-166) */
+};
+
+/* ---- opaque handles ---- */
+typedef struct srx_object_t* srx_handle; /* any handle below */
+typedef struct srx_renderer_t* srx_renderer;
+typedef struct srx_buffer_t* srx_buffer;
+typedef struct srx_texture_t* srx_texture;
+typedef struct srx_surface_t* srx_surface;
+typedef struct srx_sampler_t* srx_sampler;
+typedef struct srx_input_layout_t* srx_input_layout;
+typedef struct srx_raster_state_t* srx_raster_state;
+typedef struct srx_ds_state_t* srx_ds_state;
+typedef struct srx_query_t* srx_query;
+
+/* ---- library ---- */
+uint32_t srx_abi_version(void);
+/* "hip" or "cpu" */
+const char* srx_backend_name(void);
+/* Last error detail of this thread (hipError_t for SRX_E_HIP, else 0) and its message. */
+int32_t srx_last_error(void);
+const char* srx_last_error_string(void);
+/* Reference counting for every handle type (shared_ptr semantics). */
+void srx_retain(void* handle);
+void srx_release(void* handle);
+
+/* ---- renderer creation: create_software_renderer / create_benchmark_renderer (renderer.cpp:29-39) ---- */
+#define SRX_RENDERER_ASYNC 0u     /* create_software_renderer */
+#define SRX_RENDERER_BENCHMARK 1u /* create_benchmark_renderer (sync) */
+int32_t srx_create_renderer(uint32_t kind, srx_renderer* out);
+
+/* ---- creators (renderer.h:46-60) ---- */
+int32_t srx_create_buffer(srx_renderer r, size_t size, srx_buffer* out);
+int32_t srx_create_tex2d(srx_renderer r, size_t width, size_t height, size_t num_samples,
+                         int32_t fmt, srx_texture* out);
+int32_t srx_create_sampler(srx_renderer r, const srx_sampler_desc* desc, srx_texture tex,
+                           srx_sampler* out);
+int32_t srx_create_query(srx_renderer r, uint32_t id, srx_query* out);
+/* create_input_layout(elem_descs, count, cpp_vertex_shader) -- the shader is named by id */
+int32_t srx_create_input_layout(srx_renderer r, const srx_input_element_desc* descs, size_t count,
+                                int32_t vs_id, srx_input_layout* out);
+/* raster_state(raster_desc) (raster_state.cpp:23-33) */
+int32_t srx_create_raster_state(srx_renderer r, const srx_raster_desc* desc, srx_raster_state* out);
+/* depth_stencil_state(depth_stencil_desc) (framebuffer.cpp:168-230) */
+int32_t srx_create_depth_stencil_state(srx_renderer r, const srx_depth_stencil_desc* desc,
+                                       srx_ds_state* out);
+/* Default-initialised descriptors (the reference's constructors). */
+void srx_default_raster_desc(srx_raster_desc* d);
+void srx_default_depth_stencil_desc(srx_depth_stencil_desc* d);
+void srx_default_sampler_desc(srx_sampler_desc* d);
+
+/* ---- resources: buffer / texture / surface (buffer.h:15-35, texture.h:17-102, surface.h:22-92) ---- */
+size_t srx_buffer_size(srx_buffer b);
+/* map(write) + memcpy + unmap (resource_manager.cpp:7-113) */
+int32_t srx_buffer_write(srx_buffer b, size_t offset, const void* data, size_t size);
+/* map(read): waits for queued work that writes the buffer */
+int32_t srx_buffer_read(srx_buffer b, size_t offset, void* data, size_t size);
+/* texture::subresource(level) (texture.h:47-52): retained surface handle */
+int32_t srx_texture_surface(srx_texture t, size_t level, srx_surface* out);
+/* texture_2d::gen_mipmap(filter, auto_gen) (texture2d.cpp:25-36) */
+int32_t srx_texture_gen_mipmap(srx_texture t, int32_t filter, int32_t auto_gen);
+size_t srx_texture_max_lod(srx_texture t);
+size_t srx_texture_min_lod(srx_texture t);
+size_t srx_surface_width(srx_surface s);
+size_t srx_surface_height(srx_surface s);
+size_t srx_surface_sample_count(srx_surface s);
+int32_t srx_surface_format(srx_surface s);
+/* Linear layout ((y*w + x)*samples + s) * elem_size (surface.cpp:277-295); size = pitch*h */
+size_t srx_surface_size_bytes(srx_surface s);
+int32_t srx_surface_write(srx_surface s, const void* data, size_t size);
+int32_t srx_surface_read(srx_surface s, void* data, size_t size);
+/* surface::resolve(target) (surface.cpp:123-140); queued like a command */
+int32_t srx_surface_resolve(srx_renderer r, srx_surface src, srx_surface dst);
+
+/* ---- state set (renderer.h:66-103) ---- */
+int32_t srx_set_vertex_buffers(srx_renderer r, size_t start_slot, size_t count,
+                               const srx_buffer* buffers, const size_t* strides,
+                               const size_t* offsets);
+int32_t srx_set_index_buffer(srx_renderer r, srx_buffer b, uint32_t index_fmt);
+int32_t srx_set_input_layout(srx_renderer r, srx_input_layout layout);
+int32_t srx_set_primitive_topology(srx_renderer r, uint32_t topology);
+/* set_vertex_shader(cpp_vertex_shader) with its constants (declare_constant/set_constant) */
+int32_t srx_set_vertex_shader(srx_renderer r, int32_t vs_id, const void* consts, size_t size);
+int32_t srx_set_pixel_shader(srx_renderer r, int32_t ps_id, const void* consts, size_t size);
+int32_t srx_set_ps_sampler(srx_renderer r, uint32_t slot, srx_sampler s);
+int32_t srx_set_blend_shader(srx_renderer r, int32_t blend_id);
+int32_t srx_set_rasterizer_state(srx_renderer r, srx_raster_state rs);
+int32_t srx_set_depth_stencil_state(srx_renderer r, srx_ds_state dss, int32_t stencil_ref);
+int32_t srx_set_render_targets(srx_renderer r, size_t color_target_count,
+                               const srx_surface* color_targets, srx_surface ds_target);
+int32_t srx_set_viewport(srx_renderer r, const srx_viewport* vp);
+int32_t srx_get_viewport(srx_renderer r, srx_viewport* vp);
+
+/* ---- render operations (renderer.h:117-129) ---- */
+int32_t srx_begin(srx_renderer r, srx_query q);
+int32_t srx_end(srx_renderer r, srx_query q);
+/* data: srx_pipeline_statistics / srx_internal_statistics / uint64_t[8] profiles (ns) */
+int32_t srx_get_data(srx_renderer r, srx_query q, void* data, size_t size, int32_t do_not_wait);
+int32_t srx_draw(srx_renderer r, size_t startpos, size_t primcnt);
+int32_t srx_draw_index(srx_renderer r, size_t startpos, size_t primcnt, int32_t basevert);
+int32_t srx_clear_color(srx_renderer r, srx_surface color_target, const float rgba[4]);
+int32_t srx_clear_depth_stencil(srx_renderer r, srx_surface ds_target, uint32_t flags, float depth,
+                                uint32_t stencil);
+int32_t srx_flush(srx_renderer r);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SRX_H_ */

@@ -1,0 +1,65 @@
+// Host-side launchers of the pipeline kernels (implemented in srx_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "srx_device.h"
+
+namespace srx {
+
+struct ps_samplers {
+  sampler_desc_dev s[MAX_SAMPLERS];
+};
+
+// Device statistics slots (pipeline_statistics / internal_statistics, async_object.h:90-175)
+enum stat_slot {
+  ST_IA_VERTICES = 0, ST_IA_PRIMITIVES, ST_VS_INVOCATIONS, ST_CINVOCATIONS, ST_CPRIMITIVES,
+  ST_PS_INVOCATIONS, ST_BACKEND_PIXELS, ST_COUNT
+};
+// Setup counters
+enum counter_slot { CT_CLIP_PRIMS = 0, CT_CLIP_SLOTS, CT_CLIP_OVERFLOW, CT_TOTAL, CT_COUNT };
+
+// Per-draw device buffers (owned by the renderer, grown on demand).
+struct draw_buffers {
+  float4* vbuf;        // (nverts+1) * (nattr+1) clip-space vertex-shader outputs
+  float4* tri_rec;     // 3*prims * 5: edges, bbox, depth plane of each clipped triangle
+  uint32_t* cnt;       // 3*prims: GPU-tile records per clipped triangle (then its exclusive scan)
+  uint8_t* clip_flag;  // prims
+  int32_t* clip_slot;  // prims
+  float4* clip_verts;  // clip_capacity * 5 * (nattr+1), screen space
+  uint32_t* clip_hdr;  // clip_capacity
+  uint32_t clip_capacity;
+  uint32_t* keys[2];   // records: GPU tile id
+  uint32_t* vals[2];   // records: triangle id << 1 | tile-level accept
+  uint32_t* tstart;    // GPU tiles
+  uint32_t* tend;
+  uint32_t* hist;      // radix histogram 256 * blocks
+  uint32_t* scan_tmp;  // scan partial sums
+  uint32_t* counters;  // CT_COUNT
+  unsigned long long* stats;  // ST_COUNT
+};
+
+void upload_ewa_table(const float* table256, hipStream_t s);
+
+void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
+void launch_vs(const draw_params& P, float4* vbuf, hipStream_t s);
+void launch_setup(const draw_params& P, const draw_buffers& B, hipStream_t s);
+void launch_clip(const draw_params& P, const draw_buffers& B, hipStream_t s);
+// In-place exclusive scan of n u32; writes the total to *total_dev.  tmp holds >= scan_tmp_size(n).
+size_t scan_tmp_size(uint32_t n);
+void launch_exclusive_scan(uint32_t* data, uint32_t n, uint32_t* tmp, uint32_t* total_dev, hipStream_t s);
+void launch_emit(const draw_params& P, const draw_buffers& B, hipStream_t s);
+// Stable LSD radix sort of (keys, vals) by the low `bits` bits of keys; returns the buffer index
+// (0 or 1) that holds the result.
+int launch_sort(const draw_buffers& B, uint32_t n, int bits, hipStream_t s);
+void launch_tile_ranges(const draw_params& P, const draw_buffers& B, int which, uint32_t n, hipStream_t s);
+void launch_raster(const draw_params& P, const ps_samplers& SM, const draw_buffers& B, int which, hipStream_t s);
+
+void launch_clear(const surface_desc& s, float4 value, hipStream_t st);
+// flags: 1 depth, 2 stencil
+void launch_clear_ds(const surface_desc& s, uint32_t flags, float depth, uint32_t stencil, hipStream_t st);
+void launch_resolve(const surface_desc& src, const surface_desc& dst, hipStream_t st);
+void launch_mip(const surface_desc& src, const surface_desc& dst, int filter, hipStream_t st);
+
+}  // namespace srx

@@ -1,0 +1,1319 @@
+// HIP kernels of the Salvia rasterization pipeline for MI355X (gfx950).
+//
+// Draw pipeline (one HIP stream per renderer):
+//   vs_kernel       vertex fetch + vertex shader, one thread per vertex    (stream_assembler.cpp,
+//                   default_vertex_cache.cpp: the shader runs once per vertex instead of per cache miss)
+//   setup_kernel    index fetch, near/far trivial accept, cull, winding, viewport transform and
+//                   triangle setup, one thread per primitive                (clipper.cpp, rasterizer.cpp:864-945)
+//   clip_kernel     Sutherland-Hodgman near/far clipping for the few primitives that need it
+//   scan + emit     per-triangle GPU-tile records written in clipped-primitive order
+//   sort            stable 8-bit LSD radix sort by GPU tile -> per-tile lists in primitive order
+//                   (the reference's per-tile std::sort, rasterizer.cpp:947-994)
+//   raster_kernel   one workgroup per GPU tile, one thread per 2x2 quad; walks its tile's list in
+//                   order and evaluates the reference's 64->16->4->sample decision chain per quad
+//                   (rasterizer.cpp:298-773), early/late depth-stencil, pixel shader and blending
+//                   (framebuffer.cpp).  Opaque early-Z draws defer the pixel shader to one pass over
+//                   the final per-sample winners.
+#include "srx_kernels.h"
+#include "srx_sampler.h"
+
+namespace srx {
+
+__constant__ float c_ewa[256];
+
+enum { VS_PASSTHROUGH_COLOR = 1, VS_WVP_NORMAL_UV = 2, VS_WVP_COLOR = 3 };
+enum { PS_GOURAUD = 1, PS_LAMBERT_TEX = 2, PS_TEX_CPP = 3, PS_CONSTANT = 4 };
+enum { BLEND_OFF = 0, BLEND_ON = 1, BLEND_DISABLED = 2 };
+enum { AM_NOINTERPOLATION = 4, AM_NOPERSPECTIVE = 8 };
+enum { TOPO_TRIANGLE_LIST = 2, TOPO_TRIANGLE_STRIP = 4 };
+enum { FMT_RGBA32F = 2, FMT_RGBA32I = 4, FMT_RGB32F = 6, FMT_RG32F = 16, FMT_R32F = 41 };
+enum { CULL_NONE = 0, CULL_FRONT = 1, CULL_BACK = 2 };
+enum { SOP_KEEP = 1, SOP_ZERO, SOP_REPLACE, SOP_INCR_SAT, SOP_DECR_SAT, SOP_INVERT, SOP_INCR_WRAP, SOP_DECR_WRAP };
+
+template <int NA>
+struct vtx {
+  float4 r[NA + 1];
+};
+
+// ============================================================== vertex stage
+// eflib::transform(out, v, m): out[i] = dot_prod4(v, column i) (eflib/src/math.cpp:142-154)
+__device__ __forceinline__ float4 transform_rowvec(float4 v, const float* m) {
+  float o[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) o[i] = v.x * m[0 * 4 + i] + v.y * m[1 * 4 + i] + v.z * m[2 * 4 + i] + v.w * m[3 * 4 + i];
+  return make_float4(o[0], o[1], o[2], o[3]);
+}
+
+__device__ __forceinline__ float4 fetch_elem(const input_elem_dev& e, uint32_t v) {
+  uint32_t need;
+  switch (e.fmt) {
+  case FMT_R32F: need = 4; break;
+  case FMT_RG32F: need = 8; break;
+  case FMT_RGB32F: need = 12; break;
+  default: need = 16; break;
+  }
+  uint64_t off = static_cast<uint64_t>(e.stride) * v;
+  if (off + need > e.avail) return make_float4(0, 0, 0, 0);  // the reference reads out of bounds
+  const float* f = reinterpret_cast<const float*>(e.base + off);
+  // get_vec4 (stream_assembler.cpp:26-44)
+  switch (e.fmt) {
+  case FMT_R32F: return make_float4(f[0], 0.0f, 0.0f, e.default_w);
+  case FMT_RG32F: return make_float4(f[0], f[1], 0.0f, e.default_w);
+  case FMT_RGB32F: return make_float4(f[0], f[1], f[2], e.default_w);
+  default: return make_float4(f[0], f[1], f[2], f[3]);
+  }
+}
+
+__global__ void vs_kernel(draw_params P, float4* __restrict__ vbuf) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > P.nverts) return;  // slot nverts holds the output for an all-out-of-bounds fetch
+  float4 in[MAX_INPUTS];
+#pragma unroll
+  for (int i = 0; i < MAX_INPUTS; ++i) in[i] = make_float4(0, 0, 0, 0);
+  if (v < P.nverts)
+    for (int i = 0; i < P.nin; ++i) in[P.in[i].reg] = fetch_elem(P.in[i], v);
+  float4 out[3];
+  switch (P.vs_id) {
+  case VS_PASSTHROUGH_COLOR:
+    out[0] = in[0];
+    out[1] = in[1];
+    break;
+  case VS_WVP_COLOR:
+    out[0] = transform_rowvec(in[0], P.vs_consts);
+    out[1] = in[1];
+    break;
+  default:
+    out[0] = transform_rowvec(in[0], P.vs_consts);
+    out[1] = in[1];
+    out[2] = in[2];
+    break;
+  }
+  size_t base = static_cast<size_t>(v) * (P.nattr + 1);
+  for (int r = 0; r <= P.nattr; ++r) vbuf[base + r] = out[r];
+}
+
+// ============================================================== primitive assembly helpers
+__device__ __forceinline__ uint32_t fetch_index(const draw_params& P, uint32_t prim, int k) {
+  uint32_t id;
+  if (P.topo == TOPO_TRIANGLE_LIST) {
+    id = prim * 3 + k;
+  } else {  // strip: odd primitives swap ids 0 and 2 (index_fetcher.cpp:69-77)
+    id = prim + k;
+    if (prim & 1) id = (k == 0) ? prim + 2 : (k == 2 ? prim : prim + 1);
+  }
+  if (!P.indexed) return id + static_cast<uint32_t>(P.basevert);
+  uint64_t off = static_cast<uint64_t>(P.istart_bytes) + static_cast<uint64_t>(id) * P.istride;
+  uint32_t v = 0;
+  if (off + P.istride <= P.ib_size)
+    v = (P.istride == 2) ? *reinterpret_cast<const uint16_t*>(P.ib + off) : *reinterpret_cast<const uint32_t*>(P.ib + off);
+  return v + static_cast<uint32_t>(P.basevert);
+}
+
+template <int NA>
+__device__ __forceinline__ void load_vertex(const draw_params& P, const float4* vbuf, uint32_t index, vtx<NA>& out) {
+  uint32_t slot = index < P.nverts ? index : P.nverts;
+  const float4* src = vbuf + static_cast<size_t>(slot) * (NA + 1);
+#pragma unroll
+  for (int r = 0; r <= NA; ++r) out.r[r] = src[r];
+}
+
+__device__ __forceinline__ float dot4(float4 p, float4 v) { return p.x * v.x + p.y * v.y + p.z * v.z + p.w * v.w; }
+
+// raster_state cull functions (raster_state.cpp:10-32)
+__device__ __forceinline__ bool cull_fn(const draw_params& P, float area) {
+  switch (P.cull_mode) {
+  case CULL_FRONT: return P.front_ccw ? (area <= 0) : (area >= 0);
+  case CULL_BACK: return P.front_ccw ? (area >= 0) : (area <= 0);
+  default: return false;
+  }
+}
+
+// viewport_transform + project_n (shader.cpp:499-511, 117-135)
+template <int NA>
+__device__ __forceinline__ void viewport_project(vtx<NA>& v, const draw_params& P) {
+  float4 p = v.r[0];
+  float invw = equal_eps(p.w, 0.0f) ? 1.0f : 1.0f / p.w;
+  float px = p.x * invw, py = p.y * invw, pz = p.z * invw;
+  float ox = (P.vp[0] + P.vp[2]) * 0.5f, oy = (P.vp[1] + P.vp[3]) * 0.5f;
+  v.r[0].x = (P.vp[2] * 0.5f) * px + ox;
+  v.r[0].y = (P.vp[3] * 0.5f) * -py + oy;
+  v.r[0].z = (P.vp[5] - P.vp[4]) * pz + P.vp[4];
+  v.r[0].w = invw;
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+    if (!(P.attr_mod[a] & AM_NOPERSPECTIVE)) {
+      v.r[a + 1].x *= invw;
+      v.r[a + 1].y *= invw;
+      v.r[a + 1].z *= invw;
+      v.r[a + 1].w *= invw;
+    }
+}
+
+template <int NA>
+struct tri_setup {
+  vtx<NA> v0, ddx, ddy;
+  float e[3][3];
+  float bb[4];
+  bool front, valid;
+};
+
+// compute_triangle_info (rasterizer.cpp:864-945) + compute_derivative_n (shader.cpp:414-450)
+template <int NA>
+__device__ __forceinline__ void triangle_info(const vtx<NA>& a, const vtx<NA>& b, const vtx<NA>& c, tri_setup<NA>& t) {
+  const vtx<NA>* V[3] = {&a, &b, &c};
+  double dist[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) dist[i] = static_cast<double>(fabsf(V[i]->r[0].x) + fabsf(V[i]->r[0].y));
+  int r0;
+  if (dist[0] < dist[1]) r0 = dist[0] < dist[2] ? 0 : 2;
+  else r0 = dist[1] < dist[2] ? 1 : 2;
+  int r1 = (r0 + 1) % 3, r2 = (r1 + 1) % 3;
+  const vtx<NA>& R0 = *V[r0];
+  const vtx<NA>& R1 = *V[r1];
+  const vtx<NA>& R2 = *V[r2];
+  float e01[NA + 1][4], e02[NA + 1][4];
+#pragma unroll
+  for (int r = 0; r <= NA; ++r) {
+    e01[r][0] = R1.r[r].x - R0.r[r].x; e01[r][1] = R1.r[r].y - R0.r[r].y;
+    e01[r][2] = R1.r[r].z - R0.r[r].z; e01[r][3] = R1.r[r].w - R0.r[r].w;
+    e02[r][0] = R2.r[r].x - R0.r[r].x; e02[r][1] = R2.r[r].y - R0.r[r].y;
+    e02[r][2] = R2.r[r].z - R0.r[r].z; e02[r][3] = R2.r[r].w - R0.r[r].w;
+  }
+  float area = e02[0][0] * e01[0][1] - e02[0][1] * e01[0][0];
+  t.valid = !equal_eps(area, 0.0f);
+  if (!t.valid) return;
+  t.front = area > 0.0f;
+  float inv_area = 1.0f / area;
+  t.bb[0] = std_min(std_min(a.r[0].x, b.r[0].x), c.r[0].x);
+  t.bb[1] = std_max(std_max(a.r[0].x, b.r[0].x), c.r[0].x);
+  t.bb[2] = std_min(std_min(a.r[0].y, b.r[0].y), c.r[0].y);
+  t.bb[3] = std_max(std_max(a.r[0].y, b.r[0].y), c.r[0].y);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float4& s = V[i]->r[0];
+    const float4& e = V[(i + 1) % 3]->r[0];
+    t.e[i][0] = s.y - e.y;
+    t.e[i][1] = e.x - s.x;
+    t.e[i][2] = e.x * s.y - e.y * s.x;
+  }
+  float e01x = e01[0][0], e01y = e01[0][1], e02x = e02[0][0], e02y = e02[0][1];
+#pragma unroll
+  for (int r = 0; r <= NA; ++r) {
+    float dx[4], dy[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      float xd = e02[r][k] * e01y - e01[r][k] * e02y;
+      float yd = e01[r][k] * e02x - e02[r][k] * e01x;
+      dx[k] = xd * inv_area;
+      dy[k] = yd * inv_area;
+    }
+    t.ddx.r[r] = make_float4(dx[0], dx[1], dx[2], dx[3]);
+    t.ddy.r[r] = make_float4(dy[0], dy[1], dy[2], dy[3]);
+  }
+  t.v0 = R0;
+}
+
+// compute_front (clipper.cpp:92-101)
+__device__ __forceinline__ bool compute_front(float4 p0, float4 p1, float4 p2) {
+  float i0 = 1.0f / p0.w, i1 = 1.0f / p1.w, i2 = 1.0f / p2.w;
+  float x0 = p0.x * i0, y0 = p0.y * i0, x1 = p1.x * i1, y1 = p1.y * i1, x2 = p2.x * i2, y2 = p2.y * i2;
+  float ax = x2 - x0, ay = y2 - y0, bx = x1 - x0, by = y1 - y0;
+  return (ax * by - ay * bx) > 0.0f;
+}
+
+// Screen-space vertices of clipped triangle `id` (= 3 * primitive + fan index).
+template <int NA>
+__device__ __forceinline__ void load_clipped_triangle(const draw_params& P, const float4* vbuf, const uint8_t* clip_flag,
+                                                      const int32_t* clip_slot, const float4* clip_verts,
+                                                      const uint32_t* clip_hdr, uint32_t id, vtx<NA>* out) {
+  uint32_t prim = id / 3, k = id - prim * 3;
+  if (!clip_flag[prim]) {
+    vtx<NA> v[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) load_vertex<NA>(P, vbuf, fetch_index(P, prim, i), v[i]);
+    bool front = compute_front(v[0].r[0], v[1].r[0], v[2].r[0]);
+    int off = front ? 0 : 1;  // winding normalisation (clipper.cpp:59-65)
+    out[0] = v[0];
+    out[1] = v[1 + off];
+    out[2] = v[2 - off];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) viewport_project<NA>(out[i], P);
+    return;
+  }
+  int32_t slot = clip_slot[prim];
+  uint32_t hdr = clip_hdr[slot];
+  bool front = (hdr >> 8) & 1;
+  const float4* base = clip_verts + static_cast<size_t>(slot) * 5 * (NA + 1);
+  int ia = 0, ib = static_cast<int>(k) + 1, ic = static_cast<int>(k) + 2;
+  if (!front) {
+    int tmp = ib;
+    ib = ic;
+    ic = tmp;
+  }
+  int idx[3] = {ia, ib, ic};
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int r = 0; r <= NA; ++r) out[i].r[r] = base[idx[i] * (NA + 1) + r];
+}
+
+// ============================================================== binning geometry
+// Visits, in order, the GPU tiles that may hold pixels of a triangle, with the reference's
+// 64x64 tile-level decision (threaded_dispatch_primitive, rasterizer.cpp:775-862).
+template <class F>
+__device__ __forceinline__ void for_each_gtile(const draw_params& P, const float* e, const float* bb, F&& f) {
+  int tx = P.tiles_x, ty = P.tiles_y;
+  int sx = min(fast_floori(std_max(0.0f, bb[0]) / TILE), tx);
+  int sy = min(fast_floori(std_max(0.0f, bb[2]) / TILE), ty);
+  int ex = min(fast_ceili(std_max(0.0f, bb[1]) / TILE) + 1, tx);
+  int ey = min(fast_ceili(std_max(0.0f, bb[3]) / TILE) + 1, ty);
+  bool small = (sx + 1 == ex) && (sy + 1 == ey);
+  bool mx[3], my[3];
+  float r2a[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    mx[k] = e[k * 3 + 0] > 0;
+    my[k] = e[k * 3 + 1] > 0;
+    float stx = TILE * e[k * 3 + 0], sty = TILE * e[k * 3 + 1];
+    r2a[k] = -fabsf(stx) - fabsf(sty);
+  }
+  int sub = TILE / P.gt;
+  for (int y = sy; y < ey; ++y)
+    for (int x = sx; x < ex; ++x) {
+      bool full = false;
+      if (!small) {
+        bool rej = false, acc = true;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          float ev = e[k * 3 + 2] - ((static_cast<float>(x + mx[k]) * TILE) * e[k * 3 + 0] +
+                                     (static_cast<float>(y + my[k]) * TILE) * e[k * 3 + 1]);
+          rej |= (0 < ev);
+          acc &= (r2a[k] >= ev);
+        }
+        if (rej) continue;
+        full = acc;
+      }
+      float xmin = bb[0] - static_cast<float>(x * TILE), xmax = bb[1] - static_cast<float>(x * TILE);
+      float ymin = bb[2] - static_cast<float>(y * TILE), ymax = bb[3] - static_cast<float>(y * TILE);
+      for (int j = 0; j < sub; ++j) {
+        int gy = y * sub + j;
+        if (gy >= P.gtiles_y) break;
+        if (!full && (ymin >= static_cast<float>(j * P.gt + P.gt) || ymax < static_cast<float>(j * P.gt))) continue;
+        for (int i = 0; i < sub; ++i) {
+          int gx = x * sub + i;
+          if (gx >= P.gtiles_x) break;
+          if (!full && (xmin >= static_cast<float>(i * P.gt + P.gt) || xmax < static_cast<float>(i * P.gt))) continue;
+          f(static_cast<uint32_t>(gy * P.gtiles_x + gx), full);
+        }
+      }
+    }
+}
+
+template <int NA>
+__device__ __forceinline__ void write_tri_rec(float4* rec, const tri_setup<NA>& t) {
+  rec[0] = make_float4(t.e[0][0], t.e[0][1], t.e[0][2], t.e[1][0]);
+  rec[1] = make_float4(t.e[1][1], t.e[1][2], t.e[2][0], t.e[2][1]);
+  rec[2] = make_float4(t.e[2][2], t.bb[0], t.bb[1], t.bb[2]);
+  rec[3] = make_float4(t.bb[3], t.v0.r[0].x, t.v0.r[0].y, t.v0.r[0].z);
+  rec[4] = make_float4(t.ddx.r[0].z, t.ddy.r[0].z, t.front ? 1.0f : 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ void read_tri_rec(const float4* rec, float* e, float* bb) {
+  float4 a = rec[0], b = rec[1], c = rec[2], d = rec[3];
+  e[0] = a.x; e[1] = a.y; e[2] = a.z; e[3] = a.w; e[4] = b.x; e[5] = b.y; e[6] = b.z; e[7] = b.w; e[8] = c.x;
+  bb[0] = c.y; bb[1] = c.z; bb[2] = c.w; bb[3] = d.x;
+}
+
+__device__ __forceinline__ void wave_add(unsigned long long* dst, unsigned long long v) {
+  // one atomic per wavefront
+  unsigned long long s = v;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (__lane_id() == __builtin_ctzll(__ballot(1))) atomicAdd(dst, s);
+}
+
+// ============================================================== setup (fast path)
+template <int NA>
+__global__ void __launch_bounds__(256) setup_kernel(draw_params P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
+                                                     uint32_t* __restrict__ cnt, uint8_t* __restrict__ clip_flag,
+                                                     uint32_t* __restrict__ counters, unsigned long long* __restrict__ stats) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long cprims = 0;
+  if (p < P.prim_count) {
+    cnt[3 * p + 0] = 0;
+    cnt[3 * p + 1] = 0;
+    cnt[3 * p + 2] = 0;
+    vtx<NA> v[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) load_vertex<NA>(P, vbuf, fetch_index(P, p, i), v[i]);
+    // near (0,0,1,0) and far (0,0,-1,1) planes (clipper.cpp:21-27,104-120)
+    const float4 pn = make_float4(0.0f, 0.0f, 1.0f, 0.0f), pf = make_float4(0.0f, 0.0f, -1.0f, 1.0f);
+    bool in = true;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) in &= !(dot4(pn, v[i].r[0]) < 0);
+    if (in)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) in &= !(dot4(pf, v[i].r[0]) < 0);
+    clip_flag[p] = in ? 0 : 1;
+    if (!in) {
+      atomicAdd(&counters[CT_CLIP_PRIMS], 1u);
+    } else {
+      bool front = compute_front(v[0].r[0], v[1].r[0], v[2].r[0]);
+      if (!cull_fn(P, front ? 1.0f : -1.0f)) {
+        cprims = 1;
+        int off = front ? 0 : 1;
+        vtx<NA> o[3] = {v[0], v[1 + off], v[2 - off]};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) viewport_project<NA>(o[i], P);
+        tri_setup<NA> t;
+        triangle_info<NA>(o[0], o[1], o[2], t);
+        if (t.valid) {
+          write_tri_rec<NA>(tri_rec + static_cast<size_t>(3 * p) * 5, t);
+          float e[9], bb[4];
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) e[i * 3 + j] = t.e[i][j];
+          for (int i = 0; i < 4; ++i) bb[i] = t.bb[i];
+          uint32_t n = 0;
+          for_each_gtile(P, e, bb, [&](uint32_t, bool) { ++n; });
+          cnt[3 * p] = n;
+        }
+      }
+    }
+  }
+  wave_add(&stats[ST_CPRIMITIVES], cprims);
+}
+
+// ============================================================== clipping (slow path)
+template <int NA>
+__device__ __forceinline__ void lerp_vtx(vtx<NA>& o, const vtx<NA>& a, const vtx<NA>& b, float t, const draw_params& P) {
+  o.r[0] = make_float4(a.r[0].x + (b.r[0].x - a.r[0].x) * t, a.r[0].y + (b.r[0].y - a.r[0].y) * t,
+                       a.r[0].z + (b.r[0].z - a.r[0].z) * t, a.r[0].w + (b.r[0].w - a.r[0].w) * t);
+#pragma unroll
+  for (int i = 0; i < NA; ++i) {
+    o.r[i + 1] = a.r[i + 1];
+    if (!(P.attr_mod[i] & AM_NOINTERPOLATION))
+      o.r[i + 1] = make_float4(o.r[i + 1].x + (b.r[i + 1].x - a.r[i + 1].x) * t, o.r[i + 1].y + (b.r[i + 1].y - a.r[i + 1].y) * t,
+                               o.r[i + 1].z + (b.r[i + 1].z - a.r[i + 1].z) * t, o.r[i + 1].w + (b.r[i + 1].w - a.r[i + 1].w) * t);
+  }
+}
+
+// clip_triangle_to_poly_general + clip_solid_triangle (clipper.cpp:43-228)
+template <int NA>
+__global__ void clip_kernel(draw_params P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
+                            uint32_t* __restrict__ cnt, const uint8_t* __restrict__ clip_flag, int32_t* __restrict__ clip_slot,
+                            float4* __restrict__ clip_verts, uint32_t* __restrict__ clip_hdr, uint32_t capacity,
+                            uint32_t* __restrict__ counters, unsigned long long* __restrict__ stats) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P.prim_count || !clip_flag[p]) return;
+  vtx<NA> cv[2][5];
+  uint32_t nc[2] = {3, 0};
+  for (int i = 0; i < 3; ++i) load_vertex<NA>(P, vbuf, fetch_index(P, p, i), cv[0][i]);
+  const float4 planes[2] = {make_float4(0.0f, 0.0f, 1.0f, 0.0f), make_float4(0.0f, 0.0f, -1.0f, 1.0f)};
+  int src = 0, dst = 1;
+  bool front = false;
+  for (int pl = 0; pl < 2; ++pl) {
+    nc[dst] = 0;
+    float d0 = 0.0f;
+    if (nc[src] != 0) d0 = dot4(planes[pl], cv[src][0].r[0]);
+    for (uint32_t i = 0, j = 1; i < nc[src]; ++i, ++j) {
+      j %= nc[src];
+      float d1 = dot4(planes[pl], cv[src][j].r[0]);
+      if (d0 >= 0.0f) {
+        cv[dst][nc[dst]++] = cv[src][i];
+        if (d1 < 0.0f) lerp_vtx<NA>(cv[dst][nc[dst]++], cv[src][i], cv[src][j], d0 / (d0 - d1), P);
+      } else if (d1 >= 0.0f) {
+        lerp_vtx<NA>(cv[dst][nc[dst]++], cv[src][j], cv[src][i], d1 / (d1 - d0), P);
+      }
+      d0 = d1;
+    }
+    if (pl == 0 && nc[dst] >= 3) {
+      float px[3], py[3];
+      for (int v = 0; v < 3; ++v) {
+        float iaw = 1 / fabsf(cv[dst][v].r[0].w);
+        px[v] = cv[dst][v].r[0].x * iaw;
+        py[v] = cv[dst][v].r[0].y * iaw;
+      }
+      float ax = px[2] - px[0], ay = py[2] - py[0], bx = px[1] - px[0], by = py[1] - py[0];
+      float area = ax * by - ay * bx;
+      front = area > 0.0f;
+      if (cull_fn(P, area)) return;
+    }
+    src ^= 1;
+    dst ^= 1;
+  }
+  uint32_t n = nc[src];
+  if (n < 3) return;
+  atomicAdd(&stats[ST_CPRIMITIVES], static_cast<unsigned long long>(n - 2));
+  uint32_t slot = atomicAdd(&counters[CT_CLIP_SLOTS], 1u);
+  if (slot >= capacity) {
+    atomicAdd(&counters[CT_CLIP_OVERFLOW], 1u);
+    return;
+  }
+  clip_slot[p] = static_cast<int32_t>(slot);
+  clip_hdr[slot] = n | (front ? 0x100u : 0u);
+  float4* out = clip_verts + static_cast<size_t>(slot) * 5 * (NA + 1);
+  for (uint32_t i = 0; i < n; ++i) {
+    viewport_project<NA>(cv[src][i], P);
+    for (int r = 0; r <= NA; ++r) out[i * (NA + 1) + r] = cv[src][i].r[r];
+  }
+  for (uint32_t t = 1; t + 1 < n; ++t) {
+    const vtx<NA>& a = cv[src][0];
+    const vtx<NA>& b = front ? cv[src][t] : cv[src][t + 1];
+    const vtx<NA>& c = front ? cv[src][t + 1] : cv[src][t];
+    tri_setup<NA> ts;
+    triangle_info<NA>(a, b, c, ts);
+    uint32_t id = 3 * p + (t - 1);
+    if (!ts.valid) continue;
+    write_tri_rec<NA>(tri_rec + static_cast<size_t>(id) * 5, ts);
+    float e[9], bb[4];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) e[i * 3 + j] = ts.e[i][j];
+    for (int i = 0; i < 4; ++i) bb[i] = ts.bb[i];
+    uint32_t m = 0;
+    for_each_gtile(P, e, bb, [&](uint32_t, bool) { ++m; });
+    cnt[id] = m;
+  }
+}
+
+// ============================================================== record emission
+// off: exclusive scan of the per-triangle record counts; a triangle owns [off[id], off[id+1]).
+__global__ void emit_kernel(draw_params P, const float4* __restrict__ tri_rec, const uint32_t* __restrict__ off,
+                            const uint32_t* __restrict__ total, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t n = 3 * P.prim_count;
+  if (id >= n) return;
+  uint32_t next = (id + 1 < n) ? off[id + 1] : *total;
+  if (next == off[id]) return;
+  float e[9], bb[4];
+  read_tri_rec(tri_rec + static_cast<size_t>(id) * 5, e, bb);
+  uint32_t o = off[id];
+  for_each_gtile(P, e, bb, [&](uint32_t g, bool full) {
+    if (o < next) {  // the enumeration repeats the counting pass exactly; the bound is insurance
+      keys[o] = g;
+      vals[o] = (id << 1) | (full ? 1u : 0u);
+    }
+    ++o;
+  });
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ============================================================== scan (exclusive, u32)
+constexpr int SCAN_BLOCK = 1024;  // elements per workgroup (256 threads x 4)
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  int lane = __lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan of one 1024-element block in place; returns the block total to thread 0's sums slot.
+__global__ void __launch_bounds__(256) scan_block_kernel(uint32_t* data, uint32_t n, uint32_t* sums) {
+  __shared__ uint32_t wsum[4];
+  uint32_t base = blockIdx.x * SCAN_BLOCK + threadIdx.x * 4;
+  uint32_t v[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = (base + i < n) ? data[base + i] : 0u;
+  uint32_t t = v[0] + v[1] + v[2] + v[3];
+  uint32_t incl = wave_incl_scan(t);
+  int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  uint32_t wprefix = 0;
+  for (int w = 0; w < wave; ++w) wprefix += wsum[w];
+  uint32_t run = wprefix + incl - t;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    uint32_t x = v[i];
+    if (base + i < n) data[base + i] = run;
+    run += x;
+  }
+  if (threadIdx.x == 255) sums[blockIdx.x] = wprefix + incl;
+}
+
+__global__ void scan_add_kernel(uint32_t* data, uint32_t n, const uint32_t* sums) {
+  uint32_t i = blockIdx.x * SCAN_BLOCK + threadIdx.x * 4;
+  uint32_t add = sums[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (i + k < n) data[i + k] += add;
+}
+
+__global__ void scan_total_kernel(const uint32_t* sums_scanned, const uint32_t* sums_orig_last, uint32_t nb, uint32_t* total) {
+  *total = sums_scanned[nb - 1] + *sums_orig_last;
+}
+
+size_t scan_tmp_size(uint32_t n) {
+  size_t total = 0;
+  uint32_t m = n;
+  while (true) {
+    uint32_t nb = (m + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    total += nb + 1;
+    if (nb <= 1) break;
+    m = nb;
+  }
+  return total + 4;
+}
+
+// Recursive reduce-then-scan.  tmp layout per level: sums[nb] then one slot for the last sum.
+static void scan_rec(uint32_t* data, uint32_t n, uint32_t* tmp, uint32_t* total_dev, hipStream_t s) {
+  uint32_t nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  if (nb == 0) nb = 1;
+  uint32_t* sums = tmp;
+  uint32_t* last = tmp + nb;
+  hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(256), 0, s, data, n, sums);
+  // keep the last block's sum before the sums are scanned
+  hipMemcpyAsync(last, sums + nb - 1, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+  if (nb > 1) {
+    scan_rec(sums, nb, tmp + nb + 1, nullptr, s);
+    hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(256), 0, s, data, n, sums);
+  } else {
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, sums, 1u, 0u);
+  }
+  if (total_dev) hipLaunchKernelGGL(scan_total_kernel, dim3(1), dim3(1), 0, s, sums, last, nb, total_dev);
+}
+
+void launch_exclusive_scan(uint32_t* data, uint32_t n, uint32_t* tmp, uint32_t* total_dev, hipStream_t s) {
+  scan_rec(data, n, tmp, total_dev, s);
+}
+
+// ============================================================== radix sort (stable LSD, 8-bit digits)
+constexpr int SORT_BLOCK = 4096;  // 256 threads x 16 items
+
+__global__ void __launch_bounds__(256) radix_hist_kernel(const uint32_t* __restrict__ keys, uint32_t n, int shift,
+                                                          uint32_t nblocks, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t cnt[256];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  uint32_t b0 = blockIdx.x * SORT_BLOCK;
+  uint32_t end = min(n, b0 + SORT_BLOCK);
+  for (uint32_t i = b0 + threadIdx.x; i < end; i += 256) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+  __syncthreads();
+  hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Each wave ranks its 1024 consecutive items in 16 rounds of 64; equal digits inside a round are
+// grouped with ballots, so ranks follow input order (stability).
+__global__ void __launch_bounds__(256) radix_scatter_kernel(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                             uint32_t n, int shift, uint32_t nblocks,
+                                                             const uint32_t* __restrict__ hist_scanned,
+                                                             uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+  __shared__ uint32_t wcnt[4][256];
+  int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = threadIdx.x; i < 4 * 256; i += 256) (&wcnt[0][0])[i] = 0;
+  __syncthreads();
+  uint32_t base = blockIdx.x * SORT_BLOCK + wave * 1024;
+  uint32_t rank[16], key[16], val[16];
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    uint32_t i = base + r * 64 + lane;
+    bool valid = i < n;
+    key[r] = valid ? kin[i] : 0u;
+    val[r] = valid ? vin[i] : 0u;
+    uint32_t d = (key[r] >> shift) & 255u;
+    uint64_t active = __ballot(valid);
+    rank[r] = 0;
+    while (active) {
+      int leader = __builtin_ctzll(active);
+      uint32_t dl = __shfl(d, leader, 64);
+      uint64_t m = __ballot(valid && d == dl);
+      uint32_t before = wcnt[wave][dl];
+      if (valid && d == dl) rank[r] = before + static_cast<uint32_t>(__popcll(m & lt));
+      __builtin_amdgcn_wave_barrier();
+      if (lane == leader) wcnt[wave][dl] = before + static_cast<uint32_t>(__popcll(m));
+      __builtin_amdgcn_wave_barrier();
+      active &= ~m;
+    }
+  }
+  __syncthreads();
+  {
+    uint32_t d = threadIdx.x, s = 0;
+    for (int w = 0; w < 4; ++w) {
+      uint32_t c = wcnt[w][d];
+      wcnt[w][d] = s;
+      s += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    uint32_t i = base + r * 64 + lane;
+    if (i < n) {
+      uint32_t d = (key[r] >> shift) & 255u;
+      uint32_t pos = hist_scanned[d * nblocks + blockIdx.x] + wcnt[wave][d] + rank[r];
+      if (pos < n) {
+        kout[pos] = key[r];
+        vout[pos] = val[r];
+      }
+    }
+  }
+}
+
+int launch_sort(const draw_buffers& B, uint32_t n, int bits, hipStream_t s) {
+  int cur = 0;
+  if (n == 0) return cur;
+  uint32_t nblocks = (n + SORT_BLOCK - 1) / SORT_BLOCK;
+  for (int shift = 0; shift < bits; shift += 8) {
+    hipLaunchKernelGGL(radix_hist_kernel, dim3(nblocks), dim3(256), 0, s, B.keys[cur], n, shift, nblocks, B.hist);
+    launch_exclusive_scan(B.hist, 256 * nblocks, B.scan_tmp, nullptr, s);
+    hipLaunchKernelGGL(radix_scatter_kernel, dim3(nblocks), dim3(256), 0, s, B.keys[cur], B.vals[cur], n, shift, nblocks,
+                       B.hist, B.keys[cur ^ 1], B.vals[cur ^ 1]);
+    cur ^= 1;
+  }
+  return cur;
+}
+
+__global__ void tile_range_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ ts, uint32_t* __restrict__ te) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t k = keys[i];
+  if (i == 0 || keys[i - 1] != k) ts[k] = i;
+  if (i == n - 1 || keys[i + 1] != k) te[k] = i + 1;
+}
+
+void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(fill_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+}
+
+void launch_tile_ranges(const draw_params& P, const draw_buffers& B, int which, uint32_t n, hipStream_t s) {
+  uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((T + 255) / 256), dim3(256), 0, s, B.tstart, T, 0u);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((T + 255) / 256), dim3(256), 0, s, B.tend, T, 0u);
+  if (n) hipLaunchKernelGGL(tile_range_kernel, dim3((n + 255) / 256), dim3(256), 0, s, B.keys[which], n, B.tstart, B.tend);
+}
+
+// ============================================================== pixel shaders
+__device__ __forceinline__ void normalize3(float x, float y, float z, float& ox, float& oy, float& oz) {
+  float t = 0.0f;
+  t += x * x;
+  t += y * y;
+  t += z * z;
+  float len = sqrtf(t);
+  if (equal_eps(len, 0.0f)) len = 1.0f;
+  float inv = 1.0f / len;
+  ox = x * inv;
+  oy = y * inv;
+  oz = z * inv;
+}
+
+// Pixel shader on a 2x2 quad (cpp_pixel_shader::execute, cpp_pixel_shader.cpp:62-74); colour is
+// produced for the pixels in `need`.  The sample shaders never discard.
+template <int NA>
+__device__ __forceinline__ void run_ps(const draw_params& P, const ps_samplers& SM, const float4 (&q)[4][NA + 1],
+                                       uint32_t need, float4 (&col)[4]) {
+  switch (P.ps_id) {
+  case PS_GOURAUD:
+#pragma unroll
+    for (int i = 0; i < 4; ++i) col[i] = q[i][1];
+    return;
+  case PS_CONSTANT:
+#pragma unroll
+    for (int i = 0; i < 4; ++i) col[i] = make_float4(P.ps_consts[0], P.ps_consts[1], P.ps_consts[2], P.ps_consts[3]);
+    return;
+  case PS_LAMBERT_TEX:
+    if constexpr (NA >= 2) {
+      // SASL tex2D: ddx per row (p1-p0, p3-p2), ddy per column (p2-p0, p3-p1)
+      // (sasl/src/codegen/cgs_simd.cpp:275-312), then sample_2d_grad (sampler_api.cpp:13-18)
+      for (int i = 0; i < 4; ++i) {
+        if (!(need & (1u << i))) continue;
+        int row = i & 2, cl = i & 1;
+        float dxu = q[row + 1][2].x - q[row][2].x, dxv = q[row + 1][2].y - q[row][2].y;
+        float dyu = q[cl + 2][2].x - q[cl][2].x, dyv = q[cl + 2][2].y - q[cl][2].y;
+        float4 t = P.has_sampler[0] ? sample_2d_grad(SM.s[0], q[i][2].x, q[i][2].y, dxu, dxv, dyu, dyv, 0.0f)
+                                    : make_float4(0, 0, 0, 0);
+        float nx, ny, nz;
+        normalize3(q[i][1].x, q[i][1].y, q[i][1].z, nx, ny, nz);
+        float d = nx * P.ps_consts[0] + ny * P.ps_consts[1] + nz * P.ps_consts[2];
+        float k = std_clamp(d, 0.0f, 1.0f);
+        col[i] = make_float4(t.x * k, t.y * k, t.z * k, 1.0f);
+      }
+    }
+    return;
+  case PS_TEX_CPP: {
+    // cpp tex2d: one LOD per quad from p1 - p0 and p2 - p0 (cpp_pixel_shader.cpp:13-31)
+    float lod = 0.0f;
+    if (P.has_sampler[0]) {
+      const sampler_desc_dev& sp = SM.s[0];
+      float dxu = q[1][1].x - q[0][1].x, dxv = q[1][1].y - q[0][1].y;
+      float dyu = q[2][1].x - q[0][1].x, dyv = q[2][1].y - q[0][1].y;
+      lod = (sp.mip_filter == FILTER_ANISO && sp.max_aniso > 1) ? calc_aniso(sp, dxu, dxv, dyu, dyv, 0.0f).lod
+                                                                : calc_lod(sp, dxu, dxv, dyu, dyv, 0.0f);
+    }
+    for (int i = 0; i < 4; ++i) {
+      if (!(need & (1u << i))) continue;
+      float4 t = P.has_sampler[0] ? sample_impl(SM.s[0], q[i][1].x, q[i][1].y, lod, nullptr) : make_float4(0, 0, 0, 0);
+      t.w = 1.0f;
+      col[i] = t;
+    }
+    return;
+  }
+  default:
+#pragma unroll
+    for (int i = 0; i < 4; ++i) col[i] = make_float4(0, 0, 0, 0);
+  }
+}
+
+// step_2d_unproj_pos_quad + step_2d_unproj_attr_n_quad (shader.cpp:257-367)
+template <int NA>
+__device__ __forceinline__ void interp_quad(const tri_setup<NA>& t, float dx, float dy, const draw_params& P, float4 (&q)[4][NA + 1]) {
+  {
+    const float4 v = t.v0.r[0], a = t.ddx.r[0], b = t.ddy.r[0];
+    float4 p00 = make_float4(v.x + ((a.x * dx) + (b.x * dy)), v.y + ((a.y * dx) + (b.y * dy)),
+                             v.z + ((a.z * dx) + (b.z * dy)), v.w + ((a.w * dx) + (b.w * dy)));
+    float4 p01 = make_float4(p00.x + a.x, p00.y + a.y, p00.z + a.z, p00.w + a.w);
+    q[0][0] = p00;
+    q[1][0] = p01;
+    q[2][0] = make_float4(p00.x + b.x, p00.y + b.y, p00.z + b.z, p00.w + b.w);
+    q[3][0] = make_float4(p01.x + b.x, p01.y + b.y, p01.z + b.z, p01.w + b.w);
+  }
+  float iw[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) iw[i] = 1.0f / q[i][0].w;
+#pragma unroll
+  for (int r = 1; r <= NA; ++r) {
+    const float4 v = t.v0.r[r], a = t.ddx.r[r], b = t.ddy.r[r];
+    float4 i00, i01, i10, i11;
+    if (P.attr_mod[r - 1] & AM_NOINTERPOLATION) {
+      i00 = i01 = i10 = i11 = v;
+    } else {
+      i00 = make_float4(v.x + ((a.x * dx) + (b.x * dy)), v.y + ((a.y * dx) + (b.y * dy)),
+                        v.z + ((a.z * dx) + (b.z * dy)), v.w + ((a.w * dx) + (b.w * dy)));
+      i01 = make_float4(i00.x + a.x, i00.y + a.y, i00.z + a.z, i00.w + a.w);
+      i10 = make_float4(i00.x + b.x, i00.y + b.y, i00.z + b.z, i00.w + b.w);
+      i11 = make_float4(i01.x + b.x, i01.y + b.y, i01.z + b.z, i01.w + b.w);
+    }
+    if (P.attr_mod[r - 1] & AM_NOPERSPECTIVE) {
+      q[0][r] = i00; q[1][r] = i01; q[2][r] = i10; q[3][r] = i11;
+    } else {
+      q[0][r] = make_float4(i00.x * iw[0], i00.y * iw[0], i00.z * iw[0], i00.w * iw[0]);
+      q[1][r] = make_float4(i01.x * iw[1], i01.y * iw[1], i01.z * iw[1], i01.w * iw[1]);
+      q[2][r] = make_float4(i10.x * iw[2], i10.y * iw[2], i10.z * iw[2], i10.w * iw[2]);
+      q[3][r] = make_float4(i11.x * iw[3], i11.y * iw[3], i11.z * iw[3], i11.w * iw[3]);
+    }
+  }
+}
+
+// ============================================================== raster
+struct prim_lds {
+  float e[9];
+  float bb[4];     // relative to the 64x64 tile origin
+  float ev1[3];    // stage-16 rejection value, already offset by the 64x64 tile origin
+  float s1x[3], s1y[3], ra1[3];
+  float ev2[3], s2x[3], s2y[3], ra2[3];
+  float zx, zy, z0, dzx, dzy;
+  float aa[4];
+  uint32_t idf;    // triangle id << 1 | tile-level accept
+  uint32_t r16;    // bit k: 16x16 region k of the GPU tile rejected; bit 16+k: accepted
+  uint32_t front;
+};
+
+__device__ __forceinline__ bool cmpf(int f, float a, float b) {
+  switch (f) {
+  case 0: return false;
+  case 1: return a < b;
+  case 2: return a == b;
+  case 3: return a <= b;
+  case 4: return a > b;
+  case 5: return a != b;
+  case 6: return a >= b;
+  default: return true;
+  }
+}
+__device__ __forceinline__ bool cmpu(int f, uint32_t a, uint32_t b) {
+  switch (f) {
+  case 0: return false;
+  case 1: return a < b;
+  case 2: return a == b;
+  case 3: return a <= b;
+  case 4: return a > b;
+  case 5: return a != b;
+  case 6: return a >= b;
+  default: return true;
+  }
+}
+// framebuffer.cpp:136-166
+__device__ __forceinline__ uint32_t stencil_op(int op, uint32_t ref, uint32_t cur) {
+  switch (op) {
+  case SOP_ZERO: return 0;
+  case SOP_REPLACE: return ref;
+  case SOP_INCR_SAT: return min(0xFFu, cur + 1);
+  case SOP_DECR_SAT: return cur - 1;  // std::max<uint32_t>(0, cur - 1) never clamps
+  case SOP_INVERT: return ~cur;
+  case SOP_INCR_WRAP: return (cur + 1) & 0xFF;
+  case SOP_DECR_WRAP: return (cur - 1 + 256) & 0xFF;
+  default: return cur;
+  }
+}
+
+__device__ __forceinline__ uint8_t* sample_addr(const surface_desc& s, int x, int y, int smp) {
+  return s.data + ((static_cast<size_t>(y) * s.w + x) * s.samples + smp) * pf_size(s.pf);
+}
+
+__device__ __forceinline__ void blend_store(const draw_params& P, int x, int y, int smp, float4 src) {
+  if (!P.color.data) return;
+  uint8_t* a = sample_addr(P.color, x, y, smp);
+  if (P.blend_id == BLEND_OFF) {
+    store_texel(P.color.pf, a, src);
+  } else if (P.blend_id == BLEND_ON) {
+    store_texel(P.color.pf, a, lerp4(load_texel(P.color.pf, a), src, src.w));
+  }
+}
+
+template <int NA, int S, bool DEFERRED>
+__global__ void __launch_bounds__(256) raster_kernel(draw_params P, ps_samplers SM, const uint32_t* __restrict__ vals,
+                                                      const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ tend,
+                                                      const float4* __restrict__ tri_rec, const float4* __restrict__ vbuf,
+                                                      const uint8_t* __restrict__ clip_flag, const int32_t* __restrict__ clip_slot,
+                                                      const float4* __restrict__ clip_verts, const uint32_t* __restrict__ clip_hdr,
+                                                      unsigned long long* __restrict__ stats) {
+  extern __shared__ prim_lds lds[];
+  const int G = P.gt, QW = G / 2;
+  const int gtile = blockIdx.x;
+  const int gx = gtile % P.gtiles_x, gy = gtile / P.gtiles_x;
+  const int tid = threadIdx.x, nthreads = blockDim.x;
+  const int qx = tid % QW, qy = tid / QW;
+  const int px0 = gx * G + qx * 2, py0 = gy * G + qy * 2;  // quad origin
+  const int t64x = (gx * G) & ~(TILE - 1), t64y = (gy * G) & ~(TILE - 1);
+  const int tw = static_cast<int>(P.target_w), th = static_cast<int>(P.target_h);
+  // pixel i of the quad: (px0 + (i & 1), py0 + (i >> 1))
+  bool pin[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) pin[i] = (px0 + (i & 1)) < tw && (py0 + (i >> 1)) < th;
+
+  // depth/stencil of the quad's samples, held in registers for the whole tile
+  float depth[4][S];
+  uint32_t stencil[4][S];
+  uint32_t winner[4][S];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      depth[i][s] = 0.0f;
+      stencil[i][s] = 0;
+      winner[i][s] = 0xFFFFFFFFu;
+      if (P.ds.data && pin[i]) {
+        float2 d = *reinterpret_cast<const float2*>(sample_addr(P.ds, px0 + (i & 1), py0 + (i >> 1), s));
+        depth[i][s] = d.x;
+        stencil[i][s] = __float_as_uint(d.y);
+      }
+    }
+  const bool any_in = pin[0];
+  const bool has_ds = P.ds.data != nullptr;
+  unsigned long long ps_inv = 0, be_pix = 0;
+  uint32_t full_px = (S == 1) ? 1u : (S == 2 ? 3u : 15u);
+
+  const uint32_t t0 = tstart[gtile], t1 = tend[gtile];
+  // stage-4 and pixel-level region of this quad inside its 64x64 tile
+  const int rx64 = px0 - t64x, ry64 = py0 - t64y;
+  const int ix16 = rx64 >> 4, iy16 = ry64 >> 4;
+  const int r16k = ((py0 - gy * G) >> 4) * (G / 16) + ((px0 - gx * G) >> 4);
+  const int R16x = t64x + ix16 * 16, R16y = t64y + iy16 * 16;
+  const int ix4 = (px0 - R16x) >> 2, iy4 = (py0 - R16y) >> 2;
+  const int Bx = R16x + ix4 * 4, By = R16y + iy4 * 4;
+
+  for (uint32_t base = t0; base < t1; base += nthreads) {
+    uint32_t nb = min(static_cast<uint32_t>(nthreads), t1 - base);
+    if (tid < static_cast<int>(nb)) {
+      prim_lds& L = lds[tid];
+      uint32_t rec = vals[base + tid];
+      uint32_t id = rec >> 1;
+      const float4* tr = tri_rec + static_cast<size_t>(id) * 5;
+      float e[9], bb[4];
+      read_tri_rec(tr, e, bb);
+      float4 d3 = tr[3], d4 = tr[4];
+      for (int k = 0; k < 9; ++k) L.e[k] = e[k];
+      const float tvx = static_cast<float>(t64x), tvy = static_cast<float>(t64y);
+      L.bb[0] = bb[0] - tvx;
+      L.bb[1] = bb[1] - tvx;
+      L.bb[2] = bb[2] - tvy;
+      L.bb[3] = bb[3] - tvy;
+      // per-stage constants (rasterize_triangle, rasterizer.cpp:662-675, 704-710)
+      for (int k = 0; k < 3; ++k) {
+        float ex = e[k * 3], ey = e[k * 3 + 1], ez = e[k * 3 + 2];
+        bool mx = ex > 0, my = ey > 0;
+        float sx = TILE * ex, sy = TILE * ey;
+        float ra = -fabsf(sx) - fabsf(sy);
+        float part = static_cast<float>(mx * TILE) * ex + static_cast<float>(my * TILE) * ey;
+        sx *= 0.25f; sy *= 0.25f; ra *= 0.25f; part *= 0.25f;
+        float ev = ez - part;
+        L.s1x[k] = sx; L.s1y[k] = sy; L.ra1[k] = ra;
+        L.ev1[k] = ev - ((tvx * ex) + (tvy * ey));
+        sx *= 0.25f; sy *= 0.25f; ra *= 0.25f; part *= 0.25f;
+        L.s2x[k] = sx; L.s2y[k] = sy; L.ra2[k] = ra;
+        L.ev2[k] = ez - part;
+      }
+      L.zx = d3.y;
+      L.zy = d3.z;
+      L.z0 = d3.w;
+      L.dzx = d4.x;
+      L.dzy = d4.y;
+      L.front = d4.z != 0.0f;
+      for (int s = 0; s < 4; ++s) L.aa[s] = (S > 1 && s < S) ? (P.sp[s][0] - 0.5f) * d4.x + (P.sp[s][1] - 0.5f) * d4.y : 0.0f;
+      L.idf = rec;
+      // stage-16 decision for each 16x16 region of the GPU tile (subdivide_tile, SSE order)
+      uint32_t bits = 0;
+      int nreg = G / 16;
+      for (int ry = 0; ry < nreg; ++ry)
+        for (int rx = 0; rx < nreg; ++rx) {
+          int jx = ((gx * G - t64x) >> 4) + rx, jy = ((gy * G - t64y) >> 4) + ry;
+          bool rej = false, nacc = false;
+          for (int k = 0; k < 3; ++k) {
+            float r = (L.s1x[k] * static_cast<float>(jx)) + (L.s1y[k] * static_cast<float>(jy));
+            float a = r + L.ra1[k];
+            rej |= r < L.ev1[k];
+            nacc |= a < L.ev1[k];
+          }
+          int mxr = jx * 16, myr = jy * 16;
+          rej |= L.bb[0] >= static_cast<float>(mxr + 16);
+          rej |= L.bb[1] < static_cast<float>(mxr);
+          rej |= L.bb[2] >= static_cast<float>(myr + 16);
+          rej |= L.bb[3] < static_cast<float>(myr);
+          int k = ry * nreg + rx;
+          if (rej) bits |= 1u << k;
+          else if (!nacc) bits |= 1u << (16 + k);
+        }
+      L.r16 = bits;
+    }
+    __syncthreads();
+    if (any_in) {
+      for (uint32_t j = 0; j < nb; ++j) {
+        const prim_lds& L = lds[j];
+        uint32_t cov[4];
+        if (L.idf & 1) {
+          cov[0] = cov[1] = cov[2] = cov[3] = full_px;
+        } else {
+          uint32_t bits = L.r16;
+          if (bits & (1u << r16k)) continue;
+          if (bits & (1u << (16 + r16k))) {
+            cov[0] = cov[1] = cov[2] = cov[3] = full_px;
+          } else {
+            // stage 4: child (ix4, iy4) of the 16x16 region at (R16x, R16y)
+            bool rej = false, nacc = false;
+            const float lf = static_cast<float>(R16x), tf = static_cast<float>(R16y);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              float ev = L.ev2[k] - ((lf * L.e[k * 3]) + (tf * L.e[k * 3 + 1]));
+              float r = (L.s2x[k] * static_cast<float>(ix4)) + (L.s2y[k] * static_cast<float>(iy4));
+              float a = r + L.ra2[k];
+              rej |= r < ev;
+              nacc |= a < ev;
+            }
+            int mxr = ix16 * 16 + ix4 * 4, myr = iy16 * 16 + iy4 * 4;
+            rej |= L.bb[0] >= static_cast<float>(mxr + 4);
+            rej |= L.bb[1] < static_cast<float>(mxr);
+            rej |= L.bb[2] >= static_cast<float>(myr + 4);
+            rej |= L.bb[3] < static_cast<float>(myr);
+            if (rej) continue;
+            if (!nacc) {
+              cov[0] = cov[1] = cov[2] = cov[3] = full_px;
+            } else {
+              // pixel level: draw_partial_tile on the 4x4 block at (Bx, By)
+              float evp[3];
+              const float bl = static_cast<float>(Bx), bt = static_cast<float>(By);
+#pragma unroll
+              for (int k = 0; k < 3; ++k) evp[k] = L.e[k * 3 + 2] - ((bl * L.e[k * 3]) + (bt * L.e[k * 3 + 1]));
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                uint32_t m = 0;
+                const float fx = static_cast<float>(px0 + (i & 1) - Bx), fy = static_cast<float>(py0 + (i >> 1) - By);
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                  float mx = P.sp[s][0] + fx, my = P.sp[s][1] + fy;
+                  bool r = false;
+#pragma unroll
+                  for (int k = 0; k < 3; ++k) r |= ((mx * L.e[k * 3]) + (my * L.e[k * 3 + 1])) < evp[k];
+                  if (!r) m |= 1u << s;
+                }
+                cov[i] = m;
+              }
+              if ((cov[0] | cov[1] | cov[2] | cov[3]) == 0) continue;
+            }
+          }
+        }
+        // pixels outside the render target are never drawn
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (!pin[i]) cov[i] = 0;
+        if ((cov[0] | cov[1] | cov[2] | cov[3]) == 0) continue;
+        // depth of the quad's pixels (step_2d_unproj_pos_quad, z lane)
+        const float dx = (0.5f + static_cast<float>(px0)) - L.zx;
+        const float dy = (0.5f + static_cast<float>(py0)) - L.zy;
+        float z[4];
+        z[0] = L.z0 + ((L.dzx * dx) + (L.dzy * dy));
+        z[1] = z[0] + L.dzx;
+        z[2] = z[0] + L.dzy;
+        z[3] = z[1] + L.dzy;
+        const uint32_t id = L.idf >> 1;
+        if (P.early_z) {
+          // early_z_test (framebuffer.cpp:522-614): depth written at test time
+          uint32_t tested[4] = {0, 0, 0, 0};
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+              if (!(cov[i] & (1u << s))) continue;
+              float nd = (S > 1) ? (L.aa[s] + z[i]) : z[i];
+              float od = P.read_depth ? depth[i][s] : 0.0f;
+              bool pass = P.depth_enable ? cmpf(P.depth_func, nd, od) : true;
+              if (pass) {
+                if (P.write_depth) depth[i][s] = nd;
+                tested[i] |= 1u << s;
+              }
+            }
+          if ((tested[0] | tested[1] | tested[2] | tested[3]) == 0) continue;
+          ps_inv += 4;
+          be_pix += 4;
+          if (DEFERRED) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+              for (int s = 0; s < S; ++s)
+                if (tested[i] & (1u << s)) winner[i][s] = id;
+          } else {
+            vtx<NA> tv[3];
+            load_clipped_triangle<NA>(P, vbuf, clip_flag, clip_slot, clip_verts, clip_hdr, id, tv);
+            tri_setup<NA> ts;
+            triangle_info<NA>(tv[0], tv[1], tv[2], ts);
+            float4 q[4][NA + 1];
+            interp_quad<NA>(ts, (0.5f + static_cast<float>(px0)) - ts.v0.r[0].x, (0.5f + static_cast<float>(py0)) - ts.v0.r[0].y, P, q);
+            float4 col[4];
+            run_ps<NA>(P, SM, q, 15u, col);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+              for (int s = 0; s < S; ++s)
+                if (tested[i] & (1u << s)) blend_store(P, px0 + (i & 1), py0 + (i >> 1), s, col[i]);
+          }
+        } else {
+          // late depth/stencil (render_sample, framebuffer.cpp:445-479); tests and writes only when both pass
+          ps_inv += 4;
+          be_pix += 4;
+          vtx<NA> tv[3];
+          load_clipped_triangle<NA>(P, vbuf, clip_flag, clip_slot, clip_verts, clip_hdr, id, tv);
+          tri_setup<NA> ts;
+          triangle_info<NA>(tv[0], tv[1], tv[2], ts);
+          float4 q[4][NA + 1];
+          interp_quad<NA>(ts, (0.5f + static_cast<float>(px0)) - ts.v0.r[0].x, (0.5f + static_cast<float>(py0)) - ts.v0.r[0].y, P, q);
+          float4 col[4];
+          run_ps<NA>(P, SM, q, 15u, col);
+          const int face = L.front ? 0 : 1;
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+              if (!(cov[i] & (1u << s))) continue;
+              float nd = (S > 1) ? (z[i] + L.aa[s]) : z[i];
+              float od = P.read_depth ? depth[i][s] : 0.0f;
+              uint32_t os = P.stencil_enable ? (stencil[i][s] & P.read_mask) : 0u;
+              bool dp = has_ds ? (P.depth_enable ? cmpf(P.depth_func, nd, od) : true) : true;
+              bool sp = P.stencil_enable ? cmpu(P.st_func[face], P.stencil_ref, os) : true;
+              if (dp && sp) {
+                uint32_t ns = P.stencil_enable ? stencil_op(P.st_pass_op[face], P.stencil_ref, os) : 0u;
+                blend_store(P, px0 + (i & 1), py0 + (i >> 1), s, col[i]);
+                if (P.write_depth) depth[i][s] = nd;
+                if (P.stencil_enable) stencil[i][s] = ns & P.write_mask;
+              }
+            }
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // deferred pixel shading: one shader pass per distinct winning triangle of the quad
+  if (DEFERRED && any_in && P.blend_id == BLEND_OFF && P.color.data) {
+    uint32_t done[4][S];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int s = 0; s < S; ++s) done[i][s] = (winner[i][s] == 0xFFFFFFFFu) ? 1u : 0u;
+    for (int iter = 0; iter < 4 * S; ++iter) {
+      uint32_t w = 0xFFFFFFFFu;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+          if (!done[i][s] && w == 0xFFFFFFFFu) w = winner[i][s];
+      if (w == 0xFFFFFFFFu) break;
+      uint32_t need = 0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+          if (!done[i][s] && winner[i][s] == w) need |= 1u << i;
+      vtx<NA> tv[3];
+      load_clipped_triangle<NA>(P, vbuf, clip_flag, clip_slot, clip_verts, clip_hdr, w, tv);
+      tri_setup<NA> ts;
+      triangle_info<NA>(tv[0], tv[1], tv[2], ts);
+      float4 q[4][NA + 1];
+      interp_quad<NA>(ts, (0.5f + static_cast<float>(px0)) - ts.v0.r[0].x, (0.5f + static_cast<float>(py0)) - ts.v0.r[0].y, P, q);
+      float4 col[4];
+      run_ps<NA>(P, SM, q, need, col);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+          if (!done[i][s] && winner[i][s] == w) {
+            store_texel(P.color.pf, sample_addr(P.color, px0 + (i & 1), py0 + (i >> 1), s), col[i]);
+            done[i][s] = 1;
+          }
+    }
+  }
+
+  // write back depth/stencil
+  if (has_ds && (P.write_depth || P.stencil_enable)) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (pin[i])
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+          *reinterpret_cast<float2*>(sample_addr(P.ds, px0 + (i & 1), py0 + (i >> 1), s)) =
+              make_float2(depth[i][s], __uint_as_float(stencil[i][s]));
+  }
+  wave_add(&stats[ST_PS_INVOCATIONS], ps_inv);
+  wave_add(&stats[ST_BACKEND_PIXELS], be_pix);
+}
+
+// ============================================================== host launchers
+void upload_ewa_table(const float* table256, hipStream_t s) {
+  hipMemcpyToSymbolAsync(HIP_SYMBOL(c_ewa), table256, 256 * sizeof(float), 0, hipMemcpyHostToDevice, s);
+}
+
+void launch_vs(const draw_params& P, float4* vbuf, hipStream_t s) {
+  uint32_t n = P.nverts + 1;
+  hipLaunchKernelGGL(vs_kernel, dim3((n + 255) / 256), dim3(256), 0, s, P, vbuf);
+}
+
+#define SRX_NA_DISPATCH(NAV, CALL) \
+  switch (NAV) {                   \
+  case 1: CALL(1); break;          \
+  case 2: CALL(2); break;          \
+  default: CALL(3); break;         \
+  }
+
+void launch_setup(const draw_params& P, const draw_buffers& B, hipStream_t s) {
+  if (P.prim_count == 0) return;
+  dim3 grid((P.prim_count + 255) / 256);
+#define SRX_SETUP(NA) \
+  hipLaunchKernelGGL(setup_kernel<NA>, grid, dim3(256), 0, s, P, B.vbuf, B.tri_rec, B.cnt, B.clip_flag, B.counters, B.stats)
+  SRX_NA_DISPATCH(P.nattr, SRX_SETUP)
+#undef SRX_SETUP
+}
+
+void launch_clip(const draw_params& P, const draw_buffers& B, hipStream_t s) {
+  if (P.prim_count == 0) return;
+  dim3 grid((P.prim_count + 63) / 64);
+#define SRX_CLIP(NA)                                                                                                     \
+  hipLaunchKernelGGL(clip_kernel<NA>, grid, dim3(64), 0, s, P, B.vbuf, B.tri_rec, B.cnt, B.clip_flag, B.clip_slot,       \
+                     B.clip_verts, B.clip_hdr, B.clip_capacity, B.counters, B.stats)
+  SRX_NA_DISPATCH(P.nattr, SRX_CLIP)
+#undef SRX_CLIP
+}
+
+void launch_emit(const draw_params& P, const draw_buffers& B, hipStream_t s) {
+  uint32_t n = 3 * P.prim_count;
+  if (!n) return;
+  hipLaunchKernelGGL(emit_kernel, dim3((n + 255) / 256), dim3(256), 0, s, P, B.tri_rec, B.cnt, B.counters + CT_TOTAL,
+                     B.keys[0], B.vals[0]);
+}
+
+void launch_raster(const draw_params& P, const ps_samplers& SM, const draw_buffers& B, int which, hipStream_t s) {
+  uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
+  if (!T) return;
+  int threads = (P.gt / 2) * (P.gt / 2);
+  size_t lds = sizeof(prim_lds) * threads;
+  bool deferred = P.early_z && (P.blend_id == BLEND_OFF || P.blend_id == BLEND_DISABLED);
+#define SRX_RASTER_S(NA, S)                                                                                              \
+  do {                                                                                                                   \
+    if (deferred)                                                                                                        \
+      hipLaunchKernelGGL((raster_kernel<NA, S, true>), dim3(T), dim3(threads), lds, s, P, SM, B.vals[which], B.tstart,   \
+                         B.tend, B.tri_rec, B.vbuf, B.clip_flag, B.clip_slot, B.clip_verts, B.clip_hdr, B.stats);        \
+    else                                                                                                                 \
+      hipLaunchKernelGGL((raster_kernel<NA, S, false>), dim3(T), dim3(threads), lds, s, P, SM, B.vals[which], B.tstart,  \
+                         B.tend, B.tri_rec, B.vbuf, B.clip_flag, B.clip_slot, B.clip_verts, B.clip_hdr, B.stats);        \
+  } while (0)
+#define SRX_RASTER(NA)                     \
+  switch (P.samples) {                     \
+  case 1: SRX_RASTER_S(NA, 1); break;      \
+  case 2: SRX_RASTER_S(NA, 2); break;      \
+  default: SRX_RASTER_S(NA, 4); break;     \
+  }
+  SRX_NA_DISPATCH(P.nattr, SRX_RASTER)
+#undef SRX_RASTER
+#undef SRX_RASTER_S
+}
+
+// ============================================================== clears, resolve, mip generation
+__global__ void clear_kernel(surface_desc s, float4 v) {
+  size_t n = static_cast<size_t>(s.w) * s.h * s.samples;
+  size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  store_texel(s.pf, s.data + i * pf_size(s.pf), v);
+}
+
+__global__ void clear_ds_kernel(surface_desc s, uint32_t flags, float depth, uint32_t stencil) {
+  size_t n = static_cast<size_t>(s.w) * s.h * s.samples;
+  size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float* p = reinterpret_cast<float*>(s.data + i * 8);
+  if (flags & 1) p[0] = depth;
+  if (flags & 2) p[1] = __uint_as_float(stencil);
+}
+
+// surface::resolve (surface.cpp:123-140): sum samples in order, then multiply by 1/S
+__global__ void resolve_kernel(surface_desc src, surface_desc dst) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= src.w) return;
+  float4 c = make_float4(0, 0, 0, 0);
+  for (int s = 0; s < src.samples; ++s) {
+    float4 t = load_texel(src.pf, sample_addr(src, x, y, s));
+    c = make_float4(c.x + t.x, c.y + t.y, c.z + t.z, c.w + t.w);
+  }
+  float inv = 1 / static_cast<float>(src.samples);
+  store_texel(dst.pf, sample_addr(dst, x, y, 0), make_float4(c.x * inv, c.y * inv, c.z * inv, c.w * inv));
+}
+
+// surface::make_mip_surface (surface.cpp:53-92); odd-size edge reads clamp to the last texel
+__global__ void mip_kernel(surface_desc src, surface_desc dst, int filter) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= dst.w) return;
+  for (int s = 0; s < dst.samples; ++s) {
+    int x0 = x * 2, y0 = y * 2, x1 = min(x0 + 1, src.w - 1), y1 = min(y0 + 1, src.h - 1);
+    float4 c;
+    if (filter == FILTER_POINT) {
+      c = load_texel(src.pf, sample_addr(src, x0, y0, s));
+    } else {
+      float4 c0 = load_texel(src.pf, sample_addr(src, x0, y0, s)), c1 = load_texel(src.pf, sample_addr(src, x1, y0, s));
+      float4 c2 = load_texel(src.pf, sample_addr(src, x0, y1, s)), c3 = load_texel(src.pf, sample_addr(src, x1, y1, s));
+      c = make_float4(((c0.x + c1.x) + c2.x) + c3.x, ((c0.y + c1.y) + c2.y) + c3.y, ((c0.z + c1.z) + c2.z) + c3.z,
+                      ((c0.w + c1.w) + c2.w) + c3.w);
+      c = make_float4(c.x * 0.25f, c.y * 0.25f, c.z * 0.25f, c.w * 0.25f);
+    }
+    store_texel(dst.pf, sample_addr(dst, x, y, s), c);
+  }
+}
+
+void launch_clear(const surface_desc& s, float4 v, hipStream_t st) {
+  size_t n = static_cast<size_t>(s.w) * s.h * s.samples;
+  if (n) hipLaunchKernelGGL(clear_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, s, v);
+}
+void launch_clear_ds(const surface_desc& s, uint32_t flags, float depth, uint32_t stencil, hipStream_t st) {
+  size_t n = static_cast<size_t>(s.w) * s.h * s.samples;
+  if (n) hipLaunchKernelGGL(clear_ds_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, s, flags, depth, stencil);
+}
+void launch_resolve(const surface_desc& src, const surface_desc& dst, hipStream_t st) {
+  hipLaunchKernelGGL(resolve_kernel, dim3((src.w + 255) / 256, src.h), dim3(256), 0, st, src, dst);
+}
+void launch_mip(const surface_desc& src, const surface_desc& dst, int filter, hipStream_t st) {
+  hipLaunchKernelGGL(mip_kernel, dim3((dst.w + 255) / 256, dst.h), dim3(256), 0, st, src, dst, filter);
+}
+
+}  // namespace srx

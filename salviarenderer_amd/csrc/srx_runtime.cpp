@@ -1,0 +1,990 @@
+// Host runtime of the HIP backend: the srx C ABI (include/srx.h) over device resources.
+//
+// One HIP stream per renderer carries every command in submission order (the reference's
+// async renderer queue, async_renderer.cpp:17-110); srx_flush() waits for it.  Resources live
+// in HBM in the reference's linear layout ((y*w + x)*samples + s)*elem_size (surface.cpp:277-295),
+// so map/read-back bytes are identical.  State validation follows renderer_impl.cpp.
+#include "../../include/srx.h"
+#include "srx_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace srx;
+
+namespace {
+
+thread_local int32_t g_last_error = 0;
+thread_local std::string g_last_error_str;
+
+int32_t fail(int32_t code, const char* msg) {
+  g_last_error = 0;
+  g_last_error_str = msg;
+  return code;
+}
+int32_t hip_fail(hipError_t e, const char* where) {
+  g_last_error = static_cast<int32_t>(e);
+  g_last_error_str = std::string(where) + ": " + hipGetErrorString(e);
+  return SRX_E_HIP;
+}
+#define HIPCHK(expr)                                 \
+  do {                                               \
+    hipError_t _e = (expr);                          \
+    if (_e != hipSuccess) return hip_fail(_e, #expr); \
+  } while (0)
+
+// ---------------------------------------------------------------- objects
+enum obj_kind { K_RENDERER = 1, K_BUFFER, K_TEXTURE, K_SURFACE, K_SAMPLER, K_LAYOUT, K_RASTER, K_DS, K_QUERY };
+
+struct object {
+  std::atomic<int> refs{1};
+  int kind;
+  explicit object(int k) : kind(k) {}
+  virtual ~object() = default;
+};
+void retain(object* o) {
+  if (o) o->refs.fetch_add(1);
+}
+void release(object* o) {
+  if (o && o->refs.fetch_sub(1) == 1) delete o;
+}
+template <class T>
+struct ref {
+  T* p = nullptr;
+  ref() = default;
+  explicit ref(T* q) : p(q) { retain(p); }
+  ref(const ref& o) : p(o.p) { retain(p); }
+  ref& operator=(const ref& o) {
+    if (this != &o) {
+      retain(o.p);
+      release(p);
+      p = o.p;
+    }
+    return *this;
+  }
+  ~ref() { release(p); }
+  T* operator->() const { return p; }
+  T* get() const { return p; }
+  explicit operator bool() const { return p != nullptr; }
+};
+
+int pf_bytes(int pf) {
+  switch (pf) {
+  case PF_RGBA32F: return 16;
+  case PF_RGB32F: return 12;
+  case PF_RG32F: return 8;
+  default: return 4;
+  }
+}
+
+struct buffer_obj : object {
+  uint8_t* d = nullptr;
+  size_t size = 0;
+  buffer_obj() : object(K_BUFFER) {}
+  ~buffer_obj() override {
+    if (d) (void)hipFree(d);
+  }
+};
+
+struct surface_obj : object {
+  int w, h, samples, pf;
+  uint8_t* d = nullptr;
+  size_t bytes = 0;
+  surface_obj(int w_, int h_, int s_, int pf_) : object(K_SURFACE), w(w_), h(h_), samples(s_), pf(pf_) {}
+  ~surface_obj() override {
+    if (d) (void)hipFree(d);
+  }
+  surface_desc desc() const { return surface_desc{d, w, h, samples, pf}; }
+};
+
+int32_t make_surface(int w, int h, int s, int pf, surface_obj** out) {
+  auto* o = new surface_obj(w, h, s, pf);
+  o->bytes = static_cast<size_t>(w) * h * s * pf_bytes(pf);
+  hipError_t e = hipMalloc(&o->d, o->bytes ? o->bytes : 1);
+  if (e != hipSuccess) {
+    delete o;
+    return hip_fail(e, "hipMalloc(surface)");
+  }
+  e = hipMemset(o->d, 0, o->bytes);
+  if (e != hipSuccess) {
+    delete o;
+    return hip_fail(e, "hipMemset(surface)");
+  }
+  *out = o;
+  return SRX_OK;
+}
+
+struct texture_obj : object {
+  int pf, samples;
+  int max_lod = 0, min_lod = 0;
+  uint32_t size[3];
+  std::vector<ref<surface_obj>> surfs;
+  texture_obj() : object(K_TEXTURE) {}
+};
+
+struct sampler_obj : object {
+  srx_sampler_desc desc;
+  ref<texture_obj> tex;
+  sampler_obj() : object(K_SAMPLER) {}
+};
+
+struct layout_elem {
+  std::string sem;
+  uint32_t index, fmt, slot, offset;
+};
+struct layout_obj : object {
+  std::vector<layout_elem> elems;
+  layout_obj() : object(K_LAYOUT) {}
+};
+struct raster_obj : object {
+  srx_raster_desc desc;
+  raster_obj() : object(K_RASTER) {}
+};
+struct ds_obj : object {
+  srx_depth_stencil_desc desc;
+  ds_obj() : object(K_DS) {}
+};
+struct query_obj : object {
+  uint32_t id;
+  bool running = false, has_data = false;
+  uint64_t begin_snap[ST_COUNT] = {};
+  srx_pipeline_statistics stats{};
+  srx_internal_statistics istats{};
+  query_obj() : object(K_QUERY) {}
+};
+
+std::string lower(const char* s) {
+  std::string r(s ? s : "");
+  for (auto& c : r)
+    if ('A' <= c && c <= 'Z') c = static_cast<char>(c - 'A' + 'a');
+  return r;
+}
+// semantic_value name normalisation (shader/constants.h:52-83)
+std::string canon_sem(const std::string& l) {
+  if (l == "position" || l == "sv_position") return "position";
+  if (l == "color" || l == "sv_target") return "color";
+  return l;
+}
+
+struct vs_binding {
+  const char* sem;
+  uint32_t index;
+  int reg;
+};
+int vs_bindings(int id, vs_binding* out) {
+  switch (id) {
+  case SRX_VS_PASSTHROUGH_COLOR:
+  case SRX_VS_WVP_COLOR:
+    out[0] = {"position", 0, 0};
+    out[1] = {"color", 0, 1};
+    return 2;
+  case SRX_VS_WVP_NORMAL_UV:
+    out[0] = {"position", 0, 0};
+    out[1] = {"normal", 0, 1};
+    out[2] = {"texcoord", 0, 2};
+    return 3;
+  }
+  return -1;
+}
+int vs_output_count(int id) {
+  switch (id) {
+  case SRX_VS_PASSTHROUGH_COLOR:
+  case SRX_VS_WVP_COLOR: return 1;
+  case SRX_VS_WVP_NORMAL_UV: return 2;
+  }
+  return 0;
+}
+uint32_t fmt_bytes(uint32_t fmt) {
+  switch (fmt) {
+  case SRX_FMT_R32_FLOAT: return 4;
+  case SRX_FMT_R32G32_FLOAT: return 8;
+  case SRX_FMT_R32G32B32_FLOAT: return 12;
+  case SRX_FMT_R32G32B32A32_FLOAT:
+  case SRX_FMT_R32G32B32A32_SINT: return 16;
+  }
+  return 0;
+}
+
+// Blender's EWA weights (exp(a r) - exp(a)) / (1 - exp(a)), a = -2, r = i/255, printed with six
+// significant digits -- the values of the reference's table (sampler.cpp:618-677).
+void make_ewa(float* t) {
+  for (int i = 0; i < 256; ++i) {
+    double r = i / 255.0;
+    double w = (std::exp(-2.0 * r) - std::exp(-2.0)) / (1.0 - std::exp(-2.0));
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%g", w);
+    t[i] = std::strtof(buf, nullptr);
+  }
+}
+
+// ---------------------------------------------------------------- renderer
+struct vb_slot {
+  ref<buffer_obj> buf;
+  size_t stride = 0, offset = 0;
+};
+
+struct render_state {
+  vb_slot vbs[SRX_MAX_INPUT_SLOTS];
+  ref<buffer_obj> ib;
+  uint32_t index_fmt = SRX_FMT_R16_UINT;
+  ref<layout_obj> layout;
+  uint32_t topo = SRX_TOPOLOGY_TRIANGLE_LIST;
+  int vs_id = 0, ps_id = 0, blend_id = SRX_BLEND_OFF;
+  float vs_consts[16] = {}, ps_consts[16] = {};
+  ref<sampler_obj> samplers[SRX_MAX_SAMPLER_SLOTS];
+  srx_raster_desc raster;
+  srx_depth_stencil_desc ds;
+  int32_t stencil_ref = 0;
+  std::vector<ref<surface_obj>> color_targets;
+  ref<surface_obj> ds_target;
+  srx_viewport target_vp{0, 0, 0, 0, 0, 0};
+  size_t target_samples = 0;
+  srx_viewport vp{0, 0, 0, 0, 0, 1};
+};
+
+template <class T>
+struct dev_array {
+  T* p = nullptr;
+  size_t cap = 0;
+};
+
+struct renderer_obj : object {
+  render_state st;
+  hipStream_t stream = nullptr;
+  std::vector<ref<query_obj>> active;
+  // per-draw device scratch, grown on demand
+  dev_array<float4> vbuf, tri_rec, clip_verts;
+  dev_array<uint32_t> cnt, keys0, keys1, vals0, vals1, tstart, tend, hist, scan_tmp, clip_hdr, counters;
+  dev_array<uint8_t> clip_flag;
+  dev_array<int32_t> clip_slot;
+  unsigned long long* stats_d = nullptr;
+  uint32_t* pinned = nullptr;  // host staging for counter read-back
+  uint64_t host_stats[ST_COUNT] = {};
+  renderer_obj() : object(K_RENDERER) {
+    srx_default_raster_desc(&st.raster);
+    srx_default_depth_stencil_desc(&st.ds);
+  }
+  ~renderer_obj() override {
+    if (stream) (void)hipStreamSynchronize(stream);
+    auto fr = [](auto& a) {
+      if (a.p) (void)hipFree(a.p);
+    };
+    fr(vbuf); fr(tri_rec); fr(clip_verts); fr(cnt); fr(keys0); fr(keys1); fr(vals0); fr(vals1);
+    fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot);
+    if (stats_d) (void)hipFree(stats_d);
+    if (pinned) (void)hipHostFree(pinned);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+template <class T>
+int32_t ensure(renderer_obj* r, dev_array<T>& a, size_t n) {
+  if (n <= a.cap && a.p) return SRX_OK;
+  size_t cap = std::max<size_t>(n + n / 2, 1024);
+  if (a.p) {
+    HIPCHK(hipStreamSynchronize(r->stream));
+    HIPCHK(hipFree(a.p));
+    a.p = nullptr;
+    a.cap = 0;
+  }
+  HIPCHK(hipMalloc(&a.p, cap * sizeof(T)));
+  a.cap = cap;
+  return SRX_OK;
+}
+
+std::once_flag g_ewa_once;
+int32_t g_ewa_status = SRX_OK;
+
+int gpu_tile_size() {
+  static int g = [] {
+    const char* e = std::getenv("SRX_GPU_TILE");
+    int v = e ? std::atoi(e) : 16;
+    return (v == 32) ? 32 : 16;
+  }();
+  return g;
+}
+
+void snapshot_stats(renderer_obj* r, uint64_t* out) {
+  unsigned long long dev[ST_COUNT] = {};
+  (void)hipStreamSynchronize(r->stream);
+  (void)hipMemcpy(dev, r->stats_d, sizeof dev, hipMemcpyDeviceToHost);
+  for (int i = 0; i < ST_COUNT; ++i) out[i] = r->host_stats[i] + dev[i];
+}
+
+int32_t read_counter(renderer_obj* r, uint32_t slot, uint32_t* out) {
+  HIPCHK(hipMemcpyAsync(r->pinned, r->counters.p + slot, sizeof(uint32_t), hipMemcpyDeviceToHost, r->stream));
+  HIPCHK(hipStreamSynchronize(r->stream));
+  *out = r->pinned[0];
+  return SRX_OK;
+}
+
+#define RCHK(expr)                     \
+  do {                                 \
+    int32_t _rc = (expr);              \
+    if (_rc != SRX_OK) return _rc;     \
+  } while (0)
+
+int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t primcnt, int32_t basevert) {
+  render_state& st = r->st;
+  if (st.color_targets.empty() && !st.ds_target) return SRX_OK;  // render_core.cpp:44-46
+  if (st.topo != SRX_TOPOLOGY_TRIANGLE_LIST && st.topo != SRX_TOPOLOGY_TRIANGLE_STRIP)
+    return fail(SRX_E_UNSUPPORTED, "only triangle lists and strips are rasterised (lines are unimplemented upstream)");
+  if (st.raster.fill_mode != SRX_FILL_SOLID) return fail(SRX_E_UNSUPPORTED, "wireframe fill is unimplemented upstream");
+  int nout = vs_output_count(st.vs_id);
+  if (nout <= 0) return fail(SRX_INVALID_PARAMETER, "no vertex shader bound");
+  if (!st.layout) return fail(SRX_INVALID_PARAMETER, "no input layout");
+  if (indexed && !st.ib) return fail(SRX_INVALID_PARAMETER, "no index buffer");
+  int samples = static_cast<int>(st.target_samples);
+  if (samples != 1 && samples != 2 && samples != 4) return fail(SRX_E_UNSUPPORTED, "sample count must be 1, 2 or 4");
+
+  draw_params P;
+  std::memset(&P, 0, sizeof P);
+  P.vs_id = st.vs_id;
+  P.nattr = nout;
+  for (int i = 0; i < MAX_ATTRS; ++i) P.attr_mod[i] = SRX_AM_LINEAR;
+  std::memcpy(P.vs_consts, st.vs_consts, sizeof P.vs_consts);
+
+  // stream assembler register map (stream_assembler.cpp:57-93)
+  vs_binding binds[8];
+  int nb = vs_bindings(st.vs_id, binds);
+  uint64_t count_max = 0;
+  for (int i = 0; i < nb; ++i) {
+    const layout_elem* le = nullptr;
+    for (auto& e : st.layout->elems)
+      if (e.sem == binds[i].sem && e.index == binds[i].index) le = &e;
+    if (!le) return fail(SRX_INVALID_PARAMETER, "input layout lacks a semantic the vertex shader reads");
+    if (le->slot >= SRX_MAX_INPUT_SLOTS || !st.vbs[le->slot].buf) return fail(SRX_INVALID_PARAMETER, "vertex buffer slot unbound");
+    const vb_slot& vb = st.vbs[le->slot];
+    uint32_t need = fmt_bytes(le->fmt);
+    if (!need) return fail(SRX_E_UNSUPPORTED, "vertex element format");
+    if (vb.stride == 0) return fail(SRX_E_UNSUPPORTED, "zero vertex stride");
+    size_t off = vb.offset + le->offset;
+    input_elem_dev& d = P.in[P.nin++];
+    d.base = vb.buf->d + std::min(off, vb.buf->size);
+    d.avail = off < vb.buf->size ? vb.buf->size - off : 0;
+    d.stride = static_cast<uint32_t>(vb.stride);
+    d.fmt = le->fmt;
+    d.reg = binds[i].reg;
+    d.default_w = (std::string(binds[i].sem) == "position") ? 1.0f : 0.0f;
+    uint64_t c = d.avail >= need ? (d.avail - need) / d.stride + 1 : 0;
+    count_max = std::max(count_max, c);
+  }
+  uint64_t needed = count_max;
+  if (!indexed) needed = std::min<uint64_t>(count_max, static_cast<uint64_t>(primcnt) * 3 + 2);
+  if (needed > 0x7FFFFFFFull) return fail(SRX_E_UNSUPPORTED, "vertex count");
+  P.nverts = static_cast<uint32_t>(needed);
+
+  P.ib = indexed ? st.ib->d : nullptr;
+  P.ib_size = indexed ? st.ib->size : 0;
+  P.istride = st.index_fmt == SRX_FMT_R16_UINT ? 2 : 4;
+  P.istart_bytes = start * P.istride;
+  P.indexed = indexed ? 1 : 0;
+  P.topo = st.topo;
+  P.basevert = basevert;
+  P.prim_count = primcnt;
+
+  P.cull_mode = st.raster.cull_mode;
+  P.front_ccw = st.raster.front_ccw;
+  P.vp[0] = st.vp.x; P.vp[1] = st.vp.y; P.vp[2] = st.vp.w; P.vp[3] = st.vp.h; P.vp[4] = st.vp.minz; P.vp[5] = st.vp.maxz;
+  P.target_w = st.target_vp.w;
+  P.target_h = st.target_vp.h;
+  P.samples = samples;
+  static const float p1[1][2] = {{0.5f, 0.5f}};
+  static const float p2[2][2] = {{0.25f, 0.25f}, {0.75f, 0.75f}};
+  static const float p4[4][2] = {{0.375f, 0.125f}, {0.875f, 0.375f}, {0.125f, 0.625f}, {0.625f, 0.875f}};
+  const float(*pat)[2] = samples == 1 ? p1 : samples == 2 ? p2 : p4;
+  for (int s = 0; s < samples; ++s) {
+    P.sp[s][0] = pat[s][0];
+    P.sp[s][1] = pat[s][1];
+  }
+  // rasterizer.cpp:1106-1108
+  P.tiles_x = static_cast<int32_t>(static_cast<size_t>(st.vp.w + 64 - 1) / 64);
+  P.tiles_y = static_cast<int32_t>(static_cast<size_t>(st.vp.h + 64 - 1) / 64);
+  P.gt = gpu_tile_size();
+  float cw = std::min(st.target_vp.w, static_cast<float>(P.tiles_x * 64));
+  float ch = std::min(st.target_vp.h, static_cast<float>(P.tiles_y * 64));
+  P.gtiles_x = static_cast<int32_t>(std::ceil(std::max(cw, 0.0f) / P.gt));
+  P.gtiles_y = static_cast<int32_t>(std::ceil(std::max(ch, 0.0f) / P.gt));
+
+  surface_obj* ct = nullptr;
+  for (auto& c : st.color_targets)
+    if (c) {
+      ct = c.get();
+      break;
+    }
+  if (!st.color_targets.empty()) ct = st.color_targets[0].get();
+  if (ct) P.color = ct->desc();
+  surface_obj* ds = st.ds_target.get();
+  if (ds) P.ds = ds->desc();
+  const srx_depth_stencil_desc& d = st.ds;
+  P.depth_enable = d.depth_enable;
+  P.depth_func = d.depth_func;
+  P.stencil_enable = ds && d.stencil_enable;
+  P.read_depth = ds && d.depth_enable && d.depth_func != SRX_CMP_NEVER && d.depth_func != SRX_CMP_ALWAYS;
+  P.write_depth = ds && d.depth_enable && d.depth_write_mask && d.depth_func != SRX_CMP_NEVER;
+  P.read_mask = d.stencil_read_mask & 0xFF;
+  P.write_mask = d.stencil_write_mask & 0xFF;
+  P.stencil_ref = P.stencil_enable ? (static_cast<uint32_t>(st.stencil_ref) & P.read_mask) : 0;
+  P.st_func[0] = d.front_face.stencil_func;
+  P.st_func[1] = d.back_face.stencil_func;
+  P.st_pass_op[0] = d.front_face.stencil_pass_op;
+  P.st_pass_op[1] = d.back_face.stencil_pass_op;
+  // early-Z: stencil off and no depth-writing pixel shader (framebuffer.cpp:424), per draw
+  P.early_z = ds && !d.stencil_enable;
+  P.blend_id = st.blend_id;
+  P.ps_id = st.ps_id;
+  std::memcpy(P.ps_consts, st.ps_consts, sizeof P.ps_consts);
+
+  ps_samplers SM;
+  std::memset(&SM, 0, sizeof SM);
+  for (int i = 0; i < MAX_SAMPLERS; ++i) {
+    sampler_obj* so = st.samplers[i].get();
+    if (!so || !so->tex) continue;
+    texture_obj* t = so->tex.get();
+    sampler_desc_dev& sd = SM.s[i];
+    sd.min_filter = so->desc.min_filter;
+    sd.mag_filter = so->desc.mag_filter;
+    sd.mip_filter = so->desc.mip_filter;
+    sd.mip_qual = so->desc.mip_qual;
+    sd.addr_u = so->desc.addr_mode_u;
+    sd.addr_v = so->desc.addr_mode_v;
+    sd.lod_bias = so->desc.mip_lod_bias;
+    sd.max_aniso = so->desc.max_anisotropy;
+    for (int k = 0; k < 4; ++k) sd.border[k] = so->desc.border_color[k];
+    sd.nlevels = static_cast<int>(std::min<size_t>(t->surfs.size(), MAX_LEVELS));
+    sd.max_lod = t->max_lod;
+    sd.min_lod = std::min(t->min_lod, sd.nlevels - 1);
+    sd.size[0] = t->size[0];
+    sd.size[1] = t->size[1];
+    sd.size[2] = t->size[2];
+    for (int l = 0; l < sd.nlevels; ++l) sd.levels[l] = t->surfs[l]->desc();
+    P.has_sampler[i] = 1;
+  }
+
+  hipStream_t s = r->stream;
+  uint32_t P3 = 3 * primcnt;
+  uint64_t& hs_vs = r->host_stats[ST_VS_INVOCATIONS];
+  r->host_stats[ST_IA_VERTICES] += static_cast<uint64_t>(primcnt) * 3;
+  r->host_stats[ST_IA_PRIMITIVES] += primcnt;
+  r->host_stats[ST_CINVOCATIONS] += primcnt;
+  hs_vs += P.nverts;
+  if (primcnt == 0) return SRX_OK;
+
+  RCHK(ensure(r, r->vbuf, (static_cast<size_t>(P.nverts) + 1) * (nout + 1)));
+  RCHK(ensure(r, r->tri_rec, static_cast<size_t>(P3) * 5));
+  RCHK(ensure(r, r->cnt, P3));
+  RCHK(ensure(r, r->clip_flag, primcnt));
+  RCHK(ensure(r, r->clip_slot, primcnt));
+  RCHK(ensure(r, r->scan_tmp, scan_tmp_size(P3)));
+  uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
+  RCHK(ensure(r, r->tstart, std::max<uint32_t>(T, 1)));
+  RCHK(ensure(r, r->tend, std::max<uint32_t>(T, 1)));
+
+  draw_buffers B{};
+  B.vbuf = r->vbuf.p;
+  B.tri_rec = r->tri_rec.p;
+  B.cnt = r->cnt.p;
+  B.clip_flag = r->clip_flag.p;
+  B.clip_slot = r->clip_slot.p;
+  B.counters = r->counters.p;
+  B.stats = r->stats_d;
+  B.scan_tmp = r->scan_tmp.p;
+  B.tstart = r->tstart.p;
+  B.tend = r->tend.p;
+
+  launch_fill_u32(B.counters, CT_COUNT, 0u, s);
+  launch_vs(P, B.vbuf, s);
+  launch_setup(P, B, s);
+  uint32_t nclip = 0;
+  RCHK(read_counter(r, CT_CLIP_PRIMS, &nclip));
+  if (nclip) {
+    RCHK(ensure(r, r->clip_verts, static_cast<size_t>(nclip) * 5 * (nout + 1)));
+    RCHK(ensure(r, r->clip_hdr, nclip));
+    B.clip_verts = r->clip_verts.p;
+    B.clip_hdr = r->clip_hdr.p;
+    B.clip_capacity = nclip;
+    launch_clip(P, B, s);
+  }
+  B.clip_verts = r->clip_verts.p;
+  B.clip_hdr = r->clip_hdr.p;
+  launch_exclusive_scan(B.cnt, P3, B.scan_tmp, B.counters + CT_TOTAL, s);
+  uint32_t total = 0;
+  RCHK(read_counter(r, CT_TOTAL, &total));
+  if (nclip) {
+    uint32_t ovf = 0;
+    RCHK(read_counter(r, CT_CLIP_OVERFLOW, &ovf));
+    if (ovf) return fail(SRX_OUT_OF_MEMORY, "clip buffer overflow");
+  }
+  uint32_t nblocks = (total + 4095) / 4096;
+  RCHK(ensure(r, r->keys0, std::max<uint32_t>(total, 1)));
+  RCHK(ensure(r, r->keys1, std::max<uint32_t>(total, 1)));
+  RCHK(ensure(r, r->vals0, std::max<uint32_t>(total, 1)));
+  RCHK(ensure(r, r->vals1, std::max<uint32_t>(total, 1)));
+  RCHK(ensure(r, r->hist, static_cast<size_t>(256) * std::max<uint32_t>(nblocks, 1)));
+  RCHK(ensure(r, r->scan_tmp, std::max(scan_tmp_size(P3), scan_tmp_size(256 * std::max<uint32_t>(nblocks, 1)))));
+  B.keys[0] = r->keys0.p;
+  B.keys[1] = r->keys1.p;
+  B.vals[0] = r->vals0.p;
+  B.vals[1] = r->vals1.p;
+  B.hist = r->hist.p;
+  B.scan_tmp = r->scan_tmp.p;
+  launch_emit(P, B, s);
+  int bits = 1;
+  while ((1u << bits) < T) ++bits;
+  int which = launch_sort(B, total, bits, s);
+  launch_tile_ranges(P, B, which, total, s);
+  launch_raster(P, SM, B, which, s);
+  HIPCHK(hipGetLastError());
+  return SRX_OK;
+}
+
+template <class T>
+T* as(void* h, int kind) {
+  auto* o = static_cast<object*>(h);
+  return (o && o->kind == kind) ? static_cast<T*>(o) : nullptr;
+}
+
+}  // namespace
+
+// ==================================================================== C ABI
+extern "C" {
+
+uint32_t srx_abi_version(void) { return SRX_ABI_VERSION; }
+const char* srx_backend_name(void) { return "hip"; }
+int32_t srx_last_error(void) { return g_last_error; }
+const char* srx_last_error_string(void) { return g_last_error_str.c_str(); }
+void srx_retain(void* h) { retain(static_cast<object*>(h)); }
+void srx_release(void* h) { release(static_cast<object*>(h)); }
+
+void srx_default_raster_desc(srx_raster_desc* d) {
+  *d = srx_raster_desc{SRX_FILL_SOLID, SRX_CULL_BACK, 0, 0, 0.0f, 0.0f, 1, 0, 1, 0};
+}
+void srx_default_depth_stencil_desc(srx_depth_stencil_desc* d) {
+  srx_depth_stencil_op_desc op{SRX_SOP_KEEP, SRX_SOP_KEEP, SRX_SOP_KEEP, SRX_CMP_ALWAYS};
+  *d = srx_depth_stencil_desc{1, 1, SRX_CMP_LESS, 0, 0xFF, 0xFF, op, op};
+}
+void srx_default_sampler_desc(srx_sampler_desc* d) {
+  *d = srx_sampler_desc{SRX_FILTER_POINT, SRX_FILTER_POINT, SRX_FILTER_POINT, SRX_MIP_MI, SRX_ADDRESS_WRAP,
+                        SRX_ADDRESS_WRAP, SRX_ADDRESS_WRAP, 0.0f, 0, SRX_CMP_ALWAYS, {0, 0, 0, 0}, -1e20f, 1e20f};
+}
+
+int32_t srx_create_renderer(uint32_t kind, srx_renderer* out) {
+  (void)kind;
+  if (!out) return SRX_INVALID_PARAMETER;
+  std::call_once(g_ewa_once, [] {
+    float t[256];
+    make_ewa(t);
+    upload_ewa_table(t, nullptr);
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) g_ewa_status = hip_fail(e, "EWA table upload");
+  });
+  if (g_ewa_status != SRX_OK) return g_ewa_status;
+  auto* r = new renderer_obj();
+  hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamDefault);
+  if (e == hipSuccess) e = hipMalloc(&r->counters.p, CT_COUNT * sizeof(uint32_t));
+  if (e == hipSuccess) r->counters.cap = CT_COUNT;
+  if (e == hipSuccess) e = hipMalloc(&r->stats_d, ST_COUNT * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemset(r->stats_d, 0, ST_COUNT * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipHostMalloc(&r->pinned, 64, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    delete r;
+    return hip_fail(e, "srx_create_renderer");
+  }
+  *out = reinterpret_cast<srx_renderer>(static_cast<object*>(r));
+  return SRX_OK;
+}
+
+int32_t srx_create_buffer(srx_renderer r, size_t size, srx_buffer* out) {
+  if (!as<renderer_obj>(r, K_RENDERER) || !out) return SRX_INVALID_PARAMETER;
+  auto* b = new buffer_obj();
+  b->size = size;
+  hipError_t e = hipMalloc(&b->d, size ? size : 1);
+  if (e == hipSuccess) e = hipMemset(b->d, 0, size ? size : 1);
+  if (e != hipSuccess) {
+    delete b;
+    return hip_fail(e, "srx_create_buffer");
+  }
+  *out = reinterpret_cast<srx_buffer>(static_cast<object*>(b));
+  return SRX_OK;
+}
+size_t srx_buffer_size(srx_buffer b) {
+  auto* o = as<buffer_obj>(b, K_BUFFER);
+  return o ? o->size : 0;
+}
+// Host copies go through the null stream, which waits for every renderer stream.
+int32_t srx_buffer_write(srx_buffer b, size_t off, const void* data, size_t size) {
+  auto* o = as<buffer_obj>(b, K_BUFFER);
+  if (!o || off + size > o->size || (!data && size)) return SRX_INVALID_PARAMETER;
+  if (size) HIPCHK(hipMemcpy(o->d + off, data, size, hipMemcpyHostToDevice));
+  return SRX_OK;
+}
+int32_t srx_buffer_read(srx_buffer b, size_t off, void* data, size_t size) {
+  auto* o = as<buffer_obj>(b, K_BUFFER);
+  if (!o || off + size > o->size || (!data && size)) return SRX_INVALID_PARAMETER;
+  if (size) HIPCHK(hipMemcpy(data, o->d + off, size, hipMemcpyDeviceToHost));
+  return SRX_OK;
+}
+
+int32_t srx_create_tex2d(srx_renderer r, size_t w, size_t h, size_t ns, int32_t fmt, srx_texture* out) {
+  if (!as<renderer_obj>(r, K_RENDERER) || !out || w == 0 || h == 0 || ns == 0 || fmt < 0 || fmt > 6 ||
+      w > SRX_MAX_RENDER_TARGET_SIZE * 2 || h > SRX_MAX_RENDER_TARGET_SIZE * 2 || ns > SRX_MAX_SAMPLE_COUNT)
+    return SRX_INVALID_PARAMETER;
+  auto* t = new texture_obj();
+  t->pf = fmt;
+  t->samples = static_cast<int>(ns);
+  t->size[0] = static_cast<uint32_t>(w);
+  t->size[1] = static_cast<uint32_t>(h);
+  t->size[2] = 1;
+  surface_obj* s = nullptr;
+  int32_t rc = make_surface(static_cast<int>(w), static_cast<int>(h), static_cast<int>(ns), fmt, &s);
+  if (rc != SRX_OK) {
+    delete t;
+    return rc;
+  }
+  t->surfs.emplace_back(s);
+  release(s);
+  *out = reinterpret_cast<srx_texture>(static_cast<object*>(t));
+  return SRX_OK;
+}
+int32_t srx_texture_surface(srx_texture t, size_t level, srx_surface* out) {
+  auto* o = as<texture_obj>(t, K_TEXTURE);
+  if (!o || !out || level >= o->surfs.size()) return SRX_INVALID_PARAMETER;
+  if (static_cast<int>(level) < o->max_lod || static_cast<int>(level) > o->min_lod) return SRX_INVALID_PARAMETER;
+  retain(o->surfs[level].get());
+  *out = reinterpret_cast<srx_surface>(static_cast<object*>(o->surfs[level].get()));
+  return SRX_OK;
+}
+// texture_2d::gen_mipmap (texture2d.cpp:25-36) with the box filter on the device
+int32_t srx_texture_gen_mipmap(srx_texture t, int32_t filter, int32_t auto_gen) {
+  auto* o = as<texture_obj>(t, K_TEXTURE);
+  if (!o || (filter != SRX_FILTER_POINT && filter != SRX_FILTER_LINEAR)) return SRX_INVALID_PARAMETER;
+  if (auto_gen) {
+    uint32_t m = std::max(o->size[0], std::max(o->size[1], o->size[2]));
+    int n = 0;
+    while (m > 0) {
+      m >>= 1;
+      ++n;
+    }
+    o->max_lod = 0;
+    o->min_lod = n - 1;
+  }
+  o->surfs.resize(1);
+  for (int l = o->max_lod; l < o->min_lod; ++l) {
+    surface_obj* src = o->surfs.back().get();
+    surface_obj* dst = nullptr;
+    RCHK(make_surface((src->w + 1) / 2, (src->h + 1) / 2, src->samples, src->pf, &dst));
+    launch_mip(src->desc(), dst->desc(), filter, nullptr);
+    o->surfs.emplace_back(dst);
+    release(dst);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  return SRX_OK;
+}
+size_t srx_texture_max_lod(srx_texture t) {
+  auto* o = as<texture_obj>(t, K_TEXTURE);
+  return o ? o->max_lod : 0;
+}
+size_t srx_texture_min_lod(srx_texture t) {
+  auto* o = as<texture_obj>(t, K_TEXTURE);
+  return o ? o->min_lod : 0;
+}
+size_t srx_surface_width(srx_surface s) { auto* o = as<surface_obj>(s, K_SURFACE); return o ? o->w : 0; }
+size_t srx_surface_height(srx_surface s) { auto* o = as<surface_obj>(s, K_SURFACE); return o ? o->h : 0; }
+size_t srx_surface_sample_count(srx_surface s) { auto* o = as<surface_obj>(s, K_SURFACE); return o ? o->samples : 0; }
+int32_t srx_surface_format(srx_surface s) { auto* o = as<surface_obj>(s, K_SURFACE); return o ? o->pf : -1; }
+size_t srx_surface_size_bytes(srx_surface s) { auto* o = as<surface_obj>(s, K_SURFACE); return o ? o->bytes : 0; }
+int32_t srx_surface_write(srx_surface s, const void* d, size_t n) {
+  auto* o = as<surface_obj>(s, K_SURFACE);
+  if (!o || n != o->bytes || !d) return SRX_INVALID_PARAMETER;
+  HIPCHK(hipMemcpy(o->d, d, n, hipMemcpyHostToDevice));
+  return SRX_OK;
+}
+int32_t srx_surface_read(srx_surface s, void* d, size_t n) {
+  auto* o = as<surface_obj>(s, K_SURFACE);
+  if (!o || n != o->bytes || !d) return SRX_INVALID_PARAMETER;
+  HIPCHK(hipMemcpy(d, o->d, n, hipMemcpyDeviceToHost));
+  return SRX_OK;
+}
+int32_t srx_surface_resolve(srx_renderer rr, srx_surface src, srx_surface dst) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  auto* a = as<surface_obj>(src, K_SURFACE);
+  auto* b = as<surface_obj>(dst, K_SURFACE);
+  if (!r || !a || !b || b->samples != 1 || a->w != b->w || a->h != b->h) return SRX_INVALID_PARAMETER;
+  launch_resolve(a->desc(), b->desc(), r->stream);
+  HIPCHK(hipGetLastError());
+  return SRX_OK;
+}
+
+int32_t srx_create_sampler(srx_renderer r, const srx_sampler_desc* d, srx_texture t, srx_sampler* out) {
+  auto* tex = as<texture_obj>(t, K_TEXTURE);
+  if (!as<renderer_obj>(r, K_RENDERER) || !d || !tex || !out) return SRX_INVALID_PARAMETER;
+  auto* s = new sampler_obj();
+  s->desc = *d;
+  s->tex = ref<texture_obj>(tex);
+  *out = reinterpret_cast<srx_sampler>(static_cast<object*>(s));
+  return SRX_OK;
+}
+int32_t srx_create_query(srx_renderer r, uint32_t id, srx_query* out) {
+  if (!as<renderer_obj>(r, K_RENDERER) || !out) return SRX_INVALID_PARAMETER;
+  if (id != SRX_QUERY_PIPELINE_STATISTICS && id != SRX_QUERY_INTERNAL_STATISTICS && id != SRX_QUERY_PIPELINE_PROFILES)
+    return SRX_INVALID_PARAMETER;
+  auto* q = new query_obj();
+  q->id = id;
+  *out = reinterpret_cast<srx_query>(static_cast<object*>(q));
+  return SRX_OK;
+}
+int32_t srx_create_input_layout(srx_renderer r, const srx_input_element_desc* d, size_t n, int32_t vs_id,
+                                srx_input_layout* out) {
+  (void)vs_id;
+  if (!as<renderer_obj>(r, K_RENDERER) || (!d && n) || !out) return SRX_INVALID_PARAMETER;
+  auto* l = new layout_obj();
+  for (size_t i = 0; i < n; ++i)
+    l->elems.push_back(layout_elem{canon_sem(lower(d[i].semantic_name)), d[i].semantic_index, d[i].data_format,
+                                   d[i].input_slot, d[i].aligned_byte_offset});
+  *out = reinterpret_cast<srx_input_layout>(static_cast<object*>(l));
+  return SRX_OK;
+}
+int32_t srx_create_raster_state(srx_renderer r, const srx_raster_desc* d, srx_raster_state* out) {
+  if (!as<renderer_obj>(r, K_RENDERER) || !d || !out) return SRX_INVALID_PARAMETER;
+  if (d->cull_mode < 0 || d->cull_mode > 2) return SRX_INVALID_PARAMETER;
+  auto* o = new raster_obj();
+  o->desc = *d;
+  *out = reinterpret_cast<srx_raster_state>(static_cast<object*>(o));
+  return SRX_OK;
+}
+int32_t srx_create_depth_stencil_state(srx_renderer r, const srx_depth_stencil_desc* d, srx_ds_state* out) {
+  if (!as<renderer_obj>(r, K_RENDERER) || !d || !out) return SRX_INVALID_PARAMETER;
+  auto* o = new ds_obj();
+  o->desc = *d;
+  *out = reinterpret_cast<srx_ds_state>(static_cast<object*>(o));
+  return SRX_OK;
+}
+
+int32_t srx_set_vertex_buffers(srx_renderer rr, size_t start, size_t n, const srx_buffer* bufs, const size_t* strides,
+                               const size_t* offsets) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || start + n > SRX_MAX_INPUT_SLOTS || (n && (!bufs || !strides))) return SRX_INVALID_PARAMETER;
+  for (size_t i = 0; i < n; ++i) {
+    r->st.vbs[start + i].buf = ref<buffer_obj>(as<buffer_obj>(bufs[i], K_BUFFER));
+    r->st.vbs[start + i].stride = strides[i];
+    r->st.vbs[start + i].offset = offsets ? offsets[i] : 0;
+  }
+  return SRX_OK;
+}
+int32_t srx_set_index_buffer(srx_renderer rr, srx_buffer b, uint32_t fmt) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r) return SRX_INVALID_PARAMETER;
+  if (fmt != SRX_FMT_R16_UINT && fmt != SRX_FMT_R32_UINT) return SRX_FAILED;  // renderer_impl.cpp:49-53
+  r->st.ib = ref<buffer_obj>(as<buffer_obj>(b, K_BUFFER));
+  r->st.index_fmt = fmt;
+  return SRX_OK;
+}
+int32_t srx_set_input_layout(srx_renderer rr, srx_input_layout l) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r) return SRX_INVALID_PARAMETER;
+  r->st.layout = ref<layout_obj>(as<layout_obj>(l, K_LAYOUT));
+  return SRX_OK;
+}
+int32_t srx_set_primitive_topology(srx_renderer rr, uint32_t t) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r) return SRX_INVALID_PARAMETER;
+  switch (t) {  // renderer_impl.cpp:70-82
+  case SRX_TOPOLOGY_LINE_LIST:
+  case SRX_TOPOLOGY_LINE_STRIP:
+  case SRX_TOPOLOGY_TRIANGLE_LIST:
+  case SRX_TOPOLOGY_TRIANGLE_STRIP: r->st.topo = t; return SRX_OK;
+  }
+  return SRX_FAILED;
+}
+int32_t srx_set_vertex_shader(srx_renderer rr, int32_t id, const void* consts, size_t size) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || vs_output_count(id) <= 0) return SRX_INVALID_PARAMETER;
+  if ((id == SRX_VS_WVP_NORMAL_UV || id == SRX_VS_WVP_COLOR) && size < 64) return SRX_INVALID_PARAMETER;
+  r->st.vs_id = id;
+  std::memset(r->st.vs_consts, 0, sizeof r->st.vs_consts);
+  if (consts && size) std::memcpy(r->st.vs_consts, consts, std::min<size_t>(size, 64));
+  return SRX_OK;
+}
+int32_t srx_set_pixel_shader(srx_renderer rr, int32_t id, const void* consts, size_t size) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || id < SRX_PS_GOURAUD || id > SRX_PS_CONSTANT) return SRX_INVALID_PARAMETER;
+  if ((id == SRX_PS_LAMBERT_TEX || id == SRX_PS_CONSTANT) && size < 16) return SRX_INVALID_PARAMETER;
+  r->st.ps_id = id;
+  std::memset(r->st.ps_consts, 0, sizeof r->st.ps_consts);
+  if (consts && size) std::memcpy(r->st.ps_consts, consts, std::min<size_t>(size, 64));
+  return SRX_OK;
+}
+int32_t srx_set_ps_sampler(srx_renderer rr, uint32_t slot, srx_sampler s) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || slot >= SRX_MAX_SAMPLER_SLOTS) return SRX_INVALID_PARAMETER;
+  r->st.samplers[slot] = ref<sampler_obj>(as<sampler_obj>(s, K_SAMPLER));
+  return SRX_OK;
+}
+int32_t srx_set_blend_shader(srx_renderer rr, int32_t id) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || id < 0 || id > 2) return SRX_INVALID_PARAMETER;
+  r->st.blend_id = id;
+  return SRX_OK;
+}
+int32_t srx_set_rasterizer_state(srx_renderer rr, srx_raster_state rs) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  auto* o = as<raster_obj>(rs, K_RASTER);
+  if (!r || !o) return SRX_INVALID_PARAMETER;
+  r->st.raster = o->desc;
+  return SRX_OK;
+}
+int32_t srx_set_depth_stencil_state(srx_renderer rr, srx_ds_state d, int32_t ref_) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  auto* o = as<ds_obj>(d, K_DS);
+  if (!r || !o) return SRX_INVALID_PARAMETER;
+  r->st.ds = o->desc;
+  r->st.stencil_ref = ref_;
+  return SRX_OK;
+}
+// renderer_impl::set_render_targets (renderer_impl.cpp:159-238)
+int32_t srx_set_render_targets(srx_renderer rr, size_t n, const srx_surface* cts, srx_surface dst) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r) return SRX_INVALID_PARAMETER;
+  render_state& st = r->st;
+  st.target_vp = srx_viewport{0, 0, 0, 0, 0, 0};
+  st.target_samples = 0;
+  st.color_targets.clear();
+  st.ds_target = ref<surface_obj>();
+  if (n >= 8) return SRX_FAILED;
+  auto* ds = as<surface_obj>(dst, K_SURFACE);
+  if ((n == 0 || !cts) && !ds) return SRX_FAILED;
+  srx_viewport tvp{0, 0, 3.4028234663852886e38f, 3.4028234663852886e38f, 0, 0};
+  size_t samples = 0;
+  for (size_t i = 0; i < n; ++i) {
+    auto* c = as<surface_obj>(cts[i], K_SURFACE);
+    if (!c) continue;
+    tvp.w = std::min(static_cast<float>(c->w), tvp.w);
+    tvp.h = std::min(static_cast<float>(c->h), tvp.h);
+    if (samples == 0) samples = c->samples;
+    else if (static_cast<size_t>(c->samples) != samples) return SRX_FAILED;
+  }
+  if (ds) {
+    if (ds->pf != PF_RG32F) return SRX_FAILED;
+    if (n == 0) {
+      samples = ds->samples;
+      tvp.w = static_cast<float>(ds->w);
+      tvp.h = static_cast<float>(ds->h);
+    }
+    if (static_cast<float>(ds->w) < tvp.w || static_cast<float>(ds->h) < tvp.h) return SRX_FAILED;
+    if (static_cast<size_t>(ds->samples) != samples) return SRX_FAILED;
+  }
+  st.target_vp = tvp;
+  st.target_samples = samples;
+  for (size_t i = 0; i < n; ++i) st.color_targets.emplace_back(as<surface_obj>(cts[i], K_SURFACE));
+  st.ds_target = ref<surface_obj>(ds);
+  return SRX_OK;
+}
+int32_t srx_set_viewport(srx_renderer rr, const srx_viewport* vp) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !vp) return SRX_INVALID_PARAMETER;
+  if (vp->x < 0 || vp->y < 0 || vp->w >= SRX_MAX_RENDER_TARGET_SIZE || vp->h >= SRX_MAX_RENDER_TARGET_SIZE) return SRX_FAILED;
+  r->st.vp = *vp;
+  return SRX_OK;
+}
+int32_t srx_get_viewport(srx_renderer rr, srx_viewport* vp) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !vp) return SRX_INVALID_PARAMETER;
+  *vp = r->st.vp;
+  return SRX_OK;
+}
+
+int32_t srx_begin(srx_renderer rr, srx_query qq) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  auto* q = as<query_obj>(qq, K_QUERY);
+  if (!r || !q) return SRX_INVALID_PARAMETER;
+  if (q->running) return SRX_FAILED;
+  for (auto& a : r->active)
+    if (a->id == q->id) return SRX_FAILED;
+  q->running = true;
+  q->has_data = false;
+  snapshot_stats(r, q->begin_snap);
+  r->active.emplace_back(q);
+  return SRX_OK;
+}
+int32_t srx_end(srx_renderer rr, srx_query qq) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  auto* q = as<query_obj>(qq, K_QUERY);
+  if (!r || !q || !q->running) return SRX_FAILED;
+  uint64_t now[ST_COUNT];
+  snapshot_stats(r, now);
+  uint64_t d[ST_COUNT];
+  for (int i = 0; i < ST_COUNT; ++i) d[i] = now[i] - q->begin_snap[i];
+  q->stats = srx_pipeline_statistics{};
+  q->stats.ia_vertices = d[ST_IA_VERTICES];
+  q->stats.ia_primitives = d[ST_IA_PRIMITIVES];
+  q->stats.vs_invocations = d[ST_VS_INVOCATIONS];
+  q->stats.cinvocations = d[ST_CINVOCATIONS];
+  q->stats.cprimitives = d[ST_CPRIMITIVES];
+  q->stats.ps_invocations = d[ST_PS_INVOCATIONS];
+  q->istats.backend_input_pixels = d[ST_BACKEND_PIXELS];
+  q->running = false;
+  q->has_data = true;
+  for (size_t i = 0; i < r->active.size(); ++i)
+    if (r->active[i].get() == q) {
+      r->active.erase(r->active.begin() + static_cast<long>(i));
+      break;
+    }
+  return SRX_OK;
+}
+int32_t srx_get_data(srx_renderer rr, srx_query qq, void* data, size_t size, int32_t do_not_wait) {
+  (void)do_not_wait;
+  auto* q = as<query_obj>(qq, K_QUERY);
+  if (!as<renderer_obj>(rr, K_RENDERER) || !q || !data || !q->has_data) return SRX_ASYNC_ERROR;
+  if (q->id == SRX_QUERY_PIPELINE_STATISTICS && size >= sizeof(q->stats)) std::memcpy(data, &q->stats, sizeof(q->stats));
+  else if (q->id == SRX_QUERY_INTERNAL_STATISTICS && size >= sizeof(q->istats)) std::memcpy(data, &q->istats, sizeof(q->istats));
+  else if (q->id == SRX_QUERY_PIPELINE_PROFILES && size >= 64) std::memset(data, 0, 64);
+  else return SRX_ASYNC_ERROR;
+  return SRX_ASYNC_READY;
+}
+int32_t srx_draw(srx_renderer rr, size_t start, size_t primcnt) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r) return SRX_INVALID_PARAMETER;
+  (void)start;  // the index fetcher ignores the start of non-indexed draws (index_fetcher.cpp:103-107)
+  return execute_draw(r, false, 0, static_cast<uint32_t>(primcnt), 0);
+}
+int32_t srx_draw_index(srx_renderer rr, size_t start, size_t primcnt, int32_t basevert) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r) return SRX_INVALID_PARAMETER;
+  return execute_draw(r, true, static_cast<uint32_t>(start), static_cast<uint32_t>(primcnt), basevert);
+}
+int32_t srx_clear_color(srx_renderer rr, srx_surface s, const float* c) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  auto* o = as<surface_obj>(s, K_SURFACE);
+  if (!r || !o || !c) return SRX_INVALID_PARAMETER;
+  launch_clear(o->desc(), make_float4(c[0], c[1], c[2], c[3]), r->stream);
+  HIPCHK(hipGetLastError());
+  return SRX_OK;
+}
+int32_t srx_clear_depth_stencil(srx_renderer rr, srx_surface s, uint32_t flags, float d, uint32_t st) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  auto* o = as<surface_obj>(s, K_SURFACE);
+  if (!r || !o || o->pf != PF_RG32F) return SRX_INVALID_PARAMETER;
+  if (flags != (SRX_CLEAR_DEPTH | SRX_CLEAR_STENCIL) && flags != SRX_CLEAR_DEPTH && flags != SRX_CLEAR_STENCIL)
+    return SRX_E_UNSUPPORTED;
+  launch_clear_ds(o->desc(), flags, d, st, r->stream);
+  HIPCHK(hipGetLastError());
+  return SRX_OK;
+}
+int32_t srx_flush(srx_renderer rr) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r) return SRX_INVALID_PARAMETER;
+  HIPCHK(hipStreamSynchronize(r->stream));
+  return SRX_OK;
+}
+
+}  // extern "C"
