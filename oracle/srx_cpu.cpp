@@ -142,7 +142,13 @@ unsigned thread_count() {
     if (n > 0) return static_cast<unsigned>(n);
   }
   unsigned n = std::thread::hardware_concurrency();
-  return n ? n : 1;
+  if (n == 0) n = 1;
+  // a host share smaller than the machine (OMP_NUM_THREADS) caps the pool
+  if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+    int m = std::atoi(e);
+    if (m > 0 && static_cast<unsigned>(m) < n) n = static_cast<unsigned>(m);
+  }
+  return n;
 }
 
 pool& global_pool() {
@@ -1317,6 +1323,8 @@ extern "C" {
 
 uint32_t srx_abi_version(void) { return SRX_ABI_VERSION; }
 const char* srx_backend_name(void) { return "cpu"; }
+// threads of the oracle's pool (reported beside the CPU baseline)
+uint32_t srxo_thread_count(void) { return global_pool().size(); }
 int32_t srx_last_error(void) { return g_last_error; }
 const char* srx_last_error_string(void) { return g_last_error_str.c_str(); }
 void srx_retain(void* h) { retain(static_cast<object*>(h)); }

@@ -24,6 +24,9 @@ enum counter_slot { CT_CLIP_PRIMS = 0, CT_CLIP_SLOTS, CT_CLIP_OVERFLOW, CT_TOTAL
 struct draw_buffers {
   float4* vbuf;        // (nverts+1) * (nattr+1) clip-space vertex-shader outputs
   float4* tri_rec;     // 3*prims * 5: edges, bbox, depth plane of each clipped triangle
+  float4* tri_attr;    // 3*prims * 3*(nattr+1): v0, ddx, ddy of every register
+  uint32_t* vis;       // target w*h*samples: winning triangle per sample (opaque early-Z draws)
+  const ps_samplers* samplers;  // device copy of the bound samplers
   uint32_t* cnt;       // 3*prims: GPU-tile records per clipped triangle (then its exclusive scan)
   uint8_t* clip_flag;  // prims
   int32_t* clip_slot;  // prims
@@ -37,7 +40,8 @@ struct draw_buffers {
   uint32_t* hist;      // radix histogram 256 * blocks
   uint32_t* scan_tmp;  // scan partial sums
   uint32_t* counters;  // CT_COUNT
-  unsigned long long* stats;  // ST_COUNT
+  unsigned long long* stats;    // ST_COUNT
+  unsigned long long* partial;  // per-workgroup statistics partials (>= max(setup blocks, GPU tiles))
 };
 
 void upload_ewa_table(const float* table256, hipStream_t s);
@@ -54,7 +58,11 @@ void launch_emit(const draw_params& P, const draw_buffers& B, hipStream_t s);
 // (0 or 1) that holds the result.
 int launch_sort(const draw_buffers& B, uint32_t n, int bits, hipStream_t s);
 void launch_tile_ranges(const draw_params& P, const draw_buffers& B, int which, uint32_t n, hipStream_t s);
-void launch_raster(const draw_params& P, const ps_samplers& SM, const draw_buffers& B, int which, hipStream_t s);
+void launch_raster(const draw_params& P, const draw_buffers& B, int which, hipStream_t s);
+// Order-independent visibility for opaque early-Z draws without statistics; false if not applicable.
+bool launch_vis(const draw_params& P, const draw_buffers& B, int which, hipStream_t s);
+bool needs_shading(const draw_params& P);
+void launch_shade(const draw_params& P, const draw_buffers& B, hipStream_t s);
 
 void launch_clear(const surface_desc& s, float4 value, hipStream_t st);
 // flags: 1 depth, 2 stencil

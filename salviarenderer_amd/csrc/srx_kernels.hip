@@ -318,25 +318,59 @@ __device__ __forceinline__ void write_tri_rec(float4* rec, const tri_setup<NA>& 
   rec[4] = make_float4(t.ddx.r[0].z, t.ddy.r[0].z, t.front ? 1.0f : 0.0f, 0.0f);
 }
 
+template <int NA>
+__device__ __forceinline__ void write_tri_attr(float4* a, const tri_setup<NA>& t) {
+#pragma unroll
+  for (int r = 0; r <= NA; ++r) {
+    a[r] = t.v0.r[r];
+    a[NA + 1 + r] = t.ddx.r[r];
+    a[2 * (NA + 1) + r] = t.ddy.r[r];
+  }
+}
+
 __device__ __forceinline__ void read_tri_rec(const float4* rec, float* e, float* bb) {
   float4 a = rec[0], b = rec[1], c = rec[2], d = rec[3];
   e[0] = a.x; e[1] = a.y; e[2] = a.z; e[3] = a.w; e[4] = b.x; e[5] = b.y; e[6] = b.z; e[7] = b.w; e[8] = c.x;
   bb[0] = c.y; bb[1] = c.z; bb[2] = c.w; bb[3] = d.x;
 }
 
-__device__ __forceinline__ void wave_add(unsigned long long* dst, unsigned long long v) {
-  // one atomic per wavefront
+// Sums v over the workgroup and stores the total in partial[blockIdx.x] (no global atomics: many
+// same-address device-scope atomics from every XCD serialise badly).  All threads must call it.
+__device__ __forceinline__ void block_partial(unsigned long long* __restrict__ partial, unsigned long long v) {
+  __shared__ unsigned long long wsum[16];
   unsigned long long s = v;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  if (__lane_id() == __builtin_ctzll(__ballot(1))) atomicAdd(dst, s);
+  const int wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) wsum[wave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < nw; ++w) t += wsum[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// stats[slot] += sum(partial[0..n))
+__global__ void __launch_bounds__(256) reduce_partials_kernel(const unsigned long long* __restrict__ partial, uint32_t n,
+                                                               unsigned long long* __restrict__ stats, int slot) {
+  __shared__ unsigned long long wsum[4];
+  unsigned long long t = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += 256) t += partial[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) stats[slot] += wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
 // ============================================================== setup (fast path)
 template <int NA>
 __global__ void __launch_bounds__(256) setup_kernel(draw_params P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
+                                                     float4* __restrict__ tri_attr,
                                                      uint32_t* __restrict__ cnt, uint8_t* __restrict__ clip_flag,
-                                                     uint32_t* __restrict__ counters, unsigned long long* __restrict__ stats) {
+                                                     uint32_t* __restrict__ counters, unsigned long long* __restrict__ partial) {
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long cprims = 0;
   if (p < P.prim_count) {
@@ -369,6 +403,7 @@ __global__ void __launch_bounds__(256) setup_kernel(draw_params P, const float4*
         triangle_info<NA>(o[0], o[1], o[2], t);
         if (t.valid) {
           write_tri_rec<NA>(tri_rec + static_cast<size_t>(3 * p) * 5, t);
+          write_tri_attr<NA>(tri_attr + static_cast<size_t>(3 * p) * 3 * (NA + 1), t);
           float e[9], bb[4];
           for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) e[i * 3 + j] = t.e[i][j];
@@ -380,7 +415,7 @@ __global__ void __launch_bounds__(256) setup_kernel(draw_params P, const float4*
       }
     }
   }
-  wave_add(&stats[ST_CPRIMITIVES], cprims);
+  block_partial(partial, cprims);
 }
 
 // ============================================================== clipping (slow path)
@@ -400,7 +435,7 @@ __device__ __forceinline__ void lerp_vtx(vtx<NA>& o, const vtx<NA>& a, const vtx
 // clip_triangle_to_poly_general + clip_solid_triangle (clipper.cpp:43-228)
 template <int NA>
 __global__ void clip_kernel(draw_params P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
-                            uint32_t* __restrict__ cnt, const uint8_t* __restrict__ clip_flag, int32_t* __restrict__ clip_slot,
+                            float4* __restrict__ tri_attr, uint32_t* __restrict__ cnt, const uint8_t* __restrict__ clip_flag, int32_t* __restrict__ clip_slot,
                             float4* __restrict__ clip_verts, uint32_t* __restrict__ clip_hdr, uint32_t capacity,
                             uint32_t* __restrict__ counters, unsigned long long* __restrict__ stats) {
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -453,7 +488,7 @@ __global__ void clip_kernel(draw_params P, const float4* __restrict__ vbuf, floa
   clip_hdr[slot] = n | (front ? 0x100u : 0u);
   float4* out = clip_verts + static_cast<size_t>(slot) * 5 * (NA + 1);
   for (uint32_t i = 0; i < n; ++i) {
-    viewport_project<NA>(cv[src][i], P);
+    viewport_project<NA>(cv[src][i], P);  // once per polygon vertex, as for the unique vertices
     for (int r = 0; r <= NA; ++r) out[i * (NA + 1) + r] = cv[src][i].r[r];
   }
   for (uint32_t t = 1; t + 1 < n; ++t) {
@@ -465,6 +500,7 @@ __global__ void clip_kernel(draw_params P, const float4* __restrict__ vbuf, floa
     uint32_t id = 3 * p + (t - 1);
     if (!ts.valid) continue;
     write_tri_rec<NA>(tri_rec + static_cast<size_t>(id) * 5, ts);
+    write_tri_attr<NA>(tri_attr + static_cast<size_t>(id) * 3 * (NA + 1), ts);
     float e[9], bb[4];
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) e[i * 3 + j] = ts.e[i][j];
@@ -704,11 +740,29 @@ __device__ __forceinline__ void normalize3(float x, float y, float z, float& ox,
   oz = z * inv;
 }
 
+// Per-triangle interpolation data: v0 (the reordered first vertex) and the screen-space
+// derivatives of every register (compute_derivative_n, shader.cpp:414-450).
+template <int NA>
+struct attr_setup {
+  float4 v0[NA + 1], dx[NA + 1], dy[NA + 1];
+};
+
+template <int NA>
+__device__ __forceinline__ void load_attr(const float4* __restrict__ tri_attr, uint32_t id, attr_setup<NA>& a) {
+  const float4* src = tri_attr + static_cast<size_t>(id) * 3 * (NA + 1);
+#pragma unroll
+  for (int r = 0; r <= NA; ++r) {
+    a.v0[r] = src[r];
+    a.dx[r] = src[NA + 1 + r];
+    a.dy[r] = src[2 * (NA + 1) + r];
+  }
+}
+
 // Pixel shader on a 2x2 quad (cpp_pixel_shader::execute, cpp_pixel_shader.cpp:62-74); colour is
 // produced for the pixels in `need`.  The sample shaders never discard.
 template <int NA>
-__device__ __forceinline__ void run_ps(const draw_params& P, const ps_samplers& SM, const float4 (&q)[4][NA + 1],
-                                       uint32_t need, float4 (&col)[4]) {
+__device__ __forceinline__ void run_ps(const draw_params& P, const ps_samplers* __restrict__ SM,
+                                       const float4 (&q)[4][NA + 1], uint32_t need, float4 (&col)[4]) {
   switch (P.ps_id) {
   case PS_GOURAUD:
 #pragma unroll
@@ -727,7 +781,7 @@ __device__ __forceinline__ void run_ps(const draw_params& P, const ps_samplers& 
         int row = i & 2, cl = i & 1;
         float dxu = q[row + 1][2].x - q[row][2].x, dxv = q[row + 1][2].y - q[row][2].y;
         float dyu = q[cl + 2][2].x - q[cl][2].x, dyv = q[cl + 2][2].y - q[cl][2].y;
-        float4 t = P.has_sampler[0] ? sample_2d_grad(SM.s[0], q[i][2].x, q[i][2].y, dxu, dxv, dyu, dyv, 0.0f)
+        float4 t = P.has_sampler[0] ? sample_2d_grad(SM->s[0], q[i][2].x, q[i][2].y, dxu, dxv, dyu, dyv, 0.0f)
                                     : make_float4(0, 0, 0, 0);
         float nx, ny, nz;
         normalize3(q[i][1].x, q[i][1].y, q[i][1].z, nx, ny, nz);
@@ -741,7 +795,7 @@ __device__ __forceinline__ void run_ps(const draw_params& P, const ps_samplers& 
     // cpp tex2d: one LOD per quad from p1 - p0 and p2 - p0 (cpp_pixel_shader.cpp:13-31)
     float lod = 0.0f;
     if (P.has_sampler[0]) {
-      const sampler_desc_dev& sp = SM.s[0];
+      const sampler_desc_dev& sp = SM->s[0];
       float dxu = q[1][1].x - q[0][1].x, dxv = q[1][1].y - q[0][1].y;
       float dyu = q[2][1].x - q[0][1].x, dyv = q[2][1].y - q[0][1].y;
       lod = (sp.mip_filter == FILTER_ANISO && sp.max_aniso > 1) ? calc_aniso(sp, dxu, dxv, dyu, dyv, 0.0f).lod
@@ -749,7 +803,7 @@ __device__ __forceinline__ void run_ps(const draw_params& P, const ps_samplers& 
     }
     for (int i = 0; i < 4; ++i) {
       if (!(need & (1u << i))) continue;
-      float4 t = P.has_sampler[0] ? sample_impl(SM.s[0], q[i][1].x, q[i][1].y, lod, nullptr) : make_float4(0, 0, 0, 0);
+      float4 t = P.has_sampler[0] ? sample_impl(SM->s[0], q[i][1].x, q[i][1].y, lod, nullptr) : make_float4(0, 0, 0, 0);
       t.w = 1.0f;
       col[i] = t;
     }
@@ -761,11 +815,15 @@ __device__ __forceinline__ void run_ps(const draw_params& P, const ps_samplers& 
   }
 }
 
-// step_2d_unproj_pos_quad + step_2d_unproj_attr_n_quad (shader.cpp:257-367)
+// step_2d_unproj_pos_quad + step_2d_unproj_attr_n_quad (shader.cpp:257-367) for the quad whose
+// top-left pixel is (px0, py0)
 template <int NA>
-__device__ __forceinline__ void interp_quad(const tri_setup<NA>& t, float dx, float dy, const draw_params& P, float4 (&q)[4][NA + 1]) {
+__device__ __forceinline__ void interp_quad(const attr_setup<NA>& t, int px0, int py0, const draw_params& P,
+                                            float4 (&q)[4][NA + 1]) {
+  const float dx = (0.5f + static_cast<float>(px0)) - t.v0[0].x;
+  const float dy = (0.5f + static_cast<float>(py0)) - t.v0[0].y;
   {
-    const float4 v = t.v0.r[0], a = t.ddx.r[0], b = t.ddy.r[0];
+    const float4 v = t.v0[0], a = t.dx[0], b = t.dy[0];
     float4 p00 = make_float4(v.x + ((a.x * dx) + (b.x * dy)), v.y + ((a.y * dx) + (b.y * dy)),
                              v.z + ((a.z * dx) + (b.z * dy)), v.w + ((a.w * dx) + (b.w * dy)));
     float4 p01 = make_float4(p00.x + a.x, p00.y + a.y, p00.z + a.z, p00.w + a.w);
@@ -779,7 +837,7 @@ __device__ __forceinline__ void interp_quad(const tri_setup<NA>& t, float dx, fl
   for (int i = 0; i < 4; ++i) iw[i] = 1.0f / q[i][0].w;
 #pragma unroll
   for (int r = 1; r <= NA; ++r) {
-    const float4 v = t.v0.r[r], a = t.ddx.r[r], b = t.ddy.r[r];
+    const float4 v = t.v0[r], a = t.dx[r], b = t.dy[r];
     float4 i00, i01, i10, i11;
     if (P.attr_mod[r - 1] & AM_NOINTERPOLATION) {
       i00 = i01 = i10 = i11 = v;
@@ -867,13 +925,17 @@ __device__ __forceinline__ void blend_store(const draw_params& P, int x, int y, 
   }
 }
 
-template <int NA, int S, bool DEFERRED>
-__global__ void __launch_bounds__(256) raster_kernel(draw_params P, ps_samplers SM, const uint32_t* __restrict__ vals,
+constexpr uint32_t NO_WINNER = 0xFFFFFFFFu;
+
+// One workgroup per GPU tile, one thread per 2x2 quad.  VIS: opaque early-Z draws -- record for every
+// sample the last triangle that passed (the colour it ends with) into `vis`; the pixel shader runs
+// later in shade_kernel.  Otherwise every quad runs the pixel shader and the backend in primitive order.
+template <int NA, int S, bool VIS>
+__global__ void __launch_bounds__(256) raster_kernel(draw_params P, const ps_samplers* __restrict__ SM,
+                                                      const uint32_t* __restrict__ vals,
                                                       const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ tend,
-                                                      const float4* __restrict__ tri_rec, const float4* __restrict__ vbuf,
-                                                      const uint8_t* __restrict__ clip_flag, const int32_t* __restrict__ clip_slot,
-                                                      const float4* __restrict__ clip_verts, const uint32_t* __restrict__ clip_hdr,
-                                                      unsigned long long* __restrict__ stats) {
+                                                      const float4* __restrict__ tri_rec, const float4* __restrict__ tri_attr,
+                                                      uint32_t* __restrict__ vis, unsigned long long* __restrict__ partial) {
   extern __shared__ prim_lds lds[];
   const int G = P.gt, QW = G / 2;
   const int gtile = blockIdx.x;
@@ -889,29 +951,34 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, ps_samplers 
   for (int i = 0; i < 4; ++i) pin[i] = (px0 + (i & 1)) < tw && (py0 + (i >> 1)) < th;
 
   // depth/stencil of the quad's samples, held in registers for the whole tile
+  // VIS draws have stencil off, so only depth is kept (and written back)
   float depth[4][S];
-  uint32_t stencil[4][S];
-  uint32_t winner[4][S];
+  uint32_t stencil[4][VIS ? 1 : S];
+  uint32_t winner[4][VIS ? S : 1];
+  const bool has_ds = P.ds.data != nullptr;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       depth[i][s] = 0.0f;
-      stencil[i][s] = 0;
-      winner[i][s] = 0xFFFFFFFFu;
-      if (P.ds.data && pin[i]) {
-        float2 d = *reinterpret_cast<const float2*>(sample_addr(P.ds, px0 + (i & 1), py0 + (i >> 1), s));
-        depth[i][s] = d.x;
-        stencil[i][s] = __float_as_uint(d.y);
+      if constexpr (!VIS) stencil[i][s] = 0;
+      if constexpr (VIS) winner[i][s] = NO_WINNER;
+      if (has_ds && pin[i]) {
+        if constexpr (VIS) {
+          depth[i][s] = *reinterpret_cast<const float*>(sample_addr(P.ds, px0 + (i & 1), py0 + (i >> 1), s));
+        } else {
+          float2 d = *reinterpret_cast<const float2*>(sample_addr(P.ds, px0 + (i & 1), py0 + (i >> 1), s));
+          depth[i][s] = d.x;
+          stencil[i][s] = __float_as_uint(d.y);
+        }
       }
     }
   const bool any_in = pin[0];
-  const bool has_ds = P.ds.data != nullptr;
   unsigned long long ps_inv = 0, be_pix = 0;
-  uint32_t full_px = (S == 1) ? 1u : (S == 2 ? 3u : 15u);
+  const uint32_t full_px = (S == 1) ? 1u : (S == 2 ? 3u : 15u);
 
   const uint32_t t0 = tstart[gtile], t1 = tend[gtile];
-  // stage-4 and pixel-level region of this quad inside its 64x64 tile
+  // 16x16, 4x4 regions of this quad inside its 64x64 tile
   const int rx64 = px0 - t64x, ry64 = py0 - t64y;
   const int ix16 = rx64 >> 4, iy16 = ry64 >> 4;
   const int r16k = ((py0 - gy * G) >> 4) * (G / 16) + ((px0 - gx * G) >> 4);
@@ -1034,7 +1101,6 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, ps_samplers 
                 }
                 cov[i] = m;
               }
-              if ((cov[0] | cov[1] | cov[2] | cov[3]) == 0) continue;
             }
           }
         }
@@ -1071,19 +1137,17 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, ps_samplers 
           if ((tested[0] | tested[1] | tested[2] | tested[3]) == 0) continue;
           ps_inv += 4;
           be_pix += 4;
-          if (DEFERRED) {
+          if constexpr (VIS) {
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
               for (int s = 0; s < S; ++s)
                 if (tested[i] & (1u << s)) winner[i][s] = id;
           } else {
-            vtx<NA> tv[3];
-            load_clipped_triangle<NA>(P, vbuf, clip_flag, clip_slot, clip_verts, clip_hdr, id, tv);
-            tri_setup<NA> ts;
-            triangle_info<NA>(tv[0], tv[1], tv[2], ts);
+            attr_setup<NA> at;
+            load_attr<NA>(tri_attr, id, at);
             float4 q[4][NA + 1];
-            interp_quad<NA>(ts, (0.5f + static_cast<float>(px0)) - ts.v0.r[0].x, (0.5f + static_cast<float>(py0)) - ts.v0.r[0].y, P, q);
+            interp_quad<NA>(at, px0, py0, P, q);
             float4 col[4];
             run_ps<NA>(P, SM, q, 15u, col);
 #pragma unroll
@@ -1092,16 +1156,14 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, ps_samplers 
               for (int s = 0; s < S; ++s)
                 if (tested[i] & (1u << s)) blend_store(P, px0 + (i & 1), py0 + (i >> 1), s, col[i]);
           }
-        } else {
-          // late depth/stencil (render_sample, framebuffer.cpp:445-479); tests and writes only when both pass
+        } else if constexpr (!VIS) {
+          // late depth/stencil (render_sample, framebuffer.cpp:445-479); writes only when both pass
           ps_inv += 4;
           be_pix += 4;
-          vtx<NA> tv[3];
-          load_clipped_triangle<NA>(P, vbuf, clip_flag, clip_slot, clip_verts, clip_hdr, id, tv);
-          tri_setup<NA> ts;
-          triangle_info<NA>(tv[0], tv[1], tv[2], ts);
+          attr_setup<NA> at;
+          load_attr<NA>(tri_attr, id, at);
           float4 q[4][NA + 1];
-          interp_quad<NA>(ts, (0.5f + static_cast<float>(px0)) - ts.v0.r[0].x, (0.5f + static_cast<float>(py0)) - ts.v0.r[0].y, P, q);
+          interp_quad<NA>(at, px0, py0, P, q);
           float4 col[4];
           run_ps<NA>(P, SM, q, 15u, col);
           const int face = L.front ? 0 : 1;
@@ -1128,48 +1190,25 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, ps_samplers 
     __syncthreads();
   }
 
-  // deferred pixel shading: one shader pass per distinct winning triangle of the quad
-  if (DEFERRED && any_in && P.blend_id == BLEND_OFF && P.color.data) {
-    uint32_t done[4][S];
+  if constexpr (VIS) {
+    // winners for the shading pass, laid out like the colour target's samples
 #pragma unroll
     for (int i = 0; i < 4; ++i)
+      if (pin[i]) {
+        size_t base = (static_cast<size_t>(py0 + (i >> 1)) * tw + (px0 + (i & 1))) * S;
 #pragma unroll
-      for (int s = 0; s < S; ++s) done[i][s] = (winner[i][s] == 0xFFFFFFFFu) ? 1u : 0u;
-    for (int iter = 0; iter < 4 * S; ++iter) {
-      uint32_t w = 0xFFFFFFFFu;
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int s = 0; s < S; ++s)
-          if (!done[i][s] && w == 0xFFFFFFFFu) w = winner[i][s];
-      if (w == 0xFFFFFFFFu) break;
-      uint32_t need = 0;
+        for (int s = 0; s < S; ++s) vis[base + s] = winner[i][s];
+      }
+    if (has_ds && P.write_depth) {
 #pragma unroll
       for (int i = 0; i < 4; ++i)
+        if (pin[i])
 #pragma unroll
-        for (int s = 0; s < S; ++s)
-          if (!done[i][s] && winner[i][s] == w) need |= 1u << i;
-      vtx<NA> tv[3];
-      load_clipped_triangle<NA>(P, vbuf, clip_flag, clip_slot, clip_verts, clip_hdr, w, tv);
-      tri_setup<NA> ts;
-      triangle_info<NA>(tv[0], tv[1], tv[2], ts);
-      float4 q[4][NA + 1];
-      interp_quad<NA>(ts, (0.5f + static_cast<float>(px0)) - ts.v0.r[0].x, (0.5f + static_cast<float>(py0)) - ts.v0.r[0].y, P, q);
-      float4 col[4];
-      run_ps<NA>(P, SM, q, need, col);
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int s = 0; s < S; ++s)
-          if (!done[i][s] && winner[i][s] == w) {
-            store_texel(P.color.pf, sample_addr(P.color, px0 + (i & 1), py0 + (i >> 1), s), col[i]);
-            done[i][s] = 1;
-          }
+          for (int s = 0; s < S; ++s)
+            *reinterpret_cast<float*>(sample_addr(P.ds, px0 + (i & 1), py0 + (i >> 1), s)) = depth[i][s];
     }
-  }
-
-  // write back depth/stencil
-  if (has_ds && (P.write_depth || P.stencil_enable)) {
+  } else if (has_ds && (P.write_depth || P.stencil_enable)) {
+    // write back depth/stencil
 #pragma unroll
     for (int i = 0; i < 4; ++i)
       if (pin[i])
@@ -1178,8 +1217,289 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, ps_samplers 
           *reinterpret_cast<float2*>(sample_addr(P.ds, px0 + (i & 1), py0 + (i >> 1), s)) =
               make_float2(depth[i][s], __uint_as_float(stencil[i][s]));
   }
-  wave_add(&stats[ST_PS_INVOCATIONS], ps_inv);
-  wave_add(&stats[ST_BACKEND_PIXELS], be_pix);
+  // ps_invocations and backend_input_pixels advance together (+4 per shaded quad: the sample shaders
+  // never discard), so one partial carries both
+  (void)be_pix;
+  block_partial(partial, ps_inv);
+}
+
+// ---------------------------------------------------------------- order-independent visibility
+// For an opaque early-Z draw with blending off, the reference's sequential early-Z test (depth written
+// at test time, framebuffer.cpp:522-614) leaves in every sample the result of a running reduction:
+//   LESS/GREATER (+write): the first triangle reaching the extreme depth wins;
+//   LESS_EQUAL/GREATER_EQUAL (+write): the last triangle reaching the extreme depth wins;
+//   ALWAYS, depth off, EQUAL, or any function without depth write: among the triangles that pass
+//   against the initial depth, the last one wins.
+// Each of these is a minimum over a 64-bit key (ordered depth, primitive order), so lanes can
+// rasterise different triangles of a tile at once and merge with LDS atomic-min.
+enum { VIS_LESS = 0, VIS_LEQUAL = 1, VIS_GREATER = 2, VIS_GEQUAL = 3, VIS_LATEST = 4, VIS_NONE = 5 };
+
+__device__ __forceinline__ uint32_t ord_depth(float z) {
+  uint32_t u = __float_as_uint(z + 0.0f);  // -0 compares equal to +0
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ unsigned long long initial_key(int mode, float d0) {
+  if (mode == VIS_LATEST || mode == VIS_NONE) return 0xFFFFFFFFull;
+  if (d0 != d0) return 0ull;  // a NaN depth lets nothing pass
+  uint32_t zk = ord_depth(d0);
+  if (mode == VIS_GREATER || mode == VIS_GEQUAL) zk = ~zk;
+  uint32_t lo = (mode == VIS_LESS || mode == VIS_GREATER) ? 0u : 0xFFFFFFFFu;
+  return (static_cast<unsigned long long>(zk) << 32) | lo;
+}
+
+// key of triangle `id` at depth z, or ~0 when it cannot pass
+__device__ __forceinline__ unsigned long long sample_key(int mode, const draw_params& P, float z, float d0, uint32_t id) {
+  switch (mode) {
+  case VIS_LESS:
+  case VIS_GREATER: {
+    if (z != z) return ~0ull;
+    uint32_t zk = ord_depth(z);
+    if (mode == VIS_GREATER) zk = ~zk;
+    return (static_cast<unsigned long long>(zk) << 32) | (id + 1u);
+  }
+  case VIS_LEQUAL:
+  case VIS_GEQUAL: {
+    if (z != z) return ~0ull;
+    uint32_t zk = ord_depth(z);
+    if (mode == VIS_GEQUAL) zk = ~zk;
+    return (static_cast<unsigned long long>(zk) << 32) | (0xFFFFFFFEu - id);
+  }
+  case VIS_LATEST: {
+    float od = P.read_depth ? d0 : 0.0f;
+    bool pass = P.depth_enable ? cmpf(P.depth_func, z, od) : true;
+    return pass ? static_cast<unsigned long long>(0xFFFFFFFEu - id) : ~0ull;
+  }
+  default: return ~0ull;
+  }
+}
+
+__device__ __forceinline__ uint32_t key_winner(int mode, unsigned long long k) {
+  uint32_t lo = static_cast<uint32_t>(k);
+  if (mode == VIS_LESS || mode == VIS_GREATER) return lo == 0u ? NO_WINNER : lo - 1u;
+  return lo == 0xFFFFFFFFu ? NO_WINNER : 0xFFFFFFFEu - lo;
+}
+
+// 16x16 GPU tile, one triangle per lane.  keys: 256 pixels x S samples.
+template <int S>
+__global__ void __launch_bounds__(128) vis_kernel(draw_params P, int mode, const uint32_t* __restrict__ vals,
+                                                   const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ tend,
+                                                   const float4* __restrict__ tri_rec, uint32_t* __restrict__ vis) {
+  __shared__ unsigned long long keys[256 * S];
+  __shared__ float d0s[256 * S];
+  const int gtile = blockIdx.x;
+  const int gx = gtile % P.gtiles_x, gy = gtile / P.gtiles_x;
+  const int ox = gx * 16, oy = gy * 16;
+  const int t64x = ox & ~(TILE - 1), t64y = oy & ~(TILE - 1);
+  const int tw = static_cast<int>(P.target_w), th = static_cast<int>(P.target_h);
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const bool has_ds = P.ds.data != nullptr;
+  for (int i = tid; i < 256 * S; i += NT) {
+    int px = ox + ((i / S) & 15), py = oy + ((i / S) >> 4), smp = i % S;
+    float d0 = 0.0f;
+    if (has_ds && px < tw && py < th) d0 = *reinterpret_cast<const float*>(sample_addr(P.ds, px, py, smp));
+    d0s[i] = d0;
+    keys[i] = initial_key(mode, d0);
+  }
+  __syncthreads();
+  const uint32_t t0 = tstart[gtile], t1 = tend[gtile];
+  const uint32_t full_px = (S == 1) ? 1u : (S == 2 ? 3u : 15u);
+  const int jx = (ox - t64x) >> 4, jy = (oy - t64y) >> 4;
+  for (uint32_t base = t0 + tid; base < t1; base += NT) {
+    const uint32_t rec = vals[base];
+    const uint32_t id = rec >> 1;
+    const float4* tr = tri_rec + static_cast<size_t>(id) * 5;
+    float e[9], bb[4];
+    read_tri_rec(tr, e, bb);
+    const float4 d3 = tr[3], d4 = tr[4];
+    const float zx = d3.y, zy = d3.z, z0 = d3.w, dzx = d4.x, dzy = d4.y;
+    float aa[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) aa[s] = (S > 1) ? (P.sp[s][0] - 0.5f) * dzx + (P.sp[s][1] - 0.5f) * dzy : 0.0f;
+    const float tvx = static_cast<float>(t64x), tvy = static_cast<float>(t64y);
+    const float bx0 = bb[0] - tvx, bx1 = bb[1] - tvx, by0 = bb[2] - tvy, by1 = bb[3] - tvy;
+    // per-stage constants (rasterize_triangle, rasterizer.cpp:662-675, 704-710)
+    float s1x[3], s1y[3], ra1[3], ev1[3], s2x[3], s2y[3], ra2[3], ev2[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float ex = e[k * 3], ey = e[k * 3 + 1], ez = e[k * 3 + 2];
+      bool mx = ex > 0, my = ey > 0;
+      float sx = TILE * ex, sy = TILE * ey;
+      float ra = -fabsf(sx) - fabsf(sy);
+      float part = static_cast<float>(mx * TILE) * ex + static_cast<float>(my * TILE) * ey;
+      sx *= 0.25f; sy *= 0.25f; ra *= 0.25f; part *= 0.25f;
+      s1x[k] = sx; s1y[k] = sy; ra1[k] = ra;
+      ev1[k] = (ez - part) - ((tvx * ex) + (tvy * ey));
+      sx *= 0.25f; sy *= 0.25f; ra *= 0.25f; part *= 0.25f;
+      s2x[k] = sx; s2y[k] = sy; ra2[k] = ra;
+      ev2[k] = ez - part;
+    }
+    bool region_full = (rec & 1) != 0;
+    if (!region_full) {
+      // stage 16 (subdivide_tile on the 64x64 tile)
+      bool rej = false, nacc = false;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        float r = (s1x[k] * static_cast<float>(jx)) + (s1y[k] * static_cast<float>(jy));
+        float a = r + ra1[k];
+        rej |= r < ev1[k];
+        nacc |= a < ev1[k];
+      }
+      rej |= bx0 >= static_cast<float>(jx * 16 + 16);
+      rej |= bx1 < static_cast<float>(jx * 16);
+      rej |= by0 >= static_cast<float>(jy * 16 + 16);
+      rej |= by1 < static_cast<float>(jy * 16);
+      if (rej) continue;
+      region_full = !nacc;
+    }
+    float ev2q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      ev2q[k] = ev2[k] - ((static_cast<float>(ox) * e[k * 3]) + (static_cast<float>(oy) * e[k * 3 + 1]));
+    // candidate 4x4 blocks: all of them for an accepted region, else those the stage-4 bounding-box
+    // test keeps (bx0 < mxr + 4 and bx1 >= mxr, the same comparisons on the same floats)
+    uint32_t cols = 0, rows = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int mxr = jx * 16 + c * 4, myr = jy * 16 + c * 4;
+      if (region_full || !(bx0 >= static_cast<float>(mxr + 4) || bx1 < static_cast<float>(mxr))) cols |= 0x1111u << c;
+      if (region_full || !(by0 >= static_cast<float>(myr + 4) || by1 < static_cast<float>(myr))) rows |= 0xFu << (c * 4);
+    }
+    uint32_t blocks = cols & rows;
+    while (blocks) {
+      const int b = __builtin_ctz(blocks);
+      blocks &= blocks - 1;
+      const int ix4 = b & 3, iy4 = b >> 2;
+      const int Bx = ox + ix4 * 4, By = oy + iy4 * 4;
+      if (Bx >= tw || By >= th) continue;
+      bool block_full = region_full;
+      if (!block_full) {
+        // stage 4 (subdivide_tile on the 16x16 region); the bounding-box half was applied above
+        bool rej = false, nacc = false;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          float r = (s2x[k] * static_cast<float>(ix4)) + (s2y[k] * static_cast<float>(iy4));
+          float a = r + ra2[k];
+          rej |= r < ev2q[k];
+          nacc |= a < ev2q[k];
+        }
+        if (rej) continue;
+        block_full = !nacc;
+      }
+      float evp[3];
+      if (!block_full) {
+        // pixel level (draw_partial_tile on the 4x4 block)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          evp[k] = e[k * 3 + 2] - ((static_cast<float>(Bx) * e[k * 3]) + (static_cast<float>(By) * e[k * 3 + 1]));
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int qx0 = Bx + (q & 1) * 2, qy0 = By + (q >> 1) * 2;
+        const float dx = (0.5f + static_cast<float>(qx0)) - zx;
+        const float dy = (0.5f + static_cast<float>(qy0)) - zy;
+        float z[4];
+        z[0] = z0 + ((dzx * dx) + (dzy * dy));
+        z[1] = z[0] + dzx;
+        z[2] = z[0] + dzy;
+        z[3] = z[1] + dzy;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int px = qx0 + (i & 1), py = qy0 + (i >> 1);
+          if (px >= tw || py >= th) continue;
+          uint32_t m = full_px;
+          if (!block_full) {
+            m = 0;
+            const float fx = static_cast<float>(px - Bx), fy = static_cast<float>(py - By);
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+              float mx = P.sp[s][0] + fx, my = P.sp[s][1] + fy;
+              bool r = false;
+#pragma unroll
+              for (int k = 0; k < 3; ++k) r |= ((mx * e[k * 3]) + (my * e[k * 3 + 1])) < evp[k];
+              if (!r) m |= 1u << s;
+            }
+          }
+          const int li = ((py - oy) * 16 + (px - ox)) * S;
+#pragma unroll
+          for (int s = 0; s < S; ++s) {
+            if (!(m & (1u << s))) continue;
+            float nd = (S > 1) ? (aa[s] + z[i]) : z[i];
+            unsigned long long k = sample_key(mode, P, nd, d0s[li + s], id);
+            if (k != ~0ull) atomicMin(&keys[li + s], k);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // winners -> visibility buffer and depth
+  for (int i = tid; i < 256 * S; i += NT) {
+    const int p = i / S, smp = i % S;
+    const int px = ox + (p & 15), py = oy + (p >> 4);
+    if (px >= tw || py >= th) continue;
+    const uint32_t w = key_winner(mode, keys[i]);
+    vis[(static_cast<size_t>(py) * tw + px) * S + smp] = w;
+    if (w != NO_WINNER && has_ds && P.write_depth) {
+      // the winner's depth at this sample, as its quad computed it
+      const float4* tr = tri_rec + static_cast<size_t>(w) * 5;
+      const float4 d3 = tr[3], d4 = tr[4];
+      const int qx0 = px & ~1, qy0 = py & ~1;
+      const float dx = (0.5f + static_cast<float>(qx0)) - d3.y;
+      const float dy = (0.5f + static_cast<float>(qy0)) - d3.z;
+      float z00 = d3.w + ((d4.x * dx) + (d4.y * dy));
+      float zp;
+      if ((px & 1) == 0 && (py & 1) == 0) zp = z00;
+      else if ((py & 1) == 0) zp = z00 + d4.x;
+      else if ((px & 1) == 0) zp = z00 + d4.y;
+      else zp = (z00 + d4.x) + d4.y;
+      float nd = zp;
+      if (S > 1) nd = ((P.sp[smp][0] - 0.5f) * d4.x + (P.sp[smp][1] - 0.5f) * d4.y) + zp;
+      *reinterpret_cast<float*>(sample_addr(P.ds, px, py, smp)) = nd;
+    }
+  }
+}
+
+// Deferred pixel shading of an opaque early-Z draw: one thread per pixel (16x16 pixels per
+// workgroup).  For each distinct winning triangle of the pixel's samples, the thread evaluates that
+// triangle's quad interpolants (the values the reference's last passing draw_quad/draw_full_quad
+// computed for this quad), runs the pixel shader for its own pixel and writes the winning samples.
+template <int NA, int S>
+__global__ void __launch_bounds__(256) shade_kernel(draw_params P, const ps_samplers* __restrict__ SM,
+                                                     const float4* __restrict__ tri_attr, const uint32_t* __restrict__ vis) {
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int tw = static_cast<int>(P.target_w), th = static_cast<int>(P.target_h);
+  if (x >= tw || y >= th) return;
+  const int px0 = x & ~1, py0 = y & ~1;
+  const int pi = (x & 1) | ((y & 1) << 1);  // position in the quad
+  const size_t base = (static_cast<size_t>(y) * tw + x) * S;
+  uint32_t w[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) w[s] = vis[base + s];
+#pragma unroll
+  for (int it = 0; it < S; ++it) {
+    uint32_t cur = NO_WINNER;
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+      if (cur == NO_WINNER) cur = w[s];
+    if (cur == NO_WINNER) break;
+    attr_setup<NA> at;
+    load_attr<NA>(tri_attr, cur, at);
+    float4 q[4][NA + 1];
+    interp_quad<NA>(at, px0, py0, P, q);
+    float4 col[4];
+    run_ps<NA>(P, SM, q, 1u << pi, col);
+    float4 c = col[0];
+#pragma unroll
+    for (int k = 1; k < 4; ++k)
+      if (k == pi) c = col[k];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+      if (w[s] == cur) {
+        store_texel(P.color.pf, sample_addr(P.color, x, y, s), c);
+        w[s] = NO_WINNER;
+      }
+  }
 }
 
 // ============================================================== host launchers
@@ -1203,16 +1523,17 @@ void launch_setup(const draw_params& P, const draw_buffers& B, hipStream_t s) {
   if (P.prim_count == 0) return;
   dim3 grid((P.prim_count + 255) / 256);
 #define SRX_SETUP(NA) \
-  hipLaunchKernelGGL(setup_kernel<NA>, grid, dim3(256), 0, s, P, B.vbuf, B.tri_rec, B.cnt, B.clip_flag, B.counters, B.stats)
+  hipLaunchKernelGGL(setup_kernel<NA>, grid, dim3(256), 0, s, P, B.vbuf, B.tri_rec, B.tri_attr, B.cnt, B.clip_flag, B.counters, B.partial)
   SRX_NA_DISPATCH(P.nattr, SRX_SETUP)
 #undef SRX_SETUP
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, B.partial, grid.x, B.stats, static_cast<int>(ST_CPRIMITIVES));
 }
 
 void launch_clip(const draw_params& P, const draw_buffers& B, hipStream_t s) {
   if (P.prim_count == 0) return;
   dim3 grid((P.prim_count + 63) / 64);
 #define SRX_CLIP(NA)                                                                                                     \
-  hipLaunchKernelGGL(clip_kernel<NA>, grid, dim3(64), 0, s, P, B.vbuf, B.tri_rec, B.cnt, B.clip_flag, B.clip_slot,       \
+  hipLaunchKernelGGL(clip_kernel<NA>, grid, dim3(64), 0, s, P, B.vbuf, B.tri_rec, B.tri_attr, B.cnt, B.clip_flag, B.clip_slot, \
                      B.clip_verts, B.clip_hdr, B.clip_capacity, B.counters, B.stats)
   SRX_NA_DISPATCH(P.nattr, SRX_CLIP)
 #undef SRX_CLIP
@@ -1225,20 +1546,20 @@ void launch_emit(const draw_params& P, const draw_buffers& B, hipStream_t s) {
                      B.keys[0], B.vals[0]);
 }
 
-void launch_raster(const draw_params& P, const ps_samplers& SM, const draw_buffers& B, int which, hipStream_t s) {
+void launch_raster(const draw_params& P, const draw_buffers& B, int which, hipStream_t s) {
   uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
   if (!T) return;
   int threads = (P.gt / 2) * (P.gt / 2);
   size_t lds = sizeof(prim_lds) * threads;
-  bool deferred = P.early_z && (P.blend_id == BLEND_OFF || P.blend_id == BLEND_DISABLED);
-#define SRX_RASTER_S(NA, S)                                                                                              \
-  do {                                                                                                                   \
-    if (deferred)                                                                                                        \
-      hipLaunchKernelGGL((raster_kernel<NA, S, true>), dim3(T), dim3(threads), lds, s, P, SM, B.vals[which], B.tstart,   \
-                         B.tend, B.tri_rec, B.vbuf, B.clip_flag, B.clip_slot, B.clip_verts, B.clip_hdr, B.stats);        \
-    else                                                                                                                 \
-      hipLaunchKernelGGL((raster_kernel<NA, S, false>), dim3(T), dim3(threads), lds, s, P, SM, B.vals[which], B.tstart,  \
-                         B.tend, B.tri_rec, B.vbuf, B.clip_flag, B.clip_slot, B.clip_verts, B.clip_hdr, B.stats);        \
+  bool vis = P.early_z && (P.blend_id == BLEND_OFF || P.blend_id == BLEND_DISABLED);
+#define SRX_RASTER_S(NA, S)                                                                                          \
+  do {                                                                                                               \
+    if (vis)                                                                                                         \
+      hipLaunchKernelGGL((raster_kernel<NA, S, true>), dim3(T), dim3(threads), lds, s, P, B.samplers, B.vals[which], \
+                         B.tstart, B.tend, B.tri_rec, B.tri_attr, B.vis, B.partial);                                 \
+    else                                                                                                             \
+      hipLaunchKernelGGL((raster_kernel<NA, S, false>), dim3(T), dim3(threads), lds, s, P, B.samplers,               \
+                         B.vals[which], B.tstart, B.tend, B.tri_rec, B.tri_attr, B.vis, B.partial);                  \
   } while (0)
 #define SRX_RASTER(NA)                     \
   switch (P.samples) {                     \
@@ -1249,6 +1570,59 @@ void launch_raster(const draw_params& P, const ps_samplers& SM, const draw_buffe
   SRX_NA_DISPATCH(P.nattr, SRX_RASTER)
 #undef SRX_RASTER
 #undef SRX_RASTER_S
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, B.partial, T, B.stats, static_cast<int>(ST_PS_INVOCATIONS));
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, B.partial, T, B.stats, static_cast<int>(ST_BACKEND_PIXELS));
+}
+
+// Visibility mode of an opaque early-Z draw for vis_kernel, or -1 when only the in-order kernel
+// reproduces the reference (NOT_EQUAL with depth writes).
+int vis_mode(const draw_params& P) {
+  if (!P.depth_enable) return VIS_LATEST;
+  if (P.depth_func == 0) return VIS_NONE;
+  if (!P.write_depth) return VIS_LATEST;
+  switch (P.depth_func) {
+  case 1: return VIS_LESS;
+  case 3: return VIS_LEQUAL;
+  case 4: return VIS_GREATER;
+  case 6: return VIS_GEQUAL;
+  case 2:
+  case 7: return VIS_LATEST;  // EQUAL keeps the depth it compares against; ALWAYS ignores it
+  default: return -1;
+  }
+}
+
+bool launch_vis(const draw_params& P, const draw_buffers& B, int which, hipStream_t s) {
+  int mode = vis_mode(P);
+  uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
+  if (mode < 0 || P.gt != 16) return false;
+  if (!T) return true;
+  switch (P.samples) {
+  case 1: hipLaunchKernelGGL((vis_kernel<1>), dim3(T), dim3(128), 0, s, P, mode, B.vals[which], B.tstart, B.tend, B.tri_rec, B.vis); break;
+  case 2: hipLaunchKernelGGL((vis_kernel<2>), dim3(T), dim3(128), 0, s, P, mode, B.vals[which], B.tstart, B.tend, B.tri_rec, B.vis); break;
+  default: hipLaunchKernelGGL((vis_kernel<4>), dim3(T), dim3(128), 0, s, P, mode, B.vals[which], B.tstart, B.tend, B.tri_rec, B.vis); break;
+  }
+  return true;
+}
+
+bool needs_shading(const draw_params& P) {
+  return P.early_z && P.blend_id == BLEND_OFF && P.color.data != nullptr;
+}
+
+void launch_shade(const draw_params& P, const draw_buffers& B, hipStream_t s) {
+  int w = static_cast<int>(P.target_w), h = static_cast<int>(P.target_h);
+  if (w <= 0 || h <= 0) return;
+  dim3 grid((w + 15) / 16, (h + 15) / 16);
+#define SRX_SHADE_S(NA, S) \
+  hipLaunchKernelGGL((shade_kernel<NA, S>), grid, dim3(256), 0, s, P, B.samplers, B.tri_attr, B.vis)
+#define SRX_SHADE(NA)                    \
+  switch (P.samples) {                   \
+  case 1: SRX_SHADE_S(NA, 1); break;     \
+  case 2: SRX_SHADE_S(NA, 2); break;     \
+  default: SRX_SHADE_S(NA, 4); break;    \
+  }
+  SRX_NA_DISPATCH(P.nattr, SRX_SHADE)
+#undef SRX_SHADE
+#undef SRX_SHADE_S
 }
 
 // ============================================================== clears, resolve, mip generation

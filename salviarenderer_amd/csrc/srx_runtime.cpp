@@ -153,13 +153,25 @@ struct ds_obj : object {
   srx_depth_stencil_desc desc;
   ds_obj() : object(K_DS) {}
 };
+// pipeline_profiles slots (async_object.h:187-196): gather_vtx, vtx_proc, clipping, compact_clip,
+// vp_trans, tri_dispatch, ras; slot 7 counts the profiled draws.
+enum { PROF_GATHER = 0, PROF_VTX, PROF_CLIP, PROF_COMPACT, PROF_VP, PROF_DISPATCH, PROF_RAS, PROF_DRAWS, PROF_COUNT };
+
 struct query_obj : object {
   uint32_t id;
   bool running = false, has_data = false;
   uint64_t begin_snap[ST_COUNT] = {};
   srx_pipeline_statistics stats{};
   srx_internal_statistics istats{};
+  uint64_t prof[PROF_COUNT] = {};
   query_obj() : object(K_QUERY) {}
+};
+
+// HIP events bracketing the stages of one profiled draw, on the renderer's stream:
+// [0,1) vertex shader, [1,2) setup + clipping (viewport transform fused), [2,3) compaction scan,
+// [3,4) record emission + tile sort, [4,5) raster.
+struct draw_events {
+  hipEvent_t ev[6];
 };
 
 std::string lower(const char* s) {
@@ -262,13 +274,33 @@ struct renderer_obj : object {
   hipStream_t stream = nullptr;
   std::vector<ref<query_obj>> active;
   // per-draw device scratch, grown on demand
-  dev_array<float4> vbuf, tri_rec, clip_verts;
-  dev_array<uint32_t> cnt, keys0, keys1, vals0, vals1, tstart, tend, hist, scan_tmp, clip_hdr, counters;
+  dev_array<float4> vbuf, tri_rec, tri_attr, clip_verts;
+  dev_array<ps_samplers> samplers_d;
+  ps_samplers* samplers_h = nullptr;  // pinned staging of the per-draw sampler table
+  dev_array<uint32_t> cnt, keys0, keys1, vals0, vals1, tstart, tend, hist, scan_tmp, clip_hdr, counters, vis;
   dev_array<uint8_t> clip_flag;
   dev_array<int32_t> clip_slot;
+  dev_array<unsigned long long> partial;
   unsigned long long* stats_d = nullptr;
   uint32_t* pinned = nullptr;  // host staging for counter read-back
   uint64_t host_stats[ST_COUNT] = {};
+  std::vector<draw_events> pending_prof;  // recorded while a pipeline_profiles query runs
+  std::vector<hipEvent_t> event_pool;
+  bool profiling() const {
+    for (auto& q : active)
+      if (q->id == SRX_QUERY_PIPELINE_PROFILES) return true;
+    return false;
+  }
+  hipEvent_t take_event() {
+    if (!event_pool.empty()) {
+      hipEvent_t e = event_pool.back();
+      event_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
   renderer_obj() : object(K_RENDERER) {
     srx_default_raster_desc(&st.raster);
     srx_default_depth_stencil_desc(&st.ds);
@@ -278,10 +310,15 @@ struct renderer_obj : object {
     auto fr = [](auto& a) {
       if (a.p) (void)hipFree(a.p);
     };
-    fr(vbuf); fr(tri_rec); fr(clip_verts); fr(cnt); fr(keys0); fr(keys1); fr(vals0); fr(vals1);
-    fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot);
+    fr(vbuf); fr(tri_rec); fr(tri_attr); fr(clip_verts); fr(samplers_d); fr(vis);
+    if (samplers_h) (void)hipHostFree(samplers_h); fr(cnt); fr(keys0); fr(keys1); fr(vals0); fr(vals1);
+    fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot); fr(partial);
     if (stats_d) (void)hipFree(stats_d);
     if (pinned) (void)hipHostFree(pinned);
+    for (auto& d : pending_prof)
+      for (auto e : d.ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto e : event_pool) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -481,28 +518,53 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
 
   RCHK(ensure(r, r->vbuf, (static_cast<size_t>(P.nverts) + 1) * (nout + 1)));
   RCHK(ensure(r, r->tri_rec, static_cast<size_t>(P3) * 5));
+  RCHK(ensure(r, r->tri_attr, static_cast<size_t>(P3) * 3 * (nout + 1)));
+  bool vis_pass = P.early_z && (P.blend_id == SRX_BLEND_OFF || P.blend_id == SRX_BLEND_DISABLED);
+  if (vis_pass)
+    RCHK(ensure(r, r->vis, static_cast<size_t>(std::max(P.target_w, 1.0f)) * static_cast<size_t>(std::max(P.target_h, 1.0f)) * samples));
+  RCHK(ensure(r, r->samplers_d, 1));
   RCHK(ensure(r, r->cnt, P3));
   RCHK(ensure(r, r->clip_flag, primcnt));
   RCHK(ensure(r, r->clip_slot, primcnt));
   RCHK(ensure(r, r->scan_tmp, scan_tmp_size(P3)));
   uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
+  RCHK(ensure(r, r->partial, std::max<size_t>(T, (primcnt + 255) / 256) + 1));
   RCHK(ensure(r, r->tstart, std::max<uint32_t>(T, 1)));
   RCHK(ensure(r, r->tend, std::max<uint32_t>(T, 1)));
 
   draw_buffers B{};
   B.vbuf = r->vbuf.p;
   B.tri_rec = r->tri_rec.p;
+  B.tri_attr = r->tri_attr.p;
+  B.vis = r->vis.p;
+  B.samplers = r->samplers_d.p;
   B.cnt = r->cnt.p;
   B.clip_flag = r->clip_flag.p;
   B.clip_slot = r->clip_slot.p;
   B.counters = r->counters.p;
   B.stats = r->stats_d;
+  B.partial = r->partial.p;
   B.scan_tmp = r->scan_tmp.p;
   B.tstart = r->tstart.p;
   B.tend = r->tend.p;
 
+  bool prof = r->profiling();
+  draw_events de{};
+  if (prof)
+    for (auto& e : de.ev) {
+      e = r->take_event();
+      if (!e) return fail(SRX_E_HIP, "hipEventCreate");
+    }
+  auto mark = [&](int i) {
+    if (prof) (void)hipEventRecord(de.ev[i], s);
+  };
+  // sampler table: staged in pinned memory; the stream syncs below before the staging is reused
+  *r->samplers_h = SM;
+  HIPCHK(hipMemcpyAsync(r->samplers_d.p, r->samplers_h, sizeof(ps_samplers), hipMemcpyHostToDevice, s));
   launch_fill_u32(B.counters, CT_COUNT, 0u, s);
+  mark(0);
   launch_vs(P, B.vbuf, s);
+  mark(1);
   launch_setup(P, B, s);
   uint32_t nclip = 0;
   RCHK(read_counter(r, CT_CLIP_PRIMS, &nclip));
@@ -516,7 +578,9 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   }
   B.clip_verts = r->clip_verts.p;
   B.clip_hdr = r->clip_hdr.p;
+  mark(2);
   launch_exclusive_scan(B.cnt, P3, B.scan_tmp, B.counters + CT_TOTAL, s);
+  mark(3);
   uint32_t total = 0;
   RCHK(read_counter(r, CT_TOTAL, &total));
   if (nclip) {
@@ -542,8 +606,40 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   while ((1u << bits) < T) ++bits;
   int which = launch_sort(B, total, bits, s);
   launch_tile_ranges(P, B, which, total, s);
-  launch_raster(P, SM, B, which, s);
+  mark(4);
+  // statistics queries need the in-order walk (ps_invocations counts early-Z passes as they happen)
+  bool counting = false;
+  for (auto& q : r->active)
+    if (q->id != SRX_QUERY_PIPELINE_PROFILES) counting = true;
+  if (!(vis_pass && !counting && launch_vis(P, B, which, s))) launch_raster(P, B, which, s);
+  if (vis_pass && needs_shading(P)) launch_shade(P, B, s);
+  mark(5);
+  if (prof) r->pending_prof.push_back(de);
   HIPCHK(hipGetLastError());
+  return SRX_OK;
+}
+
+// Folds the recorded stage events into the running pipeline_profiles queries (ns).
+int32_t collect_profiles(renderer_obj* r) {
+  if (r->pending_prof.empty()) return SRX_OK;
+  HIPCHK(hipStreamSynchronize(r->stream));
+  uint64_t add[PROF_COUNT] = {};
+  for (auto& d : r->pending_prof) {
+    float ms[5];
+    for (int i = 0; i < 5; ++i) HIPCHK(hipEventElapsedTime(&ms[i], d.ev[i], d.ev[i + 1]));
+    auto ns = [](float m) { return static_cast<uint64_t>(static_cast<double>(m) * 1.0e6); };
+    add[PROF_VTX] += ns(ms[0]);
+    add[PROF_CLIP] += ns(ms[1]);
+    add[PROF_COMPACT] += ns(ms[2]);
+    add[PROF_DISPATCH] += ns(ms[3]);
+    add[PROF_RAS] += ns(ms[4]);
+    add[PROF_DRAWS] += 1;
+    for (auto e : d.ev) r->event_pool.push_back(e);
+  }
+  r->pending_prof.clear();
+  for (auto& q : r->active)
+    if (q->id == SRX_QUERY_PIPELINE_PROFILES)
+      for (int i = 0; i < PROF_COUNT; ++i) q->prof[i] += add[i];
   return SRX_OK;
 }
 
@@ -595,6 +691,7 @@ int32_t srx_create_renderer(uint32_t kind, srx_renderer* out) {
   if (e == hipSuccess) e = hipMalloc(&r->stats_d, ST_COUNT * sizeof(unsigned long long));
   if (e == hipSuccess) e = hipMemset(r->stats_d, 0, ST_COUNT * sizeof(unsigned long long));
   if (e == hipSuccess) e = hipHostMalloc(&r->pinned, 64, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc(&r->samplers_h, sizeof(ps_samplers), hipHostMallocDefault);
   if (e != hipSuccess) {
     delete r;
     return hip_fail(e, "srx_create_renderer");
@@ -910,9 +1007,11 @@ int32_t srx_begin(srx_renderer rr, srx_query qq) {
   if (q->running) return SRX_FAILED;
   for (auto& a : r->active)
     if (a->id == q->id) return SRX_FAILED;
+  RCHK(collect_profiles(r));
   q->running = true;
   q->has_data = false;
   snapshot_stats(r, q->begin_snap);
+  for (auto& v : q->prof) v = 0;
   r->active.emplace_back(q);
   return SRX_OK;
 }
@@ -920,6 +1019,7 @@ int32_t srx_end(srx_renderer rr, srx_query qq) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);
   auto* q = as<query_obj>(qq, K_QUERY);
   if (!r || !q || !q->running) return SRX_FAILED;
+  RCHK(collect_profiles(r));
   uint64_t now[ST_COUNT];
   snapshot_stats(r, now);
   uint64_t d[ST_COUNT];
@@ -947,7 +1047,7 @@ int32_t srx_get_data(srx_renderer rr, srx_query qq, void* data, size_t size, int
   if (!as<renderer_obj>(rr, K_RENDERER) || !q || !data || !q->has_data) return SRX_ASYNC_ERROR;
   if (q->id == SRX_QUERY_PIPELINE_STATISTICS && size >= sizeof(q->stats)) std::memcpy(data, &q->stats, sizeof(q->stats));
   else if (q->id == SRX_QUERY_INTERNAL_STATISTICS && size >= sizeof(q->istats)) std::memcpy(data, &q->istats, sizeof(q->istats));
-  else if (q->id == SRX_QUERY_PIPELINE_PROFILES && size >= 64) std::memset(data, 0, 64);
+  else if (q->id == SRX_QUERY_PIPELINE_PROFILES && size >= sizeof(q->prof)) std::memcpy(data, q->prof, sizeof(q->prof));
   else return SRX_ASYNC_ERROR;
   return SRX_ASYNC_READY;
 }

@@ -98,6 +98,13 @@ __device__ __forceinline__ float4 lerp_2d(const surface_desc& s, const coord2& c
   return make_float4(0, 0, 0, 0);
 }
 
+// m / size, correctly rounded.  For a power-of-two size the quotient is m * 2^-k exactly (both are the
+// exact value m * 2^-k rounded once), so the division becomes one multiply.
+__device__ __forceinline__ float div_size(float m, int size) {
+  if ((size & (size - 1)) == 0) return m * __int_as_float((127 - (__ffs(size) - 1)) << 23);
+  return m / static_cast<float>(size);
+}
+
 // ---- addressers, same mode on both axes (sampler.cpp SSE branches) ----
 __device__ __forceinline__ void wrap_point_axis(float c, int size, int& out) {
   float sz = static_cast<float>(size);
@@ -105,7 +112,7 @@ __device__ __forceinline__ void wrap_point_axis(float c, int size, int& out) {
   f = sz * f;
   float ip = sse_floor(f);
   float m = ip + sz * 8192.0f;
-  float dv = static_cast<float>(cvtt(m / sz));
+  float dv = static_cast<float>(cvtt(div_size(m, size)));
   out = cvtt(m - dv * sz);
 }
 __device__ __forceinline__ void wrap_linear_axis(float c, int size, int& lo, int& hi, float& frac) {
@@ -119,8 +126,8 @@ __device__ __forceinline__ void wrap_linear_axis(float c, int size, int& lo, int
   m0 = m0 + sz * 8192.0f;
   float m1 = ip + 1.0f;
   m1 = m1 + sz * 8192.0f;
-  lo = cvtt(m0 - static_cast<float>(cvtt(m0 / sz)) * sz);
-  hi = cvtt(m1 - static_cast<float>(cvtt(m1 / sz)) * sz);
+  lo = cvtt(m0 - static_cast<float>(cvtt(div_size(m0, size))) * sz);
+  hi = cvtt(m1 - static_cast<float>(cvtt(div_size(m1, size))) * sz);
 }
 __device__ __forceinline__ float mirror_fold(float c, int size) {
   int sel = fast_floori(c);
@@ -331,8 +338,8 @@ __device__ inline aniso_info calc_aniso(const sampler_desc_dev& p, float dxu, fl
     lsq += zz * zz;
     lsq += zz * zz;
     a.weight_D = static_cast<float>(256) * lsq * 0.25f / (long_len * long_len);
-    a.du = d0 / sw;
-    a.dv = d1 / sh;
+    a.du = div_size(d0, static_cast<int>(p.size[0]));
+    a.dv = div_size(d1, static_cast<int>(p.size[1]));
   }
   return a;
 }

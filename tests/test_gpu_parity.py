@@ -14,8 +14,15 @@ pytestmark = pytest.mark.gpu
 
 
 def render(lib, setup, **kw):
+    """Renders the scene twice: without statistics queries (opaque early-Z draws then take the
+    order-independent visibility kernel) and with them (in-order kernel, exact ps_invocations)."""
     r = lib.renderer()
     f = setup(r, **kw)
+    f.render()
+    r.flush()
+    fast = {"color": f.color.read().copy(), "ds": f.ds.read().copy()}
+    if f.resolved is not None:
+        fast["resolved"] = f.resolved.read().copy()
     q = r.create_query(A.QUERY_PIPELINE_STATISTICS)
     qi = r.create_query(A.QUERY_INTERNAL_STATISTICS)
     r.begin(q)
@@ -31,10 +38,20 @@ def render(lib, setup, **kw):
            "backend_input_pixels": sti.backend_input_pixels}
     if f.resolved is not None:
         out["resolved"] = f.resolved.read().copy()
+    out["fast"] = fast
     return out
 
 
 def compare(h, c, color_exact=True):
+    _compare(h, c, color_exact)
+    # the statistics-free path must give the same images
+    f = dict(h["fast"])
+    for k in ("ps_invocations", "cprimitives", "backend_input_pixels"):
+        f[k] = c[k]
+    _compare(f, c, color_exact)
+
+
+def _compare(h, c, color_exact=True):
     print(f"\n  coverage-bearing depth/stencil bytes differing: {int((h['ds'] != c['ds']).sum())}", flush=True)
     assert np.array_equal(h["ds"], c["ds"]), "depth/stencil differs"
     diff = np.abs(h["color"].astype(np.int32) - c["color"].astype(np.int32))
