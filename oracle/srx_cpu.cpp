@@ -1717,3 +1717,30 @@ int32_t srx_flush(srx_renderer rr) { return as<renderer_obj>(rr, K_RENDERER) ? S
 namespace srxo {
 float ewa(int i) { return ewa_table[i]; }
 }
+
+// ---- test-only entry points (the oracle's helpers, for the pinning tests) ----
+extern "C" {
+int32_t srxo_test_fast_floori(double d) { return fast_floori(d); }
+int32_t srxo_test_fast_ceili(double d) { return fast_ceili(d); }
+float srxo_test_fast_log2(float v) { return fast_log2(v); }
+float srxo_test_fast_round(float v) { return fast_round(v); }
+float srxo_test_ewa(int32_t i) { return ewa_table[i & 255]; }
+// Triangle setup of three screen-space positions (x, y, z, w each): edges[9], bbox[4], v0 index,
+// returns 0 for a degenerate triangle.
+int32_t srxo_test_triangle_setup(const float* pos12, float* edges9, float* bbox4, int32_t* v0_index) {
+  vso v[3];
+  for (int i = 0; i < 3; ++i) {
+    for (int k = 0; k < 4; ++k) v[i].r[0][k] = pos12[i * 4 + k];
+    for (int r = 1; r < MAX_REGS; ++r) v[i].r[r] = v4(0, 0, 0, 0);
+  }
+  const vso* vp[3] = {&v[0], &v[1], &v[2]};
+  tri_info t;
+  compute_triangle_info(t, vp, 0);
+  if (!t.v0) return 0;
+  for (int e = 0; e < 3; ++e)
+    for (int k = 0; k < 3; ++k) edges9[e * 3 + k] = t.edge[e][k];
+  for (int k = 0; k < 4; ++k) bbox4[k] = t.bbox[k];
+  *v0_index = static_cast<int32_t>(t.v0 - &v[0]);
+  return 1;
+}
+}

@@ -35,6 +35,7 @@ struct draw_buffers {
   uint32_t clip_capacity;
   uint32_t* keys[2];   // records: GPU tile id
   uint32_t* vals[2];   // records: triangle id << 1 | tile-level accept
+  const uint32_t* order;  // GPU tiles, most records first (workgroup launch order)
   uint32_t* tstart;    // GPU tiles
   uint32_t* tend;
   uint32_t* hist;      // radix histogram 256 * blocks
@@ -57,6 +58,9 @@ void launch_emit(const draw_params& P, const draw_buffers& B, hipStream_t s);
 // Stable LSD radix sort of (keys, vals) by the low `bits` bits of keys; returns the buffer index
 // (0 or 1) that holds the result.
 int launch_sort(const draw_buffers& B, uint32_t n, int bits, hipStream_t s);
+// Sorts the GPU tiles by record count, heaviest first, into O.vals[result]; O supplies keys/vals
+// scratch (>= tiles each) and shares hist/scan_tmp with B.
+int launch_tile_order(const draw_params& P, const draw_buffers& B, draw_buffers& O, hipStream_t s);
 void launch_tile_ranges(const draw_params& P, const draw_buffers& B, int which, uint32_t n, hipStream_t s);
 void launch_raster(const draw_params& P, const draw_buffers& B, int which, hipStream_t s);
 // Order-independent visibility for opaque early-Z draws without statistics; false if not applicable.

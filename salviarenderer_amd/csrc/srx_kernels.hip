@@ -715,6 +715,23 @@ __global__ void tile_range_kernel(const uint32_t* __restrict__ keys, uint32_t n,
   if (i == n - 1 || keys[i + 1] != k) te[k] = i + 1;
 }
 
+// Heaviest GPU tiles first: key = 65535 - min(records, 65535), value = tile id.
+__global__ void tile_order_kernel(const uint32_t* __restrict__ ts, const uint32_t* __restrict__ te, uint32_t T,
+                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  uint32_t n = te[t] - ts[t];
+  keys[t] = 65535u - min(n, 65535u);
+  vals[t] = t;
+}
+
+int launch_tile_order(const draw_params& P, const draw_buffers& B, draw_buffers& O, hipStream_t s) {
+  uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
+  if (!T) return 0;
+  hipLaunchKernelGGL(tile_order_kernel, dim3((T + 255) / 256), dim3(256), 0, s, B.tstart, B.tend, T, O.keys[0], O.vals[0]);
+  return launch_sort(O, T, 16, s);
+}
+
 void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s) {
   if (n) hipLaunchKernelGGL(fill_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
 }
@@ -932,13 +949,13 @@ constexpr uint32_t NO_WINNER = 0xFFFFFFFFu;
 // later in shade_kernel.  Otherwise every quad runs the pixel shader and the backend in primitive order.
 template <int NA, int S, bool VIS>
 __global__ void __launch_bounds__(256) raster_kernel(draw_params P, const ps_samplers* __restrict__ SM,
-                                                      const uint32_t* __restrict__ vals,
+                                                      const uint32_t* __restrict__ order, const uint32_t* __restrict__ vals,
                                                       const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ tend,
                                                       const float4* __restrict__ tri_rec, const float4* __restrict__ tri_attr,
                                                       uint32_t* __restrict__ vis, unsigned long long* __restrict__ partial) {
   extern __shared__ prim_lds lds[];
   const int G = P.gt, QW = G / 2;
-  const int gtile = blockIdx.x;
+  const int gtile = order[blockIdx.x];
   const int gx = gtile % P.gtiles_x, gy = gtile / P.gtiles_x;
   const int tid = threadIdx.x, nthreads = blockDim.x;
   const int qx = tid % QW, qy = tid / QW;
@@ -1282,12 +1299,13 @@ __device__ __forceinline__ uint32_t key_winner(int mode, unsigned long long k) {
 
 // 16x16 GPU tile, one triangle per lane.  keys: 256 pixels x S samples.
 template <int S>
-__global__ void __launch_bounds__(128) vis_kernel(draw_params P, int mode, const uint32_t* __restrict__ vals,
+__global__ void __launch_bounds__(128) vis_kernel(draw_params P, int mode, const uint32_t* __restrict__ order,
+                                                   const uint32_t* __restrict__ vals,
                                                    const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ tend,
                                                    const float4* __restrict__ tri_rec, uint32_t* __restrict__ vis) {
   __shared__ unsigned long long keys[256 * S];
   __shared__ float d0s[256 * S];
-  const int gtile = blockIdx.x;
+  const int gtile = order[blockIdx.x];
   const int gx = gtile % P.gtiles_x, gy = gtile / P.gtiles_x;
   const int ox = gx * 16, oy = gy * 16;
   const int t64x = ox & ~(TILE - 1), t64y = oy & ~(TILE - 1);
@@ -1555,10 +1573,10 @@ void launch_raster(const draw_params& P, const draw_buffers& B, int which, hipSt
 #define SRX_RASTER_S(NA, S)                                                                                          \
   do {                                                                                                               \
     if (vis)                                                                                                         \
-      hipLaunchKernelGGL((raster_kernel<NA, S, true>), dim3(T), dim3(threads), lds, s, P, B.samplers, B.vals[which], \
+      hipLaunchKernelGGL((raster_kernel<NA, S, true>), dim3(T), dim3(threads), lds, s, P, B.samplers, B.order, B.vals[which], \
                          B.tstart, B.tend, B.tri_rec, B.tri_attr, B.vis, B.partial);                                 \
     else                                                                                                             \
-      hipLaunchKernelGGL((raster_kernel<NA, S, false>), dim3(T), dim3(threads), lds, s, P, B.samplers,               \
+      hipLaunchKernelGGL((raster_kernel<NA, S, false>), dim3(T), dim3(threads), lds, s, P, B.samplers, B.order,      \
                          B.vals[which], B.tstart, B.tend, B.tri_rec, B.tri_attr, B.vis, B.partial);                  \
   } while (0)
 #define SRX_RASTER(NA)                     \
@@ -1597,9 +1615,9 @@ bool launch_vis(const draw_params& P, const draw_buffers& B, int which, hipStrea
   if (mode < 0 || P.gt != 16) return false;
   if (!T) return true;
   switch (P.samples) {
-  case 1: hipLaunchKernelGGL((vis_kernel<1>), dim3(T), dim3(128), 0, s, P, mode, B.vals[which], B.tstart, B.tend, B.tri_rec, B.vis); break;
-  case 2: hipLaunchKernelGGL((vis_kernel<2>), dim3(T), dim3(128), 0, s, P, mode, B.vals[which], B.tstart, B.tend, B.tri_rec, B.vis); break;
-  default: hipLaunchKernelGGL((vis_kernel<4>), dim3(T), dim3(128), 0, s, P, mode, B.vals[which], B.tstart, B.tend, B.tri_rec, B.vis); break;
+  case 1: hipLaunchKernelGGL((vis_kernel<1>), dim3(T), dim3(128), 0, s, P, mode, B.order, B.vals[which], B.tstart, B.tend, B.tri_rec, B.vis); break;
+  case 2: hipLaunchKernelGGL((vis_kernel<2>), dim3(T), dim3(128), 0, s, P, mode, B.order, B.vals[which], B.tstart, B.tend, B.tri_rec, B.vis); break;
+  default: hipLaunchKernelGGL((vis_kernel<4>), dim3(T), dim3(128), 0, s, P, mode, B.order, B.vals[which], B.tstart, B.tend, B.tri_rec, B.vis); break;
   }
   return true;
 }

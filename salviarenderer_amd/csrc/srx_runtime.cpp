@@ -278,6 +278,7 @@ struct renderer_obj : object {
   dev_array<ps_samplers> samplers_d;
   ps_samplers* samplers_h = nullptr;  // pinned staging of the per-draw sampler table
   dev_array<uint32_t> cnt, keys0, keys1, vals0, vals1, tstart, tend, hist, scan_tmp, clip_hdr, counters, vis;
+  dev_array<uint32_t> okeys0, okeys1, ovals0, ovals1;
   dev_array<uint8_t> clip_flag;
   dev_array<int32_t> clip_slot;
   dev_array<unsigned long long> partial;
@@ -312,7 +313,7 @@ struct renderer_obj : object {
     };
     fr(vbuf); fr(tri_rec); fr(tri_attr); fr(clip_verts); fr(samplers_d); fr(vis);
     if (samplers_h) (void)hipHostFree(samplers_h); fr(cnt); fr(keys0); fr(keys1); fr(vals0); fr(vals1);
-    fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot); fr(partial);
+    fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot); fr(partial); fr(okeys0); fr(okeys1); fr(ovals0); fr(ovals1);
     if (stats_d) (void)hipFree(stats_d);
     if (pinned) (void)hipHostFree(pinned);
     for (auto& d : pending_prof)
@@ -588,7 +589,7 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
     RCHK(read_counter(r, CT_CLIP_OVERFLOW, &ovf));
     if (ovf) return fail(SRX_OUT_OF_MEMORY, "clip buffer overflow");
   }
-  uint32_t nblocks = (total + 4095) / 4096;
+  uint32_t nblocks = (std::max(total, T) + 4095) / 4096;
   RCHK(ensure(r, r->keys0, std::max<uint32_t>(total, 1)));
   RCHK(ensure(r, r->keys1, std::max<uint32_t>(total, 1)));
   RCHK(ensure(r, r->vals0, std::max<uint32_t>(total, 1)));
@@ -606,6 +607,17 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   while ((1u << bits) < T) ++bits;
   int which = launch_sort(B, total, bits, s);
   launch_tile_ranges(P, B, which, total, s);
+  RCHK(ensure(r, r->okeys0, std::max<uint32_t>(T, 1)));
+  RCHK(ensure(r, r->okeys1, std::max<uint32_t>(T, 1)));
+  RCHK(ensure(r, r->ovals0, std::max<uint32_t>(T, 1)));
+  RCHK(ensure(r, r->ovals1, std::max<uint32_t>(T, 1)));
+  draw_buffers O = B;
+  O.keys[0] = r->okeys0.p;
+  O.keys[1] = r->okeys1.p;
+  O.vals[0] = r->ovals0.p;
+  O.vals[1] = r->ovals1.p;
+  int ow = launch_tile_order(P, B, O, s);
+  B.order = O.vals[ow];
   mark(4);
   // statistics queries need the in-order walk (ps_invocations counts early-Z passes as they happen)
   bool counting = false;

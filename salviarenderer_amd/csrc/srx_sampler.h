@@ -241,6 +241,29 @@ __device__ __forceinline__ float4 filter_linear(const surface_desc& s, const sam
   return lerp_2d(s, c);
 }
 
+// Bilinear wrap/wrap RGBA8 footprint: the common case of sample_surface, without the mode and
+// format dispatch (same operations as wrap_linear_axis + lerp_2d).
+__device__ __forceinline__ float4 bilinear_wrap_rgba8(const surface_desc& s, float x, float y) {
+  coord2 c;
+  wrap_linear_axis(x, s.w, c.x0, c.x1, c.tx);
+  wrap_linear_axis(y, s.h, c.y0, c.y1, c.ty);
+  const uint32_t u0 = *reinterpret_cast<const uint32_t*>(texel_addr(s, c.x0, c.y0));
+  const uint32_t u1 = *reinterpret_cast<const uint32_t*>(texel_addr(s, c.x1, c.y0));
+  const uint32_t u2 = *reinterpret_cast<const uint32_t*>(texel_addr(s, c.x0, c.y1));
+  const uint32_t u3 = *reinterpret_cast<const uint32_t*>(texel_addr(s, c.x1, c.y1));
+  float r[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int sh = q * 8;
+    float c0 = static_cast<float>((u0 >> sh) & 0xFF), c1 = static_cast<float>((u1 >> sh) & 0xFF);
+    float c2 = static_cast<float>((u2 >> sh) & 0xFF), c3 = static_cast<float>((u3 >> sh) & 0xFF);
+    float c01 = c0 + (c1 - c0) * c.tx;
+    float c23 = c2 + (c3 - c2) * c.tx;
+    r[q] = (c01 + (c23 - c01) * c.ty) * (1.0f / 255);
+  }
+  return make_float4(r[0], r[1], r[2], r[3]);
+}
+
 __device__ __forceinline__ float4 sample_surface(const sampler_desc_dev& p, int level, int filter, float x, float y) {
   const surface_desc& s = p.levels[level];
   return filter == FILTER_LINEAR ? filter_linear(s, p, x, y) : filter_point(s, p, x, y);
@@ -373,8 +396,10 @@ __device__ __noinline__ float4 sample_impl(const sampler_desc_dev& p, float u, f
              : (static_cast<uint64_t>(p.min_lod) < lo_s ? p.min_lod : static_cast<int>(lo_s));
   float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f, w_sum = 0.0f;
   int n = static_cast<int>(af->probe_count);
+  const surface_desc& lv = p.levels[lo_c];
+  const bool fast = p.min_filter == FILTER_LINEAR && p.addr_u == ADDR_WRAP && p.addr_v == ADDR_WRAP && lv.pf == PF_RGBA8;
   for (int i = -n + 1; i < n; i += 2) {
-    float4 c0 = sample_surface(p, lo_c, p.min_filter, x, y);
+    float4 c0 = fast ? bilinear_wrap_rgba8(lv, x, y) : sample_surface(p, lo_c, p.min_filter, x, y);
     float wf = static_cast<float>(i * i) * af->weight_D;
     int wi = (wf >= 0.0f) ? static_cast<int>(sse_min(wf, 255.0f)) : 0;  // the reference reads past its table there
     float w = c_ewa[wi];
