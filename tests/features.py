@@ -1,0 +1,332 @@
+"""Feature scenes for parity tests: each case draws into a small target through the renderer API
+and is rendered identically by the HIP library and the CPU oracle.
+
+Every case covers one state combination of the pipeline: depth functions and write masks, ties,
+blending, colour-write-disabled passes, stencil functions/operations/masks, near/far clipping,
+strips and 16-bit indices with a base vertex, culling with either winding, viewports, RGBA32F
+targets, MSAA, and the sampler's addressing/filter/mip modes through the texture pixel shader.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from salviarenderer_amd import api as A
+
+W, H = 96, 80
+
+
+class Draw:
+    """State of one draw; `issue(r)` binds it and draws (returns the objects to keep alive)."""
+
+    def __init__(self, pos, col, *, indices=None, index_fmt=A.FMT_R32_UINT, topology=A.TOPOLOGY_TRIANGLE_LIST,
+                 start=0, basevert=0, prims=None, cull=A.CULL_NONE, front_ccw=False, depth_enable=True,
+                 depth_write=True, depth_func=A.CMP_LESS, stencil=None, stencil_ref=0, ps=A.PS_GOURAUD,
+                 ps_consts=None, blend=A.BLEND_OFF, sampler=None):
+        self.v = np.ascontiguousarray(np.concatenate([np.asarray(pos, np.float32), np.asarray(col, np.float32)], 1))
+        self.indices, self.index_fmt, self.topology = indices, index_fmt, topology
+        self.start, self.basevert = start, basevert
+        if prims is None:
+            n = len(self.v) if indices is None else len(indices)
+            prims = n // 3 if topology == A.TOPOLOGY_TRIANGLE_LIST else n - 2
+        self.prims = prims
+        self.cull, self.front_ccw = cull, front_ccw
+        self.depth_enable, self.depth_write, self.depth_func = depth_enable, depth_write, depth_func
+        self.stencil, self.stencil_ref = stencil, stencil_ref
+        self.ps, self.ps_consts, self.blend, self.sampler = ps, ps_consts, blend, sampler
+
+    def issue(self, r, tex_cache):
+        lib = r.lib
+        keep = []
+        vb = r.create_buffer_from(self.v)
+        layout = r.create_input_layout([("POSITION", 0, A.FMT_R32G32B32A32_FLOAT, 0, 0),
+                                        ("COLOR", 0, A.FMT_R32G32B32A32_FLOAT, 0, 16)], A.VS_PASSTHROUGH_COLOR)
+        r.set_input_layout(layout)
+        r.set_vertex_buffers(0, [vb], [32])
+        keep += [vb, layout]
+        if self.indices is not None:
+            dt = np.uint16 if self.index_fmt == A.FMT_R16_UINT else np.uint32
+            ib = r.create_buffer_from(np.ascontiguousarray(np.asarray(self.indices, dt)))
+            r.set_index_buffer(ib, self.index_fmt)
+            keep.append(ib)
+        r.set_primitive_topology(self.topology)
+        r.set_vertex_shader(A.VS_PASSTHROUGH_COLOR)
+        r.set_pixel_shader(self.ps, self.ps_consts)
+        r.set_blend_shader(self.blend)
+        if self.sampler is not None:
+            samp = make_sampler(r, tex_cache, **self.sampler)
+            r.set_ps_sampler(0, samp)
+            keep.append(samp)
+        rd = A.default_raster_desc(lib)
+        rd.cull_mode = self.cull
+        rd.front_ccw = int(self.front_ccw)
+        rs = r.create_raster_state(rd)
+        r.set_rasterizer_state(rs)
+        dd = A.default_depth_stencil_desc(lib)
+        dd.depth_enable = int(self.depth_enable)
+        dd.depth_write_mask = int(self.depth_write)
+        dd.depth_func = self.depth_func
+        if self.stencil is not None:
+            st = self.stencil
+            dd.stencil_enable = 1
+            for face in (dd.front_face, dd.back_face):
+                face.stencil_func = st["func"]
+                face.stencil_pass_op = st.get("pass_op", A.SOP_KEEP)
+                face.stencil_fail_op = st.get("fail_op", A.SOP_KEEP)
+                face.stencil_depth_fail_op = st.get("zfail_op", A.SOP_KEEP)
+            dd.stencil_read_mask = st.get("read_mask", 0xFF)
+            dd.stencil_write_mask = st.get("write_mask", 0xFF)
+        dss = r.create_depth_stencil_state(dd)
+        r.set_depth_stencil_state(dss, self.stencil_ref)
+        keep += [rs, dss]
+        if self.indices is None:
+            r.draw(self.start, self.prims)
+        else:
+            r.draw_index(self.start, self.prims, self.basevert)
+        return keep
+
+
+def make_texture(r, size, fmt, seed):
+    rng = np.random.default_rng(seed)
+    t = r.create_tex2d(size, size, 1, fmt)
+    if fmt == A.PF_RGBA8:
+        t.surface(0).write(rng.integers(0, 256, (size, size, 4), dtype=np.uint8))
+    elif fmt == A.PF_RGBA32F:
+        t.surface(0).write(rng.random((size, size, 4), dtype=np.float32))
+    elif fmt == A.PF_R32F:
+        t.surface(0).write(rng.random((size, size, 1), dtype=np.float32))
+    t.gen_mipmap(A.FILTER_LINEAR, True)
+    return t
+
+
+def make_sampler(r, cache, *, size=64, fmt=A.PF_RGBA8, min_f=A.FILTER_LINEAR, mag_f=A.FILTER_LINEAR,
+                 mip_f=A.FILTER_LINEAR, addr=A.ADDRESS_WRAP, qual=A.MIP_MI, aniso=1, border=(0.2, 0.4, 0.6, 0.8),
+                 lod_bias=0.0):
+    key = (size, fmt)
+    if key not in cache:
+        cache[key] = make_texture(r, size, fmt, seed=size * 31 + fmt)
+    sd = A.default_sampler_desc(r.lib)
+    sd.min_filter, sd.mag_filter, sd.mip_filter = min_f, mag_f, mip_f
+    sd.mip_qual = qual
+    sd.addr_mode_u = sd.addr_mode_v = sd.addr_mode_w = addr
+    sd.max_anisotropy = aniso
+    sd.mip_lod_bias = lod_bias
+    for i in range(4):
+        sd.border_color[i] = border[i]
+    return r.create_sampler(sd, cache[key])
+
+
+# ---------------------------------------------------------------- geometry generators
+def tris(n, seed, *, z=(0.1, 0.9), flat_z=None, w=None, alpha=None, extent=1.2):
+    """n random clip-space triangles; flat_z: constant depth per triangle drawn from the given levels
+    (ties), w: (lo, hi) range of clip w (perspective), alpha: (lo, hi) colour alpha."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-1, 1, (n, 1, 2))
+    size = np.exp(rng.uniform(np.log(0.05), np.log(extent), (n, 1, 1)))
+    xy = c + rng.uniform(-1, 1, (n, 3, 2)) * size
+    if flat_z is not None:
+        zz = np.repeat(rng.choice(np.asarray(flat_z, np.float64), (n, 1)), 3, 1)
+    else:
+        zz = rng.uniform(z[0], z[1], (n, 3))
+    ww = np.ones((n, 3)) if w is None else rng.uniform(w[0], w[1], (n, 3))
+    pos = np.concatenate([xy * ww[..., None], (zz * ww)[..., None], ww[..., None]], -1).reshape(-1, 4)
+    col = rng.uniform(0, 1, (n * 3, 4))
+    if alpha is not None:
+        col[:, 3] = np.repeat(rng.uniform(alpha[0], alpha[1], n), 3)
+    return pos.astype(np.float32), col.astype(np.float32)
+
+
+def full_screen(z, colour):
+    pos = np.array([(-1, -1, z, 1), (3, -1, z, 1), (-1, 3, z, 1)], np.float32)
+    return pos, np.tile(np.asarray(colour, np.float32), (3, 1))
+
+
+def perspective_floor(uv_scale=4.0, uv_offset=(-1.5, -1.5)):
+    """Two triangles of a receding plane (clip w from 1 at the bottom to 12 at the top) with uv in
+    COLOR.xy, so the texture is magnified near the bottom and strongly minified, anisotropically,
+    towards the top."""
+    corners = [(-1.3, -1.0, 1.0), (1.3, -1.0, 1.0), (12.0 * 1.3, 12.0, 12.0), (-12.0 * 1.3, 12.0, 12.0)]
+    uvs = [(0, 0), (1, 0), (1, 1), (0, 1)]
+    pos, col = [], []
+    for i in (0, 1, 2, 0, 2, 3):
+        x, y, w = corners[i]
+        pos.append((x, y, 0.5 * w, w))
+        u, v = uvs[i]
+        col.append((uv_offset[0] + u * uv_scale, uv_offset[1] + v * uv_scale, 0.0, 1.0))
+    return np.array(pos, np.float32), np.array(col, np.float32)
+
+
+# ---------------------------------------------------------------- cases
+class Case:
+    def __init__(self, name, draws, *, samples=1, fmt=A.PF_RGBA8, clear_depth=1.0, clear_stencil=0,
+                 viewport=None, min_covered=0.05):
+        self.name, self.draws, self.samples, self.fmt = name, draws, samples, fmt
+        self.clear_depth, self.clear_stencil, self.viewport = clear_depth, clear_stencil, viewport
+        self.min_covered = min_covered
+
+
+def _depth_cases():
+    out = []
+    names = {A.CMP_NEVER: "never", A.CMP_LESS: "less", A.CMP_EQUAL: "equal", A.CMP_LESS_EQUAL: "lequal",
+             A.CMP_GREATER: "greater", A.CMP_NOT_EQUAL: "notequal", A.CMP_GREATER_EQUAL: "gequal",
+             A.CMP_ALWAYS: "always"}
+    for f, nm in names.items():
+        for write in (True, False):
+            p, c = tris(160, seed=100 + f, flat_z=(0.3, 0.5, 0.6, 0.7))
+            p2, c2 = tris(120, seed=200 + f, z=(0.2, 0.95))
+            draws = [Draw(p, c, depth_func=f, depth_write=write), Draw(p2, c2, depth_func=f, depth_write=write)]
+            out.append(Case(f"depth_{nm}_{'write' if write else 'nowrite'}", draws, clear_depth=0.6,
+                            min_covered=0.0 if f == A.CMP_NEVER else 0.05))
+    p, c = tris(200, seed=5)
+    out.append(Case("depth_disabled", [Draw(p, c, depth_enable=False)]))
+    return out
+
+
+def _blend_cases():
+    out = []
+    for samples in (1, 4):
+        p, c = tris(80, seed=11)
+        pb, cb = tris(120, seed=12, alpha=(0.2, 0.8), flat_z=(0.05, 0.5, 0.95))
+        out.append(Case(f"blend_on_msaa{samples}", [Draw(p, c), Draw(pb, cb, blend=A.BLEND_ON, depth_write=False,
+                                                                    depth_func=A.CMP_LESS_EQUAL)], samples=samples))
+    p, c = tris(150, seed=13)
+    p2, c2 = tris(150, seed=14)
+    out.append(Case("color_disabled_then_lequal", [Draw(p, c, blend=A.BLEND_DISABLED),
+                                                   Draw(p2, c2, depth_func=A.CMP_LESS_EQUAL)]))
+    p, c = tris(150, seed=15, alpha=(0.0, 1.0))
+    out.append(Case("blend_rgba32f", [Draw(p, c, blend=A.BLEND_ON, depth_enable=False)], fmt=A.PF_RGBA32F))
+    return out
+
+
+def _stencil_cases():
+    out = []
+    # mirror-style: mark with colour writes disabled, then draw only where marked
+    mp, mc = tris(6, seed=21, extent=1.5)
+    p, c = tris(200, seed=22)
+    for samples in (1, 4):
+        out.append(Case(f"stencil_mark_equal_msaa{samples}", [
+            Draw(mp, mc, blend=A.BLEND_DISABLED, depth_write=False, depth_func=A.CMP_ALWAYS,
+                 stencil={"func": A.CMP_ALWAYS, "pass_op": A.SOP_REPLACE}, stencil_ref=1),
+            Draw(p, c, stencil={"func": A.CMP_EQUAL}, stencil_ref=1)], samples=samples))
+    ops = {A.SOP_KEEP: "keep", A.SOP_ZERO: "zero", A.SOP_REPLACE: "replace", A.SOP_INCR_SAT: "incr_sat",
+           A.SOP_DECR_SAT: "decr_sat", A.SOP_INVERT: "invert", A.SOP_INCR_WRAP: "incr_wrap",
+           A.SOP_DECR_WRAP: "decr_wrap"}
+    for op, nm in ops.items():
+        p, c = tris(120, seed=30 + op, extent=1.6)
+        out.append(Case(f"stencil_op_{nm}", [Draw(p, c, depth_func=A.CMP_ALWAYS, stencil={
+            "func": A.CMP_ALWAYS, "pass_op": op, "fail_op": A.SOP_ZERO, "zfail_op": A.SOP_INVERT,
+            "read_mask": 0xF0, "write_mask": 0x3F}, stencil_ref=0x5A)], clear_stencil=0xFE if op == A.SOP_INCR_SAT else 0x03))
+    for f in (A.CMP_LESS, A.CMP_GREATER_EQUAL, A.CMP_NOT_EQUAL, A.CMP_NEVER):
+        p, c = tris(100, seed=40 + f, extent=1.6)
+        pk, ck = tris(100, seed=50 + f)
+        out.append(Case(f"stencil_func_{f}", [
+            Draw(pk, ck, depth_func=A.CMP_ALWAYS, depth_write=False, blend=A.BLEND_DISABLED,
+                 stencil={"func": A.CMP_ALWAYS, "pass_op": A.SOP_INCR_WRAP}),
+            Draw(p, c, stencil={"func": f, "pass_op": A.SOP_DECR_SAT, "read_mask": 0x0F}, stencil_ref=0x12)],
+            min_covered=0.0 if f == A.CMP_NEVER else 0.05))
+    return out
+
+
+def _geometry_cases():
+    out = []
+    p, c = tris(150, seed=60, w=(0.3, 3.0), z=(-0.6, 1.6))
+    out.append(Case("clip_near_far", [Draw(p, c)]))
+    p, c = tris(150, seed=61, w=(-1.0, 3.0), z=(-0.5, 1.2))
+    out.append(Case("clip_negative_w", [Draw(p, c)], min_covered=0.01))
+    # strip with 16-bit indices, a start index and a base vertex
+    rng = np.random.default_rng(62)
+    nv = 300
+    pos = np.concatenate([rng.uniform(-1.1, 1.1, (nv, 2)), rng.uniform(0.1, 0.9, (nv, 1)), np.ones((nv, 1))], 1)
+    col = rng.uniform(0, 1, (nv, 4))
+    idx = rng.integers(0, nv - 7, 90)
+    out.append(Case("strip_u16_basevertex", [Draw(pos, col, indices=idx, index_fmt=A.FMT_R16_UINT,
+                                                  topology=A.TOPOLOGY_TRIANGLE_STRIP, start=3, basevert=7,
+                                                  prims=80)]))
+    p, c = tris(100, seed=63)
+    idx = np.random.default_rng(64).integers(0, 300, 240)
+    out.append(Case("list_u32_indexed", [Draw(p, c, indices=idx, prims=80)]))
+    p, c = tris(60, seed=65)
+    out.append(Case("strip_nonindexed", [Draw(p, c, topology=A.TOPOLOGY_TRIANGLE_STRIP, start=5, prims=100)]))
+    for cull, cn in ((A.CULL_FRONT, "front"), (A.CULL_BACK, "back")):
+        for ccw in (False, True):
+            p, c = tris(200, seed=66)
+            out.append(Case(f"cull_{cn}_{'ccw' if ccw else 'cw'}", [Draw(p, c, cull=cull, front_ccw=ccw)]))
+    p, c = tris(200, seed=67)
+    out.append(Case("viewport_offset", [Draw(p, c)], viewport=(13, 7, 60, 50, 0.2, 0.8), min_covered=0.02))
+    p, c = tris(200, seed=68)
+    out.append(Case("msaa2_gouraud", [Draw(p, c)], samples=2))
+    p, c = tris(200, seed=69)
+    out.append(Case("rgba32f_target", [Draw(p, c)], fmt=A.PF_RGBA32F))
+    p, c = full_screen(0.5, (0.25, 0.5, 0.75, 1.0))
+    out.append(Case("ps_constant", [Draw(p, c, ps=A.PS_CONSTANT, ps_consts=np.array([0.1, 0.2, 0.3, 0.4], np.float32))]))
+    return out
+
+
+def _texture_cases():
+    out = []
+    p, c = perspective_floor()
+    pt, ct = tris(40, seed=70)
+    ct[:, :2] = ct[:, :2] * 12 - 6  # wide uv ranges: strong minification and addressing
+    specs = []
+    for addr, an in ((A.ADDRESS_WRAP, "wrap"), (A.ADDRESS_MIRROR, "mirror"), (A.ADDRESS_CLAMP, "clamp"),
+                     (A.ADDRESS_BORDER, "border")):
+        specs.append((f"tex_{an}_linear", dict(addr=addr)))
+        specs.append((f"tex_{an}_point", dict(addr=addr, min_f=A.FILTER_POINT, mag_f=A.FILTER_POINT,
+                                              mip_f=A.FILTER_POINT)))
+    specs += [
+        ("tex_mip_lo", dict(qual=A.MIP_LO)), ("tex_mip_hi", dict(qual=A.MIP_HI)),
+        ("tex_mip_point_linear", dict(mip_f=A.FILTER_POINT)),
+        ("tex_min_point_mag_linear", dict(min_f=A.FILTER_POINT)),
+        ("tex_min_linear_mag_point", dict(mag_f=A.FILTER_POINT)),
+        # sampler_desc::mip_lod_bias is declared but never read by the reference sampler (sampler.h:24)
+        ("tex_desc_lod_bias_unused", dict(lod_bias=1.5)),
+        ("tex_aniso4", dict(mip_f=A.FILTER_ANISOTROPIC, aniso=4)),
+        ("tex_aniso16", dict(mip_f=A.FILTER_ANISOTROPIC, aniso=16)),
+        ("tex_aniso16_mirror", dict(mip_f=A.FILTER_ANISOTROPIC, aniso=16, addr=A.ADDRESS_MIRROR)),
+        ("tex_rgba32f", dict(fmt=A.PF_RGBA32F)),
+        ("tex_r32f", dict(fmt=A.PF_R32F)),
+        ("tex_npot_size", dict(size=48)),
+    ]
+    for name, spec in specs:
+        draws = [Draw(p, c, ps=A.PS_TEX_CPP, sampler=spec), Draw(pt, ct, ps=A.PS_TEX_CPP, sampler=spec)]
+        out.append(Case(name, draws))
+    out.append(Case("tex_aniso16_msaa4", [Draw(p, c, ps=A.PS_TEX_CPP, sampler=dict(mip_f=A.FILTER_ANISOTROPIC, aniso=16))],
+                    samples=4))
+    return out
+
+
+CASES = _depth_cases() + _blend_cases() + _stencil_cases() + _geometry_cases() + _texture_cases()
+
+
+def render_case(lib, case: Case, stats: bool):
+    """Renders a case; with stats=True inside pipeline/internal statistics queries."""
+    r = lib.renderer()
+    ct = r.create_tex2d(W, H, case.samples, case.fmt)
+    dt = r.create_tex2d(W, H, case.samples, A.PF_RG32F)
+    color, ds = ct.surface(0), dt.surface(0)
+    r.set_render_targets([color], ds)
+    if case.viewport is None:
+        r.set_viewport(0, 0, W, H, 0.0, 1.0)
+    else:
+        r.set_viewport(*case.viewport)
+    r.clear_color(color, (0.05, 0.1, 0.15, 0.2))
+    r.clear_depth_stencil(ds, A.CLEAR_DEPTH | A.CLEAR_STENCIL, case.clear_depth, case.clear_stencil)
+    if stats:
+        q = r.create_query(A.QUERY_PIPELINE_STATISTICS)
+        qi = r.create_query(A.QUERY_INTERNAL_STATISTICS)
+        r.begin(q)
+        r.begin(qi)
+    keep, tex_cache = [], {}
+    for d in case.draws:
+        keep += d.issue(r, tex_cache)
+    out = {}
+    if stats:
+        r.end(q)
+        r.end(qi)
+        st, sti = r.get_data(q)[1], r.get_data(qi)[1]
+        out.update(ia_primitives=st.ia_primitives, cinvocations=st.cinvocations, cprimitives=st.cprimitives,
+                   ps_invocations=st.ps_invocations, backend_input_pixels=sti.backend_input_pixels)
+    r.flush()
+    out["color"] = color.read().copy()
+    out["ds"] = ds.read().copy()
+    del keep, tex_cache
+    return out
