@@ -1500,7 +1500,7 @@ __global__ void __launch_bounds__(256) shade_kernel(draw_params P, const ps_samp
 #pragma unroll
     for (int s = 0; s < S; ++s)
       if (cur == NO_WINNER) cur = w[s];
-    if (cur == NO_WINNER) break;
+    if (cur >= 3u * P.prim_count) break;  // NO_WINNER (every id is below 3 * prim_count)
     attr_setup<NA> at;
     load_attr<NA>(tri_attr, cur, at);
     float4 q[4][NA + 1];
@@ -1626,10 +1626,11 @@ bool needs_shading(const draw_params& P) {
   return P.early_z && P.blend_id == BLEND_OFF && P.color.data != nullptr;
 }
 
+// Covers exactly the 16x16 GPU tiles vis_kernel wrote: pixels of the target beyond the viewport's
+// 64x64 tile grid hold no winners of this draw.
 void launch_shade(const draw_params& P, const draw_buffers& B, hipStream_t s) {
-  int w = static_cast<int>(P.target_w), h = static_cast<int>(P.target_h);
-  if (w <= 0 || h <= 0) return;
-  dim3 grid((w + 15) / 16, (h + 15) / 16);
+  if (P.gtiles_x <= 0 || P.gtiles_y <= 0) return;
+  dim3 grid(P.gtiles_x, P.gtiles_y);
 #define SRX_SHADE_S(NA, S) \
   hipLaunchKernelGGL((shade_kernel<NA, S>), grid, dim3(256), 0, s, P, B.samplers, B.tri_attr, B.vis)
 #define SRX_SHADE(NA)                    \
