@@ -1298,8 +1298,11 @@ __device__ __forceinline__ uint32_t key_winner(int mode, unsigned long long k) {
 }
 
 // 16x16 GPU tile, one triangle per lane.  keys: 256 pixels x S samples.
+#ifndef SRX_VIS_THREADS
+#define SRX_VIS_THREADS 128
+#endif
 template <int S>
-__global__ void __launch_bounds__(128) vis_kernel(draw_params P, int mode, const uint32_t* __restrict__ order,
+__global__ void __launch_bounds__(SRX_VIS_THREADS) vis_kernel(draw_params P, int mode, const uint32_t* __restrict__ order,
                                                    const uint32_t* __restrict__ vals,
                                                    const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ tend,
                                                    const float4* __restrict__ tri_rec, uint32_t* __restrict__ vis) {
@@ -1615,9 +1618,9 @@ bool launch_vis(const draw_params& P, const draw_buffers& B, int which, hipStrea
   if (mode < 0 || P.gt != 16) return false;
   if (!T) return true;
   switch (P.samples) {
-  case 1: hipLaunchKernelGGL((vis_kernel<1>), dim3(T), dim3(128), 0, s, P, mode, B.order, B.vals[which], B.tstart, B.tend, B.tri_rec, B.vis); break;
-  case 2: hipLaunchKernelGGL((vis_kernel<2>), dim3(T), dim3(128), 0, s, P, mode, B.order, B.vals[which], B.tstart, B.tend, B.tri_rec, B.vis); break;
-  default: hipLaunchKernelGGL((vis_kernel<4>), dim3(T), dim3(128), 0, s, P, mode, B.order, B.vals[which], B.tstart, B.tend, B.tri_rec, B.vis); break;
+  case 1: hipLaunchKernelGGL((vis_kernel<1>), dim3(T), dim3(SRX_VIS_THREADS), 0, s, P, mode, B.order, B.vals[which], B.tstart, B.tend, B.tri_rec, B.vis); break;
+  case 2: hipLaunchKernelGGL((vis_kernel<2>), dim3(T), dim3(SRX_VIS_THREADS), 0, s, P, mode, B.order, B.vals[which], B.tstart, B.tend, B.tri_rec, B.vis); break;
+  default: hipLaunchKernelGGL((vis_kernel<4>), dim3(T), dim3(SRX_VIS_THREADS), 0, s, P, mode, B.order, B.vals[which], B.tstart, B.tend, B.tri_rec, B.vis); break;
   }
   return true;
 }

@@ -247,10 +247,12 @@ __device__ __forceinline__ float4 bilinear_wrap_rgba8(const surface_desc& s, flo
   coord2 c;
   wrap_linear_axis(x, s.w, c.x0, c.x1, c.tx);
   wrap_linear_axis(y, s.h, c.y0, c.y1, c.ty);
-  const uint32_t u0 = *reinterpret_cast<const uint32_t*>(texel_addr(s, c.x0, c.y0));
-  const uint32_t u1 = *reinterpret_cast<const uint32_t*>(texel_addr(s, c.x1, c.y0));
-  const uint32_t u2 = *reinterpret_cast<const uint32_t*>(texel_addr(s, c.x0, c.y1));
-  const uint32_t u3 = *reinterpret_cast<const uint32_t*>(texel_addr(s, c.x1, c.y1));
+  // single-sample RGBA8 level: texel (x, y) is word y*w + x (levels stay far below 2^32 bytes)
+  const uint32_t* t = reinterpret_cast<const uint32_t*>(s.data);
+  const uint32_t x0 = static_cast<uint32_t>(min(max(c.x0, 0), s.w - 1)), x1 = static_cast<uint32_t>(min(max(c.x1, 0), s.w - 1));
+  const uint32_t r0 = static_cast<uint32_t>(min(max(c.y0, 0), s.h - 1)) * static_cast<uint32_t>(s.w);
+  const uint32_t r1 = static_cast<uint32_t>(min(max(c.y1, 0), s.h - 1)) * static_cast<uint32_t>(s.w);
+  const uint32_t u0 = t[r0 + x0], u1 = t[r0 + x1], u2 = t[r1 + x0], u3 = t[r1 + x1];
   float r[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -397,9 +399,39 @@ __device__ __noinline__ float4 sample_impl(const sampler_desc_dev& p, float u, f
   float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f, w_sum = 0.0f;
   int n = static_cast<int>(af->probe_count);
   const surface_desc& lv = p.levels[lo_c];
-  const bool fast = p.min_filter == FILTER_LINEAR && p.addr_u == ADDR_WRAP && p.addr_v == ADDR_WRAP && lv.pf == PF_RGBA8;
+  const bool fast = p.min_filter == FILTER_LINEAR && p.addr_u == ADDR_WRAP && p.addr_v == ADDR_WRAP && lv.pf == PF_RGBA8 &&
+                    lv.samples == 1;
+  if (fast) {
+    // Four probes per step: their 16 texel loads are issued together (every address is clamped
+    // into the level, so probes past the count load harmlessly and are masked out), then the
+    // probes are accumulated in order, which keeps the reference's summation order.
+    for (int k = 0; k < n; k += 4) {
+      float4 c[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        c[j] = bilinear_wrap_rgba8(lv, x, y);
+        x += af->du;
+        y += af->dv;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int i = 2 * (k + j) - n + 1;
+        const bool live = k + j < n;
+        float wf = static_cast<float>(i * i) * af->weight_D;
+        int wi = (wf >= 0.0f) ? static_cast<int>(sse_min(wf, 255.0f)) : 0;
+        float w = c_ewa[live ? wi : 0];
+        cr = live ? cr + c[j].x * w : cr;
+        cg = live ? cg + c[j].y * w : cg;
+        cb = live ? cb + c[j].z * w : cb;
+        ca = live ? ca + c[j].w * w : ca;
+        w_sum = live ? w_sum + w : w_sum;
+      }
+    }
+    float inv = 1 / w_sum;
+    return make_float4(cr * inv, cg * inv, cb * inv, ca * inv);
+  }
   for (int i = -n + 1; i < n; i += 2) {
-    float4 c0 = fast ? bilinear_wrap_rgba8(lv, x, y) : sample_surface(p, lo_c, p.min_filter, x, y);
+    float4 c0 = sample_surface(p, lo_c, p.min_filter, x, y);
     float wf = static_cast<float>(i * i) * af->weight_D;
     int wi = (wf >= 0.0f) ? static_cast<int>(sse_min(wf, 255.0f)) : 0;  // the reference reads past its table there
     float w = c_ewa[wi];
