@@ -1459,8 +1459,19 @@ __global__ void __launch_bounds__(SRX_VIS_THREADS) vis_kernel(draw_params P, int
     const int p = i / S, smp = i % S;
     const int px = ox + (p & 15), py = oy + (p >> 4);
     if (px >= tw || py >= th) continue;
-    const uint32_t w = key_winner(mode, keys[i]);
+    const unsigned long long key = keys[i];
+    const uint32_t w = key_winner(mode, key);
     vis[(static_cast<size_t>(py) * tw + px) * S + smp] = w;
+    if (w != NO_WINNER && has_ds && P.write_depth && mode != VIS_LATEST) {
+      // depth modes carry the winning sample depth in the key; only +-0 is ambiguous there
+      uint32_t zk = static_cast<uint32_t>(key >> 32);
+      if (mode == VIS_GREATER || mode == VIS_GEQUAL) zk = ~zk;
+      const uint32_t u = (zk & 0x80000000u) ? (zk & 0x7FFFFFFFu) : ~zk;
+      if (u != 0u) {
+        *reinterpret_cast<float*>(sample_addr(P.ds, px, py, smp)) = __uint_as_float(u);
+        continue;
+      }
+    }
     if (w != NO_WINNER && has_ds && P.write_depth) {
       // the winner's depth at this sample, as its quad computed it
       const float4* tr = tri_rec + static_cast<size_t>(w) * 5;

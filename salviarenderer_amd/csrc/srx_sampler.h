@@ -369,6 +369,10 @@ __device__ inline aniso_info calc_aniso(const sampler_desc_dev& p, float dxu, fl
   return a;
 }
 
+// anisotropic probes whose texel loads are in flight together
+#ifndef SRX_ANISO_BATCH
+#define SRX_ANISO_BATCH 4
+#endif
 // sampler.cpp:680-764
 __device__ __noinline__ float4 sample_impl(const sampler_desc_dev& p, float u, float v, float miplevel, const aniso_info* af) {
   bool is_mag = (p.mip_filter == FILTER_POINT) ? (miplevel < 0.5f) : (miplevel < 0.0f);
@@ -402,19 +406,19 @@ __device__ __noinline__ float4 sample_impl(const sampler_desc_dev& p, float u, f
   const bool fast = p.min_filter == FILTER_LINEAR && p.addr_u == ADDR_WRAP && p.addr_v == ADDR_WRAP && lv.pf == PF_RGBA8 &&
                     lv.samples == 1;
   if (fast) {
-    // Four probes per step: their 16 texel loads are issued together (every address is clamped
+    // SRX_ANISO_BATCH probes per step: their texel loads are issued together (every address is clamped
     // into the level, so probes past the count load harmlessly and are masked out), then the
     // probes are accumulated in order, which keeps the reference's summation order.
-    for (int k = 0; k < n; k += 4) {
-      float4 c[4];
+    for (int k = 0; k < n; k += SRX_ANISO_BATCH) {
+      float4 c[SRX_ANISO_BATCH];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < SRX_ANISO_BATCH; ++j) {
         c[j] = bilinear_wrap_rgba8(lv, x, y);
         x += af->du;
         y += af->dv;
       }
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < SRX_ANISO_BATCH; ++j) {
         const int i = 2 * (k + j) - n + 1;
         const bool live = k + j < n;
         float wf = static_cast<float>(i * i) * af->weight_D;
