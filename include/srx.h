@@ -300,7 +300,9 @@ int32_t srx_get_viewport(srx_renderer r, srx_viewport* vp);
 /* ---- render operations (renderer.h:117-129) ---- */
 int32_t srx_begin(srx_renderer r, srx_query q);
 int32_t srx_end(srx_renderer r, srx_query q);
-/* data: srx_pipeline_statistics / srx_internal_statistics / uint64_t[8] profiles (ns) */
+/* data: srx_pipeline_statistics / srx_internal_statistics / uint64_t[8] pipeline profiles in ns
+   (gather_vtx, vtx_proc, clipping, compact_clip, vp_trans, tri_dispatch, ras, profiled draws);
+   a uint64_t[10] buffer also receives ras split into visibility (8) and deferred shading (9). */
 int32_t srx_get_data(srx_renderer r, srx_query q, void* data, size_t size, int32_t do_not_wait);
 int32_t srx_draw(srx_renderer r, size_t startpos, size_t primcnt);
 int32_t srx_draw_index(srx_renderer r, size_t startpos, size_t primcnt, int32_t basevert);

@@ -13,7 +13,17 @@
 //  - 64x64 binning into per-thread tile lists, per-tile merge + sort (rasterizer.cpp:775-994);
 //  - Larrabee 64->16->4->1 traversal with the SSE evaluation order (rasterizer.cpp:298-773);
 //  - quad interpolation, early-Z, cpp pixel shaders and the framebuffer (framebuffer.cpp).
-// Parity status of each part is listed in DESIGN.md.
+//
+// Pinning (tests/test_oracle_pins.py, tests/test_golden.py): coverage by closed-form rectangles and
+// a per-pixel brute-force evaluation of the decision chain; depth by the screen-space plane;
+// RGBA8 rounding, resolve and the mip box filter by known answers; fast_log2/floori/ceili/round by
+// their formulas; the EWA table by the reference's literals; thread-count invariance.
+// Parity unpinned (no reference test, vector or build can check them here; they follow the cited
+// SSE code line by line): the SSE addressers' cvtt + size*8192 float modulo (sampler.cpp:27-119),
+// calc_anisotropic_info and the EWA probe loop (sampler.cpp:730-760, 875-958), calc_lod and the
+// linear-mip blend (sampler.cpp:521-603), the Lambert shader's SASL derivative pairing
+// (cgs_simd.cpp:267-312), compute_triangle_info's choice of v0 (rasterizer.cpp:876-899), and
+// vs_invocations (cache- and schedule-dependent in the reference).
 #include "../include/srx.h"
 
 #include "oracle_sampler.h"

@@ -406,7 +406,7 @@ class Renderer(_Obj):
         elif q.id == QUERY_INTERNAL_STATISTICS:
             d = InternalStatistics()
         else:
-            d = (C.c_uint64 * 8)()
+            d = (C.c_uint64 * 10)()
         st = self.lib.dll.srx_get_data(self.h, q.h, C.byref(d), C.sizeof(d), int(do_not_wait))
         return st, d
 

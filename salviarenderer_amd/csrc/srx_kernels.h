@@ -60,11 +60,13 @@ void launch_emit(const draw_params& P, const draw_buffers& B, hipStream_t s);
 int launch_sort(const draw_buffers& B, uint32_t n, int bits, hipStream_t s);
 // Sorts the GPU tiles by record count, heaviest first, into O.vals[result]; O supplies keys/vals
 // scratch (>= tiles each) and shares hist/scan_tmp with B.
-int launch_tile_order(const draw_params& P, const draw_buffers& B, draw_buffers& O, hipStream_t s);
+void launch_tile_order(const draw_params& P, const draw_buffers& B, uint32_t* order, hipStream_t s);
 void launch_tile_ranges(const draw_params& P, const draw_buffers& B, int which, uint32_t n, hipStream_t s);
 void launch_raster(const draw_params& P, const draw_buffers& B, int which, hipStream_t s);
 // Order-independent visibility for opaque early-Z draws without statistics; false if not applicable.
-bool launch_vis(const draw_params& P, const draw_buffers& B, int which, hipStream_t s);
+// Order-independent visibility (and, with shade, the deferred pixel shader) of an opaque early-Z
+// draw; false when the draw needs the in-order kernel.
+bool launch_vis(const draw_params& P, const draw_buffers& B, int which, bool shade, hipStream_t s);
 bool needs_shading(const draw_params& P);
 void launch_shade(const draw_params& P, const draw_buffers& B, hipStream_t s);
 

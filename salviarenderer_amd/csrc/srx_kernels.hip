@@ -93,6 +93,13 @@ __global__ void vs_kernel(draw_params P, float4* __restrict__ vbuf) {
 }
 
 // ============================================================== primitive assembly helpers
+// Setup records of clipped triangle id = 3 * primitive + fan index live at slot
+// fan index * prim_count + primitive: the fan-0 records of consecutive primitives (all of them when
+// nothing is clipped) are contiguous, so setup writes them densely.
+__device__ __forceinline__ size_t tri_slot(const draw_params& P, uint32_t id) {
+  const uint32_t p = id / 3u;
+  return static_cast<size_t>(id - 3u * p) * P.prim_count + p;
+}
 __device__ __forceinline__ uint32_t fetch_index(const draw_params& P, uint32_t prim, int k) {
   uint32_t id;
   if (P.topo == TOPO_TRIANGLE_LIST) {
@@ -402,8 +409,8 @@ __global__ void __launch_bounds__(256) setup_kernel(draw_params P, const float4*
         tri_setup<NA> t;
         triangle_info<NA>(o[0], o[1], o[2], t);
         if (t.valid) {
-          write_tri_rec<NA>(tri_rec + static_cast<size_t>(3 * p) * 5, t);
-          write_tri_attr<NA>(tri_attr + static_cast<size_t>(3 * p) * 3 * (NA + 1), t);
+          write_tri_rec<NA>(tri_rec + static_cast<size_t>(p) * 5, t);
+          write_tri_attr<NA>(tri_attr + static_cast<size_t>(p) * 3 * (NA + 1), t);
           float e[9], bb[4];
           for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) e[i * 3 + j] = t.e[i][j];
@@ -499,8 +506,8 @@ __global__ void clip_kernel(draw_params P, const float4* __restrict__ vbuf, floa
     triangle_info<NA>(a, b, c, ts);
     uint32_t id = 3 * p + (t - 1);
     if (!ts.valid) continue;
-    write_tri_rec<NA>(tri_rec + static_cast<size_t>(id) * 5, ts);
-    write_tri_attr<NA>(tri_attr + static_cast<size_t>(id) * 3 * (NA + 1), ts);
+    write_tri_rec<NA>(tri_rec + tri_slot(P, id) * 5, ts);
+    write_tri_attr<NA>(tri_attr + tri_slot(P, id) * 3 * (NA + 1), ts);
     float e[9], bb[4];
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) e[i * 3 + j] = ts.e[i][j];
@@ -521,7 +528,7 @@ __global__ void emit_kernel(draw_params P, const float4* __restrict__ tri_rec, c
   uint32_t next = (id + 1 < n) ? off[id + 1] : *total;
   if (next == off[id]) return;
   float e[9], bb[4];
-  read_tri_rec(tri_rec + static_cast<size_t>(id) * 5, e, bb);
+  read_tri_rec(tri_rec + tri_slot(P, id) * 5, e, bb);
   uint32_t o = off[id];
   for_each_gtile(P, e, bb, [&](uint32_t g, bool full) {
     if (o < next) {  // the enumeration repeats the counting pass exactly; the bound is insurance
@@ -715,21 +722,34 @@ __global__ void tile_range_kernel(const uint32_t* __restrict__ keys, uint32_t n,
   if (i == n - 1 || keys[i + 1] != k) te[k] = i + 1;
 }
 
-// Heaviest GPU tiles first: key = 65535 - min(records, 65535), value = tile id.
-__global__ void tile_order_kernel(const uint32_t* __restrict__ ts, const uint32_t* __restrict__ te, uint32_t T,
-                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= T) return;
-  uint32_t n = te[t] - ts[t];
-  keys[t] = 65535u - min(n, 65535u);
-  vals[t] = t;
+// Scheduling order of the GPU tiles, heaviest first so that the longest tiles start early.  Tiles
+// are independent, so the order never changes a pixel.  One workgroup buckets the tiles by record
+// count / 8 (heaviest bucket first) with LDS counters.
+__device__ __forceinline__ uint32_t tile_bucket(const uint32_t* ts, const uint32_t* te, uint32_t t) {
+  return 255u - min((te[t] - ts[t]) >> 3, 255u);
+}
+__global__ void __launch_bounds__(1024) tile_order_kernel(const uint32_t* __restrict__ ts, const uint32_t* __restrict__ te,
+                                                          uint32_t T, uint32_t* __restrict__ order) {
+  __shared__ uint32_t cnt[256];
+  if (threadIdx.x < 256) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) atomicAdd(&cnt[tile_bucket(ts, te, t)], 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t sum = 0;
+    for (int b = 0; b < 256; ++b) {
+      uint32_t c = cnt[b];
+      cnt[b] = sum;
+      sum += c;
+    }
+  }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) order[atomicAdd(&cnt[tile_bucket(ts, te, t)], 1u)] = t;
 }
 
-int launch_tile_order(const draw_params& P, const draw_buffers& B, draw_buffers& O, hipStream_t s) {
+void launch_tile_order(const draw_params& P, const draw_buffers& B, uint32_t* order, hipStream_t s) {
   uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
-  if (!T) return 0;
-  hipLaunchKernelGGL(tile_order_kernel, dim3((T + 255) / 256), dim3(256), 0, s, B.tstart, B.tend, T, O.keys[0], O.vals[0]);
-  return launch_sort(O, T, 16, s);
+  if (T) hipLaunchKernelGGL(tile_order_kernel, dim3(1), dim3(1024), 0, s, B.tstart, B.tend, T, order);
 }
 
 void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s) {
@@ -765,8 +785,9 @@ struct attr_setup {
 };
 
 template <int NA>
-__device__ __forceinline__ void load_attr(const float4* __restrict__ tri_attr, uint32_t id, attr_setup<NA>& a) {
-  const float4* src = tri_attr + static_cast<size_t>(id) * 3 * (NA + 1);
+__device__ __forceinline__ void load_attr(const draw_params& P, const float4* __restrict__ tri_attr, uint32_t id,
+                                          attr_setup<NA>& a) {
+  const float4* src = tri_attr + tri_slot(P, id) * 3 * (NA + 1);
 #pragma unroll
   for (int r = 0; r <= NA; ++r) {
     a.v0[r] = src[r];
@@ -1009,7 +1030,7 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, const ps_sam
       prim_lds& L = lds[tid];
       uint32_t rec = vals[base + tid];
       uint32_t id = rec >> 1;
-      const float4* tr = tri_rec + static_cast<size_t>(id) * 5;
+      const float4* tr = tri_rec + tri_slot(P, id) * 5;
       float e[9], bb[4];
       read_tri_rec(tr, e, bb);
       float4 d3 = tr[3], d4 = tr[4];
@@ -1162,7 +1183,7 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, const ps_sam
                 if (tested[i] & (1u << s)) winner[i][s] = id;
           } else {
             attr_setup<NA> at;
-            load_attr<NA>(tri_attr, id, at);
+            load_attr<NA>(P, tri_attr, id, at);
             float4 q[4][NA + 1];
             interp_quad<NA>(at, px0, py0, P, q);
             float4 col[4];
@@ -1178,7 +1199,7 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, const ps_sam
           ps_inv += 4;
           be_pix += 4;
           attr_setup<NA> at;
-          load_attr<NA>(tri_attr, id, at);
+          load_attr<NA>(P, tri_attr, id, at);
           float4 q[4][NA + 1];
           interp_quad<NA>(at, px0, py0, P, q);
           float4 col[4];
@@ -1297,17 +1318,59 @@ __device__ __forceinline__ uint32_t key_winner(int mode, unsigned long long k) {
   return lo == 0xFFFFFFFFu ? NO_WINNER : 0xFFFFFFFEu - lo;
 }
 
-// 16x16 GPU tile, one triangle per lane.  keys: 256 pixels x S samples.
+// Depth the reference's early-Z test wrote for winner w of sample smp of pixel (px, py).
+template <int S>
+__device__ __forceinline__ void write_winner_depth(const draw_params& P, int mode, unsigned long long key, uint32_t w,
+                                                   const float4* __restrict__ tri_rec, int px, int py, int smp) {
+  if (mode != VIS_LATEST) {
+    // depth modes carry the winning sample depth in the key; only +-0 is ambiguous there
+    uint32_t zk = static_cast<uint32_t>(key >> 32);
+    if (mode == VIS_GREATER || mode == VIS_GEQUAL) zk = ~zk;
+    const uint32_t u = (zk & 0x80000000u) ? (zk & 0x7FFFFFFFu) : ~zk;
+    if (u != 0u) {
+      *reinterpret_cast<float*>(sample_addr(P.ds, px, py, smp)) = __uint_as_float(u);
+      return;
+    }
+  }
+  // the winner's depth at this sample, as its quad computed it
+  const float4* tr = tri_rec + tri_slot(P, w) * 5;
+  const float4 d3 = tr[3], d4 = tr[4];
+  const int qx0 = px & ~1, qy0 = py & ~1;
+  const float dx = (0.5f + static_cast<float>(qx0)) - d3.y;
+  const float dy = (0.5f + static_cast<float>(qy0)) - d3.z;
+  float z00 = d3.w + ((d4.x * dx) + (d4.y * dy));
+  float zp;
+  if ((px & 1) == 0 && (py & 1) == 0) zp = z00;
+  else if ((py & 1) == 0) zp = z00 + d4.x;
+  else if ((px & 1) == 0) zp = z00 + d4.y;
+  else zp = (z00 + d4.x) + d4.y;
+  float nd = zp;
+  if (S > 1) nd = ((P.sp[smp][0] - 0.5f) * d4.x + (P.sp[smp][1] - 0.5f) * d4.y) + zp;
+  *reinterpret_cast<float*>(sample_addr(P.ds, px, py, smp)) = nd;
+}
+
+// One 16x16 GPU tile per workgroup, in three phases:
+//  1. one triangle per lane: coverage with the reference's decision chain, LDS atomic-min of the
+//     (ordered depth, primitive order) key of every covered sample;
+//  2. one pixel per lane: winners -> depth/stencil target, and one shading item per distinct winner
+//     (pixel, winner, sample mask) appended to an LDS list;
+//  3. (SHADE) one item per lane: the winner's quad interpolants at the pixel's quad (the values the
+//     reference's last passing draw_quad/draw_full_quad computed), the pixel shader for that pixel,
+//     and the colour stored to the item's samples.  Items are spread evenly over the lanes however
+//     many triangles share a pixel.
 #ifndef SRX_VIS_THREADS
 #define SRX_VIS_THREADS 128
 #endif
-template <int S>
-__global__ void __launch_bounds__(SRX_VIS_THREADS) vis_kernel(draw_params P, int mode, const uint32_t* __restrict__ order,
+template <int NA, int S, bool SHADE>
+__global__ void __launch_bounds__(SRX_VIS_THREADS, 4) vis_kernel(draw_params P, int mode, const uint32_t* __restrict__ order,
                                                    const uint32_t* __restrict__ vals,
                                                    const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ tend,
-                                                   const float4* __restrict__ tri_rec, uint32_t* __restrict__ vis) {
+                                                   const float4* __restrict__ tri_rec, const float4* __restrict__ tri_attr,
+                                                   const ps_samplers* __restrict__ SM) {
   __shared__ unsigned long long keys[256 * S];
   __shared__ float d0s[256 * S];
+  __shared__ uint2 items[SHADE ? 256 * S : 1];
+  __shared__ uint32_t nitems;
   const int gtile = order[blockIdx.x];
   const int gx = gtile % P.gtiles_x, gy = gtile / P.gtiles_x;
   const int ox = gx * 16, oy = gy * 16;
@@ -1315,6 +1378,7 @@ __global__ void __launch_bounds__(SRX_VIS_THREADS) vis_kernel(draw_params P, int
   const int tw = static_cast<int>(P.target_w), th = static_cast<int>(P.target_h);
   const int tid = threadIdx.x, NT = blockDim.x;
   const bool has_ds = P.ds.data != nullptr;
+  if (tid == 0) nitems = 0;
   for (int i = tid; i < 256 * S; i += NT) {
     int px = ox + ((i / S) & 15), py = oy + ((i / S) >> 4), smp = i % S;
     float d0 = 0.0f;
@@ -1329,7 +1393,7 @@ __global__ void __launch_bounds__(SRX_VIS_THREADS) vis_kernel(draw_params P, int
   for (uint32_t base = t0 + tid; base < t1; base += NT) {
     const uint32_t rec = vals[base];
     const uint32_t id = rec >> 1;
-    const float4* tr = tri_rec + static_cast<size_t>(id) * 5;
+    const float4* tr = tri_rec + tri_slot(P, id) * 5;
     float e[9], bb[4];
     read_tri_rec(tr, e, bb);
     const float4 d3 = tr[3], d4 = tr[4];
@@ -1454,41 +1518,61 @@ __global__ void __launch_bounds__(SRX_VIS_THREADS) vis_kernel(draw_params P, int
     }
   }
   __syncthreads();
-  // winners -> visibility buffer and depth
-  for (int i = tid; i < 256 * S; i += NT) {
-    const int p = i / S, smp = i % S;
+  // phase 2: winners -> depth, shading items
+  for (int p = tid; p < 256; p += NT) {
     const int px = ox + (p & 15), py = oy + (p >> 4);
     if (px >= tw || py >= th) continue;
-    const unsigned long long key = keys[i];
-    const uint32_t w = key_winner(mode, key);
-    vis[(static_cast<size_t>(py) * tw + px) * S + smp] = w;
-    if (w != NO_WINNER && has_ds && P.write_depth && mode != VIS_LATEST) {
-      // depth modes carry the winning sample depth in the key; only +-0 is ambiguous there
-      uint32_t zk = static_cast<uint32_t>(key >> 32);
-      if (mode == VIS_GREATER || mode == VIS_GEQUAL) zk = ~zk;
-      const uint32_t u = (zk & 0x80000000u) ? (zk & 0x7FFFFFFFu) : ~zk;
-      if (u != 0u) {
-        *reinterpret_cast<float*>(sample_addr(P.ds, px, py, smp)) = __uint_as_float(u);
-        continue;
+    uint32_t w[S];
+    uint32_t left = 0;
+#pragma unroll
+    for (int smp = 0; smp < S; ++smp) {
+      const unsigned long long key = keys[p * S + smp];
+      w[smp] = key_winner(mode, key);
+      if (w[smp] == NO_WINNER) continue;
+      left |= 1u << smp;
+      if (has_ds && P.write_depth) write_winner_depth<S>(P, mode, key, w[smp], tri_rec, px, py, smp);
+    }
+    if (SHADE) {
+#pragma unroll
+      for (int it = 0; it < S; ++it) {
+        if (!left) break;
+        const int s0 = __builtin_ctz(left);
+        uint32_t cur = w[0];
+#pragma unroll
+        for (int smp = 1; smp < S; ++smp)
+          if (smp == s0) cur = w[smp];
+        uint32_t m = 0;
+#pragma unroll
+        for (int smp = 0; smp < S; ++smp)
+          if (w[smp] == cur) m |= 1u << smp;
+        left &= ~m;
+        items[atomicAdd(&nitems, 1u)] = make_uint2(cur, static_cast<uint32_t>(p) | (m << 8));
       }
     }
-    if (w != NO_WINNER && has_ds && P.write_depth) {
-      // the winner's depth at this sample, as its quad computed it
-      const float4* tr = tri_rec + static_cast<size_t>(w) * 5;
-      const float4 d3 = tr[3], d4 = tr[4];
-      const int qx0 = px & ~1, qy0 = py & ~1;
-      const float dx = (0.5f + static_cast<float>(qx0)) - d3.y;
-      const float dy = (0.5f + static_cast<float>(qy0)) - d3.z;
-      float z00 = d3.w + ((d4.x * dx) + (d4.y * dy));
-      float zp;
-      if ((px & 1) == 0 && (py & 1) == 0) zp = z00;
-      else if ((py & 1) == 0) zp = z00 + d4.x;
-      else if ((px & 1) == 0) zp = z00 + d4.y;
-      else zp = (z00 + d4.x) + d4.y;
-      float nd = zp;
-      if (S > 1) nd = ((P.sp[smp][0] - 0.5f) * d4.x + (P.sp[smp][1] - 0.5f) * d4.y) + zp;
-      *reinterpret_cast<float*>(sample_addr(P.ds, px, py, smp)) = nd;
-    }
+  }
+  if (!SHADE) return;
+  __syncthreads();
+  // phase 3: shading items
+  const uint32_t n = nitems;
+  for (uint32_t j = tid; j < n; j += NT) {
+    const uint2 item = items[j];
+    const uint32_t cur = item.x, p = item.y & 255u, m = item.y >> 8;
+    if (cur >= 3u * P.prim_count) continue;  // every winner id is below 3 * prim_count
+    const int px = ox + static_cast<int>(p & 15u), py = oy + static_cast<int>(p >> 4);
+    const int pi = (px & 1) | ((py & 1) << 1);
+    attr_setup<NA> at;
+    load_attr<NA>(P, tri_attr, cur, at);
+    float4 q[4][NA + 1];
+    interp_quad<NA>(at, px & ~1, py & ~1, P, q);
+    float4 col[4];
+    run_ps<NA>(P, SM, q, 1u << pi, col);
+    float4 c = col[0];
+#pragma unroll
+    for (int k = 1; k < 4; ++k)
+      if (k == pi) c = col[k];
+#pragma unroll
+    for (int smp = 0; smp < S; ++smp)
+      if (m & (1u << smp)) store_texel(P.color.pf, sample_addr(P.color, px, py, smp), c);
   }
 }
 
@@ -1516,7 +1600,7 @@ __global__ void __launch_bounds__(256) shade_kernel(draw_params P, const ps_samp
       if (cur == NO_WINNER) cur = w[s];
     if (cur >= 3u * P.prim_count) break;  // NO_WINNER (every id is below 3 * prim_count)
     attr_setup<NA> at;
-    load_attr<NA>(tri_attr, cur, at);
+    load_attr<NA>(P, tri_attr, cur, at);
     float4 q[4][NA + 1];
     interp_quad<NA>(at, px0, py0, P, q);
     float4 col[4];
@@ -1623,16 +1707,31 @@ int vis_mode(const draw_params& P) {
   }
 }
 
-bool launch_vis(const draw_params& P, const draw_buffers& B, int which, hipStream_t s) {
+bool launch_vis(const draw_params& P, const draw_buffers& B, int which, bool shade, hipStream_t s) {
   int mode = vis_mode(P);
   uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
   if (mode < 0 || P.gt != 16) return false;
   if (!T) return true;
-  switch (P.samples) {
-  case 1: hipLaunchKernelGGL((vis_kernel<1>), dim3(T), dim3(SRX_VIS_THREADS), 0, s, P, mode, B.order, B.vals[which], B.tstart, B.tend, B.tri_rec, B.vis); break;
-  case 2: hipLaunchKernelGGL((vis_kernel<2>), dim3(T), dim3(SRX_VIS_THREADS), 0, s, P, mode, B.order, B.vals[which], B.tstart, B.tend, B.tri_rec, B.vis); break;
-  default: hipLaunchKernelGGL((vis_kernel<4>), dim3(T), dim3(SRX_VIS_THREADS), 0, s, P, mode, B.order, B.vals[which], B.tstart, B.tend, B.tri_rec, B.vis); break;
+#define SRX_VIS_LAUNCH(NA, S, SH)                                                                                    \
+  hipLaunchKernelGGL((vis_kernel<NA, S, SH>), dim3(T), dim3(SRX_VIS_THREADS), 0, s, P, mode, B.order, B.vals[which], \
+                     B.tstart, B.tend, B.tri_rec, B.tri_attr, B.samplers)
+#define SRX_VIS_SHADE(NA)                   \
+  switch (P.samples) {                      \
+  case 1: SRX_VIS_LAUNCH(NA, 1, true); break; \
+  case 2: SRX_VIS_LAUNCH(NA, 2, true); break; \
+  default: SRX_VIS_LAUNCH(NA, 4, true); break; \
   }
+  if (shade) {
+    SRX_NA_DISPATCH(P.nattr, SRX_VIS_SHADE)
+  } else {
+    switch (P.samples) {
+    case 1: SRX_VIS_LAUNCH(1, 1, false); break;
+    case 2: SRX_VIS_LAUNCH(1, 2, false); break;
+    default: SRX_VIS_LAUNCH(1, 4, false); break;
+    }
+  }
+#undef SRX_VIS_SHADE
+#undef SRX_VIS_LAUNCH
   return true;
 }
 

@@ -154,8 +154,11 @@ struct ds_obj : object {
   ds_obj() : object(K_DS) {}
 };
 // pipeline_profiles slots (async_object.h:187-196): gather_vtx, vtx_proc, clipping, compact_clip,
-// vp_trans, tri_dispatch, ras; slot 7 counts the profiled draws.
-enum { PROF_GATHER = 0, PROF_VTX, PROF_CLIP, PROF_COMPACT, PROF_VP, PROF_DISPATCH, PROF_RAS, PROF_DRAWS, PROF_COUNT };
+// vp_trans, tri_dispatch, ras; slot 7 counts the profiled draws.  Slots 8 and 9 split ras into
+// its visibility/in-order raster kernel and the deferred shading kernel (returned when the caller's
+// buffer holds them).
+enum { PROF_GATHER = 0, PROF_VTX, PROF_CLIP, PROF_COMPACT, PROF_VP, PROF_DISPATCH, PROF_RAS, PROF_DRAWS,
+       PROF_RAS_VIS, PROF_RAS_SHADE, PROF_COUNT };
 
 struct query_obj : object {
   uint32_t id;
@@ -169,9 +172,9 @@ struct query_obj : object {
 
 // HIP events bracketing the stages of one profiled draw, on the renderer's stream:
 // [0,1) vertex shader, [1,2) setup + clipping (viewport transform fused), [2,3) compaction scan,
-// [3,4) record emission + tile sort, [4,5) raster.
+// [3,4) record emission + tile sort, [4,5) visibility or in-order raster, [5,6) deferred shading.
 struct draw_events {
-  hipEvent_t ev[6];
+  hipEvent_t ev[7];
 };
 
 std::string lower(const char* s) {
@@ -278,7 +281,7 @@ struct renderer_obj : object {
   dev_array<ps_samplers> samplers_d;
   ps_samplers* samplers_h = nullptr;  // pinned staging of the per-draw sampler table
   dev_array<uint32_t> cnt, keys0, keys1, vals0, vals1, tstart, tend, hist, scan_tmp, clip_hdr, counters, vis;
-  dev_array<uint32_t> okeys0, okeys1, ovals0, ovals1;
+  dev_array<uint32_t> order;  // GPU tile scheduling order
   dev_array<uint8_t> clip_flag;
   dev_array<int32_t> clip_slot;
   dev_array<unsigned long long> partial;
@@ -313,7 +316,7 @@ struct renderer_obj : object {
     };
     fr(vbuf); fr(tri_rec); fr(tri_attr); fr(clip_verts); fr(samplers_d); fr(vis);
     if (samplers_h) (void)hipHostFree(samplers_h); fr(cnt); fr(keys0); fr(keys1); fr(vals0); fr(vals1);
-    fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot); fr(partial); fr(okeys0); fr(okeys1); fr(ovals0); fr(ovals1);
+    fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot); fr(partial); fr(order);
     if (stats_d) (void)hipFree(stats_d);
     if (pinned) (void)hipHostFree(pinned);
     for (auto& d : pending_prof)
@@ -607,25 +610,19 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   while ((1u << bits) < T) ++bits;
   int which = launch_sort(B, total, bits, s);
   launch_tile_ranges(P, B, which, total, s);
-  RCHK(ensure(r, r->okeys0, std::max<uint32_t>(T, 1)));
-  RCHK(ensure(r, r->okeys1, std::max<uint32_t>(T, 1)));
-  RCHK(ensure(r, r->ovals0, std::max<uint32_t>(T, 1)));
-  RCHK(ensure(r, r->ovals1, std::max<uint32_t>(T, 1)));
-  draw_buffers O = B;
-  O.keys[0] = r->okeys0.p;
-  O.keys[1] = r->okeys1.p;
-  O.vals[0] = r->ovals0.p;
-  O.vals[1] = r->ovals1.p;
-  int ow = launch_tile_order(P, B, O, s);
-  B.order = O.vals[ow];
+  RCHK(ensure(r, r->order, std::max<uint32_t>(T, 1)));
+  launch_tile_order(P, B, r->order.p, s);
+  B.order = r->order.p;
   mark(4);
   // statistics queries need the in-order walk (ps_invocations counts early-Z passes as they happen)
   bool counting = false;
   for (auto& q : r->active)
     if (q->id != SRX_QUERY_PIPELINE_PROFILES) counting = true;
-  if (!(vis_pass && !counting && launch_vis(P, B, which, s))) launch_raster(P, B, which, s);
-  if (vis_pass && needs_shading(P)) launch_shade(P, B, s);
+  const bool fused = vis_pass && !counting && launch_vis(P, B, which, needs_shading(P), s);
+  if (!fused) launch_raster(P, B, which, s);
   mark(5);
+  if (!fused && vis_pass && needs_shading(P)) launch_shade(P, B, s);
+  mark(6);
   if (prof) r->pending_prof.push_back(de);
   HIPCHK(hipGetLastError());
   return SRX_OK;
@@ -637,14 +634,16 @@ int32_t collect_profiles(renderer_obj* r) {
   HIPCHK(hipStreamSynchronize(r->stream));
   uint64_t add[PROF_COUNT] = {};
   for (auto& d : r->pending_prof) {
-    float ms[5];
-    for (int i = 0; i < 5; ++i) HIPCHK(hipEventElapsedTime(&ms[i], d.ev[i], d.ev[i + 1]));
+    float ms[6];
+    for (int i = 0; i < 6; ++i) HIPCHK(hipEventElapsedTime(&ms[i], d.ev[i], d.ev[i + 1]));
     auto ns = [](float m) { return static_cast<uint64_t>(static_cast<double>(m) * 1.0e6); };
     add[PROF_VTX] += ns(ms[0]);
     add[PROF_CLIP] += ns(ms[1]);
     add[PROF_COMPACT] += ns(ms[2]);
     add[PROF_DISPATCH] += ns(ms[3]);
-    add[PROF_RAS] += ns(ms[4]);
+    add[PROF_RAS] += ns(ms[4]) + ns(ms[5]);
+    add[PROF_RAS_VIS] += ns(ms[4]);
+    add[PROF_RAS_SHADE] += ns(ms[5]);
     add[PROF_DRAWS] += 1;
     for (auto e : d.ev) r->event_pool.push_back(e);
   }
@@ -1059,7 +1058,8 @@ int32_t srx_get_data(srx_renderer rr, srx_query qq, void* data, size_t size, int
   if (!as<renderer_obj>(rr, K_RENDERER) || !q || !data || !q->has_data) return SRX_ASYNC_ERROR;
   if (q->id == SRX_QUERY_PIPELINE_STATISTICS && size >= sizeof(q->stats)) std::memcpy(data, &q->stats, sizeof(q->stats));
   else if (q->id == SRX_QUERY_INTERNAL_STATISTICS && size >= sizeof(q->istats)) std::memcpy(data, &q->istats, sizeof(q->istats));
-  else if (q->id == SRX_QUERY_PIPELINE_PROFILES && size >= sizeof(q->prof)) std::memcpy(data, q->prof, sizeof(q->prof));
+  else if (q->id == SRX_QUERY_PIPELINE_PROFILES && size >= 8 * sizeof(uint64_t))
+    std::memcpy(data, q->prof, std::min(size, sizeof(q->prof)));
   else return SRX_ASYNC_ERROR;
   return SRX_ASYNC_READY;
 }
