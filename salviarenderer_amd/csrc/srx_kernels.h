@@ -54,7 +54,7 @@ void launch_clip(const draw_params& P, const draw_buffers& B, hipStream_t s);
 // In-place exclusive scan of n u32; writes the total to *total_dev.  tmp holds >= scan_tmp_size(n).
 size_t scan_tmp_size(uint32_t n);
 void launch_exclusive_scan(uint32_t* data, uint32_t n, uint32_t* tmp, uint32_t* total_dev, hipStream_t s);
-void launch_emit(const draw_params& P, const draw_buffers& B, hipStream_t s);
+void launch_emit(const draw_params& P, const draw_buffers& B, uint32_t capacity, hipStream_t s);
 // Stable LSD radix sort of (keys, vals) by the low `bits` bits of keys; returns the buffer index
 // (0 or 1) that holds the result.
 int launch_sort(const draw_buffers& B, uint32_t n, int bits, hipStream_t s);

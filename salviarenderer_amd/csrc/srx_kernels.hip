@@ -71,7 +71,12 @@ __global__ void vs_kernel(draw_params P, float4* __restrict__ vbuf) {
 #pragma unroll
   for (int i = 0; i < MAX_INPUTS; ++i) in[i] = make_float4(0, 0, 0, 0);
   if (v < P.nverts)
-    for (int i = 0; i < P.nin; ++i) in[P.in[i].reg] = fetch_elem(P.in[i], v);
+    for (int i = 0; i < P.nin; ++i) {
+      const float4 x = fetch_elem(P.in[i], v);
+#pragma unroll
+      for (int r = 0; r < MAX_INPUTS; ++r)  // register-resident: no dynamically indexed array
+        if (P.in[i].reg == r) in[r] = x;
+    }
   float4 out[3];
   switch (P.vs_id) {
   case VS_PASSTHROUGH_COLOR:
@@ -89,7 +94,9 @@ __global__ void vs_kernel(draw_params P, float4* __restrict__ vbuf) {
     break;
   }
   size_t base = static_cast<size_t>(v) * (P.nattr + 1);
-  for (int r = 0; r <= P.nattr; ++r) vbuf[base + r] = out[r];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+    if (r <= P.nattr) vbuf[base + r] = out[r];
 }
 
 // ============================================================== primitive assembly helpers
@@ -165,9 +172,29 @@ struct tri_setup {
 };
 
 // compute_triangle_info (rasterizer.cpp:864-945) + compute_derivative_n (shader.cpp:414-450)
+// Register selects of whole vertices, one float at a time: a conditional over vertex lvalues
+// would select addresses and push the vertices into scratch memory.
+__device__ __forceinline__ float4 sel4(bool c, float4 a, float4 b) {
+  return make_float4(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.w : b.w);
+}
+template <int NA>
+__device__ __forceinline__ vtx<NA> vsel(bool c, const vtx<NA>& a, const vtx<NA>& b) {
+  vtx<NA> o;
+#pragma unroll
+  for (int r = 0; r <= NA; ++r) o.r[r] = sel4(c, a.r[r], b.r[r]);
+  return o;
+}
+template <int NA>
+__device__ __forceinline__ vtx<NA> pick3(int i, const vtx<NA>& a, const vtx<NA>& b, const vtx<NA>& c) {
+  vtx<NA> o;
+#pragma unroll
+  for (int r = 0; r <= NA; ++r) o.r[r] = sel4(i == 0, a.r[r], sel4(i == 1, b.r[r], c.r[r]));
+  return o;
+}
+
 template <int NA>
 __device__ __forceinline__ void triangle_info(const vtx<NA>& a, const vtx<NA>& b, const vtx<NA>& c, tri_setup<NA>& t) {
-  const vtx<NA>* V[3] = {&a, &b, &c};
+  const vtx<NA>* V[3] = {&a, &b, &c};  // constant indices only (after unrolling)
   double dist[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) dist[i] = static_cast<double>(fabsf(V[i]->r[0].x) + fabsf(V[i]->r[0].y));
@@ -175,9 +202,10 @@ __device__ __forceinline__ void triangle_info(const vtx<NA>& a, const vtx<NA>& b
   if (dist[0] < dist[1]) r0 = dist[0] < dist[2] ? 0 : 2;
   else r0 = dist[1] < dist[2] ? 1 : 2;
   int r1 = (r0 + 1) % 3, r2 = (r1 + 1) % 3;
-  const vtx<NA>& R0 = *V[r0];
-  const vtx<NA>& R1 = *V[r1];
-  const vtx<NA>& R2 = *V[r2];
+  // cyclic reorder by selects, so the vertices stay in registers
+  const vtx<NA> R0 = pick3<NA>(r0, a, b, c);
+  const vtx<NA> R1 = pick3<NA>(r1, a, b, c);
+  const vtx<NA> R2 = pick3<NA>(r2, a, b, c);
   float e01[NA + 1][4], e02[NA + 1][4];
 #pragma unroll
   for (int r = 0; r <= NA; ++r) {
@@ -239,10 +267,10 @@ __device__ __forceinline__ void load_clipped_triangle(const draw_params& P, cons
 #pragma unroll
     for (int i = 0; i < 3; ++i) load_vertex<NA>(P, vbuf, fetch_index(P, prim, i), v[i]);
     bool front = compute_front(v[0].r[0], v[1].r[0], v[2].r[0]);
-    int off = front ? 0 : 1;  // winding normalisation (clipper.cpp:59-65)
+    // winding normalisation (clipper.cpp:59-65) by selects
     out[0] = v[0];
-    out[1] = v[1 + off];
-    out[2] = v[2 - off];
+    out[1] = vsel<NA>(front, v[1], v[2]);
+    out[2] = vsel<NA>(front, v[2], v[1]);
 #pragma unroll
     for (int i = 0; i < 3; ++i) viewport_project<NA>(out[i], P);
     return;
@@ -402,8 +430,8 @@ __global__ void __launch_bounds__(256) setup_kernel(draw_params P, const float4*
       bool front = compute_front(v[0].r[0], v[1].r[0], v[2].r[0]);
       if (!cull_fn(P, front ? 1.0f : -1.0f)) {
         cprims = 1;
-        int off = front ? 0 : 1;
-        vtx<NA> o[3] = {v[0], v[1 + off], v[2 - off]};
+        // winding normalisation (clipper.cpp:59-65) by selects
+        vtx<NA> o[3] = {v[0], vsel<NA>(front, v[1], v[2]), vsel<NA>(front, v[2], v[1])};
 #pragma unroll
         for (int i = 0; i < 3; ++i) viewport_project<NA>(o[i], P);
         tri_setup<NA> t;
@@ -445,6 +473,7 @@ __global__ void clip_kernel(draw_params P, const float4* __restrict__ vbuf, floa
                             float4* __restrict__ tri_attr, uint32_t* __restrict__ cnt, const uint8_t* __restrict__ clip_flag, int32_t* __restrict__ clip_slot,
                             float4* __restrict__ clip_verts, uint32_t* __restrict__ clip_hdr, uint32_t capacity,
                             uint32_t* __restrict__ counters, unsigned long long* __restrict__ stats) {
+  if (counters[CT_CLIP_PRIMS] == 0) return;  // setup flagged nothing
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P.prim_count || !clip_flag[p]) return;
   vtx<NA> cv[2][5];
@@ -520,8 +549,10 @@ __global__ void clip_kernel(draw_params P, const float4* __restrict__ vbuf, floa
 
 // ============================================================== record emission
 // off: exclusive scan of the per-triangle record counts; a triangle owns [off[id], off[id+1]).
+// Records beyond `capacity` are dropped; the host compares the total with it and emits again.
 __global__ void emit_kernel(draw_params P, const float4* __restrict__ tri_rec, const uint32_t* __restrict__ off,
-                            const uint32_t* __restrict__ total, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                            const uint32_t* __restrict__ total, uint32_t capacity, uint32_t* __restrict__ keys,
+                            uint32_t* __restrict__ vals) {
   uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t n = 3 * P.prim_count;
   if (id >= n) return;
@@ -531,7 +562,7 @@ __global__ void emit_kernel(draw_params P, const float4* __restrict__ tri_rec, c
   read_tri_rec(tri_rec + tri_slot(P, id) * 5, e, bb);
   uint32_t o = off[id];
   for_each_gtile(P, e, bb, [&](uint32_t g, bool full) {
-    if (o < next) {  // the enumeration repeats the counting pass exactly; the bound is insurance
+    if (o < next && o < capacity) {  // the enumeration repeats the counting pass exactly
       keys[o] = g;
       vals[o] = (id << 1) | (full ? 1u : 0u);
     }
@@ -545,7 +576,8 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
 }
 
 // ============================================================== scan (exclusive, u32)
-constexpr int SCAN_BLOCK = 1024;  // elements per workgroup (256 threads x 4)
+constexpr int SCAN_PER_THREAD = 16;
+constexpr int SCAN_BLOCK = 256 * SCAN_PER_THREAD;  // elements per workgroup
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   int lane = __lane_id();
@@ -557,40 +589,46 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   return v;
 }
 
-// Exclusive scan of one 1024-element block in place; returns the block total to thread 0's sums slot.
-__global__ void __launch_bounds__(256) scan_block_kernel(uint32_t* data, uint32_t n, uint32_t* sums) {
+// Exclusive scan of one SCAN_BLOCK-element block in place.  The block total goes to sums[block]
+// (when sums is given); a single-block launch also stores it to *total.
+__global__ void __launch_bounds__(256) scan_block_kernel(uint32_t* data, uint32_t n, uint32_t* sums, uint32_t* total) {
   __shared__ uint32_t wsum[4];
-  uint32_t base = blockIdx.x * SCAN_BLOCK + threadIdx.x * 4;
-  uint32_t v[4];
+  const uint32_t base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_PER_THREAD;
+  uint32_t v[SCAN_PER_THREAD];
+  uint32_t t = 0;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) v[i] = (base + i < n) ? data[base + i] : 0u;
-  uint32_t t = v[0] + v[1] + v[2] + v[3];
-  uint32_t incl = wave_incl_scan(t);
-  int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = 0; i < SCAN_PER_THREAD; ++i) {
+    v[i] = (base + i < n) ? data[base + i] : 0u;
+    t += v[i];
+  }
+  const uint32_t incl = wave_incl_scan(t);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (lane == 63) wsum[wave] = incl;
   __syncthreads();
   uint32_t wprefix = 0;
   for (int w = 0; w < wave; ++w) wprefix += wsum[w];
   uint32_t run = wprefix + incl - t;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    uint32_t x = v[i];
+  for (int i = 0; i < SCAN_PER_THREAD; ++i) {
+    const uint32_t x = v[i];
     if (base + i < n) data[base + i] = run;
     run += x;
   }
-  if (threadIdx.x == 255) sums[blockIdx.x] = wprefix + incl;
+  if (threadIdx.x == 255) {
+    const uint32_t block_total = wprefix + incl;
+    if (sums) sums[blockIdx.x] = block_total;
+    if (total && gridDim.x == 1) *total = block_total;
+  }
 }
 
-__global__ void scan_add_kernel(uint32_t* data, uint32_t n, const uint32_t* sums) {
-  uint32_t i = blockIdx.x * SCAN_BLOCK + threadIdx.x * 4;
-  uint32_t add = sums[blockIdx.x];
+__global__ void __launch_bounds__(256) scan_add_kernel(uint32_t* data, uint32_t n, const uint32_t* sums) {
+  const uint32_t add = sums[blockIdx.x];
+  const uint32_t b0 = blockIdx.x * SCAN_BLOCK;
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
-    if (i + k < n) data[i + k] += add;
-}
-
-__global__ void scan_total_kernel(const uint32_t* sums_scanned, const uint32_t* sums_orig_last, uint32_t nb, uint32_t* total) {
-  *total = sums_scanned[nb - 1] + *sums_orig_last;
+  for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+    const uint32_t i = b0 + k * 256 + threadIdx.x;
+    if (i < n) data[i] += add;
+  }
 }
 
 size_t scan_tmp_size(uint32_t n) {
@@ -598,29 +636,25 @@ size_t scan_tmp_size(uint32_t n) {
   uint32_t m = n;
   while (true) {
     uint32_t nb = (m + SCAN_BLOCK - 1) / SCAN_BLOCK;
-    total += nb + 1;
     if (nb <= 1) break;
+    total += nb;
     m = nb;
   }
   return total + 4;
 }
 
-// Recursive reduce-then-scan.  tmp layout per level: sums[nb] then one slot for the last sum.
+// Reduce-then-scan: blocks scan in place, the block totals are scanned the same way (one block up
+// to SCAN_BLOCK^2 elements), then added back.  The top-level single block yields the total.
 static void scan_rec(uint32_t* data, uint32_t n, uint32_t* tmp, uint32_t* total_dev, hipStream_t s) {
   uint32_t nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
-  if (nb == 0) nb = 1;
-  uint32_t* sums = tmp;
-  uint32_t* last = tmp + nb;
-  hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(256), 0, s, data, n, sums);
-  // keep the last block's sum before the sums are scanned
-  hipMemcpyAsync(last, sums + nb - 1, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
-  if (nb > 1) {
-    scan_rec(sums, nb, tmp + nb + 1, nullptr, s);
-    hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(256), 0, s, data, n, sums);
-  } else {
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, sums, 1u, 0u);
+  if (nb <= 1) {
+    hipLaunchKernelGGL(scan_block_kernel, dim3(1), dim3(256), 0, s, data, n, nullptr, total_dev);
+    return;
   }
-  if (total_dev) hipLaunchKernelGGL(scan_total_kernel, dim3(1), dim3(1), 0, s, sums, last, nb, total_dev);
+  uint32_t* sums = tmp;
+  hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(256), 0, s, data, n, sums, nullptr);
+  scan_rec(sums, nb, tmp + nb, total_dev, s);
+  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(256), 0, s, data, n, sums);
 }
 
 void launch_exclusive_scan(uint32_t* data, uint32_t n, uint32_t* tmp, uint32_t* total_dev, hipStream_t s) {
@@ -1655,11 +1689,11 @@ void launch_clip(const draw_params& P, const draw_buffers& B, hipStream_t s) {
 #undef SRX_CLIP
 }
 
-void launch_emit(const draw_params& P, const draw_buffers& B, hipStream_t s) {
+void launch_emit(const draw_params& P, const draw_buffers& B, uint32_t capacity, hipStream_t s) {
   uint32_t n = 3 * P.prim_count;
   if (!n) return;
   hipLaunchKernelGGL(emit_kernel, dim3((n + 255) / 256), dim3(256), 0, s, P, B.tri_rec, B.cnt, B.counters + CT_TOTAL,
-                     B.keys[0], B.vals[0]);
+                     capacity, B.keys[0], B.vals[0]);
 }
 
 void launch_raster(const draw_params& P, const draw_buffers& B, int which, hipStream_t s) {

@@ -287,6 +287,7 @@ struct renderer_obj : object {
   dev_array<unsigned long long> partial;
   unsigned long long* stats_d = nullptr;
   uint32_t* pinned = nullptr;  // host staging for counter read-back
+  uint32_t last_records = 0;   // binned records of the previous draw (sizes the next emission)
   uint64_t host_stats[ST_COUNT] = {};
   std::vector<draw_events> pending_prof;  // recorded while a pipeline_profiles query runs
   std::vector<hipEvent_t> event_pool;
@@ -361,10 +362,11 @@ void snapshot_stats(renderer_obj* r, uint64_t* out) {
   for (int i = 0; i < ST_COUNT; ++i) out[i] = r->host_stats[i] + dev[i];
 }
 
-int32_t read_counter(renderer_obj* r, uint32_t slot, uint32_t* out) {
-  HIPCHK(hipMemcpyAsync(r->pinned, r->counters.p + slot, sizeof(uint32_t), hipMemcpyDeviceToHost, r->stream));
+// Reads the draw's device counters (waits for the queued work).
+int32_t read_counters(renderer_obj* r, uint32_t* out) {
+  HIPCHK(hipMemcpyAsync(r->pinned, r->counters.p, CT_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, r->stream));
   HIPCHK(hipStreamSynchronize(r->stream));
-  *out = r->pinned[0];
+  for (int i = 0; i < CT_COUNT; ++i) out[i] = r->pinned[i];
   return SRX_OK;
 }
 
@@ -569,43 +571,48 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   mark(0);
   launch_vs(P, B.vbuf, s);
   mark(1);
-  launch_setup(P, B, s);
-  uint32_t nclip = 0;
-  RCHK(read_counter(r, CT_CLIP_PRIMS, &nclip));
-  if (nclip) {
-    RCHK(ensure(r, r->clip_verts, static_cast<size_t>(nclip) * 5 * (nout + 1)));
-    RCHK(ensure(r, r->clip_hdr, nclip));
-    B.clip_verts = r->clip_verts.p;
-    B.clip_hdr = r->clip_hdr.p;
-    B.clip_capacity = nclip;
-    launch_clip(P, B, s);
-  }
+  // clipping capacity for every primitive, so no read-back decides whether clipping runs
+  RCHK(ensure(r, r->clip_verts, static_cast<size_t>(primcnt) * 5 * (nout + 1)));
+  RCHK(ensure(r, r->clip_hdr, primcnt));
   B.clip_verts = r->clip_verts.p;
   B.clip_hdr = r->clip_hdr.p;
+  B.clip_capacity = primcnt;
+  launch_setup(P, B, s);
+  launch_clip(P, B, s);  // returns at once on the device when setup flagged no primitive
   mark(2);
   launch_exclusive_scan(B.cnt, P3, B.scan_tmp, B.counters + CT_TOTAL, s);
   mark(3);
-  uint32_t total = 0;
-  RCHK(read_counter(r, CT_TOTAL, &total));
-  if (nclip) {
-    uint32_t ovf = 0;
-    RCHK(read_counter(r, CT_CLIP_OVERFLOW, &ovf));
-    if (ovf) return fail(SRX_OUT_OF_MEMORY, "clip buffer overflow");
+  // Records are emitted into a grow-only buffer before their count is read back, so the read-back
+  // overlaps the emission; a buffer found too small is grown and the emission repeated.
+  const uint32_t guess = std::max<uint32_t>(std::max<uint32_t>(r->last_records + r->last_records / 4, P3 + T), 1u << 16);
+  if (r->keys0.cap < guess) {
+    RCHK(ensure(r, r->keys0, guess));
+    RCHK(ensure(r, r->vals0, guess));
+  }
+  B.keys[0] = r->keys0.p;
+  B.vals[0] = r->vals0.p;
+  launch_emit(P, B, static_cast<uint32_t>(std::min<size_t>(r->keys0.cap, r->vals0.cap)), s);
+  uint32_t ctr[CT_COUNT];
+  RCHK(read_counters(r, ctr));
+  if (ctr[CT_CLIP_OVERFLOW]) return fail(SRX_OUT_OF_MEMORY, "clip buffer overflow");
+  const uint32_t total = ctr[CT_TOTAL];
+  r->last_records = total;
+  if (total > std::min<size_t>(r->keys0.cap, r->vals0.cap)) {
+    RCHK(ensure(r, r->keys0, total));
+    RCHK(ensure(r, r->vals0, total));
+    B.keys[0] = r->keys0.p;
+    B.vals[0] = r->vals0.p;
+    launch_emit(P, B, total, s);
   }
   uint32_t nblocks = (std::max(total, T) + 4095) / 4096;
-  RCHK(ensure(r, r->keys0, std::max<uint32_t>(total, 1)));
   RCHK(ensure(r, r->keys1, std::max<uint32_t>(total, 1)));
-  RCHK(ensure(r, r->vals0, std::max<uint32_t>(total, 1)));
   RCHK(ensure(r, r->vals1, std::max<uint32_t>(total, 1)));
   RCHK(ensure(r, r->hist, static_cast<size_t>(256) * std::max<uint32_t>(nblocks, 1)));
   RCHK(ensure(r, r->scan_tmp, std::max(scan_tmp_size(P3), scan_tmp_size(256 * std::max<uint32_t>(nblocks, 1)))));
-  B.keys[0] = r->keys0.p;
   B.keys[1] = r->keys1.p;
-  B.vals[0] = r->vals0.p;
   B.vals[1] = r->vals1.p;
   B.hist = r->hist.p;
   B.scan_tmp = r->scan_tmp.p;
-  launch_emit(P, B, s);
   int bits = 1;
   while ((1u << bits) < T) ++bits;
   int which = launch_sort(B, total, bits, s);
