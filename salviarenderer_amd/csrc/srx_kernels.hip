@@ -1401,16 +1401,25 @@ __global__ void __launch_bounds__(SRX_VIS_THREADS, 4) vis_kernel(draw_params P, 
                                                    const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ tend,
                                                    const float4* __restrict__ tri_rec, const float4* __restrict__ tri_attr,
                                                    const ps_samplers* __restrict__ SM) {
+  constexpr int NT = SRX_VIS_THREADS;
+  // LDS shared by phase 1 (a chunk of NT records + their block work list) and phase 3 (shading items)
+  constexpr int PH1_BYTES = NT * 64 + NT * 16 * 2;
+  constexpr int PH3_BYTES = SHADE ? 256 * S * 8 : 0;
+  constexpr int UNION_BYTES = PH1_BYTES > PH3_BYTES ? PH1_BYTES : PH3_BYTES;
   __shared__ unsigned long long keys[256 * S];
   __shared__ float d0s[256 * S];
-  __shared__ uint2 items[SHADE ? 256 * S : 1];
+  __shared__ __align__(16) unsigned char lds_union[UNION_BYTES];
+  __shared__ uint32_t wsum[NT / 64];
   __shared__ uint32_t nitems;
+  float4* recs = reinterpret_cast<float4*>(lds_union);                           // [NT][4]
+  uint16_t* work = reinterpret_cast<uint16_t*>(lds_union + NT * 64);             // [NT * 16]
+  uint2* items = reinterpret_cast<uint2*>(lds_union);                            // [256 * S]
   const int gtile = order[blockIdx.x];
   const int gx = gtile % P.gtiles_x, gy = gtile / P.gtiles_x;
   const int ox = gx * 16, oy = gy * 16;
   const int t64x = ox & ~(TILE - 1), t64y = oy & ~(TILE - 1);
   const int tw = static_cast<int>(P.target_w), th = static_cast<int>(P.target_h);
-  const int tid = threadIdx.x, NT = blockDim.x;
+  const int tid = threadIdx.x;
   const bool has_ds = P.ds.data != nullptr;
   if (tid == 0) nitems = 0;
   for (int i = tid; i < 256 * S; i += NT) {
@@ -1420,98 +1429,138 @@ __global__ void __launch_bounds__(SRX_VIS_THREADS, 4) vis_kernel(draw_params P, 
     d0s[i] = d0;
     keys[i] = initial_key(mode, d0);
   }
-  __syncthreads();
   const uint32_t t0 = tstart[gtile], t1 = tend[gtile];
   const uint32_t full_px = (S == 1) ? 1u : (S == 2 ? 3u : 15u);
   const int jx = (ox - t64x) >> 4, jy = (oy - t64y) >> 4;
-  for (uint32_t base = t0 + tid; base < t1; base += NT) {
-    const uint32_t rec = vals[base];
-    const uint32_t id = rec >> 1;
-    const float4* tr = tri_rec + tri_slot(P, id) * 5;
-    float e[9], bb[4];
-    read_tri_rec(tr, e, bb);
-    const float4 d3 = tr[3], d4 = tr[4];
-    const float zx = d3.y, zy = d3.z, z0 = d3.w, dzx = d4.x, dzy = d4.y;
-    float aa[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) aa[s] = (S > 1) ? (P.sp[s][0] - 0.5f) * dzx + (P.sp[s][1] - 0.5f) * dzy : 0.0f;
-    const float tvx = static_cast<float>(t64x), tvy = static_cast<float>(t64y);
-    const float bx0 = bb[0] - tvx, bx1 = bb[1] - tvx, by0 = bb[2] - tvy, by1 = bb[3] - tvy;
-    // per-stage constants (rasterize_triangle, rasterizer.cpp:662-675, 704-710)
-    float s1x[3], s1y[3], ra1[3], ev1[3], s2x[3], s2y[3], ra2[3], ev2[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      float ex = e[k * 3], ey = e[k * 3 + 1], ez = e[k * 3 + 2];
-      bool mx = ex > 0, my = ey > 0;
-      float sx = TILE * ex, sy = TILE * ey;
-      float ra = -fabsf(sx) - fabsf(sy);
-      float part = static_cast<float>(mx * TILE) * ex + static_cast<float>(my * TILE) * ey;
-      sx *= 0.25f; sy *= 0.25f; ra *= 0.25f; part *= 0.25f;
-      s1x[k] = sx; s1y[k] = sy; ra1[k] = ra;
-      ev1[k] = (ez - part) - ((tvx * ex) + (tvy * ey));
-      sx *= 0.25f; sy *= 0.25f; ra *= 0.25f; part *= 0.25f;
-      s2x[k] = sx; s2y[k] = sy; ra2[k] = ra;
-      ev2[k] = ez - part;
-    }
-    bool region_full = (rec & 1) != 0;
-    if (!region_full) {
-      // stage 16 (subdivide_tile on the 64x64 tile)
-      bool rej = false, nacc = false;
+  const float tvx = static_cast<float>(t64x), tvy = static_cast<float>(t64y);
+  for (uint32_t c0 = t0; c0 < t1; c0 += NT) {
+    __syncthreads();  // the previous chunk's work list is consumed (and the keys are initialised)
+    // phase 1a: one record per lane -> the 4x4 blocks of this GPU tile it reaches (full or partial)
+    uint32_t bm = 0, fm = 0;
+    const uint32_t ri = c0 + tid;
+    if (ri < t1) {
+      const uint32_t rec = vals[ri];
+      const uint32_t id = rec >> 1;
+      const float4* tr = tri_rec + tri_slot(P, id) * 5;
+      float e[9], bb[4];
+      read_tri_rec(tr, e, bb);
+      const float4 d3 = tr[3], d4 = tr[4];
+      const float bx0 = bb[0] - tvx, bx1 = bb[1] - tvx, by0 = bb[2] - tvy, by1 = bb[3] - tvy;
+      // per-stage constants (rasterize_triangle, rasterizer.cpp:662-675, 704-710)
+      float s1x[3], s1y[3], ra1[3], ev1[3], s2x[3], s2y[3], ra2[3], ev2[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        float r = (s1x[k] * static_cast<float>(jx)) + (s1y[k] * static_cast<float>(jy));
-        float a = r + ra1[k];
-        rej |= r < ev1[k];
-        nacc |= a < ev1[k];
+        float ex = e[k * 3], ey = e[k * 3 + 1], ez = e[k * 3 + 2];
+        bool mx = ex > 0, my = ey > 0;
+        float sx = TILE * ex, sy = TILE * ey;
+        float ra = -fabsf(sx) - fabsf(sy);
+        float part = static_cast<float>(mx * TILE) * ex + static_cast<float>(my * TILE) * ey;
+        sx *= 0.25f; sy *= 0.25f; ra *= 0.25f; part *= 0.25f;
+        s1x[k] = sx; s1y[k] = sy; ra1[k] = ra;
+        ev1[k] = (ez - part) - ((tvx * ex) + (tvy * ey));
+        sx *= 0.25f; sy *= 0.25f; ra *= 0.25f; part *= 0.25f;
+        s2x[k] = sx; s2y[k] = sy; ra2[k] = ra;
+        ev2[k] = ez - part;
       }
-      rej |= bx0 >= static_cast<float>(jx * 16 + 16);
-      rej |= bx1 < static_cast<float>(jx * 16);
-      rej |= by0 >= static_cast<float>(jy * 16 + 16);
-      rej |= by1 < static_cast<float>(jy * 16);
-      if (rej) continue;
-      region_full = !nacc;
-    }
-    float ev2q[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-      ev2q[k] = ev2[k] - ((static_cast<float>(ox) * e[k * 3]) + (static_cast<float>(oy) * e[k * 3 + 1]));
-    // candidate 4x4 blocks: all of them for an accepted region, else those the stage-4 bounding-box
-    // test keeps (bx0 < mxr + 4 and bx1 >= mxr, the same comparisons on the same floats)
-    uint32_t cols = 0, rows = 0;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int mxr = jx * 16 + c * 4, myr = jy * 16 + c * 4;
-      if (region_full || !(bx0 >= static_cast<float>(mxr + 4) || bx1 < static_cast<float>(mxr))) cols |= 0x1111u << c;
-      if (region_full || !(by0 >= static_cast<float>(myr + 4) || by1 < static_cast<float>(myr))) rows |= 0xFu << (c * 4);
-    }
-    uint32_t blocks = cols & rows;
-    while (blocks) {
-      const int b = __builtin_ctz(blocks);
-      blocks &= blocks - 1;
-      const int ix4 = b & 3, iy4 = b >> 2;
-      const int Bx = ox + ix4 * 4, By = oy + iy4 * 4;
-      if (Bx >= tw || By >= th) continue;
-      bool block_full = region_full;
-      if (!block_full) {
-        // stage 4 (subdivide_tile on the 16x16 region); the bounding-box half was applied above
+      bool region_full = (rec & 1) != 0;
+      bool keep = true;
+      if (!region_full) {
+        // stage 16 (subdivide_tile on the 64x64 tile)
         bool rej = false, nacc = false;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          float r = (s2x[k] * static_cast<float>(ix4)) + (s2y[k] * static_cast<float>(iy4));
-          float a = r + ra2[k];
-          rej |= r < ev2q[k];
-          nacc |= a < ev2q[k];
+          float r = (s1x[k] * static_cast<float>(jx)) + (s1y[k] * static_cast<float>(jy));
+          float a = r + ra1[k];
+          rej |= r < ev1[k];
+          nacc |= a < ev1[k];
         }
-        if (rej) continue;
-        block_full = !nacc;
+        rej |= bx0 >= static_cast<float>(jx * 16 + 16);
+        rej |= bx1 < static_cast<float>(jx * 16);
+        rej |= by0 >= static_cast<float>(jy * 16 + 16);
+        rej |= by1 < static_cast<float>(jy * 16);
+        keep = !rej;
+        region_full = !nacc;
       }
-      float evp[3];
-      if (!block_full) {
-        // pixel level (draw_partial_tile on the 4x4 block)
+      if (keep) {
+        float ev2q[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k)
-          evp[k] = e[k * 3 + 2] - ((static_cast<float>(Bx) * e[k * 3]) + (static_cast<float>(By) * e[k * 3 + 1]));
+          ev2q[k] = ev2[k] - ((static_cast<float>(ox) * e[k * 3]) + (static_cast<float>(oy) * e[k * 3 + 1]));
+        // candidate 4x4 blocks: all of them for an accepted region, else those the stage-4
+        // bounding-box test keeps (bx0 < mxr + 4 and bx1 >= mxr, the same comparisons on the same floats)
+        uint32_t cols = 0, rows = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int mxr = jx * 16 + c * 4, myr = jy * 16 + c * 4;
+          if (region_full || !(bx0 >= static_cast<float>(mxr + 4) || bx1 < static_cast<float>(mxr))) cols |= 0x1111u << c;
+          if (region_full || !(by0 >= static_cast<float>(myr + 4) || by1 < static_cast<float>(myr))) rows |= 0xFu << (c * 4);
+          if (ox + c * 4 >= tw) cols &= ~(0x1111u << c);  // blocks outside the target
+          if (oy + c * 4 >= th) rows &= ~(0xFu << (c * 4));
+        }
+        uint32_t blocks = cols & rows;
+        while (blocks) {
+          const int b = __builtin_ctz(blocks);
+          blocks &= blocks - 1;
+          bool block_full = region_full;
+          if (!block_full) {
+            // stage 4 (subdivide_tile on the 16x16 region); the bounding-box half was applied above
+            const int ix4 = b & 3, iy4 = b >> 2;
+            bool rej = false, nacc = false;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              float r = (s2x[k] * static_cast<float>(ix4)) + (s2y[k] * static_cast<float>(iy4));
+              float a = r + ra2[k];
+              rej |= r < ev2q[k];
+              nacc |= a < ev2q[k];
+            }
+            if (rej) continue;
+            block_full = !nacc;
+          }
+          bm |= 1u << b;
+          if (block_full) fm |= 1u << b;
+        }
       }
+      float4* rs = recs + tid * 4;
+      rs[0] = make_float4(e[0], e[1], e[2], e[3]);
+      rs[1] = make_float4(e[4], e[5], e[6], e[7]);
+      rs[2] = make_float4(e[8], d3.w, d3.y, d3.z);          // z0, zx, zy
+      rs[3] = make_float4(d4.x, d4.y, __uint_as_float(id), 0.0f);  // dzx, dzy, id
+    }
+    // phase 1b: work list of (record, block) pairs, spread evenly over the lanes
+    const uint32_t cnt = __popc(bm);
+    const uint32_t incl = wave_incl_scan(cnt);
+    if ((tid & 63) == 63) wsum[tid >> 6] = incl;
+    __syncthreads();
+    uint32_t off = incl - cnt, total = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) {
+      if (w < (tid >> 6)) off += wsum[w];
+      total += wsum[w];
+    }
+    while (bm) {
+      const uint32_t b = __builtin_ctz(bm);
+      bm &= bm - 1;
+      work[off++] = static_cast<uint16_t>((tid << 5) | (b << 1) | ((fm >> b) & 1u));
+    }
+    __syncthreads();
+    // phase 1c: one 4x4 block per lane: per-sample coverage (draw_partial_tile, rasterizer.cpp:298-439),
+    // quad depth (step_2d_unproj_pos_quad) and the key atomic-min
+    for (uint32_t wi = tid; wi < total; wi += NT) {
+      const uint32_t wk = work[wi];
+      const int j = static_cast<int>(wk >> 5), b = static_cast<int>((wk >> 1) & 15u);
+      const bool block_full = (wk & 1u) != 0;
+      const float4 r0 = recs[j * 4 + 0], r1 = recs[j * 4 + 1], r2 = recs[j * 4 + 2], r3 = recs[j * 4 + 3];
+      const float e[9] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x};
+      const float z0 = r2.y, zx = r2.z, zy = r2.w, dzx = r3.x, dzy = r3.y;
+      const uint32_t id = __float_as_uint(r3.z);
+      float aa[S];
+#pragma unroll
+      for (int s = 0; s < S; ++s) aa[s] = (S > 1) ? (P.sp[s][0] - 0.5f) * dzx + (P.sp[s][1] - 0.5f) * dzy : 0.0f;
+      const int Bx = ox + (b & 3) * 4, By = oy + (b >> 2) * 4;
+      float evp[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        evp[k] = e[k * 3 + 2] - ((static_cast<float>(Bx) * e[k * 3]) + (static_cast<float>(By) * e[k * 3 + 1]));
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int qx0 = Bx + (q & 1) * 2, qy0 = By + (q >> 1) * 2;
