@@ -36,6 +36,10 @@ struct draw_buffers {
   uint32_t* keys[2];   // records: GPU tile id
   uint32_t* vals[2];   // records: triangle id << 1 | tile-level accept
   const uint32_t* order;  // GPU tiles, most records first (workgroup launch order)
+  uint4* vis_entries;     // visibility work table (vis_table_bounds)
+  uint4* vis_merges;
+  uint32_t* vis_counts;   // entries, merges, partial slots
+  unsigned long long* vis_partial;  // partial per-sample keys of split tiles
   uint32_t* tstart;    // GPU tiles
   uint32_t* tend;
   uint32_t* hist;      // radix histogram 256 * blocks
@@ -66,7 +70,10 @@ void launch_raster(const draw_params& P, const draw_buffers& B, int which, hipSt
 // Order-independent visibility for opaque early-Z draws without statistics; false if not applicable.
 // Order-independent visibility (and, with shade, the deferred pixel shader) of an opaque early-Z
 // draw; false when the draw needs the in-order kernel.
-bool launch_vis(const draw_params& P, const draw_buffers& B, int which, bool shade, hipStream_t s);
+bool launch_vis(const draw_params& P, const draw_buffers& B, int which, bool shade, uint32_t records, hipStream_t s);
+// Upper bounds of the visibility work table for T GPU tiles and `records` binned records: entries
+// (first launch), merges (second launch) and partial-key slots (256 * samples keys each).
+void vis_table_bounds(uint32_t T, uint32_t records, uint32_t* entries, uint32_t* merges, uint32_t* slots);
 bool needs_shading(const draw_params& P);
 void launch_shade(const draw_params& P, const draw_buffers& B, hipStream_t s);
 

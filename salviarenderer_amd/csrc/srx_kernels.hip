@@ -786,6 +786,62 @@ void launch_tile_order(const draw_params& P, const draw_buffers& B, uint32_t* or
   if (T) hipLaunchKernelGGL(tile_order_kernel, dim3(1), dim3(1024), 0, s, B.tstart, B.tend, T, order);
 }
 
+// records per workgroup beyond which a tile of a depth-only draw is split
+constexpr uint32_t VIS_PART = 256;
+
+// Work table of the order-independent visibility kernel for draws without shading (depth-only
+// passes), whose time is the longest tile's: a GPU tile with more than VIS_PART records is split
+// into parts of at most VIS_PART records, each a workgroup of the first launch that leaves its
+// partial per-sample keys in `partial`; the second launch merges a split tile's parts (a minimum,
+// so the order of the parts is immaterial) and finishes it.  (Draws that shade keep whole tiles:
+// their shading fills the GPU while the long tiles run.)  Entries follow `order`, heaviest first.
+//   entries[i] = (tile, part, parts, first partial slot)   i < counts[0]
+//   merges[i]  = (tile, parts, first partial slot, 0)      i < counts[1]
+__global__ void __launch_bounds__(1024) vis_table_kernel(const uint32_t* __restrict__ ts, const uint32_t* __restrict__ te,
+                                                         const uint32_t* __restrict__ order, uint32_t T,
+                                                         uint4* __restrict__ entries, uint4* __restrict__ merges,
+                                                         uint32_t* __restrict__ counts) {
+  __shared__ uint32_t wtot[3][16];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t per = (T + 1023) / 1024, p0 = tid * per, p1 = min(T, p0 + per);
+  uint32_t c[3] = {0, 0, 0};  // blocks, partial slots, merges in this thread's segment of `order`
+  for (uint32_t p = p0; p < p1; ++p) {
+    const uint32_t t = order[p], n = te[t] - ts[t];
+    const uint32_t parts = n > VIS_PART ? (n + VIS_PART - 1) / VIS_PART : 1u;
+    c[0] += parts;
+    if (parts > 1) { c[1] += parts; c[2] += 1; }
+  }
+  uint32_t incl[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    incl[k] = wave_incl_scan(c[k]);
+    if (lane == 63) wtot[k][wave] = incl[k];
+  }
+  __syncthreads();
+  uint32_t base[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    uint32_t before = 0, total = 0;
+    for (uint32_t w = 0; w < 16; ++w) {
+      if (w < wave) before += wtot[k][w];
+      total += wtot[k][w];
+    }
+    base[k] = before + incl[k] - c[k];
+    if (tid == 0) counts[k == 0 ? 0 : (k == 1 ? 2 : 1)] = total;
+  }
+  uint32_t b = base[0], sl = base[1], m = base[2];
+  for (uint32_t p = p0; p < p1; ++p) {
+    const uint32_t t = order[p], n = te[t] - ts[t];
+    const uint32_t parts = n > VIS_PART ? (n + VIS_PART - 1) / VIS_PART : 1u;
+    for (uint32_t k = 0; k < parts; ++k) entries[b + k] = make_uint4(t, k, parts, sl);
+    b += parts;
+    if (parts > 1) {
+      merges[m++] = make_uint4(t, parts, sl, 0u);
+      sl += parts;
+    }
+  }
+}
+
 void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s) {
   if (n) hipLaunchKernelGGL(fill_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
 }
@@ -1396,7 +1452,11 @@ __device__ __forceinline__ void write_winner_depth(const draw_params& P, int mod
 #define SRX_VIS_THREADS 128
 #endif
 template <int NA, int S, bool SHADE>
-__global__ void __launch_bounds__(SRX_VIS_THREADS, 4) vis_kernel(draw_params P, int mode, const uint32_t* __restrict__ order,
+__global__ void __launch_bounds__(SRX_VIS_THREADS, 4) vis_kernel(draw_params P, int mode, int pass,
+                                                   const uint32_t* __restrict__ order,
+                                                   const uint4* __restrict__ entries, const uint4* __restrict__ merges,
+                                                   const uint32_t* __restrict__ counts,
+                                                   unsigned long long* __restrict__ partial,
                                                    const uint32_t* __restrict__ vals,
                                                    const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ tend,
                                                    const float4* __restrict__ tri_rec, const float4* __restrict__ tri_attr,
@@ -1414,7 +1474,25 @@ __global__ void __launch_bounds__(SRX_VIS_THREADS, 4) vis_kernel(draw_params P, 
   float4* recs = reinterpret_cast<float4*>(lds_union);                           // [NT][4]
   uint16_t* work = reinterpret_cast<uint16_t*>(lds_union + NT * 64);             // [NT * 16]
   uint2* items = reinterpret_cast<uint2*>(lds_union);                            // [256 * S]
-  const int gtile = order[blockIdx.x];
+  // pass 0: one tile, or one part of a split tile; pass 1: merge and finish a split tile
+  int gtile;
+  uint32_t part = 0, parts = 1, slot = 0;
+  if (entries == nullptr) {  // whole tiles in `order`
+    gtile = static_cast<int>(order[blockIdx.x]);
+  } else if (pass == 0) {
+    if (blockIdx.x >= counts[0]) return;
+    const uint4 en = entries[blockIdx.x];
+    gtile = static_cast<int>(en.x);
+    part = en.y;
+    parts = en.z;
+    slot = en.w + en.y;
+  } else {
+    if (blockIdx.x >= counts[1]) return;
+    const uint4 mg = merges[blockIdx.x];
+    gtile = static_cast<int>(mg.x);
+    parts = mg.y;
+    slot = mg.z;
+  }
   const int gx = gtile % P.gtiles_x, gy = gtile / P.gtiles_x;
   const int ox = gx * 16, oy = gy * 16;
   const int t64x = ox & ~(TILE - 1), t64y = oy & ~(TILE - 1);
@@ -1422,14 +1500,30 @@ __global__ void __launch_bounds__(SRX_VIS_THREADS, 4) vis_kernel(draw_params P, 
   const int tid = threadIdx.x;
   const bool has_ds = P.ds.data != nullptr;
   if (tid == 0) nitems = 0;
-  for (int i = tid; i < 256 * S; i += NT) {
-    int px = ox + ((i / S) & 15), py = oy + ((i / S) >> 4), smp = i % S;
-    float d0 = 0.0f;
-    if (has_ds && px < tw && py < th) d0 = *reinterpret_cast<const float*>(sample_addr(P.ds, px, py, smp));
-    d0s[i] = d0;
-    keys[i] = initial_key(mode, d0);
+  if (pass == 1) {
+    // the parts' keys, each started from the target's depth: their minimum is the tile's result
+    const unsigned long long* src = partial + static_cast<size_t>(slot) * (256 * S);
+    for (int i = tid; i < 256 * S; i += NT) {
+      unsigned long long k = src[i];
+      for (uint32_t q = 1; q < parts; ++q) {
+        const unsigned long long kq = src[static_cast<size_t>(q) * (256 * S) + i];
+        k = kq < k ? kq : k;
+      }
+      keys[i] = k;
+    }
+  } else {
+    for (int i = tid; i < 256 * S; i += NT) {
+      int px = ox + ((i / S) & 15), py = oy + ((i / S) >> 4), smp = i % S;
+      float d0 = 0.0f;
+      if (has_ds && px < tw && py < th) d0 = *reinterpret_cast<const float*>(sample_addr(P.ds, px, py, smp));
+      d0s[i] = d0;
+      keys[i] = initial_key(mode, d0);
+    }
   }
-  const uint32_t t0 = tstart[gtile], t1 = tend[gtile];
+  // this workgroup's records: the whole tile, or one part of it (none when merging)
+  const uint32_t tr0 = tstart[gtile], tr1 = tend[gtile];
+  const uint32_t t0 = pass == 1 ? tr1 : min(tr1, tr0 + part * VIS_PART);
+  const uint32_t t1 = (pass == 1 || entries == nullptr) ? tr1 : min(tr1, t0 + VIS_PART);
   const uint32_t full_px = (S == 1) ? 1u : (S == 2 ? 3u : 15u);
   const int jx = (ox - t64x) >> 4, jy = (oy - t64y) >> 4;
   const float tvx = static_cast<float>(t64x), tvy = static_cast<float>(t64y);
@@ -1601,6 +1695,11 @@ __global__ void __launch_bounds__(SRX_VIS_THREADS, 4) vis_kernel(draw_params P, 
     }
   }
   __syncthreads();
+  if (pass == 0 && parts > 1) {  // one part of a split tile: leave the partial keys for the merge
+    unsigned long long* dst = partial + static_cast<size_t>(slot) * (256 * S);
+    for (int i = tid; i < 256 * S; i += NT) dst[i] = keys[i];
+    return;
+  }
   // phase 2: winners -> depth, shading items
   for (int p = tid; p < 256; p += NT) {
     const int px = ox + (p & 15), py = oy + (p >> 4);
@@ -1790,14 +1889,30 @@ int vis_mode(const draw_params& P) {
   }
 }
 
-bool launch_vis(const draw_params& P, const draw_buffers& B, int which, bool shade, hipStream_t s) {
+void vis_table_bounds(uint32_t T, uint32_t records, uint32_t* entries, uint32_t* merges, uint32_t* slots) {
+  const uint32_t chunks = (records + VIS_PART - 1) / VIS_PART;
+  *entries = T + chunks;            // sum over tiles of max(1, ceil(n / VIS_PART))
+  *merges = records / (VIS_PART + 1) + 1;  // tiles with more than VIS_PART records
+  *slots = 2 * chunks + 1;          // parts of split tiles: ceil(n / VIS_PART) <= 2n / VIS_PART
+}
+
+bool launch_vis(const draw_params& P, const draw_buffers& B, int which, bool shade, uint32_t records, hipStream_t s) {
   int mode = vis_mode(P);
   uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
   if (mode < 0 || P.gt != 16) return false;
   if (!T) return true;
-#define SRX_VIS_LAUNCH(NA, S, SH)                                                                                    \
-  hipLaunchKernelGGL((vis_kernel<NA, S, SH>), dim3(T), dim3(SRX_VIS_THREADS), 0, s, P, mode, B.order, B.vals[which], \
-                     B.tstart, B.tend, B.tri_rec, B.tri_attr, B.samplers)
+  uint32_t ne = T, nm = 0, nsl;
+  if (!shade) {
+    vis_table_bounds(T, records, &ne, &nm, &nsl);
+    hipLaunchKernelGGL(vis_table_kernel, dim3(1), dim3(1024), 0, s, B.tstart, B.tend, B.order, T, B.vis_entries,
+                       B.vis_merges, B.vis_counts);
+  }
+  const uint4* entries = shade ? nullptr : B.vis_entries;
+#define SRX_VIS_LAUNCH(NA, S, SH)                                                                                 \
+  for (int pass = 0; pass < (SH ? 1 : 2); ++pass)                                                                 \
+    hipLaunchKernelGGL((vis_kernel<NA, S, SH>), dim3(pass == 0 ? ne : nm), dim3(SRX_VIS_THREADS), 0, s, P, mode, pass, \
+                       B.order, entries, B.vis_merges, B.vis_counts, B.vis_partial, B.vals[which], B.tstart,      \
+                       B.tend, B.tri_rec, B.tri_attr, B.samplers)
 #define SRX_VIS_SHADE(NA)                   \
   switch (P.samples) {                      \
   case 1: SRX_VIS_LAUNCH(NA, 1, true); break; \

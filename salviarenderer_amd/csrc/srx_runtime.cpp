@@ -282,6 +282,9 @@ struct renderer_obj : object {
   ps_samplers* samplers_h = nullptr;  // pinned staging of the per-draw sampler table
   dev_array<uint32_t> cnt, keys0, keys1, vals0, vals1, tstart, tend, hist, scan_tmp, clip_hdr, counters, vis;
   dev_array<uint32_t> order;  // GPU tile scheduling order
+  dev_array<uint4> vis_entries, vis_merges;  // visibility work table
+  dev_array<uint32_t> vis_counts;
+  dev_array<unsigned long long> vis_partial;
   dev_array<uint8_t> clip_flag;
   dev_array<int32_t> clip_slot;
   dev_array<unsigned long long> partial;
@@ -317,7 +320,7 @@ struct renderer_obj : object {
     };
     fr(vbuf); fr(tri_rec); fr(tri_attr); fr(clip_verts); fr(samplers_d); fr(vis);
     if (samplers_h) (void)hipHostFree(samplers_h); fr(cnt); fr(keys0); fr(keys1); fr(vals0); fr(vals1);
-    fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot); fr(partial); fr(order);
+    fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot); fr(partial); fr(order); fr(vis_entries); fr(vis_merges); fr(vis_counts); fr(vis_partial);
     if (stats_d) (void)hipFree(stats_d);
     if (pinned) (void)hipHostFree(pinned);
     for (auto& d : pending_prof)
@@ -625,7 +628,19 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   bool counting = false;
   for (auto& q : r->active)
     if (q->id != SRX_QUERY_PIPELINE_PROFILES) counting = true;
-  const bool fused = vis_pass && !counting && launch_vis(P, B, which, needs_shading(P), s);
+  if (vis_pass && !counting) {
+    uint32_t ne, nm, nsl;
+    vis_table_bounds(T, total, &ne, &nm, &nsl);
+    RCHK(ensure(r, r->vis_entries, ne));
+    RCHK(ensure(r, r->vis_merges, nm));
+    RCHK(ensure(r, r->vis_counts, 4));
+    RCHK(ensure(r, r->vis_partial, static_cast<size_t>(nsl) * 256 * samples));
+    B.vis_entries = r->vis_entries.p;
+    B.vis_merges = r->vis_merges.p;
+    B.vis_counts = r->vis_counts.p;
+    B.vis_partial = r->vis_partial.p;
+  }
+  const bool fused = vis_pass && !counting && launch_vis(P, B, which, needs_shading(P), total, s);
   if (!fused) launch_raster(P, B, which, s);
   mark(5);
   if (!fused && vis_pass && needs_shading(P)) launch_shade(P, B, s);

@@ -192,6 +192,20 @@ def _blend_cases():
     p2, c2 = tris(150, seed=14)
     out.append(Case("color_disabled_then_lequal", [Draw(p, c, blend=A.BLEND_DISABLED),
                                                    Draw(p2, c2, depth_func=A.CMP_LESS_EQUAL)]))
+    # depth-only pass over many small triangles crowded into a few tiles (more than 256 records per
+    # 16x16 tile, so the visibility work of a tile is split and merged), ties included
+    rng = np.random.default_rng(16)
+    n = 2400
+    ctr = rng.uniform(-0.45, -0.05, (n, 1, 2))
+    xy = ctr + rng.uniform(-0.06, 0.06, (n, 3, 2))
+    zz = np.repeat(rng.choice([0.3, 0.45, 0.6], (n, 1)), 3, 1)
+    pd = np.concatenate([xy, zz[..., None], np.ones((n, 3, 1))], -1).reshape(-1, 4).astype(np.float32)
+    cd = rng.uniform(0, 1, (n * 3, 4)).astype(np.float32)
+    p2, c2 = full_screen(0.45, (0.9, 0.2, 0.4, 1.0))
+    for f, nm in ((A.CMP_LESS, "less"), (A.CMP_LESS_EQUAL, "lequal"), (A.CMP_GREATER, "greater")):
+        out.append(Case(f"depth_only_crowded_{nm}", [Draw(pd, cd, blend=A.BLEND_DISABLED, depth_func=f),
+                                                     Draw(p2, c2, depth_func=A.CMP_LESS_EQUAL)],
+                        clear_depth=0.5 if f == A.CMP_GREATER else 1.0))
     p, c = tris(150, seed=15, alpha=(0.0, 1.0))
     out.append(Case("blend_rgba32f", [Draw(p, c, blend=A.BLEND_ON, depth_enable=False)], fmt=A.PF_RGBA32F))
     return out
