@@ -32,6 +32,9 @@ struct sampler_desc_dev {
   float border[4];
   int32_t nlevels, max_lod, min_lod;
   uint32_t size[3];
+  // 1: anisotropic mip filter (max_aniso > 1), linear min/mag, wrap/wrap, every level single-sample
+  // RGBA8 -- the sampler's common case, which has a compact inline path
+  int32_t fast_aniso;
   surface_desc levels[MAX_LEVELS];
 };
 

@@ -943,6 +943,64 @@ __device__ __forceinline__ void run_ps(const draw_params& P, const ps_samplers* 
   }
 }
 
+// Register r of quad pixel i (0..3) by selects (a dynamically indexed register array would go to
+// scratch memory).
+template <int NA>
+__device__ __forceinline__ float4 quad_reg(const float4 (&q)[4][NA + 1], int i, int r) {
+  return sel4(i == 0, q[0][r], sel4(i == 1, q[1][r], sel4(i == 2, q[2][r], q[3][r])));
+}
+
+// The pixel shader's colour for quad pixel pi only (the deferred shading paths need one pixel of
+// the quad; the other three supply derivatives).  Same arithmetic as run_ps.
+template <int NA>
+__device__ __forceinline__ float4 run_ps_pixel(const draw_params& P, const ps_samplers* __restrict__ SM,
+                                               const float4 (&q)[4][NA + 1], int pi) {
+  switch (P.ps_id) {
+  case PS_GOURAUD:
+    return quad_reg<NA>(q, pi, 1);
+  case PS_CONSTANT:
+    return make_float4(P.ps_consts[0], P.ps_consts[1], P.ps_consts[2], P.ps_consts[3]);
+  case PS_LAMBERT_TEX:
+    if constexpr (NA >= 2) {
+      // SASL tex2D: ddx per row (p1-p0, p3-p2), ddy per column (p2-p0, p3-p1)
+      // (sasl/src/codegen/cgs_simd.cpp:275-312), then sample_2d_grad (sampler_api.cpp:13-18)
+      const bool row1 = (pi & 2) != 0, col1 = (pi & 1) != 0;
+      const float4 r0 = sel4(row1, q[2][2], q[0][2]), r1 = sel4(row1, q[3][2], q[1][2]);
+      const float4 c0 = sel4(col1, q[1][2], q[0][2]), c1 = sel4(col1, q[3][2], q[2][2]);
+      const float dxu = r1.x - r0.x, dxv = r1.y - r0.y;
+      const float dyu = c1.x - c0.x, dyv = c1.y - c0.y;
+      const float4 uv = quad_reg<NA>(q, pi, 2), nrm = quad_reg<NA>(q, pi, 1);
+      float4 t = make_float4(0, 0, 0, 0);
+      if (P.has_sampler[0]) {
+        const sampler_desc_dev& sp = SM->s[0];
+        t = sp.fast_aniso ? sample_2d_grad_fast(sp, uv.x, uv.y, dxu, dxv, dyu, dyv, 0.0f)
+                          : sample_2d_grad(sp, uv.x, uv.y, dxu, dxv, dyu, dyv, 0.0f);
+      }
+      float nx, ny, nz;
+      normalize3(nrm.x, nrm.y, nrm.z, nx, ny, nz);
+      float d = nx * P.ps_consts[0] + ny * P.ps_consts[1] + nz * P.ps_consts[2];
+      float k = std_clamp(d, 0.0f, 1.0f);
+      return make_float4(t.x * k, t.y * k, t.z * k, 1.0f);
+    }
+    return make_float4(0, 0, 0, 0);
+  case PS_TEX_CPP: {
+    // cpp tex2d: one LOD per quad from p1 - p0 and p2 - p0 (cpp_pixel_shader.cpp:13-31)
+    if (!P.has_sampler[0]) return make_float4(0, 0, 0, 1.0f);
+    const sampler_desc_dev& sp = SM->s[0];
+    float dxu = q[1][1].x - q[0][1].x, dxv = q[1][1].y - q[0][1].y;
+    float dyu = q[2][1].x - q[0][1].x, dyv = q[2][1].y - q[0][1].y;
+    float lod = (sp.mip_filter == FILTER_ANISO && sp.max_aniso > 1) ? calc_aniso(sp, dxu, dxv, dyu, dyv, 0.0f).lod
+                                                                    : calc_lod(sp, dxu, dxv, dyu, dyv, 0.0f);
+    const float4 uv = quad_reg<NA>(q, pi, 1);
+    float4 t = sample_impl(sp, uv.x, uv.y, lod, nullptr);
+    t.w = 1.0f;
+    return t;
+  }
+  default:
+    return make_float4(0, 0, 0, 0);
+  }
+}
+
 // step_2d_unproj_pos_quad + step_2d_unproj_attr_n_quad (shader.cpp:257-367) for the quad whose
 // top-left pixel is (px0, py0)
 template <int NA>
@@ -1746,12 +1804,7 @@ __global__ void __launch_bounds__(SRX_VIS_THREADS, 4) vis_kernel(draw_params P, 
     load_attr<NA>(P, tri_attr, cur, at);
     float4 q[4][NA + 1];
     interp_quad<NA>(at, px & ~1, py & ~1, P, q);
-    float4 col[4];
-    run_ps<NA>(P, SM, q, 1u << pi, col);
-    float4 c = col[0];
-#pragma unroll
-    for (int k = 1; k < 4; ++k)
-      if (k == pi) c = col[k];
+    const float4 c = run_ps_pixel<NA>(P, SM, q, pi);
 #pragma unroll
     for (int smp = 0; smp < S; ++smp)
       if (m & (1u << smp)) store_texel(P.color.pf, sample_addr(P.color, px, py, smp), c);
@@ -1785,12 +1838,7 @@ __global__ void __launch_bounds__(256) shade_kernel(draw_params P, const ps_samp
     load_attr<NA>(P, tri_attr, cur, at);
     float4 q[4][NA + 1];
     interp_quad<NA>(at, px0, py0, P, q);
-    float4 col[4];
-    run_ps<NA>(P, SM, q, 1u << pi, col);
-    float4 c = col[0];
-#pragma unroll
-    for (int k = 1; k < 4; ++k)
-      if (k == pi) c = col[k];
+    const float4 c = run_ps_pixel<NA>(P, SM, q, pi);
 #pragma unroll
     for (int s = 0; s < S; ++s)
       if (w[s] == cur) {

@@ -513,6 +513,11 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
     sd.size[1] = t->size[1];
     sd.size[2] = t->size[2];
     for (int l = 0; l < sd.nlevels; ++l) sd.levels[l] = t->surfs[l]->desc();
+    bool rgba8 = sd.nlevels > 0;
+    for (int l = 0; l < sd.nlevels; ++l) rgba8 &= sd.levels[l].pf == SRX_PF_RGBA8 && sd.levels[l].samples == 1;
+    sd.fast_aniso = rgba8 && sd.mip_filter == SRX_FILTER_ANISOTROPIC && sd.max_aniso > 1 &&
+                    sd.min_filter == SRX_FILTER_LINEAR && sd.mag_filter == SRX_FILTER_LINEAR &&
+                    sd.addr_u == SRX_ADDRESS_WRAP && sd.addr_v == SRX_ADDRESS_WRAP;
     P.has_sampler[i] = 1;
   }
 

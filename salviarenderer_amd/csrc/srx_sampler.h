@@ -373,6 +373,38 @@ __device__ inline aniso_info calc_aniso(const sampler_desc_dev& p, float dxu, fl
 #ifndef SRX_ANISO_BATCH
 #define SRX_ANISO_BATCH 4
 #endif
+// The probes of the anisotropic filter on one bilinear/wrap RGBA8 level (sampler.cpp:730-760),
+// SRX_ANISO_BATCH per step: their texel loads are issued together (every address is clamped into
+// the level, so probes past the count load harmlessly and are masked out), then the probes are
+// accumulated in order, which keeps the reference's summation order.  (x, y) is the first probe.
+__device__ __forceinline__ float4 aniso_probes_rgba8(const surface_desc& lv, float x, float y, int n, const aniso_info* af) {
+  float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f, w_sum = 0.0f;
+  for (int k = 0; k < n; k += SRX_ANISO_BATCH) {
+    float4 c[SRX_ANISO_BATCH];
+#pragma unroll
+    for (int j = 0; j < SRX_ANISO_BATCH; ++j) {
+      c[j] = bilinear_wrap_rgba8(lv, x, y);
+      x += af->du;
+      y += af->dv;
+    }
+#pragma unroll
+    for (int j = 0; j < SRX_ANISO_BATCH; ++j) {
+      const int i = 2 * (k + j) - n + 1;
+      const bool live = k + j < n;
+      float wf = static_cast<float>(i * i) * af->weight_D;
+      int wi = (wf >= 0.0f) ? static_cast<int>(sse_min(wf, 255.0f)) : 0;
+      float w = c_ewa[live ? wi : 0];
+      cr = live ? cr + c[j].x * w : cr;
+      cg = live ? cg + c[j].y * w : cg;
+      cb = live ? cb + c[j].z * w : cb;
+      ca = live ? ca + c[j].w * w : ca;
+      w_sum = live ? w_sum + w : w_sum;
+    }
+  }
+  float inv = 1 / w_sum;
+  return make_float4(cr * inv, cg * inv, cb * inv, ca * inv);
+}
+
 // sampler.cpp:680-764
 __device__ __noinline__ float4 sample_impl(const sampler_desc_dev& p, float u, float v, float miplevel, const aniso_info* af) {
   bool is_mag = (p.mip_filter == FILTER_POINT) ? (miplevel < 0.5f) : (miplevel < 0.0f);
@@ -405,35 +437,7 @@ __device__ __noinline__ float4 sample_impl(const sampler_desc_dev& p, float u, f
   const surface_desc& lv = p.levels[lo_c];
   const bool fast = p.min_filter == FILTER_LINEAR && p.addr_u == ADDR_WRAP && p.addr_v == ADDR_WRAP && lv.pf == PF_RGBA8 &&
                     lv.samples == 1;
-  if (fast) {
-    // SRX_ANISO_BATCH probes per step: their texel loads are issued together (every address is clamped
-    // into the level, so probes past the count load harmlessly and are masked out), then the
-    // probes are accumulated in order, which keeps the reference's summation order.
-    for (int k = 0; k < n; k += SRX_ANISO_BATCH) {
-      float4 c[SRX_ANISO_BATCH];
-#pragma unroll
-      for (int j = 0; j < SRX_ANISO_BATCH; ++j) {
-        c[j] = bilinear_wrap_rgba8(lv, x, y);
-        x += af->du;
-        y += af->dv;
-      }
-#pragma unroll
-      for (int j = 0; j < SRX_ANISO_BATCH; ++j) {
-        const int i = 2 * (k + j) - n + 1;
-        const bool live = k + j < n;
-        float wf = static_cast<float>(i * i) * af->weight_D;
-        int wi = (wf >= 0.0f) ? static_cast<int>(sse_min(wf, 255.0f)) : 0;
-        float w = c_ewa[live ? wi : 0];
-        cr = live ? cr + c[j].x * w : cr;
-        cg = live ? cg + c[j].y * w : cg;
-        cb = live ? cb + c[j].z * w : cb;
-        ca = live ? ca + c[j].w * w : ca;
-        w_sum = live ? w_sum + w : w_sum;
-      }
-    }
-    float inv = 1 / w_sum;
-    return make_float4(cr * inv, cg * inv, cb * inv, ca * inv);
-  }
+  if (fast) return aniso_probes_rgba8(lv, x, y, n, af);
   for (int i = -n + 1; i < n; i += 2) {
     float4 c0 = sample_surface(p, lo_c, p.min_filter, x, y);
     float wf = static_cast<float>(i * i) * af->weight_D;
@@ -449,6 +453,21 @@ __device__ __noinline__ float4 sample_impl(const sampler_desc_dev& p, float u, f
   }
   float inv = 1 / w_sum;
   return make_float4(cr * inv, cg * inv, cb * inv, ca * inv);
+}
+
+// sample_2d_grad for a sampler with fast_aniso set: the same decisions as sample_2d_grad and
+// sample_impl (magnification below lod 0, else the probes on the rounded level), inline and compact.
+__device__ __forceinline__ float4 sample_2d_grad_fast(const sampler_desc_dev& p, float u, float v, float dxu, float dxv,
+                                                      float dyu, float dyv, float bias) {
+  const aniso_info af = calc_aniso(p, dxu, dxv, dyu, dyv, bias);
+  const float miplevel = af.lod;
+  if (miplevel < 0.0f) return bilinear_wrap_rgba8(p.levels[p.max_lod], u, v);
+  const float start = -0.5f * (af.probe_count - 1.0f);
+  const int lo = fast_roundi(miplevel);
+  const uint64_t lo_s = static_cast<uint64_t>(static_cast<int64_t>(lo));
+  const int lo_c = lo_s < static_cast<uint64_t>(p.max_lod) ? p.max_lod
+                   : (static_cast<uint64_t>(p.min_lod) < lo_s ? p.min_lod : static_cast<int>(lo_s));
+  return aniso_probes_rgba8(p.levels[lo_c], u + af.du * start, v + af.dv * start, static_cast<int>(af.probe_count), &af);
 }
 
 // sample_2d_grad (sampler.cpp:854-873)
