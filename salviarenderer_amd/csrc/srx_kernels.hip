@@ -549,25 +549,35 @@ __global__ void clip_kernel(draw_params P, const float4* __restrict__ vbuf, floa
 
 // ============================================================== record emission
 // off: exclusive scan of the per-triangle record counts; a triangle owns [off[id], off[id+1]).
-// Records beyond `capacity` are dropped; the host compares the total with it and emits again.
-__global__ void emit_kernel(draw_params P, const float4* __restrict__ tri_rec, const uint32_t* __restrict__ off,
-                            const uint32_t* __restrict__ total, uint32_t capacity, uint32_t* __restrict__ keys,
-                            uint32_t* __restrict__ vals) {
-  uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t n = 3 * P.prim_count;
-  if (id >= n) return;
-  uint32_t next = (id + 1 < n) ? off[id + 1] : *total;
-  if (next == off[id]) return;
-  float e[9], bb[4];
-  read_tri_rec(tri_rec + tri_slot(P, id) * 5, e, bb);
-  uint32_t o = off[id];
-  for_each_gtile(P, e, bb, [&](uint32_t g, bool full) {
-    if (o < next && o < capacity) {  // the enumeration repeats the counting pass exactly
-      keys[o] = g;
-      vals[o] = (id << 1) | (full ? 1u : 0u);
-    }
-    ++o;
-  });
+// One thread per primitive, over its (at most 3) clipped triangles in fan order.  Records beyond
+// `capacity` are dropped; the host compares the total with it and emits again.
+__global__ void __launch_bounds__(256) emit_kernel(draw_params P, const float4* __restrict__ tri_rec,
+                                                   const uint32_t* __restrict__ off, const uint32_t* __restrict__ total,
+                                                   uint32_t capacity, uint32_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = 3 * P.prim_count;
+  if (p >= P.prim_count) return;
+  const uint32_t id3 = 3 * p;
+  const uint32_t o0 = off[id3], o1 = off[id3 + 1], o2 = off[id3 + 2];
+  const uint32_t o3 = (id3 + 3 < n) ? off[id3 + 3] : *total;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const uint32_t id = id3 + k;
+    const uint32_t first = k == 0 ? o0 : (k == 1 ? o1 : o2);
+    const uint32_t next = k == 0 ? o1 : (k == 1 ? o2 : o3);
+    if (next == first) continue;
+    float e[9], bb[4];
+    read_tri_rec(tri_rec + tri_slot(P, id) * 5, e, bb);
+    uint32_t o = first;
+    for_each_gtile(P, e, bb, [&](uint32_t g, bool full) {
+      if (o < next && o < capacity) {  // the enumeration repeats the counting pass exactly
+        keys[o] = g;
+        vals[o] = (id << 1) | (full ? 1u : 0u);
+      }
+      ++o;
+    });
+  }
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
@@ -1507,7 +1517,7 @@ __device__ __forceinline__ void write_winner_depth(const draw_params& P, int mod
 //     and the colour stored to the item's samples.  Items are spread evenly over the lanes however
 //     many triangles share a pixel.
 #ifndef SRX_VIS_THREADS
-#define SRX_VIS_THREADS 128
+#define SRX_VIS_THREADS 256
 #endif
 template <int NA, int S, bool SHADE>
 __global__ void __launch_bounds__(SRX_VIS_THREADS, 4) vis_kernel(draw_params P, int mode, int pass,
@@ -1886,7 +1896,7 @@ void launch_clip(const draw_params& P, const draw_buffers& B, hipStream_t s) {
 }
 
 void launch_emit(const draw_params& P, const draw_buffers& B, uint32_t capacity, hipStream_t s) {
-  uint32_t n = 3 * P.prim_count;
+  const uint32_t n = P.prim_count;
   if (!n) return;
   hipLaunchKernelGGL(emit_kernel, dim3((n + 255) / 256), dim3(256), 0, s, P, B.tri_rec, B.cnt, B.counters + CT_TOTAL,
                      capacity, B.keys[0], B.vals[0]);
