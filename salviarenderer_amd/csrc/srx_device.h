@@ -77,6 +77,7 @@ struct draw_params {
   uint32_t stencil_ref, read_mask, write_mask;
   int32_t st_func[2], st_pass_op[2];  // [0] front face, [1] back face
   int32_t early_z, blend_id;
+  int32_t count_stats;  // a statistics query is active: accumulate the device-side counters
   // pixel shader
   int32_t ps_id;
   float ps_consts[16];
@@ -137,7 +138,7 @@ __device__ __forceinline__ uint32_t unorm8(float x) {
 }
 
 // ------------------------------------------------------------------ texel formats
-__device__ __forceinline__ int pf_size(int pf) {
+__host__ __device__ __forceinline__ int pf_size(int pf) {
   switch (pf) {
   case PF_RGBA32F: return 16;
   case PF_RGB32F: return 12;
@@ -189,6 +190,30 @@ __device__ __forceinline__ void store_texel(int pf, uint8_t* p, float4 c) {
   case PF_R32F: *reinterpret_cast<float*>(p) = c.x; return;
   case PF_RG32F: *reinterpret_cast<float2*>(p) = make_float2(c.x, c.y); return;
   case PF_R32I: *reinterpret_cast<int32_t*>(p) = static_cast<int32_t>(c.x + 0.5f); return;
+  }
+}
+
+// The 16 bytes of a fill with texel value c: the texel's words repeated (texel sizes 4, 8, 16).
+__device__ __forceinline__ uint4 fill_words(int pf, float4 c) {
+  switch (pf) {
+  case PF_RGBA8: {
+    const uint32_t u = unorm8(c.x) | (unorm8(c.y) << 8) | (unorm8(c.z) << 16) | (unorm8(c.w) << 24);
+    return make_uint4(u, u, u, u);
+  }
+  case PF_BGRA8: {
+    const uint32_t u = unorm8(c.z) | (unorm8(c.y) << 8) | (unorm8(c.x) << 16) | (unorm8(c.w) << 24);
+    return make_uint4(u, u, u, u);
+  }
+  case PF_RGBA32F: return make_uint4(__float_as_uint(c.x), __float_as_uint(c.y), __float_as_uint(c.z), __float_as_uint(c.w));
+  case PF_RG32F: return make_uint4(__float_as_uint(c.x), __float_as_uint(c.y), __float_as_uint(c.x), __float_as_uint(c.y));
+  case PF_R32I: {
+    const uint32_t u = static_cast<uint32_t>(static_cast<int32_t>(c.x + 0.5f));
+    return make_uint4(u, u, u, u);
+  }
+  default: {  // R32F
+    const uint32_t u = __float_as_uint(c.x);
+    return make_uint4(u, u, u, u);
+  }
   }
 }
 

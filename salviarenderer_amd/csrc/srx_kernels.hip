@@ -450,7 +450,7 @@ __global__ void __launch_bounds__(256) setup_kernel(draw_params P, const float4*
       }
     }
   }
-  block_partial(partial, cprims);
+  if (P.count_stats) block_partial(partial, cprims);  // uniform over the launch
 }
 
 // ============================================================== clipping (slow path)
@@ -467,15 +467,12 @@ __device__ __forceinline__ void lerp_vtx(vtx<NA>& o, const vtx<NA>& a, const vtx
   }
 }
 
-// clip_triangle_to_poly_general + clip_solid_triangle (clipper.cpp:43-228)
+// clip_triangle_to_poly_general + clip_solid_triangle (clipper.cpp:43-228) for primitive p
 template <int NA>
-__global__ void clip_kernel(draw_params P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
-                            float4* __restrict__ tri_attr, uint32_t* __restrict__ cnt, const uint8_t* __restrict__ clip_flag, int32_t* __restrict__ clip_slot,
-                            float4* __restrict__ clip_verts, uint32_t* __restrict__ clip_hdr, uint32_t capacity,
-                            uint32_t* __restrict__ counters, unsigned long long* __restrict__ stats) {
-  if (counters[CT_CLIP_PRIMS] == 0) return;  // setup flagged nothing
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P.prim_count || !clip_flag[p]) return;
+__device__ void clip_primitive(const draw_params& P, uint32_t p, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
+                               float4* __restrict__ tri_attr, uint32_t* __restrict__ cnt, int32_t* __restrict__ clip_slot,
+                               float4* __restrict__ clip_verts, uint32_t* __restrict__ clip_hdr, uint32_t capacity,
+                               uint32_t* __restrict__ counters, unsigned long long* __restrict__ stats) {
   vtx<NA> cv[2][5];
   uint32_t nc[2] = {3, 0};
   for (int i = 0; i < 3; ++i) load_vertex<NA>(P, vbuf, fetch_index(P, p, i), cv[0][i]);
@@ -545,6 +542,18 @@ __global__ void clip_kernel(draw_params P, const float4* __restrict__ vbuf, floa
     for_each_gtile(P, e, bb, [&](uint32_t, bool) { ++m; });
     cnt[id] = m;
   }
+}
+
+// Clipping launch: a bounded grid strides over the primitives setup flagged (none, usually).
+template <int NA>
+__global__ void clip_kernel(draw_params P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
+                            float4* __restrict__ tri_attr, uint32_t* __restrict__ cnt, const uint8_t* __restrict__ clip_flag, int32_t* __restrict__ clip_slot,
+                            float4* __restrict__ clip_verts, uint32_t* __restrict__ clip_hdr, uint32_t capacity,
+                            uint32_t* __restrict__ counters, unsigned long long* __restrict__ stats) {
+  if (counters[CT_CLIP_PRIMS] == 0) return;  // setup flagged nothing
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P.prim_count; p += gridDim.x * blockDim.x)
+    if (clip_flag[p]) clip_primitive<NA>(P, p, vbuf, tri_rec, tri_attr, cnt, clip_slot, clip_verts, clip_hdr, capacity,
+                                         counters, stats);
 }
 
 // ============================================================== record emission
@@ -758,12 +767,23 @@ int launch_sort(const draw_buffers& B, uint32_t n, int bits, hipStream_t s) {
   return cur;
 }
 
-__global__ void tile_range_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ ts, uint32_t* __restrict__ te) {
+// [ts, te) of every GPU tile in the sorted records; the thread at the last record of a tile (or
+// before the first record) also writes the empty ranges of the tiles up to the next record's tile,
+// so every tile is written exactly once without a separate clear.
+__global__ void tile_range_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t T, uint32_t* __restrict__ ts,
+                                  uint32_t* __restrict__ te) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t k = keys[i];
+  if (i == 0) {
+    for (uint32_t t = 0; t < k; ++t) ts[t] = te[t] = 0;
+  }
   if (i == 0 || keys[i - 1] != k) ts[k] = i;
-  if (i == n - 1 || keys[i + 1] != k) te[k] = i + 1;
+  if (i == n - 1 || keys[i + 1] != k) {
+    te[k] = i + 1;
+    const uint32_t next = (i == n - 1) ? T : keys[i + 1];
+    for (uint32_t t = k + 1; t < next; ++t) ts[t] = te[t] = i + 1;
+  }
 }
 
 // Scheduling order of the GPU tiles, heaviest first so that the longest tiles start early.  Tiles
@@ -858,9 +878,13 @@ void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s) {
 
 void launch_tile_ranges(const draw_params& P, const draw_buffers& B, int which, uint32_t n, hipStream_t s) {
   uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
-  hipLaunchKernelGGL(fill_u32_kernel, dim3((T + 255) / 256), dim3(256), 0, s, B.tstart, T, 0u);
-  hipLaunchKernelGGL(fill_u32_kernel, dim3((T + 255) / 256), dim3(256), 0, s, B.tend, T, 0u);
-  if (n) hipLaunchKernelGGL(tile_range_kernel, dim3((n + 255) / 256), dim3(256), 0, s, B.keys[which], n, B.tstart, B.tend);
+  if (!T) return;
+  if (!n) {  // no records: every tile is empty
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((T + 255) / 256), dim3(256), 0, s, B.tstart, T, 0u);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((T + 255) / 256), dim3(256), 0, s, B.tend, T, 0u);
+    return;
+  }
+  hipLaunchKernelGGL(tile_range_kernel, dim3((n + 255) / 256), dim3(256), 0, s, B.keys[which], n, T, B.tstart, B.tend);
 }
 
 // ============================================================== pixel shaders
@@ -1882,12 +1906,13 @@ void launch_setup(const draw_params& P, const draw_buffers& B, hipStream_t s) {
   hipLaunchKernelGGL(setup_kernel<NA>, grid, dim3(256), 0, s, P, B.vbuf, B.tri_rec, B.tri_attr, B.cnt, B.clip_flag, B.counters, B.partial)
   SRX_NA_DISPATCH(P.nattr, SRX_SETUP)
 #undef SRX_SETUP
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, B.partial, grid.x, B.stats, static_cast<int>(ST_CPRIMITIVES));
+  if (P.count_stats)
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, B.partial, grid.x, B.stats, static_cast<int>(ST_CPRIMITIVES));
 }
 
 void launch_clip(const draw_params& P, const draw_buffers& B, hipStream_t s) {
   if (P.prim_count == 0) return;
-  dim3 grid((P.prim_count + 63) / 64);
+  dim3 grid(std::min<uint32_t>((P.prim_count + 63) / 64, 1024));
 #define SRX_CLIP(NA)                                                                                                     \
   hipLaunchKernelGGL(clip_kernel<NA>, grid, dim3(64), 0, s, P, B.vbuf, B.tri_rec, B.tri_attr, B.cnt, B.clip_flag, B.clip_slot, \
                      B.clip_verts, B.clip_hdr, B.clip_capacity, B.counters, B.stats)
@@ -2014,16 +2039,39 @@ void launch_shade(const draw_params& P, const draw_buffers& B, hipStream_t s) {
 }
 
 // ============================================================== clears, resolve, mip generation
+// surface::fill with the converted colour: the texel bytes are built once and stored 16 bytes at a
+// time (texel sizes 4, 8 and 16 tile a 16-byte word; RGB32F takes the per-texel path).
 __global__ void clear_kernel(surface_desc s, float4 v) {
-  size_t n = static_cast<size_t>(s.w) * s.h * s.samples;
-  size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  store_texel(s.pf, s.data + i * pf_size(s.pf), v);
+  const size_t n = static_cast<size_t>(s.w) * s.h * s.samples;
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int es = pf_size(s.pf);
+  if (es == 12) {
+    if (i < n) store_texel(s.pf, s.data + i * 12, v);
+    return;
+  }
+  const uint4 w = fill_words(s.pf, v);
+  const size_t words = n * es / 16, tail = n * es % 16;
+  if (i < words) reinterpret_cast<uint4*>(s.data)[i] = w;
+  if (i == 0 && tail) {  // whole texels (es divides 16): the tail starts a fresh 16-byte word
+    const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+    for (size_t b = 0; b < tail; b += 4) *reinterpret_cast<uint32_t*>(s.data + words * 16 + b) = ws[b / 4];
+  }
 }
 
+// depth and/or stencil of every RG32F sample; both at once are 16-byte stores of two samples
 __global__ void clear_ds_kernel(surface_desc s, uint32_t flags, float depth, uint32_t stencil) {
-  size_t n = static_cast<size_t>(s.w) * s.h * s.samples;
-  size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t n = static_cast<size_t>(s.w) * s.h * s.samples;
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (flags == 3) {
+    const uint32_t d = __float_as_uint(depth);
+    if (i < n / 2) reinterpret_cast<uint4*>(s.data)[i] = make_uint4(d, stencil, d, stencil);
+    if (i == 0 && (n & 1)) {
+      float* p = reinterpret_cast<float*>(s.data + (n - 1) * 8);
+      p[0] = depth;
+      p[1] = __uint_as_float(stencil);
+    }
+    return;
+  }
   if (i >= n) return;
   float* p = reinterpret_cast<float*>(s.data + i * 8);
   if (flags & 1) p[0] = depth;
@@ -2041,6 +2089,24 @@ __global__ void resolve_kernel(surface_desc src, surface_desc dst) {
   }
   float inv = 1 / static_cast<float>(src.samples);
   store_texel(dst.pf, sample_addr(dst, x, y, 0), make_float4(c.x * inv, c.y * inv, c.z * inv, c.w * inv));
+}
+
+// The same for 4-sample RGBA8 into RGBA8: one 16-byte load per pixel.
+__global__ void resolve_rgba8x4_kernel(surface_desc src, surface_desc dst) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<size_t>(src.w) * src.h) return;
+  const uint4 q = reinterpret_cast<const uint4*>(src.data)[i];
+  const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+  float4 c = make_float4(0, 0, 0, 0);
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const uint32_t u = w[s];
+    const float4 t = make_float4(static_cast<float>(u & 0xFF) * (1.0f / 255), static_cast<float>((u >> 8) & 0xFF) * (1.0f / 255),
+                                 static_cast<float>((u >> 16) & 0xFF) * (1.0f / 255), static_cast<float>(u >> 24) * (1.0f / 255));
+    c = make_float4(c.x + t.x, c.y + t.y, c.z + t.z, c.w + t.w);
+  }
+  const float inv = 1 / 4.0f;
+  store_texel(PF_RGBA8, dst.data + i * 4, make_float4(c.x * inv, c.y * inv, c.z * inv, c.w * inv));
 }
 
 // surface::make_mip_surface (surface.cpp:53-92); odd-size edge reads clamp to the last texel
@@ -2064,14 +2130,21 @@ __global__ void mip_kernel(surface_desc src, surface_desc dst, int filter) {
 }
 
 void launch_clear(const surface_desc& s, float4 v, hipStream_t st) {
-  size_t n = static_cast<size_t>(s.w) * s.h * s.samples;
-  if (n) hipLaunchKernelGGL(clear_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, s, v);
+  const size_t n = static_cast<size_t>(s.w) * s.h * s.samples;
+  const size_t threads = pf_size(s.pf) == 12 ? n : std::max<size_t>(n * pf_size(s.pf) / 16, 1);
+  if (n) hipLaunchKernelGGL(clear_kernel, dim3(static_cast<unsigned>((threads + 255) / 256)), dim3(256), 0, st, s, v);
 }
 void launch_clear_ds(const surface_desc& s, uint32_t flags, float depth, uint32_t stencil, hipStream_t st) {
-  size_t n = static_cast<size_t>(s.w) * s.h * s.samples;
-  if (n) hipLaunchKernelGGL(clear_ds_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, s, flags, depth, stencil);
+  const size_t n = static_cast<size_t>(s.w) * s.h * s.samples;
+  const size_t threads = flags == 3 ? std::max<size_t>(n / 2, 1) : n;
+  if (n) hipLaunchKernelGGL(clear_ds_kernel, dim3(static_cast<unsigned>((threads + 255) / 256)), dim3(256), 0, st, s, flags, depth, stencil);
 }
 void launch_resolve(const surface_desc& src, const surface_desc& dst, hipStream_t st) {
+  if (src.pf == PF_RGBA8 && dst.pf == PF_RGBA8 && src.samples == 4 && dst.samples == 1 && src.w == dst.w && src.h == dst.h) {
+    const size_t n = static_cast<size_t>(src.w) * src.h;
+    if (n) hipLaunchKernelGGL(resolve_rgba8x4_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, src, dst);
+    return;
+  }
   hipLaunchKernelGGL(resolve_kernel, dim3((src.w + 255) / 256, src.h), dim3(256), 0, st, src, dst);
 }
 void launch_mip(const surface_desc& src, const surface_desc& dst, int filter, hipStream_t st) {

@@ -486,6 +486,9 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   P.st_pass_op[1] = d.back_face.stencil_pass_op;
   // early-Z: stencil off and no depth-writing pixel shader (framebuffer.cpp:424), per draw
   P.early_z = ds && !d.stencil_enable;
+  P.count_stats = 0;
+  for (auto& q : r->active)
+    if (q->id != SRX_QUERY_PIPELINE_PROFILES) P.count_stats = 1;
   P.blend_id = st.blend_id;
   P.ps_id = st.ps_id;
   std::memcpy(P.ps_consts, st.ps_consts, sizeof P.ps_consts);
@@ -630,9 +633,7 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   B.order = r->order.p;
   mark(4);
   // statistics queries need the in-order walk (ps_invocations counts early-Z passes as they happen)
-  bool counting = false;
-  for (auto& q : r->active)
-    if (q->id != SRX_QUERY_PIPELINE_PROFILES) counting = true;
+  const bool counting = P.count_stats != 0;
   if (vis_pass && !counting) {
     uint32_t ne, nm, nsl;
     vis_table_bounds(T, total, &ne, &nm, &nsl);
