@@ -185,7 +185,11 @@ enum srx_vertex_shader_id {
      consts: float wvp[16] (row-major, row-vector convention of eflib::transform). */
   SRX_VS_WVP_NORMAL_UV = 2,
   /* in: POSITION0, COLOR0.  out: pos = mul(P, wvp); attr0 = COLOR0. consts: float wvp[16] */
-  SRX_VS_WVP_COLOR = 3
+  SRX_VS_WVP_COLOR = 3,
+  /* in: POSITION0, COLOR0.  out: pos = mul(P, wvp); attr0 = COLOR0; attr1 = mul(P, light_wvp), the
+     light's clip-space position for a shadow-map lookup (StandardShadowMap.cpp draw_cpp_vs).
+     consts: float wvp[16], float light_wvp[16] */
+  SRX_VS_WVP_COLOR_LIGHT = 4
 };
 enum srx_pixel_shader_id {
   /* color0 = attr0 (Gouraud). consts: none */
@@ -199,7 +203,14 @@ enum srx_pixel_shader_id {
      consts: none */
   SRX_PS_TEX_CPP = 3,
   /* color0 = constant. consts: float c[4] */
-  SRX_PS_CONSTANT = 4
+  SRX_PS_CONSTANT = 4,
+  /* Shadowed vertex colour with a 3x3 percentage-closer lookup: l = attr1.xyz / attr1.w, shadow-map
+     centre ((l.x + 1) * 0.5, 1 - (l.y + 1) * 0.5) and the 3x3 tap layout of StandardShadowMap.cpp:89-100
+     with spacing consts[2]; each tap is tex2dlod(s0, tap, lod 0).r and counts as lit when
+     l.z - consts[1] <= tap; lit = count / 9; k = consts[0] + (1 - consts[0]) * lit;
+     color0 = (attr0.r * k, attr0.g * k, attr0.b * k, attr0.a).
+     consts: float ambient, bias, spacing, unused */
+  SRX_PS_SHADOW_PCF = 5
 };
 enum srx_blend_shader_id {
   SRX_BLEND_OFF = 0,      /* ts_blend_off: target = src (TextureAndBlending.cpp:182-193) */

@@ -346,6 +346,7 @@ int vs_bindings(int id, vs_binding* out) {
   switch (id) {
   case SRX_VS_PASSTHROUGH_COLOR:
   case SRX_VS_WVP_COLOR:
+  case SRX_VS_WVP_COLOR_LIGHT:
     out[0] = {"position", 0, 0};
     out[1] = {"color", 0, 1};
     return 2;
@@ -361,7 +362,8 @@ int vs_output_count(int id) {
   switch (id) {
   case SRX_VS_PASSTHROUGH_COLOR:
   case SRX_VS_WVP_COLOR: return 1;
-  case SRX_VS_WVP_NORMAL_UV: return 2;
+  case SRX_VS_WVP_NORMAL_UV:
+  case SRX_VS_WVP_COLOR_LIGHT: return 2;
   }
   return 0;
 }
@@ -386,6 +388,11 @@ void run_vs(int id, const float* consts, const vec4* in, vso& out) {
     out.r[1] = in[1];
     out.r[2] = in[2];
     break;
+  case SRX_VS_WVP_COLOR_LIGHT:  // light-space position for the shadow lookup (StandardShadowMap.cpp)
+    out.r[0] = transform_rowvec(in[0], consts);
+    out.r[1] = in[1];
+    out.r[2] = transform_rowvec(in[0], consts + 16);
+    break;
   }
 }
 
@@ -406,6 +413,28 @@ struct ps_context {
   sampler_params sp[SRX_MAX_SAMPLER_SLOTS];
   bool has_tex[SRX_MAX_SAMPLER_SLOTS];
 };
+
+// 3x3 percentage-closer shadow lookup (the tap layout of StandardShadowMap.cpp:86-106; each tap is a
+// tex2dlod at lod 0 and is lit when the light-space depth minus the bias does not exceed it).
+vec4 shadow_pcf(const ps_context& c, const vso& v) {
+  const vec4& a = v.r[1];
+  const vec4& l = v.r[2];
+  float lx = l[0] / l[3], ly = l[1] / l[3], lz = l[2] / l[3];
+  float cu = (lx + 1.0f) * 0.5f, cv = 1.0f - (ly + 1.0f) * 0.5f;
+  float o = c.consts[2];
+  int count = 0;
+  if (c.has_tex[0]) {
+    for (int ty = -1; ty <= 1; ++ty)
+      for (int tx = -1; tx <= 1; ++tx) {
+        float u = cu + static_cast<float>(tx) * o, vv = cv + static_cast<float>(ty) * o;
+        vec4 t = sample_impl(c.tex[0], c.sp[0], u, vv, 0.0f, nullptr);
+        if (lz - c.consts[1] <= t[0]) ++count;
+      }
+  }
+  float lit = static_cast<float>(count) / 9.0f;
+  float k = c.consts[0] + (1.0f - c.consts[0]) * lit;
+  return v4(a[0] * k, a[1] * k, a[2] * k, a[3]);
+}
 
 // Runs the pixel shader on the 4 pixels of a quad (cpp_pixel_shader::execute,
 // cpp_pixel_shader.cpp:62-74).  Returns the per-pixel pass mask (the sample shaders never discard).
@@ -447,6 +476,9 @@ void run_ps_quad(const ps_context& c, const vso* q, vec4* color) {
     }
     break;
   }
+  case SRX_PS_SHADOW_PCF:
+    for (int i = 0; i < 4; ++i) color[i] = shadow_pcf(c, q[i]);
+    break;
   default:
     for (int i = 0; i < 4; ++i) color[i] = v4(0, 0, 0, 0);
   }
@@ -1551,14 +1583,15 @@ int32_t srx_set_vertex_shader(srx_renderer rr, int32_t id, const void* consts, s
   if (!r || vs_output_count(id) <= 0) return SRX_INVALID_PARAMETER;
   if ((id == SRX_VS_WVP_NORMAL_UV || id == SRX_VS_WVP_COLOR) && size < 64) return SRX_INVALID_PARAMETER;
   r->st.vs_id = id;
-  r->st.vs_consts.assign(16, 0.0f);
-  if (consts && size) std::memcpy(r->st.vs_consts.data(), consts, std::min<size_t>(size, 64));
+  if (id == SRX_VS_WVP_COLOR_LIGHT && size < 128) return SRX_INVALID_PARAMETER;
+  r->st.vs_consts.assign(32, 0.0f);
+  if (consts && size) std::memcpy(r->st.vs_consts.data(), consts, std::min<size_t>(size, 128));
   return SRX_OK;
 }
 int32_t srx_set_pixel_shader(srx_renderer rr, int32_t id, const void* consts, size_t size) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);
-  if (!r || id < SRX_PS_GOURAUD || id > SRX_PS_CONSTANT) return SRX_INVALID_PARAMETER;
-  if ((id == SRX_PS_LAMBERT_TEX || id == SRX_PS_CONSTANT) && size < 16) return SRX_INVALID_PARAMETER;
+  if (!r || id < SRX_PS_GOURAUD || id > SRX_PS_SHADOW_PCF) return SRX_INVALID_PARAMETER;
+  if ((id == SRX_PS_LAMBERT_TEX || id == SRX_PS_CONSTANT || id == SRX_PS_SHADOW_PCF) && size < 16) return SRX_INVALID_PARAMETER;
   r->st.ps_id = id;
   r->st.ps_consts.assign(16, 0.0f);
   if (consts && size) std::memcpy(r->st.ps_consts.data(), consts, std::min<size_t>(size, 64));

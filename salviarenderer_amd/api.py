@@ -27,8 +27,8 @@ CMP_NEVER, CMP_LESS, CMP_EQUAL, CMP_LESS_EQUAL, CMP_GREATER, CMP_NOT_EQUAL, CMP_
 SOP_KEEP, SOP_ZERO, SOP_REPLACE, SOP_INCR_SAT, SOP_DECR_SAT, SOP_INVERT, SOP_INCR_WRAP, SOP_DECR_WRAP = range(1, 9)
 CLEAR_DEPTH, CLEAR_STENCIL = 1, 2
 QUERY_PIPELINE_STATISTICS, QUERY_INTERNAL_STATISTICS, QUERY_PIPELINE_PROFILES = 3, 5, 6
-VS_PASSTHROUGH_COLOR, VS_WVP_NORMAL_UV, VS_WVP_COLOR = 1, 2, 3
-PS_GOURAUD, PS_LAMBERT_TEX, PS_TEX_CPP, PS_CONSTANT = 1, 2, 3, 4
+VS_PASSTHROUGH_COLOR, VS_WVP_NORMAL_UV, VS_WVP_COLOR, VS_WVP_COLOR_LIGHT = 1, 2, 3, 4
+PS_GOURAUD, PS_LAMBERT_TEX, PS_TEX_CPP, PS_CONSTANT, PS_SHADOW_PCF = 1, 2, 3, 4, 5
 BLEND_OFF, BLEND_ON, BLEND_DISABLED = 0, 1, 2
 RENDERER_ASYNC, RENDERER_BENCHMARK = 0, 1
 

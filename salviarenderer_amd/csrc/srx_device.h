@@ -52,7 +52,7 @@ struct draw_params {
   // vertex stage
   int32_t vs_id, nin, nattr;
   uint32_t attr_mod[MAX_ATTRS];
-  float vs_consts[16];
+  float vs_consts[32];
   input_elem_dev in[MAX_INPUTS];
   uint32_t nverts;  // vertex-shader outputs stored (one more slot holds an all-zero input)
   // index fetch

@@ -21,8 +21,8 @@ namespace srx {
 
 __constant__ float c_ewa[256];
 
-enum { VS_PASSTHROUGH_COLOR = 1, VS_WVP_NORMAL_UV = 2, VS_WVP_COLOR = 3 };
-enum { PS_GOURAUD = 1, PS_LAMBERT_TEX = 2, PS_TEX_CPP = 3, PS_CONSTANT = 4 };
+enum { VS_PASSTHROUGH_COLOR = 1, VS_WVP_NORMAL_UV = 2, VS_WVP_COLOR = 3, VS_WVP_COLOR_LIGHT = 4 };
+enum { PS_GOURAUD = 1, PS_LAMBERT_TEX = 2, PS_TEX_CPP = 3, PS_CONSTANT = 4, PS_SHADOW_PCF = 5 };
 enum { BLEND_OFF = 0, BLEND_ON = 1, BLEND_DISABLED = 2 };
 enum { AM_NOINTERPOLATION = 4, AM_NOPERSPECTIVE = 8 };
 enum { TOPO_TRIANGLE_LIST = 2, TOPO_TRIANGLE_STRIP = 4 };
@@ -86,6 +86,11 @@ __global__ void vs_kernel(draw_params P, float4* __restrict__ vbuf) {
   case VS_WVP_COLOR:
     out[0] = transform_rowvec(in[0], P.vs_consts);
     out[1] = in[1];
+    break;
+  case VS_WVP_COLOR_LIGHT:  // light-space position for the shadow lookup (StandardShadowMap.cpp)
+    out[0] = transform_rowvec(in[0], P.vs_consts);
+    out[1] = in[1];
+    out[2] = transform_rowvec(in[0], P.vs_consts + 16);
     break;
   default:
     out[0] = transform_rowvec(in[0], P.vs_consts);
@@ -971,10 +976,37 @@ __device__ __forceinline__ void run_ps(const draw_params& P, const ps_samplers* 
     }
     return;
   }
+  case PS_SHADOW_PCF:
+    if constexpr (NA >= 2) {
+      for (int i = 0; i < 4; ++i)
+        col[i] = (need & (1u << i)) ? shadow_pcf(P, SM, q[i][1], q[i][2]) : make_float4(0, 0, 0, 0);
+    }
+    return;
   default:
 #pragma unroll
     for (int i = 0; i < 4; ++i) col[i] = make_float4(0, 0, 0, 0);
   }
+}
+
+// 3x3 percentage-closer shadow lookup of one pixel (attr0 colour, attr1 light clip position): the
+// tap layout of StandardShadowMap.cpp:86-106, each tap a tex2dlod at lod 0, lit when the light-space
+// depth minus the bias does not exceed it.
+__device__ __forceinline__ float4 shadow_pcf(const draw_params& P, const ps_samplers* __restrict__ SM, float4 a, float4 l) {
+  const float lx = l.x / l.w, ly = l.y / l.w, lz = l.z / l.w;
+  const float cu = (lx + 1.0f) * 0.5f, cv = 1.0f - (ly + 1.0f) * 0.5f;
+  const float o = P.ps_consts[2];
+  int count = 0;
+  if (P.has_sampler[0]) {
+    for (int ty = -1; ty <= 1; ++ty)
+      for (int tx = -1; tx <= 1; ++tx) {
+        const float u = cu + static_cast<float>(tx) * o, v = cv + static_cast<float>(ty) * o;
+        const float4 t = sample_impl(SM->s[0], u, v, 0.0f, nullptr);
+        if (lz - P.ps_consts[1] <= t.x) ++count;
+      }
+  }
+  const float lit = static_cast<float>(count) / 9.0f;
+  const float k = P.ps_consts[0] + (1.0f - P.ps_consts[0]) * lit;
+  return make_float4(a.x * k, a.y * k, a.z * k, a.w);
 }
 
 // Register r of quad pixel i (0..3) by selects (a dynamically indexed register array would go to
@@ -1030,6 +1062,9 @@ __device__ __forceinline__ float4 run_ps_pixel(const draw_params& P, const ps_sa
     t.w = 1.0f;
     return t;
   }
+  case PS_SHADOW_PCF:
+    if constexpr (NA >= 2) return shadow_pcf(P, SM, quad_reg<NA>(q, pi, 1), quad_reg<NA>(q, pi, 2));
+    return make_float4(0, 0, 0, 0);
   default:
     return make_float4(0, 0, 0, 0);
   }

@@ -199,6 +199,7 @@ int vs_bindings(int id, vs_binding* out) {
   switch (id) {
   case SRX_VS_PASSTHROUGH_COLOR:
   case SRX_VS_WVP_COLOR:
+  case SRX_VS_WVP_COLOR_LIGHT:
     out[0] = {"position", 0, 0};
     out[1] = {"color", 0, 1};
     return 2;
@@ -214,7 +215,8 @@ int vs_output_count(int id) {
   switch (id) {
   case SRX_VS_PASSTHROUGH_COLOR:
   case SRX_VS_WVP_COLOR: return 1;
-  case SRX_VS_WVP_NORMAL_UV: return 2;
+  case SRX_VS_WVP_NORMAL_UV:
+  case SRX_VS_WVP_COLOR_LIGHT: return 2;
   }
   return 0;
 }
@@ -254,7 +256,7 @@ struct render_state {
   ref<layout_obj> layout;
   uint32_t topo = SRX_TOPOLOGY_TRIANGLE_LIST;
   int vs_id = 0, ps_id = 0, blend_id = SRX_BLEND_OFF;
-  float vs_consts[16] = {}, ps_consts[16] = {};
+  float vs_consts[32] = {}, ps_consts[16] = {};
   ref<sampler_obj> samplers[SRX_MAX_SAMPLER_SLOTS];
   srx_raster_desc raster;
   srx_depth_stencil_desc ds;
@@ -946,15 +948,16 @@ int32_t srx_set_vertex_shader(srx_renderer rr, int32_t id, const void* consts, s
   auto* r = as<renderer_obj>(rr, K_RENDERER);
   if (!r || vs_output_count(id) <= 0) return SRX_INVALID_PARAMETER;
   if ((id == SRX_VS_WVP_NORMAL_UV || id == SRX_VS_WVP_COLOR) && size < 64) return SRX_INVALID_PARAMETER;
+  if (id == SRX_VS_WVP_COLOR_LIGHT && size < 128) return SRX_INVALID_PARAMETER;
   r->st.vs_id = id;
   std::memset(r->st.vs_consts, 0, sizeof r->st.vs_consts);
-  if (consts && size) std::memcpy(r->st.vs_consts, consts, std::min<size_t>(size, 64));
+  if (consts && size) std::memcpy(r->st.vs_consts, consts, std::min<size_t>(size, sizeof r->st.vs_consts));
   return SRX_OK;
 }
 int32_t srx_set_pixel_shader(srx_renderer rr, int32_t id, const void* consts, size_t size) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);
-  if (!r || id < SRX_PS_GOURAUD || id > SRX_PS_CONSTANT) return SRX_INVALID_PARAMETER;
-  if ((id == SRX_PS_LAMBERT_TEX || id == SRX_PS_CONSTANT) && size < 16) return SRX_INVALID_PARAMETER;
+  if (!r || id < SRX_PS_GOURAUD || id > SRX_PS_SHADOW_PCF) return SRX_INVALID_PARAMETER;
+  if ((id == SRX_PS_LAMBERT_TEX || id == SRX_PS_CONSTANT || id == SRX_PS_SHADOW_PCF) && size < 16) return SRX_INVALID_PARAMETER;
   r->st.ps_id = id;
   std::memset(r->st.ps_consts, 0, sizeof r->st.ps_consts);
   if (consts && size) std::memcpy(r->st.ps_consts, consts, std::min<size_t>(size, 64));

@@ -10,6 +10,8 @@ Parameters the configurations leave open are fixed here once (seeds, camera, lig
                  with the full box-filtered mip chain; min/mag linear, mip anisotropic x16, wrap;
                  Lambert with L = normalize(1,1,-1); depth LESS; cull none; camera at
                  (0, 0.8, -0.6) looking at (0, 0.5, 0), vertical fov pi/2, near 0.1, far 100.
+  multipass    : shadow map + shaded, blended and mirrored passes over 200,000 random triangles
+                 (seed 0x5A11A003; setup_multipass below has the exact recipe).
 Random numbers come from splitmix64; a float in [0,1) is (u >> 40) * 2^-24.
 """
 from __future__ import annotations
@@ -223,3 +225,166 @@ def setup_mesh(r: A.Renderer, width=1920, height=1080, samples=4, grid=512, tex_
     n_tris = idx.size // 3
     keep += [vb, ib, layout, tex, samp, rs, dss]
     return Frame(r, color, ds, resolved, lambda: r.draw_index(0, n_tris, 0), n_tris, width, height, samples, keep)
+
+
+# ---- multipass ---------------------------------------------------------------------------
+SEED_MULTIPASS = 0x5A11A003
+
+
+def multipass_triangles(n_tris: int, width: int, height: int, seed: int = SEED_MULTIPASS):
+    """Positions (3n, 3) in the camera's clip space (w = 1) and colours (3n, 4).
+
+    Per triangle: centroid ~ U[-1,1]^2, area ~ log-uniform over [1, 4096] pixels of a width x height
+    target, an isotropic random shape (three standard-normal offsets scaled to that area), depth
+    ~ U[0.1, 0.9] per vertex, RGB ~ U[0,1]^3; 30% of the triangles are translucent with alpha ~
+    U[0.2, 0.8] (the rest have alpha 1).  Returns (pos, col, translucent mask per triangle)."""
+    u = uniform01(seed, n_tris * 17).reshape(n_tris, 17)
+    c = -1.0 + 2.0 * u[:, 0:2]
+    area_px = np.exp(np.log(4096.0) * u[:, 2])
+    # three isotropic offsets from two uniforms each (Box-Muller)
+    r = np.sqrt(-2.0 * np.log(np.maximum(u[:, 3:6], 1e-12)))
+    th = 2.0 * math.pi * u[:, 6:9]
+    off = np.stack([r * np.cos(th), r * np.sin(th)], -1)  # (n, 3, 2)
+    off -= off.mean(1, keepdims=True)
+    a0 = 0.5 * np.abs((off[:, 1, 0] - off[:, 0, 0]) * (off[:, 2, 1] - off[:, 0, 1])
+                      - (off[:, 2, 0] - off[:, 0, 0]) * (off[:, 1, 1] - off[:, 0, 1]))
+    scale = np.sqrt(area_px / np.maximum(a0, 1e-6))
+    xy = c[:, None, :] + off * scale[:, None, None] * np.array([2.0 / width, 2.0 / height])
+    z = 0.1 + 0.8 * u[:, 9:12]
+    pos = np.concatenate([xy, z[..., None]], -1).reshape(-1, 3)
+    translucent = u[:, 12] < 0.3
+    alpha = np.where(translucent, 0.2 + 0.6 * u[:, 13], 1.0)
+    rgba = np.concatenate([u[:, 14:17], alpha[:, None]], 1)
+    col = np.repeat(rgba, 3, 0)
+    return pos.astype(np.float32), col.astype(np.float32), translucent
+
+
+def light_matrix() -> np.ndarray:
+    """Row-vector orthographic light: rotate about y by 20 and x by 15 degrees around (0, 0, 0.5),
+    scale x, y by 0.8 and map the rotated depth to [0.1, 0.9]."""
+    ay, ax = math.radians(20.0), math.radians(15.0)
+    t0 = np.eye(4); t0[3, 2] = -0.5
+    ry = np.array([[math.cos(ay), 0, -math.sin(ay), 0], [0, 1, 0, 0], [math.sin(ay), 0, math.cos(ay), 0], [0, 0, 0, 1]])
+    rx = np.array([[1, 0, 0, 0], [0, math.cos(ax), math.sin(ax), 0], [0, -math.sin(ax), math.cos(ax), 0], [0, 0, 0, 1]])
+    sc = np.diag([0.8, 0.8, 0.4, 1.0]); sc[3, 2] = 0.5
+    return (t0 @ ry @ rx @ sc).astype(np.float32)
+
+
+def reflect_matrix(plane) -> np.ndarray:
+    """eflib mat_reflect (eflib/src/math.cpp:429-456) of the plane (a, b, c, d), row-vector form."""
+    n = np.asarray(plane[:3], np.float64)
+    ln = np.linalg.norm(n)
+    x, y, z = n / ln
+    w = plane[3] / ln
+    return np.array([[-2 * x * x + 1, -2 * y * x, -2 * z * x, 0], [-2 * x * y, -2 * y * y + 1, -2 * z * y, 0],
+                     [-2 * x * z, -2 * y * z, -2 * z * z + 1, 0], [-2 * x * w, -2 * y * w, -2 * z * w, 1]], np.float32)
+
+
+class MultipassFrame:
+    """One frame: shadow pass (depth only, from the light) -> opaque shaded pass with a 3x3 PCF
+    shadow lookup -> translucent triangles back-to-front (blend on, no depth write) -> mirror quad at
+    depth 0.95 marking stencil 1 (StencilMirror's stencil-replace state with depth ALWAYS, so the
+    mirror's depth replaces the scene's; colour writes disabled) -> the opaque triangles reflected
+    in the mirror plane x = 0.55 where stencil equals 1 -> MSAA resolve."""
+
+    def __init__(self, r: A.Renderer, width=1920, height=1080, samples=4, shadow_size=2048, n_tris=200_000,
+                 seed=SEED_MULTIPASS):
+        lib = r.lib
+        self.r, self.width, self.height, self.samples, self.n_tris = r, width, height, samples, n_tris
+        self.color, self.ds, self.resolved, self.keep = _targets(r, width, height, samples)
+        self.shadow_tex = r.create_tex2d(shadow_size, shadow_size, 1, A.PF_RG32F)
+        self.shadow = self.shadow_tex.surface(0)
+        pos, col, tr = multipass_triangles(n_tris, width, height, seed)
+        tri_pos = pos.reshape(-1, 3, 3)
+        tri_col = col.reshape(-1, 3, 4)
+        opaque = ~tr
+        # translucent triangles back-to-front: larger mean depth first (stable)
+        tidx = np.nonzero(tr)[0]
+        tidx = tidx[np.argsort(-tri_pos[tidx, :, 2].mean(1), kind="stable")]
+        order = np.concatenate([np.nonzero(opaque)[0], tidx])
+        verts = np.concatenate([tri_pos[order].reshape(-1, 3), tri_col[order].reshape(-1, 4)], 1).astype(np.float32)
+        self.n_opaque, self.n_trans = int(opaque.sum()), int(tr.sum())
+        self.vb = r.create_buffer_from(np.ascontiguousarray(verts))
+        self.layout = r.create_input_layout([("POSITION", 0, A.FMT_R32G32B32_FLOAT, 0, 0),
+                                             ("COLOR", 0, A.FMT_R32G32B32A32_FLOAT, 0, 12)], A.VS_WVP_COLOR_LIGHT)
+        mirror = np.array([[0.2, -0.9, 0.95, 1, 1, 1, 1], [0.9, -0.9, 0.95, 1, 1, 1, 1], [0.9, -0.2, 0.95, 1, 1, 1, 1],
+                           [0.2, -0.9, 0.95, 1, 1, 1, 1], [0.9, -0.2, 0.95, 1, 1, 1, 1], [0.2, -0.2, 0.95, 1, 1, 1, 1]],
+                          np.float32)
+        self.mvb = r.create_buffer_from(mirror)
+        self.L = light_matrix()
+        self.I = np.eye(4, dtype=np.float32)
+        self.M = reflect_matrix((1.0, 0.0, 0.0, -0.55))
+        sd = A.default_sampler_desc(lib)
+        sd.min_filter = sd.mag_filter = sd.mip_filter = A.FILTER_POINT
+        sd.addr_mode_u = sd.addr_mode_v = sd.addr_mode_w = A.ADDRESS_BORDER
+        for k in range(4):
+            sd.border_color[k] = 1.0
+        self.samp = r.create_sampler(sd, self.shadow_tex)
+        rd = A.default_raster_desc(lib)
+        rd.cull_mode = A.CULL_NONE
+        self.rs = r.create_raster_state(rd)
+        d = A.default_depth_stencil_desc(lib)
+        self.dss_normal = r.create_depth_stencil_state(d)
+        dt = A.default_depth_stencil_desc(lib)
+        dt.depth_write_mask = 0
+        self.dss_translucent = r.create_depth_stencil_state(dt)
+        dw = A.default_depth_stencil_desc(lib)  # StencilMirror.cpp:197-201, depth reset to the mirror plane
+        dw.stencil_enable = 1
+        dw.depth_func = A.CMP_ALWAYS
+        for f in (dw.front_face, dw.back_face):
+            f.stencil_pass_op = A.SOP_REPLACE
+        self.dss_mark = r.create_depth_stencil_state(dw)
+        de = A.default_depth_stencil_desc(lib)  # StencilMirror.cpp:203-206
+        de.stencil_enable = 1
+        for f in (de.front_face, de.back_face):
+            f.stencil_func = A.CMP_EQUAL
+        self.dss_test = r.create_depth_stencil_state(de)
+        self.pcf = np.array([0.3, 0.002, 1.0 / shadow_size, 0.0], np.float32)
+        self.shadow_size = shadow_size
+
+    def _bind_geometry(self, vb, wvp, light):
+        r = self.r
+        r.set_input_layout(self.layout)
+        r.set_vertex_buffers(0, [vb], [28])
+        r.set_primitive_topology(A.TOPOLOGY_TRIANGLE_LIST)
+        r.set_vertex_shader(A.VS_WVP_COLOR_LIGHT, np.concatenate([wvp.ravel(), light.ravel()]).astype(np.float32))
+        r.set_rasterizer_state(self.rs)
+
+    def render(self, clear=True, resolve=True):
+        r = self.r
+        # (a) shadow map, depth only, from the light
+        r.set_render_targets([], self.shadow)
+        r.set_viewport(0, 0, self.shadow_size, self.shadow_size, 0.0, 1.0)
+        r.clear_depth_stencil(self.shadow, A.CLEAR_DEPTH | A.CLEAR_STENCIL, 1.0, 0)
+        self._bind_geometry(self.vb, self.L, self.L)
+        r.set_pixel_shader(A.PS_GOURAUD)
+        r.set_blend_shader(A.BLEND_OFF)
+        r.set_depth_stencil_state(self.dss_normal, 0)
+        r.draw(0, self.n_opaque)
+        # (b) shaded opaque, then translucent back-to-front
+        r.set_render_targets([self.color], self.ds)
+        r.set_viewport(0, 0, self.width, self.height, 0.0, 1.0)
+        if clear:
+            r.clear_color(self.color, (0.0, 0.0, 0.0, 1.0))
+            r.clear_depth_stencil(self.ds, A.CLEAR_DEPTH | A.CLEAR_STENCIL, 1.0, 0)
+        self._bind_geometry(self.vb, self.I, self.L)
+        r.set_pixel_shader(A.PS_SHADOW_PCF, self.pcf)
+        r.set_ps_sampler(0, self.samp)
+        r.draw(0, self.n_opaque)
+        r.set_blend_shader(A.BLEND_ON)
+        r.set_depth_stencil_state(self.dss_translucent, 0)
+        r.draw(self.n_opaque * 3, self.n_trans)
+        # (c) mirror quad: stencil 1, colour writes disabled
+        self._bind_geometry(self.mvb, self.I, self.L)
+        r.set_pixel_shader(A.PS_CONSTANT, np.array([1, 1, 1, 1], np.float32))
+        r.set_blend_shader(A.BLEND_DISABLED)
+        r.set_depth_stencil_state(self.dss_mark, 1)
+        r.draw(0, 2)
+        # (d) the opaque triangles reflected in the mirror plane x = 0.55, where stencil == 1
+        self._bind_geometry(self.vb, self.M, self.L)
+        r.set_pixel_shader(A.PS_SHADOW_PCF, self.pcf)
+        r.set_blend_shader(A.BLEND_OFF)
+        r.set_depth_stencil_state(self.dss_test, 1)
+        r.draw(0, self.n_opaque)
+        if resolve and self.resolved is not None:
+            r.resolve(self.color, self.resolved)

@@ -90,3 +90,42 @@ def test_textured_mesh_small(hip_lib, cpu_lib):
 def test_textured_mesh_full_config(hip_lib, cpu_lib):
     """1920x1080 4xMSAA, 522,242 triangles, 2048^2 texture, 16x anisotropic."""
     compare(render(hip_lib, S.setup_mesh), render(cpu_lib, S.setup_mesh), color_exact=False)
+
+
+def render_multipass(lib, stats, **kw):
+    r = lib.renderer()
+    f = S.MultipassFrame(r, **kw)
+    if stats:
+        q = r.create_query(A.QUERY_PIPELINE_STATISTICS)
+        qi = r.create_query(A.QUERY_INTERNAL_STATISTICS)
+        r.begin(q)
+        r.begin(qi)
+    f.render()
+    out = {}
+    if stats:
+        r.end(q)
+        r.end(qi)
+        st, sti = r.get_data(q)[1], r.get_data(qi)[1]
+        out.update(ps_invocations=st.ps_invocations, cprimitives=st.cprimitives,
+                   backend_input_pixels=sti.backend_input_pixels)
+    r.flush()
+    out.update(color=f.color.read().copy(), ds=f.ds.read().copy(), shadow=f.shadow.read().copy(),
+               resolved=f.resolved.read().copy())
+    return out
+
+
+@pytest.mark.parametrize("kw", [dict(width=320, height=180, samples=4, shadow_size=256, n_tris=3000),
+                                dict(width=640, height=360, samples=2, shadow_size=512, n_tris=20000)],
+                         ids=["small", "medium"])
+def test_multipass_shadow_blend_stencil_mirror(hip_lib, cpu_lib, kw):
+    """Shadow map (depth only, split tiles) -> PCF-shaded opaque pass -> translucent blend pass ->
+    stencil mirror mark -> mirrored redraw, bit-exact on both GPU paths."""
+    want = render_multipass(cpu_lib, True, **kw)
+    for stats in (False, True):
+        got = render_multipass(hip_lib, stats, **kw)
+        for k in ("shadow", "ds", "color", "resolved"):
+            d = int((got[k] != want[k]).sum())
+            assert d == 0, f"stats={stats}: {k} differs in {d} bytes"
+        if stats:
+            for k in ("ps_invocations", "cprimitives", "backend_input_pixels"):
+                assert got[k] == want[k], (k, got[k], want[k])
