@@ -1000,7 +1000,7 @@ __device__ __forceinline__ float4 shadow_pcf(const draw_params& P, const ps_samp
     for (int ty = -1; ty <= 1; ++ty)
       for (int tx = -1; tx <= 1; ++tx) {
         const float u = cu + static_cast<float>(tx) * o, v = cv + static_cast<float>(ty) * o;
-        const float4 t = sample_impl(SM->s[0], u, v, 0.0f, nullptr);
+        const float4 t = sample_lod0(SM->s[0], u, v);
         if (lz - P.ps_consts[1] <= t.x) ++count;
       }
   }
@@ -1403,8 +1403,11 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, const ps_sam
             load_attr<NA>(P, tri_attr, id, at);
             float4 q[4][NA + 1];
             interp_quad<NA>(at, px0, py0, P, q);
+            // colours only for the pixels that store one (interpolants, not neighbours' colours,
+            // supply the shaders' derivatives)
             float4 col[4];
-            run_ps<NA>(P, SM, q, 15u, col);
+            run_ps<NA>(P, SM, q, (tested[0] ? 1u : 0u) | (tested[1] ? 2u : 0u) | (tested[2] ? 4u : 0u) | (tested[3] ? 8u : 0u),
+                       col);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -1412,16 +1415,13 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, const ps_sam
                 if (tested[i] & (1u << s)) blend_store(P, px0 + (i & 1), py0 + (i >> 1), s, col[i]);
           }
         } else if constexpr (!VIS) {
-          // late depth/stencil (render_sample, framebuffer.cpp:445-479); writes only when both pass
+          // late depth/stencil (render_sample, framebuffer.cpp:445-479); writes only when both pass.
+          // The tests of a sample depend only on that sample's depth/stencil, not on the colour, so
+          // they run first and the pixel shader (side-effect free) runs only when some sample passes.
           ps_inv += 4;
           be_pix += 4;
-          attr_setup<NA> at;
-          load_attr<NA>(P, tri_attr, id, at);
-          float4 q[4][NA + 1];
-          interp_quad<NA>(at, px0, py0, P, q);
-          float4 col[4];
-          run_ps<NA>(P, SM, q, 15u, col);
           const int face = L.front ? 0 : 1;
+          uint32_t passed[4] = {0, 0, 0, 0};
 #pragma unroll
           for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -1432,12 +1432,27 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, const ps_sam
               uint32_t os = P.stencil_enable ? (stencil[i][s] & P.read_mask) : 0u;
               bool dp = has_ds ? (P.depth_enable ? cmpf(P.depth_func, nd, od) : true) : true;
               bool sp = P.stencil_enable ? cmpu(P.st_func[face], P.stencil_ref, os) : true;
-              if (dp && sp) {
-                uint32_t ns = P.stencil_enable ? stencil_op(P.st_pass_op[face], P.stencil_ref, os) : 0u;
-                blend_store(P, px0 + (i & 1), py0 + (i >> 1), s, col[i]);
-                if (P.write_depth) depth[i][s] = nd;
-                if (P.stencil_enable) stencil[i][s] = ns & P.write_mask;
-              }
+              if (dp && sp) passed[i] |= 1u << s;
+            }
+          if ((passed[0] | passed[1] | passed[2] | passed[3]) == 0) continue;
+          attr_setup<NA> at;
+          load_attr<NA>(P, tri_attr, id, at);
+          float4 q[4][NA + 1];
+          interp_quad<NA>(at, px0, py0, P, q);
+          float4 col[4];
+          run_ps<NA>(P, SM, q, (passed[0] ? 1u : 0u) | (passed[1] ? 2u : 0u) | (passed[2] ? 4u : 0u) | (passed[3] ? 8u : 0u),
+                     col);
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+              if (!(passed[i] & (1u << s))) continue;
+              float nd = (S > 1) ? (z[i] + L.aa[s]) : z[i];
+              uint32_t os = P.stencil_enable ? (stencil[i][s] & P.read_mask) : 0u;
+              uint32_t ns = P.stencil_enable ? stencil_op(P.st_pass_op[face], P.stencil_ref, os) : 0u;
+              blend_store(P, px0 + (i & 1), py0 + (i >> 1), s, col[i]);
+              if (P.write_depth) depth[i][s] = nd;
+              if (P.stencil_enable) stencil[i][s] = ns & P.write_mask;
             }
         }
       }

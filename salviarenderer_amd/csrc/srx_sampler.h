@@ -455,6 +455,13 @@ __device__ __noinline__ float4 sample_impl(const sampler_desc_dev& p, float u, f
   return make_float4(cr * inv, cg * inv, cb * inv, ca * inv);
 }
 
+// sample_impl at lod 0 (tex2dlod with a zero lod): a point mip filter magnifies there (0 < 0.5), and a
+// point mag filter is one texel of the largest level -- inline; anything else takes sample_impl.
+__device__ __forceinline__ float4 sample_lod0(const sampler_desc_dev& p, float u, float v) {
+  if (p.mip_filter == FILTER_POINT && p.mag_filter == FILTER_POINT) return filter_point(p.levels[p.max_lod], p, u, v);
+  return sample_impl(p, u, v, 0.0f, nullptr);
+}
+
 // sample_2d_grad for a sampler with fast_aniso set: the same decisions as sample_2d_grad and
 // sample_impl (magnification below lod 0, else the probes on the rounded level), inline and compact.
 __device__ __forceinline__ float4 sample_2d_grad_fast(const sampler_desc_dev& p, float u, float v, float dxu, float dxv,
