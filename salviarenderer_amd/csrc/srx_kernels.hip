@@ -349,6 +349,58 @@ __device__ __forceinline__ void for_each_gtile(const draw_params& P, const float
     }
 }
 
+// The number of GPU tiles for_each_gtile visits, with each 64x64 tile's GPU tiles counted as
+// (rows in the bounding box) x (columns in the bounding box) instead of one by one.
+__device__ __forceinline__ uint32_t count_gtiles(const draw_params& P, const float* e, const float* bb) {
+  int tx = P.tiles_x, ty = P.tiles_y;
+  int sx = min(fast_floori(std_max(0.0f, bb[0]) / TILE), tx);
+  int sy = min(fast_floori(std_max(0.0f, bb[2]) / TILE), ty);
+  int ex = min(fast_ceili(std_max(0.0f, bb[1]) / TILE) + 1, tx);
+  int ey = min(fast_ceili(std_max(0.0f, bb[3]) / TILE) + 1, ty);
+  bool small = (sx + 1 == ex) && (sy + 1 == ey);
+  bool mx[3], my[3];
+  float r2a[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    mx[k] = e[k * 3 + 0] > 0;
+    my[k] = e[k * 3 + 1] > 0;
+    float stx = TILE * e[k * 3 + 0], sty = TILE * e[k * 3 + 1];
+    r2a[k] = -fabsf(stx) - fabsf(sty);
+  }
+  const int sub = TILE / P.gt;
+  uint32_t n = 0;
+  for (int y = sy; y < ey; ++y) {
+    const float ymin = bb[2] - static_cast<float>(y * TILE), ymax = bb[3] - static_cast<float>(y * TILE);
+    for (int x = sx; x < ex; ++x) {
+      bool full = false;
+      if (!small) {
+        bool rej = false, acc = true;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          float ev = e[k * 3 + 2] - ((static_cast<float>(x + mx[k]) * TILE) * e[k * 3 + 0] +
+                                     (static_cast<float>(y + my[k]) * TILE) * e[k * 3 + 1]);
+          rej |= (0 < ev);
+          acc &= (r2a[k] >= ev);
+        }
+        if (rej) continue;
+        full = acc;
+      }
+      const float xmin = bb[0] - static_cast<float>(x * TILE), xmax = bb[1] - static_cast<float>(x * TILE);
+      uint32_t rows = 0, cols = 0;
+      for (int j = 0; j < sub; ++j) {
+        if (y * sub + j >= P.gtiles_y) break;
+        if (full || !(ymin >= static_cast<float>(j * P.gt + P.gt) || ymax < static_cast<float>(j * P.gt))) ++rows;
+      }
+      for (int i = 0; i < sub; ++i) {
+        if (x * sub + i >= P.gtiles_x) break;
+        if (full || !(xmin >= static_cast<float>(i * P.gt + P.gt) || xmax < static_cast<float>(i * P.gt))) ++cols;
+      }
+      n += rows * cols;
+    }
+  }
+  return n;
+}
+
 template <int NA>
 __device__ __forceinline__ void write_tri_rec(float4* rec, const tri_setup<NA>& t) {
   rec[0] = make_float4(t.e[0][0], t.e[0][1], t.e[0][2], t.e[1][0]);
@@ -448,9 +500,7 @@ __global__ void __launch_bounds__(256) setup_kernel(draw_params P, const float4*
           for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) e[i * 3 + j] = t.e[i][j];
           for (int i = 0; i < 4; ++i) bb[i] = t.bb[i];
-          uint32_t n = 0;
-          for_each_gtile(P, e, bb, [&](uint32_t, bool) { ++n; });
-          cnt[3 * p] = n;
+          cnt[3 * p] = count_gtiles(P, e, bb);
         }
       }
     }
@@ -543,9 +593,7 @@ __device__ void clip_primitive(const draw_params& P, uint32_t p, const float4* _
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) e[i * 3 + j] = ts.e[i][j];
     for (int i = 0; i < 4; ++i) bb[i] = ts.bb[i];
-    uint32_t m = 0;
-    for_each_gtile(P, e, bb, [&](uint32_t, bool) { ++m; });
-    cnt[id] = m;
+    cnt[id] = count_gtiles(P, e, bb);
   }
 }
 
@@ -772,21 +820,22 @@ int launch_sort(const draw_buffers& B, uint32_t n, int bits, hipStream_t s) {
   return cur;
 }
 
-// [ts, te) of every GPU tile in the sorted records; the thread at the last record of a tile (or
-// before the first record) also writes the empty ranges of the tiles up to the next record's tile,
-// so every tile is written exactly once without a separate clear.
+// [ts, te) of every GPU tile in the sorted records.  With T > 0 the thread at the last record of a
+// tile (or before the first record) also writes the empty ranges of the tiles up to the next
+// record's tile, so every tile is written exactly once without a separate clear (dense records).
 __global__ void tile_range_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t T, uint32_t* __restrict__ ts,
                                   uint32_t* __restrict__ te) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  // T == 0: the ranges were cleared beforehand, only the non-empty tiles are written
   uint32_t k = keys[i];
-  if (i == 0) {
+  if (i == 0 && T) {
     for (uint32_t t = 0; t < k; ++t) ts[t] = te[t] = 0;
   }
   if (i == 0 || keys[i - 1] != k) ts[k] = i;
   if (i == n - 1 || keys[i + 1] != k) {
     te[k] = i + 1;
-    const uint32_t next = (i == n - 1) ? T : keys[i + 1];
+    const uint32_t next = !T ? k + 1 : ((i == n - 1) ? T : keys[i + 1]);
     for (uint32_t t = k + 1; t < next; ++t) ts[t] = te[t] = i + 1;
   }
 }
@@ -884,9 +933,10 @@ void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s) {
 void launch_tile_ranges(const draw_params& P, const draw_buffers& B, int which, uint32_t n, hipStream_t s) {
   uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
   if (!T) return;
-  if (!n) {  // no records: every tile is empty
+  if (n < 4 * T) {  // sparse records: clear every range first, so no thread walks a long empty run
     hipLaunchKernelGGL(fill_u32_kernel, dim3((T + 255) / 256), dim3(256), 0, s, B.tstart, T, 0u);
     hipLaunchKernelGGL(fill_u32_kernel, dim3((T + 255) / 256), dim3(256), 0, s, B.tend, T, 0u);
+    if (n) hipLaunchKernelGGL(tile_range_kernel, dim3((n + 255) / 256), dim3(256), 0, s, B.keys[which], n, 0u, B.tstart, B.tend);
     return;
   }
   hipLaunchKernelGGL(tile_range_kernel, dim3((n + 255) / 256), dim3(256), 0, s, B.keys[which], n, T, B.tstart, B.tend);
@@ -1228,6 +1278,23 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, const ps_sam
         }
       }
     }
+  if constexpr (!VIS) {
+    // Stencil changes only where the stencil test passes, so a tile none of whose samples passes it
+    // now can never pass it during this draw: nothing in the tile changes (exact early-out, taken
+    // only when no statistics are counted).
+    if (P.stencil_enable && !P.count_stats) {
+      int can = 0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+          if (pin[i]) {
+            const uint32_t os = stencil[i][s] & P.read_mask;
+            can |= cmpu(P.st_func[0], P.stencil_ref, os) || cmpu(P.st_func[1], P.stencil_ref, os);
+          }
+      if (!__syncthreads_or(can)) return;
+    }
+  }
   const bool any_in = pin[0];
   unsigned long long ps_inv = 0, be_pix = 0;
   const uint32_t full_px = (S == 1) ? 1u : (S == 2 ? 3u : 15u);
@@ -1490,7 +1557,7 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, const ps_sam
   // ps_invocations and backend_input_pixels advance together (+4 per shaded quad: the sample shaders
   // never discard), so one partial carries both
   (void)be_pix;
-  block_partial(partial, ps_inv);
+  if (P.count_stats) block_partial(partial, ps_inv);  // uniform over the launch
 }
 
 // ---------------------------------------------------------------- order-independent visibility
@@ -2001,8 +2068,10 @@ void launch_raster(const draw_params& P, const draw_buffers& B, int which, hipSt
   SRX_NA_DISPATCH(P.nattr, SRX_RASTER)
 #undef SRX_RASTER
 #undef SRX_RASTER_S
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, B.partial, T, B.stats, static_cast<int>(ST_PS_INVOCATIONS));
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, B.partial, T, B.stats, static_cast<int>(ST_BACKEND_PIXELS));
+  if (P.count_stats) {
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, B.partial, T, B.stats, static_cast<int>(ST_PS_INVOCATIONS));
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, B.partial, T, B.stats, static_cast<int>(ST_BACKEND_PIXELS));
+  }
 }
 
 // Visibility mode of an opaque early-Z draw for vis_kernel, or -1 when only the in-order kernel
