@@ -78,6 +78,7 @@ struct draw_params {
   int32_t st_func[2], st_pass_op[2];  // [0] front face, [1] back face
   int32_t early_z, blend_id;
   int32_t count_stats;  // a statistics query is active: accumulate the device-side counters
+  int32_t vis_stencil;  // stencil on with KEEP on pass for both faces: the stencil test is a fixed mask
   // pixel shader
   int32_t ps_id;
   float ps_consts[16];

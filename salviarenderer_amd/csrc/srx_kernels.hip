@@ -1724,9 +1724,20 @@ __global__ void __launch_bounds__(SRX_VIS_THREADS, 4) vis_kernel(draw_params P, 
     for (int i = tid; i < 256 * S; i += NT) {
       int px = ox + ((i / S) & 15), py = oy + ((i / S) >> 4), smp = i % S;
       float d0 = 0.0f;
-      if (has_ds && px < tw && py < th) d0 = *reinterpret_cast<const float*>(sample_addr(P.ds, px, py, smp));
+      bool blocked = false;
+      if (has_ds && px < tw && py < th) {
+        const float2 ds = *reinterpret_cast<const float2*>(sample_addr(P.ds, px, py, smp));
+        d0 = ds.x;
+        if (P.vis_stencil) {
+          // framebuffer.cpp:445-479 (both faces share the function here); a failing sample takes
+          // no colour and no depth
+          const uint32_t os = __float_as_uint(ds.y) & P.read_mask;
+          blocked = !cmpu(P.st_func[1], P.stencil_ref, os);
+        }
+      }
       d0s[i] = d0;
-      keys[i] = initial_key(mode, d0);
+      // a blocked sample's key is below every candidate's and decodes as no winner
+      keys[i] = blocked ? ((mode == VIS_LEQUAL || mode == VIS_GEQUAL) ? 0xFFFFFFFFull : 0ull) : initial_key(mode, d0);
     }
   }
   // this workgroup's records: the whole tile, or one part of it (none when merging)
@@ -2077,6 +2088,9 @@ void launch_raster(const draw_params& P, const draw_buffers& B, int which, hipSt
 // Visibility mode of an opaque early-Z draw for vis_kernel, or -1 when only the in-order kernel
 // reproduces the reference (NOT_EQUAL with depth writes).
 int vis_mode(const draw_params& P) {
+  if (P.vis_stencil && !(P.depth_enable && P.write_depth && (P.depth_func == 1 || P.depth_func == 3 ||
+                                                            P.depth_func == 4 || P.depth_func == 6)))
+    return -1;  // a stencil mask is expressed through the depth keys only
   if (!P.depth_enable) return VIS_LATEST;
   if (P.depth_func == 0) return VIS_NONE;
   if (!P.write_depth) return VIS_LATEST;
@@ -2136,7 +2150,7 @@ bool launch_vis(const draw_params& P, const draw_buffers& B, int which, bool sha
 }
 
 bool needs_shading(const draw_params& P) {
-  return P.early_z && P.blend_id == BLEND_OFF && P.color.data != nullptr;
+  return (P.early_z || P.vis_stencil) && P.blend_id == BLEND_OFF && P.color.data != nullptr;
 }
 
 // Covers exactly the 16x16 GPU tiles vis_kernel wrote: pixels of the target beyond the viewport's

@@ -538,7 +538,12 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   RCHK(ensure(r, r->vbuf, (static_cast<size_t>(P.nverts) + 1) * (nout + 1)));
   RCHK(ensure(r, r->tri_rec, static_cast<size_t>(P3) * 5));
   RCHK(ensure(r, r->tri_attr, static_cast<size_t>(P3) * 3 * (nout + 1)));
-  bool vis_pass = P.early_z && (P.blend_id == SRX_BLEND_OFF || P.blend_id == SRX_BLEND_DISABLED);
+  // Opaque draws whose samples keep or lose a colour in one depth reduction: early-Z draws, and
+  // draws whose stencil never changes (KEEP on pass; fail ops never run), whose stencil test is
+  // then a fixed per-sample mask.
+  P.vis_stencil = P.stencil_enable && P.st_pass_op[0] == SRX_SOP_KEEP && P.st_pass_op[1] == SRX_SOP_KEEP &&
+                  P.st_func[0] == P.st_func[1];
+  bool vis_pass = (P.early_z || P.vis_stencil) && (P.blend_id == SRX_BLEND_OFF || P.blend_id == SRX_BLEND_DISABLED);
   if (vis_pass)
     RCHK(ensure(r, r->vis, static_cast<size_t>(std::max(P.target_w, 1.0f)) * static_cast<size_t>(std::max(P.target_h, 1.0f)) * samples));
   RCHK(ensure(r, r->samplers_d, 1));
@@ -651,7 +656,8 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   const bool fused = vis_pass && !counting && launch_vis(P, B, which, needs_shading(P), total, s);
   if (!fused) launch_raster(P, B, which, s);
   mark(5);
-  if (!fused && vis_pass && needs_shading(P)) launch_shade(P, B, s);
+  // the in-order kernel defers shading (to shade_kernel) only for early-Z draws
+  if (!fused && vis_pass && P.early_z && needs_shading(P)) launch_shade(P, B, s);
   mark(6);
   if (prof) r->pending_prof.push_back(de);
   HIPCHK(hipGetLastError());

@@ -221,6 +221,18 @@ def _stencil_cases():
             Draw(mp, mc, blend=A.BLEND_DISABLED, depth_write=False, depth_func=A.CMP_ALWAYS,
                  stencil={"func": A.CMP_ALWAYS, "pass_op": A.SOP_REPLACE}, stencil_ref=1),
             Draw(p, c, stencil={"func": A.CMP_EQUAL}, stencil_ref=1)], samples=samples))
+    # stencil tests with KEEP: the order-independent path with a per-sample stencil mask
+    for f, fn, smp, cd in ((A.CMP_LESS_EQUAL, "lequal", 1, 1.0), (A.CMP_GREATER_EQUAL, "gequal", 4, 0.0),
+                           (A.CMP_GREATER, "greater", 2, 0.0)):
+        pk = np.array([(-1, -1, 0.5, 1), (1.2, -1, 0.5, 1), (-1, 1.2, 0.5, 1),
+                       (-0.3, -0.6, 0.5, 1), (0.9, 0.9, 0.5, 1), (-0.8, 0.7, 0.5, 1)], np.float32)
+        ck = np.ones((6, 4), np.float32)
+        p2, c2 = tris(200, seed=25, flat_z=(0.3, 0.5, 0.7))
+        out.append(Case(f"stencil_mask_{fn}_msaa{smp}", [
+            Draw(pk, ck, blend=A.BLEND_DISABLED, depth_write=False, depth_func=A.CMP_ALWAYS,
+                 stencil={"func": A.CMP_ALWAYS, "pass_op": A.SOP_INCR_SAT}),
+            Draw(p2, c2, depth_func=f, stencil={"func": A.CMP_NOT_EQUAL, "read_mask": 0xFF}, stencil_ref=0)],
+            samples=smp, clear_depth=cd))
     ops = {A.SOP_KEEP: "keep", A.SOP_ZERO: "zero", A.SOP_REPLACE: "replace", A.SOP_INCR_SAT: "incr_sat",
            A.SOP_DECR_SAT: "decr_sat", A.SOP_INVERT: "invert", A.SOP_INCR_WRAP: "incr_wrap",
            A.SOP_DECR_WRAP: "decr_wrap"}
