@@ -93,6 +93,30 @@ def run_frames(lib, steps, warmup, profile=False):
     return total, np.array(lat), prof, f.n_tris
 
 
+def multipass_frames(lib, steps, warmup):
+    """Secondary workload (not the headline): shadow map + PCF-shaded, translucent and stencil-mirror
+    passes over 200,000 random triangles at 1920x1080 4xMSAA (scenes.MultipassFrame)."""
+    r = lib.renderer(A.RENDERER_BENCHMARK)
+    f = scenes.MultipassFrame(r)
+    for _ in range(warmup):
+        f.render()
+        r.flush()
+    q = r.create_query(A.QUERY_PIPELINE_PROFILES)
+    r.begin(q)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f.render()
+        r.flush()
+    total = time.perf_counter() - t0
+    r.end(q)
+    prof = list(r.get_data(q)[1])
+    return {"workload": "multipass 1920x1080 4xMSAA: 2048^2 shadow map, 140k opaque PCF-shaded + 60k translucent "
+                        "triangles, stencil mirror mark and mirrored redraw (5 draws)",
+            "ms_per_frame": total / steps * 1e3, "frames_per_s": steps / total, "frames": steps,
+            "stage_ms_per_frame": {"setup_clip": prof[2] / steps / 1e6, "bin_sort": prof[5] / steps / 1e6,
+                                   "raster": prof[6] / steps / 1e6}}
+
+
 def cpu_baseline(seconds):
     oracle = os.path.join(ROOT, "oracle", "libsrx_cpu.so")
     if not os.path.exists(oracle):
@@ -122,6 +146,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-multipass", action="store_true", help="skip the secondary multipass measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel used to calibrate HBM byte counters, then exit")
     args = ap.parse_args()
@@ -167,6 +192,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": RASTER_KERNEL, "algorithmic_bytes": RASTER_BYTES, "kernel_ms": vis_ns / 1e6},
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(args.cpu_seconds),
+        "secondary": None if args.no_multipass else {"multipass": multipass_frames(lib, 20, 3)},
     }
     print(json.dumps(out), flush=True)
 
