@@ -49,7 +49,7 @@ struct draw_buffers {
   unsigned long long* partial;  // per-workgroup statistics partials (>= max(setup blocks, GPU tiles))
 };
 
-void upload_ewa_table(const float* table256, hipStream_t s);
+hipError_t upload_ewa_table(const float* table256, hipStream_t s);
 
 void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
 void launch_vs(const draw_params& P, float4* vbuf, hipStream_t s);
@@ -65,6 +65,12 @@ int launch_sort(const draw_buffers& B, uint32_t n, int bits, hipStream_t s);
 // Sorts the GPU tiles by record count, heaviest first, into O.vals[result]; O supplies keys/vals
 // scratch (>= tiles each) and shares hist/scan_tmp with B.
 void launch_tile_order(const draw_params& P, const draw_buffers& B, uint32_t* order, hipStream_t s);
+// One-pass (unstable within a tile) bucket sort of the n emitted records by GPU tile into vals[1],
+// with the tile ranges; false when the tile count exceeds its LDS histogram (then radix-sort).
+bool launch_bucket_sort(const draw_params& P, const draw_buffers& B, uint32_t n, hipStream_t s);
+uint32_t bucket_hist_size(uint32_t T, uint32_t n);
+// Whether launch_vis will take the draw (its visibility mode exists and the tiles are 16x16).
+bool vis_applicable(const draw_params& P);
 void launch_tile_ranges(const draw_params& P, const draw_buffers& B, int which, uint32_t n, hipStream_t s);
 void launch_raster(const draw_params& P, const draw_buffers& B, int which, hipStream_t s);
 // Order-independent visibility for opaque early-Z draws without statistics; false if not applicable.

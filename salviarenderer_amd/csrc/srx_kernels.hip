@@ -823,6 +823,89 @@ int launch_sort(const draw_buffers& B, uint32_t n, int bits, hipStream_t s) {
 // [ts, te) of every GPU tile in the sorted records.  With T > 0 the thread at the last record of a
 // tile (or before the first record) also writes the empty ranges of the tiles up to the next
 // record's tile, so every tile is written exactly once without a separate clear (dense records).
+// ---------------------------------------------------------------- one-pass bucket sort by GPU tile
+// For the order-independent path the order of a tile's records is immaterial, so one counting pass
+// replaces the stable radix passes: per-workgroup tile histograms in LDS, one scan of the
+// tile-major histogram, and a scatter whose in-workgroup ranks come from LDS atomics.
+constexpr int BUCKET_PER_THREAD = 16;
+constexpr int BUCKET_ITEMS = 256 * BUCKET_PER_THREAD;  // records per workgroup
+constexpr uint32_t BUCKET_MAX_TILES = 8192;
+
+__global__ void __launch_bounds__(256) bucket_hist_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t T,
+                                                          uint32_t nblocks, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t cnt[BUCKET_MAX_TILES];
+  for (uint32_t d = threadIdx.x; d < T; d += 256) cnt[d] = 0;
+  __syncthreads();
+  const uint32_t b0 = blockIdx.x * BUCKET_ITEMS;
+  uint32_t k[BUCKET_PER_THREAD];
+#pragma unroll
+  for (int j = 0; j < BUCKET_PER_THREAD; ++j) {
+    const uint32_t i = b0 + j * 256 + threadIdx.x;
+    k[j] = i < n ? keys[i] : 0xFFFFFFFFu;
+  }
+#pragma unroll
+  for (int j = 0; j < BUCKET_PER_THREAD; ++j)
+    if (k[j] != 0xFFFFFFFFu) atomicAdd(&cnt[k[j]], 1u);
+  __syncthreads();
+  for (uint32_t d = threadIdx.x; d < T; d += 256) hist[d * nblocks + blockIdx.x] = cnt[d];
+}
+
+// Only the tiles present in the workgroup's records fetch their (strided) offsets.
+__global__ void __launch_bounds__(256) bucket_scatter_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                             uint32_t n, uint32_t T, uint32_t nblocks,
+                                                             const uint32_t* __restrict__ hist_scanned,
+                                                             uint32_t* __restrict__ vout) {
+  __shared__ uint32_t off[BUCKET_MAX_TILES];
+  for (uint32_t d = threadIdx.x; d < T; d += 256) off[d] = 0xFFFFFFFFu;
+  __syncthreads();
+  const uint32_t b0 = blockIdx.x * BUCKET_ITEMS;
+  uint32_t k[BUCKET_PER_THREAD], v[BUCKET_PER_THREAD];
+#pragma unroll
+  for (int j = 0; j < BUCKET_PER_THREAD; ++j) {
+    const uint32_t i = b0 + j * 256 + threadIdx.x;
+    k[j] = i < n ? keys[i] : 0xFFFFFFFFu;
+    v[j] = i < n ? vals[i] : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < BUCKET_PER_THREAD; ++j)
+    if (k[j] != 0xFFFFFFFFu) off[k[j]] = 0u;  // mark the tiles present
+  __syncthreads();
+  for (uint32_t d = threadIdx.x; d < T; d += 256)
+    if (off[d] == 0u) off[d] = hist_scanned[d * nblocks + blockIdx.x];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < BUCKET_PER_THREAD; ++j)
+    if (k[j] != 0xFFFFFFFFu) {
+      const uint32_t pos = atomicAdd(&off[k[j]], 1u);
+      if (pos < n) vout[pos] = v[j];
+    }
+}
+
+// tile t's records are [scanned[t * nblocks], scanned[(t + 1) * nblocks]) (n for the last tile)
+__global__ void bucket_range_kernel(const uint32_t* __restrict__ hist_scanned, uint32_t n, uint32_t T, uint32_t nblocks,
+                                    uint32_t* __restrict__ ts, uint32_t* __restrict__ te) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  ts[t] = hist_scanned[t * nblocks];
+  te[t] = (t + 1 < T) ? hist_scanned[(t + 1) * nblocks] : n;
+}
+
+uint32_t bucket_hist_size(uint32_t T, uint32_t n) {
+  return T * std::max<uint32_t>((n + BUCKET_ITEMS - 1) / BUCKET_ITEMS, 1);
+}
+
+bool launch_bucket_sort(const draw_params& P, const draw_buffers& B, uint32_t n, hipStream_t s) {
+  const uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
+  if (T == 0 || T > BUCKET_MAX_TILES) return false;
+  const uint32_t nblocks = std::max<uint32_t>((n + BUCKET_ITEMS - 1) / BUCKET_ITEMS, 1);
+  hipLaunchKernelGGL(bucket_hist_kernel, dim3(nblocks), dim3(256), 0, s, B.keys[0], n, T, nblocks, B.hist);
+  launch_exclusive_scan(B.hist, T * nblocks, B.scan_tmp, nullptr, s);
+  if (n) hipLaunchKernelGGL(bucket_scatter_kernel, dim3(nblocks), dim3(256), 0, s, B.keys[0], B.vals[0], n, T, nblocks, B.hist,
+                            B.vals[1]);
+  hipLaunchKernelGGL(bucket_range_kernel, dim3((T + 255) / 256), dim3(256), 0, s, B.hist, n, T, nblocks, B.tstart, B.tend);
+  return true;
+}
+
 __global__ void tile_range_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t T, uint32_t* __restrict__ ts,
                                   uint32_t* __restrict__ te) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2011,8 +2094,8 @@ __global__ void __launch_bounds__(256) shade_kernel(draw_params P, const ps_samp
 }
 
 // ============================================================== host launchers
-void upload_ewa_table(const float* table256, hipStream_t s) {
-  hipMemcpyToSymbolAsync(HIP_SYMBOL(c_ewa), table256, 256 * sizeof(float), 0, hipMemcpyHostToDevice, s);
+hipError_t upload_ewa_table(const float* table256, hipStream_t s) {
+  return hipMemcpyToSymbolAsync(HIP_SYMBOL(c_ewa), table256, 256 * sizeof(float), 0, hipMemcpyHostToDevice, s);
 }
 
 void launch_vs(const draw_params& P, float4* vbuf, hipStream_t s) {
@@ -2148,6 +2231,8 @@ bool launch_vis(const draw_params& P, const draw_buffers& B, int which, bool sha
 #undef SRX_VIS_LAUNCH
   return true;
 }
+
+bool vis_applicable(const draw_params& P) { return vis_mode(P) >= 0 && P.gt == 16; }
 
 bool needs_shading(const draw_params& P) {
   return (P.early_z || P.vis_stencil) && P.blend_id == BLEND_OFF && P.color.data != nullptr;

@@ -622,25 +622,30 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
     B.vals[0] = r->vals0.p;
     launch_emit(P, B, total, s);
   }
+  // statistics queries need the in-order walk (ps_invocations counts early-Z passes as they happen)
+  const bool counting = P.count_stats != 0;
+  const bool order_free = vis_pass && !counting && vis_applicable(P);  // tile lists need no primitive order
   uint32_t nblocks = (std::max(total, T) + 4095) / 4096;
+  const size_t hist_n = std::max<size_t>(static_cast<size_t>(256) * std::max<uint32_t>(nblocks, 1), bucket_hist_size(T, total));
   RCHK(ensure(r, r->keys1, std::max<uint32_t>(total, 1)));
   RCHK(ensure(r, r->vals1, std::max<uint32_t>(total, 1)));
-  RCHK(ensure(r, r->hist, static_cast<size_t>(256) * std::max<uint32_t>(nblocks, 1)));
-  RCHK(ensure(r, r->scan_tmp, std::max(scan_tmp_size(P3), scan_tmp_size(256 * std::max<uint32_t>(nblocks, 1)))));
+  RCHK(ensure(r, r->hist, hist_n));
+  RCHK(ensure(r, r->scan_tmp, std::max(scan_tmp_size(P3), scan_tmp_size(static_cast<uint32_t>(hist_n)))));
   B.keys[1] = r->keys1.p;
   B.vals[1] = r->vals1.p;
   B.hist = r->hist.p;
   B.scan_tmp = r->scan_tmp.p;
-  int bits = 1;
-  while ((1u << bits) < T) ++bits;
-  int which = launch_sort(B, total, bits, s);
-  launch_tile_ranges(P, B, which, total, s);
+  int which = 1;
+  if (!(order_free && launch_bucket_sort(P, B, total, s))) {
+    int bits = 1;
+    while ((1u << bits) < T) ++bits;
+    which = launch_sort(B, total, bits, s);
+    launch_tile_ranges(P, B, which, total, s);
+  }
   RCHK(ensure(r, r->order, std::max<uint32_t>(T, 1)));
   launch_tile_order(P, B, r->order.p, s);
   B.order = r->order.p;
   mark(4);
-  // statistics queries need the in-order walk (ps_invocations counts early-Z passes as they happen)
-  const bool counting = P.count_stats != 0;
   if (vis_pass && !counting) {
     uint32_t ne, nm, nsl;
     vis_table_bounds(T, total, &ne, &nm, &nsl);
@@ -726,8 +731,8 @@ int32_t srx_create_renderer(uint32_t kind, srx_renderer* out) {
   std::call_once(g_ewa_once, [] {
     float t[256];
     make_ewa(t);
-    upload_ewa_table(t, nullptr);
-    hipError_t e = hipDeviceSynchronize();
+    hipError_t e = upload_ewa_table(t, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) g_ewa_status = hip_fail(e, "EWA table upload");
   });
   if (g_ewa_status != SRX_OK) return g_ewa_status;
