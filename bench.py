@@ -117,6 +117,50 @@ def multipass_frames(lib, steps, warmup):
                                    "raster": prof[6] / steps / 1e6}}
 
 
+def batched_jobs(lib, launches, plain_passes=1, autotune_reps=1):
+    """Secondary workload: 256 independent jobs (scenes.batch_job_specs: 256^2..1920x1080, 1k-1M
+    triangles, plain-scene shaders, resident noise textures), issued one by one through the
+    renderer API and as one recorded batch (one launch per stage, HIP graph), both timed on the
+    wall clock around complete passes."""
+    r = lib.renderer(A.RENDERER_BENCHMARK)
+    specs = scenes.batch_job_specs()
+    t = time.perf_counter()
+    jobs = scenes.BatchedJobs(r, specs)
+    r.flush()
+    setup_s = time.perf_counter() - t
+    print(f"batched: {len(specs)} jobs resident ({setup_s:.1f} s)", file=sys.stderr, flush=True)
+    jobs.render_all()  # warm-up
+    r.flush()
+    t0 = time.perf_counter()
+    for _ in range(plain_passes):
+        jobs.render_all()
+        r.flush()
+    plain_s = (time.perf_counter() - t0) / plain_passes
+    print(f"batched: one-by-one pass {plain_s * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    b = jobs.record()
+    sweep = b.autotune(reps=autotune_reps)
+    info = b.info()
+    b.launch()
+    r.flush()
+    t0 = time.perf_counter()
+    for _ in range(launches):
+        b.launch()
+    r.flush()
+    batch_s = (time.perf_counter() - t0) / launches
+    status = b.status()
+    print(f"batched: batch launch {batch_s * 1e3:.1f} ms, status {status}", file=sys.stderr, flush=True)
+    n = len(specs)
+    return {"workload": f"{n} independent jobs: 256^2/512^2/1024^2/1920x1080 RGBA8 + depth, 1k-1M triangles "
+                        "(37.4M total), pass-through VS, Gouraud PS, depth LESS, 1 sample",
+            "frames_per_s": n / batch_s, "batch_ms": batch_s * 1e3, "launches": launches,
+            "one_by_one_frames_per_s": n / plain_s, "one_by_one_ms": plain_s * 1e3,
+            "speedup": plain_s / batch_s, "status": status,
+            "mtri_per_s": info.primitives / batch_s / 1e6,
+            "tuning": {"bin_items": info.bin_items, "vis_threads": info.vis_threads},
+            "sweep_ms": [[bi, th, round(ms, 3)] for bi, th, ms in sweep],
+            "device_bytes": info.device_bytes, "records": info.records}
+
+
 def cpu_baseline(seconds):
     oracle = os.path.join(ROOT, "oracle", "libsrx_cpu.so")
     if not os.path.exists(oracle):
@@ -147,6 +191,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-multipass", action="store_true", help="skip the secondary multipass measurement")
+    ap.add_argument("--no-batched", action="store_true", help="skip the secondary batched-jobs measurement")
+    ap.add_argument("--batched-only", action="store_true", help="only the batched-jobs measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel used to calibrate HBM byte counters, then exit")
     args = ap.parse_args()
@@ -159,6 +205,9 @@ def main():
         sys.exit(2)
 
     lib = hip_lib()
+    if args.batched_only:
+        print(json.dumps(batched_jobs(lib, 3)), flush=True)
+        return
     total, lat, prof, n_tris = run_frames(lib, args.steps, args.warmup, profile=True)
     fps = args.steps / total
     draws = max(prof[7], 1)
@@ -192,8 +241,12 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": RASTER_KERNEL, "algorithmic_bytes": RASTER_BYTES, "kernel_ms": vis_ns / 1e6},
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(args.cpu_seconds),
-        "secondary": None if args.no_multipass else {"multipass": multipass_frames(lib, 20, 3)},
+        "secondary": {},
     }
+    if not args.no_multipass:
+        out["secondary"]["multipass"] = multipass_frames(lib, 20, 3)
+    if not args.no_batched:
+        out["secondary"]["batched"] = batched_jobs(lib, 3)
     print(json.dumps(out), flush=True)
 
 

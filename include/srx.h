@@ -322,6 +322,45 @@ int32_t srx_clear_depth_stencil(srx_renderer r, srx_surface ds_target, uint32_t 
                                 uint32_t stencil);
 int32_t srx_flush(srx_renderer r);
 
+/* ---- batches of independent render jobs ----
+ * No counterpart in the reference, whose renderer executes one command at a time
+ * (render_core.cpp:30-61).  Between srx_batch_begin and srx_batch_end the renderer records
+ * srx_clear_color, srx_clear_depth_stencil (both planes), srx_draw / srx_draw_index and
+ * srx_surface_resolve instead of executing them; srx_batch_launch replays all of them as one
+ * launch per pipeline stage (every job's workgroups in one grid) from a HIP graph: clears first,
+ * then the draws, then the resolves.  Each job's output is the output of the same commands
+ * executed one by one.  Requirements, checked while recording (SRX_E_UNSUPPORTED otherwise, also
+ * returned by srx_batch_end): every draw is opaque (blend off or disabled) with an
+ * order-independent depth/stencil test (early-Z, or a stencil that keeps its value), no query is
+ * running, targets of at most 8192 16x16 tiles (e.g. 1920x1088), and no surface is a target of two
+ * draws, written after it is read or sampled while written.  Resources stay referenced by the
+ * batch; their contents may change between launches (vertex positions included: a launch whose
+ * records outgrow the prepared capacity stays in bounds and sets status bit 0). */
+typedef struct srx_batch_t* srx_batch;
+typedef struct srx_batch_info {
+  uint32_t jobs, clears, resolves;
+  uint32_t bin_items;    /* records binned per workgroup (2048, 4096, 8192) */
+  uint32_t vis_threads;  /* lanes per 16x16 tile in the visibility kernel (64, 128, 256) */
+  uint32_t vis_launches; /* visibility launches (one per shader/sample-count variant) */
+  uint64_t primitives, records, device_bytes;
+} srx_batch_info;
+typedef struct srx_batch_tuning {
+  int32_t bin_items, vis_threads;
+  float ms; /* mean time of one launch */
+} srx_batch_tuning;
+int32_t srx_batch_begin(srx_renderer r);
+/* Sizes the batch with one counting pass over its geometry (synchronous) and captures its graph. */
+int32_t srx_batch_end(srx_renderer r, srx_batch* out);
+int32_t srx_batch_launch(srx_batch b); /* queued on the renderer's stream */
+/* Waits for the renderer's stream; status bit 0: records beyond the prepared capacity were
+ * dropped, bit 1: clipping overflow.  Zero after every complete launch. */
+int32_t srx_batch_status(srx_batch b, uint32_t* status);
+int32_t srx_batch_configure(srx_batch b, int32_t bin_items, int32_t vis_threads);
+/* Times `reps` launches of every (bin_items, vis_threads) pair, reports them in results (up to
+ * capacity; *count = pairs tried) and leaves the batch configured with the fastest. */
+int32_t srx_batch_autotune(srx_batch b, uint32_t reps, srx_batch_tuning* results, size_t capacity, size_t* count);
+int32_t srx_batch_get_info(srx_batch b, srx_batch_info* info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -167,7 +167,7 @@ pool& global_pool() {
 }
 
 // ---------------------------------------------------------------- objects
-enum obj_kind { K_RENDERER = 1, K_BUFFER, K_TEXTURE, K_SURFACE, K_SAMPLER, K_LAYOUT, K_RASTER, K_DS, K_QUERY };
+enum obj_kind { K_RENDERER = 1, K_BUFFER, K_TEXTURE, K_SURFACE, K_SAMPLER, K_LAYOUT, K_RASTER, K_DS, K_QUERY, K_BATCH };
 
 struct object {
   std::atomic<int> refs{1};
@@ -522,13 +522,16 @@ struct stats_acc {
       ps_invocations{0}, backend_input_pixels{0};
 };
 
+struct batch_obj;
 struct renderer_obj : object {
   render_state st;
   std::vector<ref<query_obj>> active;
+  batch_obj* recording = nullptr;  // srx_batch_begin .. srx_batch_end
   renderer_obj() : object(K_RENDERER) {
     srx_default_raster_desc(&st.raster);
     srx_default_depth_stencil_desc(&st.ds);
   }
+  ~renderer_obj() override;
 };
 
 // ---------------------------------------------------------------- pipeline
@@ -1352,6 +1355,23 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   return SRX_OK;
 }
 
+// ---------------------------------------------------------------- batches
+// The batch contract is that every recorded command has the effect of its one-by-one execution;
+// the restatement records each command with the state it saw and replays them in order.
+struct batch_obj : object {
+  renderer_obj* r = nullptr;
+  std::vector<std::function<int32_t()>> cmds;
+  uint32_t draws = 0, clears = 0, resolves = 0;
+  uint64_t primitives = 0;
+  batch_obj() : object(K_BATCH) {}
+  ~batch_obj() override {
+    if (r) release(r);
+  }
+};
+renderer_obj::~renderer_obj() {
+  if (recording) release(recording);
+}
+
 template <class T>
 T* as(void* h, int kind) {
   auto* o = static_cast<object*>(h);
@@ -1482,8 +1502,19 @@ int32_t srx_surface_read(srx_surface s, void* d, size_t n) {
 int32_t srx_surface_resolve(srx_renderer r, srx_surface src, srx_surface dst) {
   auto* a = as<surface_obj>(src, K_SURFACE);
   auto* b = as<surface_obj>(dst, K_SURFACE);
-  if (!as<renderer_obj>(r, K_RENDERER) || !a || !b || b->samples != 1 || a->w != b->w || a->h != b->h)
+  auto* ro = as<renderer_obj>(r, K_RENDERER);
+  if (!ro || !a || !b || b->samples != 1 || a->w != b->w || a->h != b->h)
     return SRX_INVALID_PARAMETER;
+  if (ro->recording) {
+    batch_obj* bo = ro->recording;
+    ref<surface_obj> ka(a), kb(b);
+    bo->cmds.push_back([r, ka, kb]() {
+      return srx_surface_resolve(r, reinterpret_cast<srx_surface>(static_cast<object*>(ka.get())),
+                                 reinterpret_cast<srx_surface>(static_cast<object*>(kb.get())));
+    });
+    ++bo->resolves;
+    return SRX_OK;
+  }
   global_pool().run(static_cast<size_t>(a->h), 8, [&](unsigned, size_t y0, size_t y1) {
     for (size_t y = y0; y < y1; ++y)
       for (size_t x = 0; x < static_cast<size_t>(a->w); ++x) {
@@ -1715,9 +1746,23 @@ int32_t srx_get_data(srx_renderer rr, srx_query qq, void* data, size_t size, int
   else return SRX_ASYNC_ERROR;
   return SRX_ASYNC_READY;
 }
+static int32_t record_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t primcnt, int32_t basevert) {
+  batch_obj* b = r->recording;
+  render_state st = r->st;
+  b->cmds.push_back([r, st, indexed, start, primcnt, basevert]() mutable {
+    std::swap(r->st, st);
+    const int32_t rc = execute_draw(r, indexed, start, primcnt, basevert);
+    std::swap(r->st, st);
+    return rc;
+  });
+  ++b->draws;
+  b->primitives += primcnt;
+  return SRX_OK;
+}
 int32_t srx_draw(srx_renderer rr, size_t start, size_t primcnt) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);
   if (!r) return SRX_INVALID_PARAMETER;
+  if (r->recording) return record_draw(r, false, 0, static_cast<uint32_t>(primcnt), 0);
   // renderer_impl::draw: base_vertex 0; the index fetcher ignores start for non-indexed draws
   (void)start;
   return execute_draw(r, false, 0, static_cast<uint32_t>(primcnt), 0);
@@ -1725,18 +1770,42 @@ int32_t srx_draw(srx_renderer rr, size_t start, size_t primcnt) {
 int32_t srx_draw_index(srx_renderer rr, size_t start, size_t primcnt, int32_t basevert) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);
   if (!r) return SRX_INVALID_PARAMETER;
+  if (r->recording) return record_draw(r, true, static_cast<uint32_t>(start), static_cast<uint32_t>(primcnt), basevert);
   return execute_draw(r, true, static_cast<uint32_t>(start), static_cast<uint32_t>(primcnt), basevert);
 }
 int32_t srx_clear_color(srx_renderer rr, srx_surface s, const float* c) {
   auto* o = as<surface_obj>(s, K_SURFACE);
-  if (!as<renderer_obj>(rr, K_RENDERER) || !o || !c) return SRX_INVALID_PARAMETER;
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !o || !c) return SRX_INVALID_PARAMETER;
+  if (r->recording) {
+    ref<surface_obj> keep(o);
+    const float cc[4] = {c[0], c[1], c[2], c[3]};
+    r->recording->cmds.push_back([keep, cc]() {
+      keep->fill(v4(cc[0], cc[1], cc[2], cc[3]));
+      return SRX_OK;
+    });
+    ++r->recording->clears;
+    return SRX_OK;
+  }
   o->fill(v4(c[0], c[1], c[2], c[3]));
   return SRX_OK;
 }
 // render_core::clear_depth_stencil (render_core.cpp:101-111), framebuffer::clear_depth_stencil (:616-644)
 int32_t srx_clear_depth_stencil(srx_renderer rr, srx_surface s, uint32_t flags, float d, uint32_t st) {
   auto* o = as<surface_obj>(s, K_SURFACE);
-  if (!as<renderer_obj>(rr, K_RENDERER) || !o || o->pf != PF_RG32F) return SRX_INVALID_PARAMETER;
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !o || o->pf != PF_RG32F) return SRX_INVALID_PARAMETER;
+  if (r->recording) {
+    batch_obj* b = r->recording;
+    r->recording = nullptr;  // validate now, replay later
+    ref<surface_obj> keep(o);
+    b->cmds.push_back([rr, keep, flags, d, st]() {
+      return srx_clear_depth_stencil(rr, reinterpret_cast<srx_surface>(static_cast<object*>(keep.get())), flags, d, st);
+    });
+    ++b->clears;
+    r->recording = b;
+    return SRX_OK;
+  }
   if (flags == (SRX_CLEAR_DEPTH | SRX_CLEAR_STENCIL)) {
     float sf;
     std::memcpy(&sf, &st, 4);
@@ -1753,6 +1822,62 @@ int32_t srx_clear_depth_stencil(srx_renderer rr, srx_surface s, uint32_t flags, 
   return SRX_OK;
 }
 int32_t srx_flush(srx_renderer rr) { return as<renderer_obj>(rr, K_RENDERER) ? SRX_OK : SRX_INVALID_PARAMETER; }
+
+int32_t srx_batch_begin(srx_renderer rr) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || r->recording) return SRX_INVALID_PARAMETER;
+  r->recording = new batch_obj();
+  return SRX_OK;
+}
+int32_t srx_batch_end(srx_renderer rr, srx_batch* out) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !out || !r->recording) return SRX_INVALID_PARAMETER;
+  batch_obj* b = r->recording;
+  r->recording = nullptr;
+  retain(r);
+  b->r = r;
+  *out = reinterpret_cast<srx_batch>(static_cast<object*>(b));
+  return SRX_OK;
+}
+int32_t srx_batch_launch(srx_batch bb) {
+  auto* b = as<batch_obj>(bb, K_BATCH);
+  if (!b) return SRX_INVALID_PARAMETER;
+  batch_obj* rec = b->r->recording;
+  b->r->recording = nullptr;
+  int32_t rc = SRX_OK;
+  for (auto& c : b->cmds)
+    if ((rc = c()) != SRX_OK) break;
+  b->r->recording = rec;
+  return rc;
+}
+int32_t srx_batch_status(srx_batch bb, uint32_t* status) {
+  if (!as<batch_obj>(bb, K_BATCH) || !status) return SRX_INVALID_PARAMETER;
+  *status = 0;
+  return SRX_OK;
+}
+int32_t srx_batch_configure(srx_batch bb, int32_t bin_items, int32_t vis_threads) {
+  (void)bin_items;
+  (void)vis_threads;
+  return as<batch_obj>(bb, K_BATCH) ? SRX_OK : SRX_INVALID_PARAMETER;
+}
+int32_t srx_batch_autotune(srx_batch bb, uint32_t reps, srx_batch_tuning* results, size_t capacity, size_t* count) {
+  (void)reps;
+  (void)results;
+  (void)capacity;
+  if (!as<batch_obj>(bb, K_BATCH)) return SRX_INVALID_PARAMETER;
+  if (count) *count = 0;  // nothing to tune on the host
+  return SRX_OK;
+}
+int32_t srx_batch_get_info(srx_batch bb, srx_batch_info* info) {
+  auto* b = as<batch_obj>(bb, K_BATCH);
+  if (!b || !info) return SRX_INVALID_PARAMETER;
+  *info = srx_batch_info{};
+  info->jobs = b->draws;
+  info->clears = b->clears;
+  info->resolves = b->resolves;
+  info->primitives = b->primitives;
+  return SRX_OK;
+}
 
 }  // extern "C"
 

@@ -88,6 +88,16 @@ class InternalStatistics(C.Structure):
     _fields_ = [("backend_input_pixels", C.c_uint64)]
 
 
+class BatchInfo(C.Structure):
+    _fields_ = [("jobs", C.c_uint32), ("clears", C.c_uint32), ("resolves", C.c_uint32), ("bin_items", C.c_uint32),
+                ("vis_threads", C.c_uint32), ("vis_launches", C.c_uint32), ("primitives", C.c_uint64),
+                ("records", C.c_uint64), ("device_bytes", C.c_uint64)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("bin_items", C.c_int32), ("vis_threads", C.c_int32), ("ms", C.c_float)]
+
+
 _H = C.c_void_p
 _SZ = C.c_size_t
 _I32 = C.c_int32
@@ -149,6 +159,13 @@ PROTOTYPES = {
     "srx_clear_color": (_I32, [_H, _H, C.POINTER(C.c_float)]),
     "srx_clear_depth_stencil": (_I32, [_H, _H, _U32, C.c_float, _U32]),
     "srx_flush": (_I32, [_H]),
+    "srx_batch_begin": (_I32, [_H]),
+    "srx_batch_end": (_I32, [_H, C.POINTER(_H)]),
+    "srx_batch_launch": (_I32, [_H]),
+    "srx_batch_status": (_I32, [_H, C.POINTER(_U32)]),
+    "srx_batch_configure": (_I32, [_H, _I32, _I32]),
+    "srx_batch_autotune": (_I32, [_H, _U32, C.POINTER(BatchTuning), _SZ, C.POINTER(_SZ)]),
+    "srx_batch_get_info": (_I32, [_H, C.POINTER(BatchInfo)]),
 }
 
 
@@ -264,6 +281,33 @@ class Query(_Obj):
     def __init__(self, lib, handle, qid):
         super().__init__(lib, handle)
         self.id = qid
+
+
+class Batch(_Obj):
+    """Recorded independent render jobs replayed as one launch per pipeline stage."""
+
+    def launch(self):
+        self.lib.call("srx_batch_launch", self.h)
+
+    def status(self) -> int:
+        st = C.c_uint32()
+        self.lib.call("srx_batch_status", self.h, C.byref(st))
+        return st.value
+
+    def configure(self, bin_items, vis_threads):
+        self.lib.call("srx_batch_configure", self.h, bin_items, vis_threads)
+
+    def autotune(self, reps=3):
+        """Times every (bin_items, vis_threads) pair; the batch keeps the fastest."""
+        res = (BatchTuning * 16)()
+        n = C.c_size_t()
+        self.lib.call("srx_batch_autotune", self.h, reps, res, 16, C.byref(n))
+        return [(r.bin_items, r.vis_threads, r.ms) for r in res[: min(n.value, 16)]]
+
+    def info(self) -> BatchInfo:
+        d = BatchInfo()
+        self.lib.call("srx_batch_get_info", self.h, C.byref(d))
+        return d
 
 
 def default_raster_desc(lib: Lib) -> RasterDesc:
@@ -428,3 +472,12 @@ class Renderer(_Obj):
 
     def flush(self):
         self.lib.call("srx_flush", self.h)
+
+    # batches: commands between batch_begin and batch_end are recorded, not executed
+    def batch_begin(self):
+        self.lib.call("srx_batch_begin", self.h)
+
+    def batch_end(self) -> Batch:
+        h = C.c_void_p()
+        self.lib.call("srx_batch_end", self.h, C.byref(h))
+        return Batch(self.lib, h)

@@ -49,6 +49,61 @@ struct draw_buffers {
   unsigned long long* partial;  // per-workgroup statistics partials (>= max(setup blocks, GPU tiles))
 };
 
+// ---------------------------------------------------------------- batched jobs
+// Workgroup -> job map of one batched launch: job[b] for every workgroup b; range[j] = (first
+// workgroup, workgroups) of job j (no workgroups: the job has no work in this launch).
+struct stage_map {
+  const uint32_t* job;
+  const uint2* range;
+};
+// One draw of a batch: its parameters and its slices of the batch's buffers.  B.keys/B.vals are the
+// batch-wide record arrays; B.cnt is the job's slice of the batch-wide per-triangle counts (followed
+// by the next job's slice or the trailing total slot); B.hist likewise.
+struct batch_job {
+  draw_params P;
+  draw_buffers B;
+  int32_t vis_mode;
+  uint32_t tiles;          // GPU tiles
+  uint32_t bucket_blocks;  // binning workgroups; the histogram is tiles x bucket_blocks
+  uint32_t pad;
+};
+// A clear as a 16-byte pattern fill of `value` converted to format pf (texel sizes 4, 8, 16):
+// `words` 16-byte stores, then `tail` bytes (whole texels).  A depth/stencil clear is the RG32F
+// value (depth, stencil bits).
+struct fill_item {
+  uint8_t* data;
+  size_t words;
+  uint32_t tail;
+  int32_t pf;
+  float4 value;
+};
+struct resolve_item {
+  surface_desc src, dst;
+};
+
+void launch_batch_reset(uint32_t* counters, uint32_t n, uint32_t* cnt_tail, uint32_t* hist_tail, uint32_t* status,
+                        hipStream_t s);
+void launch_fill_batch(const fill_item* items, stage_map m, uint32_t blocks, hipStream_t s);
+void launch_resolve_batch(const resolve_item* items, stage_map m, uint32_t blocks, hipStream_t s);
+void launch_vs_batch(const batch_job* jobs, stage_map m, uint32_t blocks, hipStream_t s);
+void launch_setup_batch(int na, const batch_job* jobs, stage_map m, uint32_t blocks, hipStream_t s);
+void launch_clip_batch(int na, const batch_job* jobs, stage_map m, uint32_t blocks, hipStream_t s);
+void launch_emit_batch(const batch_job* jobs, stage_map m, uint32_t blocks, uint32_t capacity, hipStream_t s);
+// binning with pt records per thread (256 * pt per workgroup): phase 0 histograms, 1 scatter
+bool batch_bin_items_valid(int pt);
+void launch_bucket_batch(int pt, int phase, const batch_job* jobs, stage_map m, uint32_t blocks, uint32_t capacity,
+                         hipStream_t s);
+void launch_range_batch(const batch_job* jobs, stage_map m, uint32_t blocks, hipStream_t s);
+void launch_tile_order_batch(const batch_job* jobs, uint32_t njobs, hipStream_t s);
+// threads: 64, 128 or 256 lanes per 16x16 tile
+void launch_vis_batch(int na, int samples, bool shade, int threads, const batch_job* jobs, stage_map m, uint32_t blocks,
+                      hipStream_t s);
+void launch_batch_check(const batch_job* jobs, uint32_t njobs, uint32_t capacity, int pt, uint32_t* status, hipStream_t s);
+// out[j]: records of job j; out[njobs + j]: its clip-overflow counter
+void launch_batch_counts(const batch_job* jobs, uint32_t njobs, uint32_t* out, hipStream_t s);
+int vis_mode(const draw_params& P);
+constexpr uint32_t BATCH_MAX_TILES = 8192;  // GPU tiles of one batched draw (its LDS histogram)
+
 hipError_t upload_ewa_table(const float* table256, hipStream_t s);
 
 void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);

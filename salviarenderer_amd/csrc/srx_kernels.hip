@@ -64,8 +64,10 @@ __device__ __forceinline__ float4 fetch_elem(const input_elem_dev& e, uint32_t v
   }
 }
 
-__global__ void vs_kernel(draw_params P, float4* __restrict__ vbuf) {
-  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+// Kernel bodies take their workgroup index `bx` (and grid size where they stride) as arguments:
+// the single-draw kernels pass blockIdx.x, the batched kernels a job's local workgroup index.
+__device__ __forceinline__ void vs_body(const draw_params& P, float4* __restrict__ vbuf, uint32_t bx) {
+  uint32_t v = bx * blockDim.x + threadIdx.x;
   if (v > P.nverts) return;  // slot nverts holds the output for an all-out-of-bounds fetch
   float4 in[MAX_INPUTS];
 #pragma unroll
@@ -103,6 +105,7 @@ __global__ void vs_kernel(draw_params P, float4* __restrict__ vbuf) {
   for (int r = 0; r < 3; ++r)
     if (r <= P.nattr) vbuf[base + r] = out[r];
 }
+__global__ void vs_kernel(draw_params P, float4* __restrict__ vbuf) { vs_body(P, vbuf, blockIdx.x); }
 
 // ============================================================== primitive assembly helpers
 // Setup records of clipped triangle id = 3 * primitive + fan index live at slot
@@ -459,11 +462,11 @@ __global__ void __launch_bounds__(256) reduce_partials_kernel(const unsigned lon
 
 // ============================================================== setup (fast path)
 template <int NA>
-__global__ void __launch_bounds__(256) setup_kernel(draw_params P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
-                                                     float4* __restrict__ tri_attr,
-                                                     uint32_t* __restrict__ cnt, uint8_t* __restrict__ clip_flag,
-                                                     uint32_t* __restrict__ counters, unsigned long long* __restrict__ partial) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void setup_body(const draw_params& P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
+                                           float4* __restrict__ tri_attr, uint32_t* __restrict__ cnt,
+                                           uint8_t* __restrict__ clip_flag, uint32_t* __restrict__ counters,
+                                           unsigned long long* __restrict__ partial, uint32_t bx) {
+  uint32_t p = bx * blockDim.x + threadIdx.x;
   unsigned long long cprims = 0;
   if (p < P.prim_count) {
     cnt[3 * p + 0] = 0;
@@ -506,6 +509,13 @@ __global__ void __launch_bounds__(256) setup_kernel(draw_params P, const float4*
     }
   }
   if (P.count_stats) block_partial(partial, cprims);  // uniform over the launch
+}
+template <int NA>
+__global__ void __launch_bounds__(256) setup_kernel(draw_params P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
+                                                     float4* __restrict__ tri_attr,
+                                                     uint32_t* __restrict__ cnt, uint8_t* __restrict__ clip_flag,
+                                                     uint32_t* __restrict__ counters, unsigned long long* __restrict__ partial) {
+  setup_body<NA>(P, vbuf, tri_rec, tri_attr, cnt, clip_flag, counters, partial, blockIdx.x);
 }
 
 // ============================================================== clipping (slow path)
@@ -599,25 +609,35 @@ __device__ void clip_primitive(const draw_params& P, uint32_t p, const float4* _
 
 // Clipping launch: a bounded grid strides over the primitives setup flagged (none, usually).
 template <int NA>
+__device__ __forceinline__ void clip_body(const draw_params& P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
+                                          float4* __restrict__ tri_attr, uint32_t* __restrict__ cnt,
+                                          const uint8_t* __restrict__ clip_flag, int32_t* __restrict__ clip_slot,
+                                          float4* __restrict__ clip_verts, uint32_t* __restrict__ clip_hdr, uint32_t capacity,
+                                          uint32_t* __restrict__ counters, unsigned long long* __restrict__ stats,
+                                          uint32_t bx, uint32_t gx) {
+  if (counters[CT_CLIP_PRIMS] == 0) return;  // setup flagged nothing
+  for (uint32_t p = bx * blockDim.x + threadIdx.x; p < P.prim_count; p += gx * blockDim.x)
+    if (clip_flag[p]) clip_primitive<NA>(P, p, vbuf, tri_rec, tri_attr, cnt, clip_slot, clip_verts, clip_hdr, capacity,
+                                         counters, stats);
+}
+template <int NA>
 __global__ void clip_kernel(draw_params P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
                             float4* __restrict__ tri_attr, uint32_t* __restrict__ cnt, const uint8_t* __restrict__ clip_flag, int32_t* __restrict__ clip_slot,
                             float4* __restrict__ clip_verts, uint32_t* __restrict__ clip_hdr, uint32_t capacity,
                             uint32_t* __restrict__ counters, unsigned long long* __restrict__ stats) {
-  if (counters[CT_CLIP_PRIMS] == 0) return;  // setup flagged nothing
-  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P.prim_count; p += gridDim.x * blockDim.x)
-    if (clip_flag[p]) clip_primitive<NA>(P, p, vbuf, tri_rec, tri_attr, cnt, clip_slot, clip_verts, clip_hdr, capacity,
-                                         counters, stats);
+  clip_body<NA>(P, vbuf, tri_rec, tri_attr, cnt, clip_flag, clip_slot, clip_verts, clip_hdr, capacity, counters, stats,
+                blockIdx.x, gridDim.x);
 }
 
 // ============================================================== record emission
 // off: exclusive scan of the per-triangle record counts; a triangle owns [off[id], off[id+1]).
 // One thread per primitive, over its (at most 3) clipped triangles in fan order.  Records beyond
 // `capacity` are dropped; the host compares the total with it and emits again.
-__global__ void __launch_bounds__(256) emit_kernel(draw_params P, const float4* __restrict__ tri_rec,
-                                                   const uint32_t* __restrict__ off, const uint32_t* __restrict__ total,
-                                                   uint32_t capacity, uint32_t* __restrict__ keys,
-                                                   uint32_t* __restrict__ vals) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void emit_body(const draw_params& P, const float4* __restrict__ tri_rec,
+                                          const uint32_t* __restrict__ off, const uint32_t* __restrict__ total,
+                                          uint32_t capacity, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                          uint32_t bx) {
+  const uint32_t p = bx * blockDim.x + threadIdx.x;
   const uint32_t n = 3 * P.prim_count;
   if (p >= P.prim_count) return;
   const uint32_t id3 = 3 * p;
@@ -640,6 +660,12 @@ __global__ void __launch_bounds__(256) emit_kernel(draw_params P, const float4* 
       ++o;
     });
   }
+}
+__global__ void __launch_bounds__(256) emit_kernel(draw_params P, const float4* __restrict__ tri_rec,
+                                                   const uint32_t* __restrict__ off, const uint32_t* __restrict__ total,
+                                                   uint32_t capacity, uint32_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals) {
+  emit_body(P, tri_rec, off, total, capacity, keys, vals, blockIdx.x);
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
@@ -831,63 +857,81 @@ constexpr int BUCKET_PER_THREAD = 16;
 constexpr int BUCKET_ITEMS = 256 * BUCKET_PER_THREAD;  // records per workgroup
 constexpr uint32_t BUCKET_MAX_TILES = 8192;
 
-__global__ void __launch_bounds__(256) bucket_hist_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t T,
-                                                          uint32_t nblocks, uint32_t* __restrict__ hist) {
+// PT records per thread: 256 * PT records per workgroup (the batched path tunes PT)
+template <int PT>
+__device__ __forceinline__ void bucket_hist_body(const uint32_t* __restrict__ keys, uint32_t n, uint32_t T,
+                                                 uint32_t nblocks, uint32_t* __restrict__ hist, uint32_t bx) {
   __shared__ uint32_t cnt[BUCKET_MAX_TILES];
   for (uint32_t d = threadIdx.x; d < T; d += 256) cnt[d] = 0;
   __syncthreads();
-  const uint32_t b0 = blockIdx.x * BUCKET_ITEMS;
-  uint32_t k[BUCKET_PER_THREAD];
+  const uint32_t b0 = bx * (256 * PT);
+  uint32_t k[PT];
 #pragma unroll
-  for (int j = 0; j < BUCKET_PER_THREAD; ++j) {
+  for (int j = 0; j < PT; ++j) {
     const uint32_t i = b0 + j * 256 + threadIdx.x;
     k[j] = i < n ? keys[i] : 0xFFFFFFFFu;
   }
 #pragma unroll
-  for (int j = 0; j < BUCKET_PER_THREAD; ++j)
+  for (int j = 0; j < PT; ++j)
     if (k[j] != 0xFFFFFFFFu) atomicAdd(&cnt[k[j]], 1u);
   __syncthreads();
-  for (uint32_t d = threadIdx.x; d < T; d += 256) hist[d * nblocks + blockIdx.x] = cnt[d];
+  for (uint32_t d = threadIdx.x; d < T; d += 256) hist[d * nblocks + bx] = cnt[d];
+}
+__global__ void __launch_bounds__(256) bucket_hist_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t T,
+                                                          uint32_t nblocks, uint32_t* __restrict__ hist) {
+  bucket_hist_body<BUCKET_PER_THREAD>(keys, n, T, nblocks, hist, blockIdx.x);
 }
 
-// Only the tiles present in the workgroup's records fetch their (strided) offsets.
-__global__ void __launch_bounds__(256) bucket_scatter_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                                             uint32_t n, uint32_t T, uint32_t nblocks,
-                                                             const uint32_t* __restrict__ hist_scanned,
-                                                             uint32_t* __restrict__ vout) {
+// Only the tiles present in the workgroup's records fetch their (strided) offsets.  Positions at
+// or beyond `limit` (the output's size) are dropped.
+template <int PT>
+__device__ __forceinline__ void bucket_scatter_body(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                    uint32_t n, uint32_t T, uint32_t nblocks,
+                                                    const uint32_t* __restrict__ hist_scanned, uint32_t* __restrict__ vout,
+                                                    uint32_t limit, uint32_t bx) {
   __shared__ uint32_t off[BUCKET_MAX_TILES];
   for (uint32_t d = threadIdx.x; d < T; d += 256) off[d] = 0xFFFFFFFFu;
   __syncthreads();
-  const uint32_t b0 = blockIdx.x * BUCKET_ITEMS;
-  uint32_t k[BUCKET_PER_THREAD], v[BUCKET_PER_THREAD];
+  const uint32_t b0 = bx * (256 * PT);
+  uint32_t k[PT], v[PT];
 #pragma unroll
-  for (int j = 0; j < BUCKET_PER_THREAD; ++j) {
+  for (int j = 0; j < PT; ++j) {
     const uint32_t i = b0 + j * 256 + threadIdx.x;
     k[j] = i < n ? keys[i] : 0xFFFFFFFFu;
     v[j] = i < n ? vals[i] : 0u;
   }
 #pragma unroll
-  for (int j = 0; j < BUCKET_PER_THREAD; ++j)
+  for (int j = 0; j < PT; ++j)
     if (k[j] != 0xFFFFFFFFu) off[k[j]] = 0u;  // mark the tiles present
   __syncthreads();
   for (uint32_t d = threadIdx.x; d < T; d += 256)
-    if (off[d] == 0u) off[d] = hist_scanned[d * nblocks + blockIdx.x];
+    if (off[d] == 0u) off[d] = hist_scanned[d * nblocks + bx];
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < BUCKET_PER_THREAD; ++j)
+  for (int j = 0; j < PT; ++j)
     if (k[j] != 0xFFFFFFFFu) {
       const uint32_t pos = atomicAdd(&off[k[j]], 1u);
-      if (pos < n) vout[pos] = v[j];
+      if (pos < limit) vout[pos] = v[j];
     }
 }
+__global__ void __launch_bounds__(256) bucket_scatter_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                             uint32_t n, uint32_t T, uint32_t nblocks,
+                                                             const uint32_t* __restrict__ hist_scanned,
+                                                             uint32_t* __restrict__ vout) {
+  bucket_scatter_body<BUCKET_PER_THREAD>(keys, vals, n, T, nblocks, hist_scanned, vout, n, blockIdx.x);
+}
 
-// tile t's records are [scanned[t * nblocks], scanned[(t + 1) * nblocks]) (n for the last tile)
-__global__ void bucket_range_kernel(const uint32_t* __restrict__ hist_scanned, uint32_t n, uint32_t T, uint32_t nblocks,
-                                    uint32_t* __restrict__ ts, uint32_t* __restrict__ te) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+// tile t's records are [scanned[t * nblocks], scanned[(t + 1) * nblocks]) (`end` for the last tile)
+__device__ __forceinline__ void bucket_range_body(const uint32_t* __restrict__ hist_scanned, uint32_t end, uint32_t T,
+                                                  uint32_t nblocks, uint32_t* __restrict__ ts, uint32_t* __restrict__ te,
+                                                  uint32_t t) {
   if (t >= T) return;
   ts[t] = hist_scanned[t * nblocks];
-  te[t] = (t + 1 < T) ? hist_scanned[(t + 1) * nblocks] : n;
+  te[t] = (t + 1 < T) ? hist_scanned[(t + 1) * nblocks] : end;
+}
+__global__ void bucket_range_kernel(const uint32_t* __restrict__ hist_scanned, uint32_t n, uint32_t T, uint32_t nblocks,
+                                    uint32_t* __restrict__ ts, uint32_t* __restrict__ te) {
+  bucket_range_body(hist_scanned, n, T, nblocks, ts, te, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 uint32_t bucket_hist_size(uint32_t T, uint32_t n) {
@@ -929,8 +973,8 @@ __global__ void tile_range_kernel(const uint32_t* __restrict__ keys, uint32_t n,
 __device__ __forceinline__ uint32_t tile_bucket(const uint32_t* ts, const uint32_t* te, uint32_t t) {
   return 255u - min((te[t] - ts[t]) >> 3, 255u);
 }
-__global__ void __launch_bounds__(1024) tile_order_kernel(const uint32_t* __restrict__ ts, const uint32_t* __restrict__ te,
-                                                          uint32_t T, uint32_t* __restrict__ order) {
+__device__ __forceinline__ void tile_order_body(const uint32_t* __restrict__ ts, const uint32_t* __restrict__ te,
+                                                uint32_t T, uint32_t* __restrict__ order) {
   __shared__ uint32_t cnt[256];
   if (threadIdx.x < 256) cnt[threadIdx.x] = 0;
   __syncthreads();
@@ -946,6 +990,10 @@ __global__ void __launch_bounds__(1024) tile_order_kernel(const uint32_t* __rest
   }
   __syncthreads();
   for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) order[atomicAdd(&cnt[tile_bucket(ts, te, t)], 1u)] = t;
+}
+__global__ void __launch_bounds__(1024) tile_order_kernel(const uint32_t* __restrict__ ts, const uint32_t* __restrict__ te,
+                                                          uint32_t T, uint32_t* __restrict__ order) {
+  tile_order_body(ts, te, T, order);
 }
 
 void launch_tile_order(const draw_params& P, const draw_buffers& B, uint32_t* order, hipStream_t s) {
@@ -1743,25 +1791,28 @@ __device__ __forceinline__ void write_winner_depth(const draw_params& P, int mod
 #ifndef SRX_VIS_THREADS
 #define SRX_VIS_THREADS 256
 #endif
-template <int NA, int S, bool SHADE>
-__global__ void __launch_bounds__(SRX_VIS_THREADS, 4) vis_kernel(draw_params P, int mode, int pass,
-                                                   const uint32_t* __restrict__ order,
-                                                   const uint4* __restrict__ entries, const uint4* __restrict__ merges,
-                                                   const uint32_t* __restrict__ counts,
-                                                   unsigned long long* __restrict__ partial,
-                                                   const uint32_t* __restrict__ vals,
-                                                   const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ tend,
-                                                   const float4* __restrict__ tri_rec, const float4* __restrict__ tri_attr,
-                                                   const ps_samplers* __restrict__ SM) {
-  constexpr int NT = SRX_VIS_THREADS;
-  // LDS shared by phase 1 (a chunk of NT records + their block work list) and phase 3 (shading items)
-  constexpr int PH1_BYTES = NT * 64 + NT * 16 * 2;
-  constexpr int PH3_BYTES = SHADE ? 256 * S * 8 : 0;
-  constexpr int UNION_BYTES = PH1_BYTES > PH3_BYTES ? PH1_BYTES : PH3_BYTES;
+// LDS shared by phase 1 (a chunk of NT records + their block work list) and phase 3 (shading items)
+__host__ __device__ constexpr int vis_union_bytes(int nt, int s, bool shade) {
+  return (nt * 64 + nt * 16 * 2) > (shade ? 256 * s * 8 : 0) ? (nt * 64 + nt * 16 * 2) : 256 * s * 8;
+}
+// NTC > 0: NTC lanes per workgroup, static LDS.  NTC == 0: blockDim.x lanes (64, 128 or 256) with
+// the phase-1/3 union in dynamic LDS of vis_union_bytes(blockDim.x, S, SHADE) bytes.
+template <int NA, int S, bool SHADE, int NTC>
+__device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pass, const uint32_t* __restrict__ order,
+                                         const uint4* __restrict__ entries, const uint4* __restrict__ merges,
+                                         const uint32_t* __restrict__ counts, unsigned long long* __restrict__ partial,
+                                         const uint32_t* __restrict__ vals, const uint32_t* __restrict__ tstart,
+                                         const uint32_t* __restrict__ tend, const float4* __restrict__ tri_rec,
+                                         const float4* __restrict__ tri_attr, const ps_samplers* __restrict__ SM,
+                                         uint32_t bx) {
+  const int NT = NTC > 0 ? NTC : static_cast<int>(blockDim.x);
+  constexpr int UNION_BYTES = NTC > 0 ? vis_union_bytes(NTC, S, SHADE) : 16;
   __shared__ unsigned long long keys[256 * S];
   __shared__ float d0s[256 * S];
-  __shared__ __align__(16) unsigned char lds_union[UNION_BYTES];
-  __shared__ uint32_t wsum[NT / 64];
+  __shared__ __align__(16) unsigned char lds_static[UNION_BYTES];
+  extern __shared__ __align__(16) unsigned char lds_dynamic[];
+  unsigned char* const lds_union = NTC > 0 ? lds_static : lds_dynamic;
+  __shared__ uint32_t wsum[4];
   __shared__ uint32_t nitems;
   float4* recs = reinterpret_cast<float4*>(lds_union);                           // [NT][4]
   uint16_t* work = reinterpret_cast<uint16_t*>(lds_union + NT * 64);             // [NT * 16]
@@ -1770,17 +1821,17 @@ __global__ void __launch_bounds__(SRX_VIS_THREADS, 4) vis_kernel(draw_params P, 
   int gtile;
   uint32_t part = 0, parts = 1, slot = 0;
   if (entries == nullptr) {  // whole tiles in `order`
-    gtile = static_cast<int>(order[blockIdx.x]);
+    gtile = static_cast<int>(order[bx]);
   } else if (pass == 0) {
-    if (blockIdx.x >= counts[0]) return;
-    const uint4 en = entries[blockIdx.x];
+    if (bx >= counts[0]) return;
+    const uint4 en = entries[bx];
     gtile = static_cast<int>(en.x);
     part = en.y;
     parts = en.z;
     slot = en.w + en.y;
   } else {
-    if (blockIdx.x >= counts[1]) return;
-    const uint4 mg = merges[blockIdx.x];
+    if (bx >= counts[1]) return;
+    const uint4 mg = merges[bx];
     gtile = static_cast<int>(mg.x);
     parts = mg.y;
     slot = mg.z;
@@ -1930,7 +1981,8 @@ __global__ void __launch_bounds__(SRX_VIS_THREADS, 4) vis_kernel(draw_params P, 
     __syncthreads();
     uint32_t off = incl - cnt, total = 0;
 #pragma unroll
-    for (int w = 0; w < NT / 64; ++w) {
+    for (int w = 0; w < 4; ++w) {
+      if (w >= NT / 64) break;
       if (w < (tid >> 6)) off += wsum[w];
       total += wsum[w];
     }
@@ -2055,6 +2107,20 @@ __global__ void __launch_bounds__(SRX_VIS_THREADS, 4) vis_kernel(draw_params P, 
       if (m & (1u << smp)) store_texel(P.color.pf, sample_addr(P.color, px, py, smp), c);
   }
 }
+template <int NA, int S, bool SHADE>
+__global__ void __launch_bounds__(SRX_VIS_THREADS, 4) vis_kernel(draw_params P, int mode, int pass,
+                                                   const uint32_t* __restrict__ order,
+                                                   const uint4* __restrict__ entries, const uint4* __restrict__ merges,
+                                                   const uint32_t* __restrict__ counts,
+                                                   unsigned long long* __restrict__ partial,
+                                                   const uint32_t* __restrict__ vals,
+                                                   const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ tend,
+                                                   const float4* __restrict__ tri_rec, const float4* __restrict__ tri_attr,
+                                                   const ps_samplers* __restrict__ SM) {
+  vis_body<NA, S, SHADE, SRX_VIS_THREADS>(P, mode, pass, order, entries, merges, counts, partial, vals, tstart, tend,
+                                          tri_rec, tri_attr, SM, blockIdx.x);
+}
+
 
 // Deferred pixel shading of an opaque early-Z draw: one thread per pixel (16x16 pixels per
 // workgroup).  For each distinct winning triangle of the pixel's samples, the thread evaluates that
@@ -2345,6 +2411,243 @@ __global__ void mip_kernel(surface_desc src, surface_desc dst, int filter) {
     }
     store_texel(dst.pf, sample_addr(dst, x, y, s), c);
   }
+}
+
+// ============================================================== batched jobs
+// Every stage of a batch of independent draws is one launch: each workgroup looks up its job and
+// its index within the job's grid (stage_map) and runs the single-draw kernel body on the job's
+// parameters and buffers.  Record counts are read on the device (from the batch-wide scan of the
+// per-triangle counts), clamped to the capacities the batch was prepared with, so a replay whose
+// geometry changed stays in bounds and reports the overflow through the status word.
+__device__ __forceinline__ uint32_t stage_job(const stage_map& m, uint32_t& local, uint32_t& blocks) {
+  const uint32_t j = m.job[blockIdx.x];
+  const uint2 r = m.range[j];
+  local = blockIdx.x - r.x;
+  blocks = r.y;
+  return j;
+}
+
+__global__ void batch_reset_kernel(uint32_t* __restrict__ counters, uint32_t n, uint32_t* __restrict__ cnt_tail,
+                                   uint32_t* __restrict__ hist_tail, uint32_t* __restrict__ status) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) counters[i] = 0;
+  if (i == 0) {
+    *cnt_tail = 0;
+    *hist_tail = 0;
+    *status = 0;
+  }
+}
+
+__global__ void __launch_bounds__(256) fill_batch_kernel(const fill_item* __restrict__ items, stage_map m) {
+  uint32_t lb, nb;
+  const fill_item& it = items[stage_job(m, lb, nb)];
+  const size_t i = static_cast<size_t>(lb) * 256 + threadIdx.x;
+  const uint4 w = fill_words(it.pf, it.value);
+  if (i < it.words) reinterpret_cast<uint4*>(it.data)[i] = w;
+  if (i == 0 && it.tail) {
+    const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+    for (uint32_t b = 0; b < it.tail; b += 4) *reinterpret_cast<uint32_t*>(it.data + it.words * 16 + b) = ws[b / 4];
+  }
+}
+
+__global__ void __launch_bounds__(256) resolve_batch_kernel(const resolve_item* __restrict__ items, stage_map m) {
+  uint32_t lb, nb;
+  const resolve_item& it = items[stage_job(m, lb, nb)];
+  const surface_desc src = it.src, dst = it.dst;
+  const size_t i = static_cast<size_t>(lb) * 256 + threadIdx.x;
+  if (i >= static_cast<size_t>(src.w) * src.h) return;
+  const int x = static_cast<int>(i % src.w), y = static_cast<int>(i / src.w);
+  float4 c = make_float4(0, 0, 0, 0);
+  for (int smp = 0; smp < src.samples; ++smp) {
+    const float4 t = load_texel(src.pf, sample_addr(src, x, y, smp));
+    c = make_float4(c.x + t.x, c.y + t.y, c.z + t.z, c.w + t.w);
+  }
+  const float inv = 1 / static_cast<float>(src.samples);
+  store_texel(dst.pf, sample_addr(dst, x, y, 0), make_float4(c.x * inv, c.y * inv, c.z * inv, c.w * inv));
+}
+
+__global__ void __launch_bounds__(256) vs_batch_kernel(const batch_job* __restrict__ jobs, stage_map m) {
+  uint32_t lb, nb;
+  const batch_job& J = jobs[stage_job(m, lb, nb)];
+  vs_body(J.P, J.B.vbuf, lb);
+}
+
+template <int NA>
+__global__ void __launch_bounds__(256) setup_batch_kernel(const batch_job* __restrict__ jobs, stage_map m) {
+  uint32_t lb, nb;
+  const batch_job& J = jobs[stage_job(m, lb, nb)];
+  setup_body<NA>(J.P, J.B.vbuf, J.B.tri_rec, J.B.tri_attr, J.B.cnt, J.B.clip_flag, J.B.counters, J.B.partial, lb);
+}
+
+template <int NA>
+__global__ void clip_batch_kernel(const batch_job* __restrict__ jobs, stage_map m) {
+  uint32_t lb, nb;
+  const batch_job& J = jobs[stage_job(m, lb, nb)];
+  clip_body<NA>(J.P, J.B.vbuf, J.B.tri_rec, J.B.tri_attr, J.B.cnt, J.B.clip_flag, J.B.clip_slot, J.B.clip_verts,
+                J.B.clip_hdr, J.B.clip_capacity, J.B.counters, J.B.stats, lb, nb);
+}
+
+// The job's counts are followed by the next job's first scanned count (or the batch total): the
+// end of its last triangle's records.
+__global__ void __launch_bounds__(256) emit_batch_kernel(const batch_job* __restrict__ jobs, stage_map m, uint32_t capacity) {
+  uint32_t lb, nb;
+  const batch_job& J = jobs[stage_job(m, lb, nb)];
+  emit_body(J.P, J.B.tri_rec, J.B.cnt, J.B.cnt + 3 * J.P.prim_count, capacity, J.B.keys[0], J.B.vals[0], lb);
+}
+
+// The job's records [begin, begin + n): its scanned counts clamped to the record capacity and to the
+// records its binning workgroups cover.
+__device__ __forceinline__ void job_records(const batch_job& J, uint32_t capacity, uint32_t items, uint32_t& begin,
+                                            uint32_t& n) {
+  const uint32_t b = min(J.B.cnt[0], capacity), e = min(J.B.cnt[3 * J.P.prim_count], capacity);
+  begin = b;
+  n = min(e - b, J.bucket_blocks * items);
+}
+
+template <int PT>
+__global__ void __launch_bounds__(256) bucket_hist_batch_kernel(const batch_job* __restrict__ jobs, stage_map m,
+                                                                uint32_t capacity) {
+  uint32_t lb, nb, begin, n;
+  const batch_job& J = jobs[stage_job(m, lb, nb)];
+  job_records(J, capacity, 256 * PT, begin, n);
+  bucket_hist_body<PT>(J.B.keys[0] + begin, n, J.tiles, J.bucket_blocks, J.B.hist, lb);
+}
+
+template <int PT>
+__global__ void __launch_bounds__(256) bucket_scatter_batch_kernel(const batch_job* __restrict__ jobs, stage_map m,
+                                                                   uint32_t capacity) {
+  uint32_t lb, nb, begin, n;
+  const batch_job& J = jobs[stage_job(m, lb, nb)];
+  job_records(J, capacity, 256 * PT, begin, n);
+  bucket_scatter_body<PT>(J.B.keys[0] + begin, J.B.vals[0] + begin, n, J.tiles, J.bucket_blocks, J.B.hist, J.B.vals[1],
+                          capacity, lb);
+}
+
+// The job's histogram is followed by the next job's first scanned entry (or the batch total).
+__global__ void __launch_bounds__(256) bucket_range_batch_kernel(const batch_job* __restrict__ jobs, stage_map m) {
+  uint32_t lb, nb;
+  const batch_job& J = jobs[stage_job(m, lb, nb)];
+  bucket_range_body(J.B.hist, J.B.hist[J.tiles * J.bucket_blocks], J.tiles, J.bucket_blocks, J.B.tstart, J.B.tend,
+                    lb * 256 + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(1024) tile_order_batch_kernel(const batch_job* __restrict__ jobs) {
+  const batch_job& J = jobs[blockIdx.x];
+  tile_order_body(J.B.tstart, J.B.tend, J.tiles, const_cast<uint32_t*>(J.B.order));
+}
+
+template <int NA, int S, bool SHADE>
+__global__ void __launch_bounds__(256, 4) vis_batch_kernel(const batch_job* __restrict__ jobs, stage_map m) {
+  uint32_t lb, nb;
+  const batch_job& J = jobs[stage_job(m, lb, nb)];
+  vis_body<NA, S, SHADE, 0>(J.P, J.vis_mode, 0, J.B.order, nullptr, nullptr, nullptr, nullptr, J.B.vals[1], J.B.tstart,
+                            J.B.tend, J.B.tri_rec, J.B.tri_attr, J.B.samplers, lb);
+}
+
+// status bit 0: a job's records exceeded the capacity it was prepared with; bit 1: clip overflow
+__global__ void batch_check_kernel(const batch_job* __restrict__ jobs, uint32_t njobs, uint32_t capacity, uint32_t items,
+                                   uint32_t* __restrict__ status) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= njobs) return;
+  const batch_job& J = jobs[j];
+  const uint32_t b = J.B.cnt[0], e = J.B.cnt[3 * J.P.prim_count];
+  uint32_t st = 0;
+  if (e > capacity || e - b > J.bucket_blocks * items) st |= 1u;
+  if (J.B.counters[CT_CLIP_OVERFLOW]) st |= 2u;
+  if (st) atomicOr(status, st);
+}
+
+// records[j] of every job (unclamped), for sizing a batch
+__global__ void batch_counts_kernel(const batch_job* __restrict__ jobs, uint32_t njobs, uint32_t* __restrict__ out) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= njobs) return;
+  const batch_job& J = jobs[j];
+  out[j] = J.B.cnt[3 * J.P.prim_count] - J.B.cnt[0];
+  out[njobs + j] = J.B.counters[CT_CLIP_OVERFLOW];
+}
+void launch_batch_counts(const batch_job* jobs, uint32_t njobs, uint32_t* out, hipStream_t s) {
+  if (njobs) hipLaunchKernelGGL(batch_counts_kernel, dim3((njobs + 255) / 256), dim3(256), 0, s, jobs, njobs, out);
+}
+
+void launch_batch_reset(uint32_t* counters, uint32_t n, uint32_t* cnt_tail, uint32_t* hist_tail, uint32_t* status,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(batch_reset_kernel, dim3((std::max<uint32_t>(n, 1) + 255) / 256), dim3(256), 0, s, counters, n,
+                     cnt_tail, hist_tail, status);
+}
+void launch_fill_batch(const fill_item* items, stage_map m, uint32_t blocks, hipStream_t s) {
+  if (blocks) hipLaunchKernelGGL(fill_batch_kernel, dim3(blocks), dim3(256), 0, s, items, m);
+}
+void launch_resolve_batch(const resolve_item* items, stage_map m, uint32_t blocks, hipStream_t s) {
+  if (blocks) hipLaunchKernelGGL(resolve_batch_kernel, dim3(blocks), dim3(256), 0, s, items, m);
+}
+void launch_vs_batch(const batch_job* jobs, stage_map m, uint32_t blocks, hipStream_t s) {
+  if (blocks) hipLaunchKernelGGL(vs_batch_kernel, dim3(blocks), dim3(256), 0, s, jobs, m);
+}
+void launch_setup_batch(int na, const batch_job* jobs, stage_map m, uint32_t blocks, hipStream_t s) {
+  if (!blocks) return;
+#define SRX_SETUP_B(NA) hipLaunchKernelGGL(setup_batch_kernel<NA>, dim3(blocks), dim3(256), 0, s, jobs, m)
+  SRX_NA_DISPATCH(na, SRX_SETUP_B)
+#undef SRX_SETUP_B
+}
+void launch_clip_batch(int na, const batch_job* jobs, stage_map m, uint32_t blocks, hipStream_t s) {
+  if (!blocks) return;
+#define SRX_CLIP_B(NA) hipLaunchKernelGGL(clip_batch_kernel<NA>, dim3(blocks), dim3(64), 0, s, jobs, m)
+  SRX_NA_DISPATCH(na, SRX_CLIP_B)
+#undef SRX_CLIP_B
+}
+void launch_emit_batch(const batch_job* jobs, stage_map m, uint32_t blocks, uint32_t capacity, hipStream_t s) {
+  if (blocks) hipLaunchKernelGGL(emit_batch_kernel, dim3(blocks), dim3(256), 0, s, jobs, m, capacity);
+}
+bool batch_bin_items_valid(int pt) { return pt == 8 || pt == 16 || pt == 32; }
+void launch_bucket_batch(int pt, int phase, const batch_job* jobs, stage_map m, uint32_t blocks, uint32_t capacity,
+                         hipStream_t s) {
+  if (!blocks) return;
+#define SRX_BUCKET_B(PT)                                                                                          \
+  if (phase == 0)                                                                                                  \
+    hipLaunchKernelGGL(bucket_hist_batch_kernel<PT>, dim3(blocks), dim3(256), 0, s, jobs, m, capacity);          \
+  else                                                                                                             \
+    hipLaunchKernelGGL(bucket_scatter_batch_kernel<PT>, dim3(blocks), dim3(256), 0, s, jobs, m, capacity)
+  switch (pt) {
+  case 8: SRX_BUCKET_B(8); break;
+  case 32: SRX_BUCKET_B(32); break;
+  default: SRX_BUCKET_B(16); break;
+  }
+#undef SRX_BUCKET_B
+}
+void launch_range_batch(const batch_job* jobs, stage_map m, uint32_t blocks, hipStream_t s) {
+  if (blocks) hipLaunchKernelGGL(bucket_range_batch_kernel, dim3(blocks), dim3(256), 0, s, jobs, m);
+}
+void launch_tile_order_batch(const batch_job* jobs, uint32_t njobs, hipStream_t s) {
+  if (njobs) hipLaunchKernelGGL(tile_order_batch_kernel, dim3(njobs), dim3(1024), 0, s, jobs);
+}
+void launch_vis_batch(int na, int samples, bool shade, int threads, const batch_job* jobs, stage_map m, uint32_t blocks,
+                      hipStream_t s) {
+  if (!blocks) return;
+  const size_t lds = static_cast<size_t>(vis_union_bytes(threads, samples, shade));
+#define SRX_VIS_B(NA, S, SH) \
+  hipLaunchKernelGGL((vis_batch_kernel<NA, S, SH>), dim3(blocks), dim3(threads), lds, s, jobs, m)
+#define SRX_VIS_BS(NA)                   \
+  switch (samples) {                     \
+  case 1: SRX_VIS_B(NA, 1, true); break; \
+  case 2: SRX_VIS_B(NA, 2, true); break; \
+  default: SRX_VIS_B(NA, 4, true); break; \
+  }
+  if (shade) {
+    SRX_NA_DISPATCH(na, SRX_VIS_BS)
+  } else {
+    switch (samples) {
+    case 1: SRX_VIS_B(1, 1, false); break;
+    case 2: SRX_VIS_B(1, 2, false); break;
+    default: SRX_VIS_B(1, 4, false); break;
+    }
+  }
+#undef SRX_VIS_BS
+#undef SRX_VIS_B
+}
+void launch_batch_check(const batch_job* jobs, uint32_t njobs, uint32_t capacity, int pt, uint32_t* status, hipStream_t s) {
+  if (njobs)
+    hipLaunchKernelGGL(batch_check_kernel, dim3((njobs + 255) / 256), dim3(256), 0, s, jobs, njobs, capacity,
+                       static_cast<uint32_t>(256 * pt), status);
 }
 
 void launch_clear(const surface_desc& s, float4 v, hipStream_t st) {

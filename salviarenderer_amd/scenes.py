@@ -31,11 +31,18 @@ MASK64 = (1 << 64) - 1
 def splitmix64(seed: int, n: int) -> np.ndarray:
     """n consecutive outputs of splitmix64 started at `seed`."""
     with np.errstate(over="ignore"):
-        idx = np.arange(1, n + 1, dtype=np.uint64)
-        z = np.uint64(seed & MASK64) + idx * np.uint64(0x9E3779B97F4A7C15)
-        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        return z ^ (z >> np.uint64(31))
+        z = np.arange(1, n + 1, dtype=np.uint64)
+        z *= np.uint64(0x9E3779B97F4A7C15)
+        z += np.uint64(seed & MASK64)
+        t = z >> np.uint64(30)
+        z ^= t
+        z *= np.uint64(0xBF58476D1CE4E5B9)
+        np.right_shift(z, np.uint64(27), out=t)
+        z ^= t
+        z *= np.uint64(0x94D049BB133111EB)
+        np.right_shift(z, np.uint64(31), out=t)
+        z ^= t
+        return z
 
 
 def uniform01(seed: int, n: int) -> np.ndarray:
@@ -388,3 +395,142 @@ class MultipassFrame:
         r.draw(0, self.n_opaque)
         if resolve and self.resolved is not None:
             r.resolve(self.color, self.resolved)
+
+
+# ---- batched jobs -----------------------------------------------------------------------
+# 256 independent jobs (seed 0x5A11A004 + job).  Per job, four draws from the job's splitmix64
+# stream pick the resolution from BATCH_RESOLUTIONS, the triangle count log-uniform in
+# [1,000, 1,000,000], the resident texture's size from BATCH_TEX_SIZES and the sample count from
+# the caller's choices; the triangles follow (plain_triangles' recipe on the same stream).  Each
+# triangle with n > 2,000 is shrunk about its centroid by sqrt(2,000 / n), so every job keeps the
+# plain scene's depth complexity (~150 layers at 2,000 triangles) instead of growing it to ~76,000
+# at 1,000,000.  Shaders and states are the plain scene's; the texture (uniform noise, full mip
+# chain) only occupies memory, as the plain shaders sample nothing.
+SEED_BATCH = 0x5A11A004
+BATCH_RESOLUTIONS = ((256, 256), (512, 512), (1024, 1024), (1920, 1080))
+BATCH_TEX_SIZES = (256, 512, 1024, 2048)
+BATCH_REF_TRIS = 2000
+
+
+@dataclass
+class JobSpec:
+    seed: int
+    width: int
+    height: int
+    n_tris: int
+    tex_size: int
+    samples: int
+
+
+def batch_job_specs(n_jobs=256, seed=SEED_BATCH, min_tris=1000, max_tris=1_000_000, samples=(1,)):
+    specs = []
+    lo, hi = math.log(min_tris), math.log(max_tris)
+    for j in range(n_jobs):
+        u = uniform01(seed + j, 4)
+        w, h = BATCH_RESOLUTIONS[min(int(u[0] * 4), 3)]
+        n = int(round(math.exp(lo + u[1] * (hi - lo))))
+        tex = BATCH_TEX_SIZES[min(int(u[2] * 4), 3)]
+        s = samples[min(int(u[3] * len(samples)), len(samples) - 1)]
+        specs.append(JobSpec(seed + j, w, h, n, tex, s))
+    return specs
+
+
+def batch_triangles(spec: JobSpec) -> np.ndarray:
+    """(3*n, 8) float32 x y z w r g b a: plain_triangles' recipe after the four spec draws, with
+    x, y scaled about each triangle's centroid by min(1, sqrt(2,000 / n))."""
+    n = spec.n_tris
+    # floats in [0,1) with 24 bits: exact in float32
+    u = ((splitmix64(spec.seed, 4 + n * 21)[4:] >> np.uint64(40)).astype(np.float32) * np.float32(2.0 ** -24))
+    u = u.reshape(n * 3, 7)
+    v = np.empty((n * 3, 8), np.float32)
+    xy = (np.float32(-1.0) + np.float32(2.0) * u[:, 0:2]).reshape(n, 3, 2)
+    k = np.float32(min(1.0, math.sqrt(BATCH_REF_TRIS / n)))
+    if k < 1.0:
+        c = (xy[:, 0:1] + xy[:, 1:2] + xy[:, 2:3]) * np.float32(1.0 / 3.0)
+        xy = c + (xy - c) * k
+    v[:, 0:2] = xy.reshape(n * 3, 2)
+    v[:, 2] = np.float32(0.1) + np.float32(0.8) * u[:, 2]
+    v[:, 3] = 1.0
+    v[:, 4:8] = u[:, 3:7]
+    return v
+
+
+class BatchedJobs:
+    """The jobs' resources on one renderer; render_all() issues every job's commands (clears,
+    draw, resolve) one by one, record() the same commands into a batch."""
+
+    def __init__(self, r: A.Renderer, specs, with_textures=True):
+        lib = r.lib
+        self.r = r
+        self.specs = specs
+        self.jobs = []
+        self.layout = r.create_input_layout([("POSITION", 0, A.FMT_R32G32B32A32_FLOAT, 0, 0),
+                                             ("COLOR", 0, A.FMT_R32G32B32A32_FLOAT, 0, 16)], A.VS_PASSTHROUGH_COLOR)
+        rd = A.default_raster_desc(lib)
+        rd.cull_mode = A.CULL_NONE
+        self.rs = r.create_raster_state(rd)
+        self.dss = r.create_depth_stencil_state(A.default_depth_stencil_desc(lib))
+        from concurrent.futures import ThreadPoolExecutor
+
+        with ThreadPoolExecutor(8) as ex:  # numpy releases the GIL in the generators
+            geo = ex.map(batch_triangles, specs)
+            texels = ex.map(lambda sp: noise_texture(sp.tex_size, sp.seed) if with_textures else None, specs)
+            data = list(zip(geo, texels))
+        for sp, (tris, noise) in zip(specs, data):
+            ct = r.create_tex2d(sp.width, sp.height, sp.samples, A.PF_RGBA8)
+            dt = r.create_tex2d(sp.width, sp.height, sp.samples, A.PF_RG32F)
+            rt = r.create_tex2d(sp.width, sp.height, 1, A.PF_RGBA8) if sp.samples > 1 else None
+            vb = r.create_buffer_from(tris)
+            tex = None
+            if with_textures:
+                tex = r.create_tex2d(sp.tex_size, sp.tex_size, 1, A.PF_RGBA8)
+                tex.surface(0).write(noise)
+                tex.gen_mipmap(A.FILTER_LINEAR, True)
+            self.jobs.append(dict(ct=ct, dt=dt, rt=rt, color=ct.surface(0), ds=dt.surface(0),
+                                  resolved=rt.surface(0) if rt else None, vb=vb, tex=tex))
+
+    def bind_common(self):
+        r = self.r
+        r.set_input_layout(self.layout)
+        r.set_primitive_topology(A.TOPOLOGY_TRIANGLE_LIST)
+        r.set_vertex_shader(A.VS_PASSTHROUGH_COLOR)
+        r.set_pixel_shader(A.PS_GOURAUD)
+        r.set_blend_shader(A.BLEND_OFF)
+        r.set_rasterizer_state(self.rs)
+        r.set_depth_stencil_state(self.dss, 0)
+
+    def issue(self, j):
+        r, sp, jb = self.r, self.specs[j], self.jobs[j]
+        r.set_render_targets([jb["color"]], jb["ds"])
+        r.set_viewport(0, 0, sp.width, sp.height, 0.0, 1.0)
+        r.set_vertex_buffers(0, [jb["vb"]], [32])
+        r.clear_color(jb["color"], (0.0, 0.0, 0.0, 0.0))
+        r.clear_depth_stencil(jb["ds"], A.CLEAR_DEPTH | A.CLEAR_STENCIL, 1.0, 0)
+        r.draw(0, sp.n_tris)
+        if jb["resolved"] is not None:
+            r.resolve(jb["color"], jb["resolved"])
+
+    def render_all(self):
+        self.bind_common()
+        for j in range(len(self.jobs)):
+            self.issue(j)
+
+    def record(self) -> A.Batch:
+        self.r.batch_begin()
+        try:
+            self.render_all()
+        except Exception:
+            self.r.batch_end().release()
+            raise
+        return self.r.batch_end()
+
+    def output(self, j):
+        jb = self.jobs[j]
+        return (jb["resolved"] or jb["color"]).read()
+
+    def footprint(self, j) -> int:
+        """Device bytes the job keeps resident: geometry, targets and texture chain."""
+        sp = self.specs[j]
+        px = sp.width * sp.height
+        tex = sum(max(sp.tex_size >> l, 1) ** 2 for l in range(int(math.log2(sp.tex_size)) + 1)) * 4
+        return sp.n_tris * 96 + px * sp.samples * 12 + (px * 4 if sp.samples > 1 else 0) + tex
