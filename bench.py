@@ -117,7 +117,7 @@ def multipass_frames(lib, steps, warmup):
                                    "raster": prof[6] / steps / 1e6}}
 
 
-def batched_jobs(lib, launches, plain_passes=1, autotune_reps=1):
+def batched_jobs(lib, launches, plain_passes=1, autotune_reps=1, config=None):
     """Secondary workload: 256 independent jobs (scenes.batch_job_specs: 256^2..1920x1080, 1k-1M
     triangles, plain-scene shaders, resident noise textures), issued one by one through the
     renderer API and as one recorded batch (one launch per stage, HIP graph), both timed on the
@@ -138,7 +138,11 @@ def batched_jobs(lib, launches, plain_passes=1, autotune_reps=1):
     plain_s = (time.perf_counter() - t0) / plain_passes
     print(f"batched: one-by-one pass {plain_s * 1e3:.1f} ms", file=sys.stderr, flush=True)
     b = jobs.record()
-    sweep = b.autotune(reps=autotune_reps)
+    if config:
+        b.configure(*config)
+        sweep = []
+    else:
+        sweep = b.autotune(reps=autotune_reps)
     info = b.info()
     b.launch()
     r.flush()
@@ -193,6 +197,8 @@ def main():
     ap.add_argument("--no-multipass", action="store_true", help="skip the secondary multipass measurement")
     ap.add_argument("--no-batched", action="store_true", help="skip the secondary batched-jobs measurement")
     ap.add_argument("--batched-only", action="store_true", help="only the batched-jobs measurement")
+    ap.add_argument("--batch-config", default="", metavar="BIN_ITEMS,THREADS",
+                    help="configure the batch instead of autotuning it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel used to calibrate HBM byte counters, then exit")
     args = ap.parse_args()
@@ -205,8 +211,9 @@ def main():
         sys.exit(2)
 
     lib = hip_lib()
+    config = tuple(int(x) for x in args.batch_config.split(",")) if args.batch_config else None
     if args.batched_only:
-        print(json.dumps(batched_jobs(lib, 3)), flush=True)
+        print(json.dumps(batched_jobs(lib, 3, config=config)), flush=True)
         return
     total, lat, prof, n_tris = run_frames(lib, args.steps, args.warmup, profile=True)
     fps = args.steps / total
@@ -246,7 +253,7 @@ def main():
     if not args.no_multipass:
         out["secondary"]["multipass"] = multipass_frames(lib, 20, 3)
     if not args.no_batched:
-        out["secondary"]["batched"] = batched_jobs(lib, 3)
+        out["secondary"]["batched"] = batched_jobs(lib, 3, config=config)
     print(json.dumps(out), flush=True)
 
 

@@ -1813,6 +1813,7 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
   extern __shared__ __align__(16) unsigned char lds_dynamic[];
   unsigned char* const lds_union = NTC > 0 ? lds_static : lds_dynamic;
   __shared__ uint32_t wsum[4];
+  __shared__ uint32_t hiz[16];
   __shared__ uint32_t nitems;
   float4* recs = reinterpret_cast<float4*>(lds_union);                           // [NT][4]
   uint16_t* work = reinterpret_cast<uint16_t*>(lds_union + NT * 64);             // [NT * 16]
@@ -1881,8 +1882,29 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
   const uint32_t full_px = (S == 1) ? 1u : (S == 2 ? 3u : 15u);
   const int jx = (ox - t64x) >> 4, jy = (oy - t64y) >> 4;
   const float tvx = static_cast<float>(t64x), tvy = static_cast<float>(t64y);
+  // Depth culling from the second chunk on: hiz[b] is the largest key depth word of 4x4 block b's
+  // samples after the previous chunks.  Keys only decrease, so a record whose smallest possible
+  // depth word over the block exceeds it cannot win any sample there and skips the block.
+  const bool depth_keys = mode == VIS_LESS || mode == VIS_LEQUAL || mode == VIS_GREATER || mode == VIS_GEQUAL;
+  const bool far_is_min = mode == VIS_GREATER || mode == VIS_GEQUAL;
   for (uint32_t c0 = t0; c0 < t1; c0 += NT) {
     __syncthreads();  // the previous chunk's work list is consumed (and the keys are initialised)
+    const bool hiz_on = depth_keys && c0 > t0;
+    if (hiz_on) {
+      // 64 consecutive pixels are one row of 4x4 blocks; lanes differing in bits 0, 1 (x) and 4, 5
+      // (y) share a block
+      for (int i = tid; i < 256; i += NT) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int smp = 0; smp < S; ++smp) m = max(m, static_cast<uint32_t>(keys[i * S + smp] >> 32));
+        m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), 1, 64)));
+        m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), 2, 64)));
+        m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), 16, 64)));
+        m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), 32, 64)));
+        if ((i & 0x33) == 0) hiz[(i >> 6) * 4 + ((i >> 2) & 3)] = m;
+      }
+      __syncthreads();
+    }
     // phase 1a: one record per lane -> the 4x4 blocks of this GPU tile it reaches (full or partial)
     uint32_t bm = 0, fm = 0;
     const uint32_t ri = c0 + tid;
@@ -1963,6 +1985,20 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
             }
             if (rej) continue;
             block_full = !nacc;
+          }
+          if (hiz_on) {
+            // the depth plane's extreme over the block's sample positions [Bx, Bx+4) x [By, By+4),
+            // widened by far more than the rounding of the per-sample evaluation
+            const float z0 = d3.w, zx = d3.y, zy = d3.z, dzx = d4.x, dzy = d4.y;
+            const float ax = static_cast<float>(ox + (b & 3) * 4) - zx, ay = static_cast<float>(oy + (b >> 2) * 4) - zy;
+            const float tx0 = dzx * ax, tx1 = dzx * (ax + 4.0f), ty0 = dzy * ay, ty1 = dzy * (ay + 4.0f);
+            const float margin = (fabsf(z0) + fmaxf(fabsf(tx0), fabsf(tx1)) + fmaxf(fabsf(ty0), fabsf(ty1))) * 1.52587890625e-05f;
+            const float zext = far_is_min ? (z0 + fmaxf(tx0, tx1) + fmaxf(ty0, ty1)) + margin
+                                          : (z0 + fminf(tx0, tx1) + fminf(ty0, ty1)) - margin;
+            if (fabsf(zext) < 3.0e38f) {  // finite: planes with inf/nan terms are never culled
+              const uint32_t zk = far_is_min ? ~ord_depth(zext) : ord_depth(zext);
+              if (zk > hiz[b]) continue;
+            }
           }
           bm |= 1u << b;
           if (block_full) fm |= 1u << b;

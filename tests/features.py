@@ -206,6 +206,21 @@ def _blend_cases():
         out.append(Case(f"depth_only_crowded_{nm}", [Draw(pd, cd, blend=A.BLEND_DISABLED, depth_func=f),
                                                      Draw(p2, c2, depth_func=A.CMP_LESS_EQUAL)],
                         clear_depth=0.5 if f == A.CMP_GREATER else 1.0))
+    # shaded draws over crowded tiles (several record chunks per tile, so later chunks are depth
+    # culled against the earlier ones), random and tied depths, every depth-key mode
+    n = 3000
+    ctr = rng.uniform(-0.5, 0.0, (n, 1, 2))
+    xy = ctr + rng.uniform(-0.08, 0.08, (n, 3, 2))
+    zz = rng.uniform(0.1, 0.9, (n, 3))
+    tied = rng.uniform(0, 1, n) < 0.3
+    zz[tied] = np.repeat(rng.choice([0.3, 0.45, 0.6], (int(tied.sum()), 1)), 3, 1)
+    ps_ = np.concatenate([xy, zz[..., None], np.ones((n, 3, 1))], -1).reshape(-1, 4).astype(np.float32)
+    cs_ = rng.uniform(0, 1, (n * 3, 4)).astype(np.float32)
+    for f, nm in ((A.CMP_LESS, "less"), (A.CMP_LESS_EQUAL, "lequal"), (A.CMP_GREATER, "greater"),
+                  (A.CMP_GREATER_EQUAL, "gequal")):
+        for samples in (1, 4):
+            out.append(Case(f"shaded_crowded_{nm}_msaa{samples}", [Draw(ps_, cs_, depth_func=f)], samples=samples,
+                            clear_depth=0.0 if f in (A.CMP_GREATER, A.CMP_GREATER_EQUAL) else 1.0))
     p, c = tris(150, seed=15, alpha=(0.0, 1.0))
     out.append(Case("blend_rgba32f", [Draw(p, c, blend=A.BLEND_ON, depth_enable=False)], fmt=A.PF_RGBA32F))
     return out
