@@ -165,6 +165,67 @@ def batched_jobs(lib, launches, plain_passes=1, autotune_reps=1, config=None):
             "device_bytes": info.device_bytes, "records": info.records}
 
 
+def serving(lib, n_requests=10000, rates=(0.25, 0.5, 0.75, 1.0), seed=0x5A11A005):
+    """Secondary workload: Poisson streams of render requests drawn from the batched jobs (resident
+    meshes and textures) with 1x or 4x MSAA at random, through the request server (continuous
+    batching); first all at once (capacity), then at fractions of that capacity."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    r = lib.renderer(A.RENDERER_BENCHMARK)
+    desc = A.default_server_desc(lib)
+    desc.staging_bytes = 4 << 30
+    srv = r.create_server(desc)
+    specs = scenes.batch_job_specs()
+    t = time.perf_counter()
+    with ThreadPoolExecutor(8) as ex:
+        for sp, geo, tex in zip(specs, ex.map(scenes.batch_triangles, specs),
+                                ex.map(lambda sp: scenes.noise_texture(sp.tex_size, sp.seed), specs)):
+            srv.put_mesh(sp.seed, geo)
+            srv.put_texture(sp.seed, tex)
+    print(f"serving: {len(specs)} meshes + textures resident ({time.perf_counter() - t:.1f} s)", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(seed)
+
+    def run(rate):
+        jobs = rng.integers(0, len(specs), n_requests)
+        samples = rng.choice([1, 4], n_requests)
+        gaps = rng.exponential(1e9 / rate, n_requests) if rate else np.zeros(n_requests)
+        start = srv.now() + 300_000_000
+        arrivals = start + np.cumsum(gaps).astype(np.int64)
+        tickets = {}
+        for j, s_, a in zip(jobs, samples, arrivals):
+            sp = specs[int(j)]
+            tickets[srv.submit(sp.seed, sp.n_tris, sp.width, sp.height, int(s_), texture_key=sp.seed,
+                               arrival_ns=int(a))] = (int(j), int(s_))
+        lat, sizes, done_last, fails = [], [], 0, 0
+        while len(lat) < n_requests:
+            for c in srv.poll(256, wait_ms=50, copy=False):
+                lat.append((c["done_ns"] - c["arrival_ns"]) / 1e6)
+                sizes.append(c["batch_jobs"])
+                done_last = max(done_last, c["done_ns"])
+                fails += c["status"] != 0
+                srv.release(c["ticket"])
+        span = (done_last - int(arrivals[0])) / 1e9
+        lat = np.array(lat)
+        return {"offered_per_s": None if not rate else rate, "frames_per_s": n_requests / span,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "mean_batch": float(np.mean(sizes)), "failed": int(fails)}
+
+    cap = run(None)
+    print(f"serving: capacity {cap['frames_per_s']:.0f} frames/s", file=sys.stderr, flush=True)
+    sweep = []
+    for f in rates:
+        res = run(f * cap["frames_per_s"])
+        res["load"] = f
+        sweep.append(res)
+        print(f"serving: load {f} -> {res['frames_per_s']:.0f} frames/s p50 {res['p50_ms']:.2f} ms "
+              f"p99 {res['p99_ms']:.2f} ms", file=sys.stderr, flush=True)
+    st = srv.stats()
+    return {"workload": f"{n_requests} requests per rate from the 256 batched jobs, 1x/4x MSAA at random, "
+                        "Poisson arrivals; meshes and textures resident (LRU pool cap 200 GiB)",
+            "capacity": cap, "sweep": sweep, "batches": st.batches, "retries": st.retries,
+            "arena_peak_bytes": st.arena_peak_bytes, "pool_bytes": st.pool_bytes}
+
+
 def cpu_baseline(seconds):
     oracle = os.path.join(ROOT, "oracle", "libsrx_cpu.so")
     if not os.path.exists(oracle):
@@ -197,6 +258,9 @@ def main():
     ap.add_argument("--no-multipass", action="store_true", help="skip the secondary multipass measurement")
     ap.add_argument("--no-batched", action="store_true", help="skip the secondary batched-jobs measurement")
     ap.add_argument("--batched-only", action="store_true", help="only the batched-jobs measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the secondary serving measurement")
+    ap.add_argument("--serving-only", action="store_true", help="only the serving measurement")
+    ap.add_argument("--serving-requests", type=int, default=10000)
     ap.add_argument("--batch-config", default="", metavar="BIN_ITEMS,THREADS",
                     help="configure the batch instead of autotuning it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -214,6 +278,9 @@ def main():
     config = tuple(int(x) for x in args.batch_config.split(",")) if args.batch_config else None
     if args.batched_only:
         print(json.dumps(batched_jobs(lib, 3, config=config)), flush=True)
+        return
+    if args.serving_only:
+        print(json.dumps(serving(lib, args.serving_requests)), flush=True)
         return
     total, lat, prof, n_tris = run_frames(lib, args.steps, args.warmup, profile=True)
     fps = args.steps / total
@@ -254,6 +321,8 @@ def main():
         out["secondary"]["multipass"] = multipass_frames(lib, 20, 3)
     if not args.no_batched:
         out["secondary"]["batched"] = batched_jobs(lib, 3, config=config)
+    if not args.no_serving:
+        out["secondary"]["serving"] = serving(lib, args.serving_requests)
     print(json.dumps(out), flush=True)
 
 

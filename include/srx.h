@@ -361,6 +361,65 @@ int32_t srx_batch_configure(srx_batch b, int32_t bin_items, int32_t vis_threads)
 int32_t srx_batch_autotune(srx_batch b, uint32_t reps, srx_batch_tuning* results, size_t capacity, size_t* count);
 int32_t srx_batch_get_info(srx_batch b, srx_batch_info* info);
 
+/* ---- request server ----
+ * No counterpart in the reference.  A server renders requests -- a resident mesh of non-indexed
+ * triangles (per vertex: position xyzw, colour rgba, 8 floats) drawn with the pass-through vertex
+ * shader, the Gouraud pixel shader, cull none and depth LESS into a fresh target of the requested
+ * size and sample count (cleared to clear_color and depth 1.0), then resolved -- and returns the
+ * resolved RGBA8 image (and, on request, the RG32F depth/stencil target), byte for byte what the
+ * renderer API produces for the same commands.  A dispatcher thread admits arrived requests in
+ * arrival order by their device footprint (targets, geometry scratch, bins) into batches of up to
+ * max_batch_jobs, and launches a new batch on a free stream while earlier ones still run.  Meshes
+ * and textures are kept resident under an LRU cap (pool_bytes); a request names them by key and
+ * may carry their data for a miss.  Textures only occupy memory: the Gouraud shader samples
+ * nothing.  All entry points are thread-safe. */
+typedef struct srx_server_t* srx_server;
+typedef struct srx_server_desc {
+  uint64_t pool_bytes;     /* LRU cap of resident meshes and textures */
+  uint64_t arena_bytes;    /* device memory of the batches in flight */
+  uint64_t staging_bytes;  /* pinned host memory: batch tables and returned images */
+  uint32_t max_batch_jobs; /* requests per batch */
+  uint32_t streams;        /* batches in flight (1..8) */
+  int32_t bin_items;       /* as srx_batch_configure */
+  int32_t vis_threads;
+} srx_server_desc;
+typedef struct srx_request {
+  uint64_t mesh_key;
+  const void* mesh_data; /* 3 * n_tris vertices, or NULL when the mesh is resident */
+  uint32_t n_tris;
+  uint32_t texture_size;  /* RGBA8 texture_size^2 level 0, chain built on the device */
+  uint64_t texture_key;   /* 0: none */
+  const void* texture_data;
+  uint32_t width, height, samples, want_depth;
+  float clear_color[4];
+  int64_t arrival_ns; /* server clock (srx_server_now); not admitted before */
+} srx_request;
+typedef struct srx_completion {
+  uint64_t ticket;
+  int32_t status;      /* SRX_OK or why the request failed */
+  uint32_t batch_jobs; /* requests in the batch that carried it */
+  int64_t arrival_ns, admitted_ns, done_ns;
+  const void* color; /* resolved RGBA8, width*height*4 bytes, valid until srx_server_release */
+  uint64_t color_bytes;
+  const void* depth; /* RG32F per sample when want_depth */
+  uint64_t depth_bytes;
+} srx_completion;
+typedef struct srx_server_stats {
+  uint64_t batches, jobs, retries, uploads, evictions, max_batch_jobs;
+  uint64_t pool_bytes, resident_meshes, resident_textures, arena_peak_bytes, queued, running;
+} srx_server_stats;
+void srx_default_server_desc(srx_server_desc* d); /* 200 GiB pool, 64 GiB arena, 2 GiB staging */
+int32_t srx_create_server(srx_renderer r, const srx_server_desc* d, srx_server* out);
+int32_t srx_server_put_mesh(srx_server s, uint64_t key, const void* vertices, uint32_t n_tris);
+int32_t srx_server_put_texture(srx_server s, uint64_t key, const void* rgba8, uint32_t size);
+int32_t srx_server_submit(srx_server s, const srx_request* req, uint64_t* ticket);
+/* Up to capacity finished requests (waits up to wait_ms for the first); each stays valid, and keeps
+ * its staging, until srx_server_release. */
+int32_t srx_server_poll(srx_server s, srx_completion* out, size_t capacity, size_t* count, int32_t wait_ms);
+int32_t srx_server_release(srx_server s, uint64_t ticket);
+int64_t srx_server_now(srx_server s);
+int32_t srx_server_get_stats(srx_server s, srx_server_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,6 +36,7 @@
 #include <cstdlib>
 #include <deque>
 #include <functional>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -167,7 +168,7 @@ pool& global_pool() {
 }
 
 // ---------------------------------------------------------------- objects
-enum obj_kind { K_RENDERER = 1, K_BUFFER, K_TEXTURE, K_SURFACE, K_SAMPLER, K_LAYOUT, K_RASTER, K_DS, K_QUERY, K_BATCH };
+enum obj_kind { K_RENDERER = 1, K_BUFFER, K_TEXTURE, K_SURFACE, K_SAMPLER, K_LAYOUT, K_RASTER, K_DS, K_QUERY, K_BATCH, K_SERVER };
 
 struct object {
   std::atomic<int> refs{1};
@@ -1372,6 +1373,29 @@ renderer_obj::~renderer_obj() {
   if (recording) release(recording);
 }
 
+// The request server's contract is the plain pipeline applied to each request; the restatement
+// renders every request at submission with the renderer API and reports it at once.
+struct served {
+  srx_request req;
+  std::vector<uint8_t> color, depth;
+  int64_t done_ns = 0;
+  int32_t status = SRX_OK;
+};
+struct server_obj : object {
+  srx_renderer r = nullptr;
+  std::map<uint64_t, std::vector<float>> meshes;
+  std::map<uint64_t, uint32_t> textures;
+  std::map<uint64_t, served> finished;
+  std::deque<uint64_t> unreported;
+  uint64_t next_ticket = 1, jobs = 0, uploads = 0;
+  std::chrono::steady_clock::time_point epoch = std::chrono::steady_clock::now();
+  server_obj() : object(K_SERVER) {}
+  ~server_obj() override { srx_release(r); }
+  int64_t now() const {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - epoch).count();
+  }
+};
+
 template <class T>
 T* as(void* h, int kind) {
   auto* o = static_cast<object*>(h);
@@ -1822,6 +1846,162 @@ int32_t srx_clear_depth_stencil(srx_renderer rr, srx_surface s, uint32_t flags, 
   return SRX_OK;
 }
 int32_t srx_flush(srx_renderer rr) { return as<renderer_obj>(rr, K_RENDERER) ? SRX_OK : SRX_INVALID_PARAMETER; }
+
+void srx_default_server_desc(srx_server_desc* d) {
+  *d = srx_server_desc{200ull << 30, 64ull << 30, 2ull << 30, 64, 3, 8192, 64};
+}
+int32_t srx_create_server(srx_renderer rr, const srx_server_desc* d, srx_server* out) {
+  if (!as<renderer_obj>(rr, K_RENDERER) || !d || !out) return SRX_INVALID_PARAMETER;
+  auto* S = new server_obj();
+  srx_retain(rr);
+  S->r = rr;
+  *out = reinterpret_cast<srx_server>(static_cast<object*>(S));
+  return SRX_OK;
+}
+int32_t srx_server_put_mesh(srx_server ss, uint64_t key, const void* vertices, uint32_t n_tris) {
+  auto* S = as<server_obj>(ss, K_SERVER);
+  if (!S || n_tris == 0) return SRX_INVALID_PARAMETER;
+  auto it = S->meshes.find(key);
+  if (it != S->meshes.end()) return it->second.size() == n_tris * 24ull ? SRX_OK : SRX_INVALID_PARAMETER;
+  if (!vertices) return SRX_INVALID_PARAMETER;
+  const float* f = static_cast<const float*>(vertices);
+  S->meshes[key].assign(f, f + n_tris * 24ull);
+  ++S->uploads;
+  return SRX_OK;
+}
+int32_t srx_server_put_texture(srx_server ss, uint64_t key, const void* rgba8, uint32_t size) {
+  auto* S = as<server_obj>(ss, K_SERVER);
+  if (!S || key == 0 || size == 0) return SRX_INVALID_PARAMETER;
+  if (!S->textures.count(key)) {
+    if (!rgba8) return SRX_INVALID_PARAMETER;
+    S->textures[key] = size;
+    ++S->uploads;
+  }
+  return SRX_OK;
+}
+static int32_t serve_one(server_obj* S, const srx_request& q, served& o) {
+  srx_renderer r = S->r;
+  const std::vector<float>& m = S->meshes[q.mesh_key];
+  srx_buffer vb = nullptr;
+  srx_texture ct = nullptr, dt = nullptr, rt = nullptr;
+  srx_surface cs = nullptr, ds = nullptr, rs = nullptr;
+  srx_input_layout lay = nullptr;
+  srx_raster_state ras = nullptr;
+  srx_ds_state dss = nullptr;
+  int32_t rc = srx_create_buffer(r, m.size() * 4, &vb);
+  if (rc == SRX_OK) rc = srx_buffer_write(vb, 0, m.data(), m.size() * 4);
+  if (rc == SRX_OK) rc = srx_create_tex2d(r, q.width, q.height, q.samples, SRX_PF_RGBA8, &ct);
+  if (rc == SRX_OK) rc = srx_create_tex2d(r, q.width, q.height, q.samples, SRX_PF_RG32F, &dt);
+  if (rc == SRX_OK) rc = srx_texture_surface(ct, 0, &cs);
+  if (rc == SRX_OK) rc = srx_texture_surface(dt, 0, &ds);
+  if (rc == SRX_OK && q.samples > 1) rc = srx_create_tex2d(r, q.width, q.height, 1, SRX_PF_RGBA8, &rt);
+  if (rc == SRX_OK && q.samples > 1) rc = srx_texture_surface(rt, 0, &rs);
+  srx_input_element_desc el[2] = {{"POSITION", 0, SRX_FMT_R32G32B32A32_FLOAT, 0, 0, 0, 0},
+                                  {"COLOR", 0, SRX_FMT_R32G32B32A32_FLOAT, 0, 16, 0, 0}};
+  if (rc == SRX_OK) rc = srx_create_input_layout(r, el, 2, SRX_VS_PASSTHROUGH_COLOR, &lay);
+  srx_raster_desc rd;
+  srx_default_raster_desc(&rd);
+  rd.cull_mode = SRX_CULL_NONE;
+  if (rc == SRX_OK) rc = srx_create_raster_state(r, &rd, &ras);
+  srx_depth_stencil_desc dd;
+  srx_default_depth_stencil_desc(&dd);
+  if (rc == SRX_OK) rc = srx_create_depth_stencil_state(r, &dd, &dss);
+  if (rc == SRX_OK) {
+    size_t stride = 32, offs = 0;
+    srx_set_input_layout(r, lay);
+    srx_set_vertex_buffers(r, 0, 1, &vb, &stride, &offs);
+    srx_set_primitive_topology(r, SRX_TOPOLOGY_TRIANGLE_LIST);
+    srx_set_vertex_shader(r, SRX_VS_PASSTHROUGH_COLOR, nullptr, 0);
+    srx_set_pixel_shader(r, SRX_PS_GOURAUD, nullptr, 0);
+    srx_set_blend_shader(r, SRX_BLEND_OFF);
+    srx_set_rasterizer_state(r, ras);
+    srx_set_depth_stencil_state(r, dss, 0);
+    srx_set_render_targets(r, 1, &cs, ds);
+    srx_viewport vp{0, 0, static_cast<float>(q.width), static_cast<float>(q.height), 0, 1};
+    srx_set_viewport(r, &vp);
+    rc = srx_clear_color(r, cs, q.clear_color);
+    if (rc == SRX_OK) rc = srx_clear_depth_stencil(r, ds, SRX_CLEAR_DEPTH | SRX_CLEAR_STENCIL, 1.0f, 0);
+    if (rc == SRX_OK) rc = srx_draw(r, 0, q.n_tris);
+    if (rc == SRX_OK && q.samples > 1) rc = srx_surface_resolve(r, cs, rs);
+  }
+  if (rc == SRX_OK) {
+    o.color.resize(static_cast<size_t>(q.width) * q.height * 4);
+    rc = srx_surface_read(q.samples > 1 ? rs : cs, o.color.data(), o.color.size());
+  }
+  if (rc == SRX_OK && q.want_depth) {
+    o.depth.resize(static_cast<size_t>(q.width) * q.height * q.samples * 8);
+    rc = srx_surface_read(ds, o.depth.data(), o.depth.size());
+  }
+  for (void* h : {static_cast<void*>(vb), static_cast<void*>(ct), static_cast<void*>(dt), static_cast<void*>(rt),
+                  static_cast<void*>(cs), static_cast<void*>(ds), static_cast<void*>(rs), static_cast<void*>(lay),
+                  static_cast<void*>(ras), static_cast<void*>(dss)})
+    if (h) srx_release(h);
+  return rc;
+}
+int32_t srx_server_submit(srx_server ss, const srx_request* q, uint64_t* ticket) {
+  auto* S = as<server_obj>(ss, K_SERVER);
+  if (!S || !q || !ticket || q->width == 0 || q->height == 0 || q->n_tris == 0 ||
+      !(q->samples == 1 || q->samples == 2 || q->samples == 4))
+    return SRX_INVALID_PARAMETER;
+  int32_t rc = srx_server_put_mesh(ss, q->mesh_key, q->mesh_data, q->n_tris);
+  if (rc == SRX_OK && q->texture_key) rc = srx_server_put_texture(ss, q->texture_key, q->texture_data, q->texture_size);
+  if (rc != SRX_OK) return rc;
+  served o;
+  o.req = *q;
+  o.status = serve_one(S, *q, o);
+  o.done_ns = S->now();
+  *ticket = S->next_ticket++;
+  S->finished[*ticket] = std::move(o);
+  S->unreported.push_back(*ticket);
+  ++S->jobs;
+  return SRX_OK;
+}
+int32_t srx_server_poll(srx_server ss, srx_completion* out, size_t capacity, size_t* count, int32_t wait_ms) {
+  (void)wait_ms;
+  auto* S = as<server_obj>(ss, K_SERVER);
+  if (!S || !count || (capacity && !out)) return SRX_INVALID_PARAMETER;
+  size_t n = 0;
+  while (n < capacity && !S->unreported.empty()) {
+    const uint64_t tk = S->unreported.front();
+    S->unreported.pop_front();
+    const served& o = S->finished[tk];
+    srx_completion& c = out[n++];
+    c = srx_completion{};
+    c.ticket = tk;
+    c.status = o.status;
+    c.batch_jobs = 1;
+    c.arrival_ns = o.req.arrival_ns;
+    c.admitted_ns = o.req.arrival_ns;
+    c.done_ns = o.done_ns;
+    c.color = o.color.empty() ? nullptr : o.color.data();
+    c.color_bytes = o.color.size();
+    c.depth = o.depth.empty() ? nullptr : o.depth.data();
+    c.depth_bytes = o.depth.size();
+  }
+  *count = n;
+  return SRX_OK;
+}
+int32_t srx_server_release(srx_server ss, uint64_t ticket) {
+  auto* S = as<server_obj>(ss, K_SERVER);
+  if (!S || !S->finished.erase(ticket)) return SRX_INVALID_PARAMETER;
+  return SRX_OK;
+}
+int64_t srx_server_now(srx_server ss) {
+  auto* S = as<server_obj>(ss, K_SERVER);
+  return S ? S->now() : -1;
+}
+int32_t srx_server_get_stats(srx_server ss, srx_server_stats* st) {
+  auto* S = as<server_obj>(ss, K_SERVER);
+  if (!S || !st) return SRX_INVALID_PARAMETER;
+  *st = srx_server_stats{};
+  st->batches = S->jobs;
+  st->jobs = S->jobs;
+  st->uploads = S->uploads;
+  st->max_batch_jobs = S->jobs ? 1 : 0;
+  st->resident_meshes = S->meshes.size();
+  st->resident_textures = S->textures.size();
+  return SRX_OK;
+}
 
 int32_t srx_batch_begin(srx_renderer rr) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);

@@ -98,6 +98,31 @@ class BatchTuning(C.Structure):
     _fields_ = [("bin_items", C.c_int32), ("vis_threads", C.c_int32), ("ms", C.c_float)]
 
 
+class ServerDesc(C.Structure):
+    _fields_ = [("pool_bytes", C.c_uint64), ("arena_bytes", C.c_uint64), ("staging_bytes", C.c_uint64),
+                ("max_batch_jobs", C.c_uint32), ("streams", C.c_uint32), ("bin_items", C.c_int32),
+                ("vis_threads", C.c_int32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("mesh_key", C.c_uint64), ("mesh_data", C.c_void_p), ("n_tris", C.c_uint32),
+                ("texture_size", C.c_uint32), ("texture_key", C.c_uint64), ("texture_data", C.c_void_p),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("samples", C.c_uint32), ("want_depth", C.c_uint32),
+                ("clear_color", C.c_float * 4), ("arrival_ns", C.c_int64)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("status", C.c_int32), ("batch_jobs", C.c_uint32),
+                ("arrival_ns", C.c_int64), ("admitted_ns", C.c_int64), ("done_ns", C.c_int64),
+                ("color", C.c_void_p), ("color_bytes", C.c_uint64), ("depth", C.c_void_p), ("depth_bytes", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("batches", "jobs", "retries", "uploads", "evictions", "max_batch_jobs",
+                                          "pool_bytes", "resident_meshes", "resident_textures", "arena_peak_bytes",
+                                          "queued", "running")]
+
+
 _H = C.c_void_p
 _SZ = C.c_size_t
 _I32 = C.c_int32
@@ -166,6 +191,15 @@ PROTOTYPES = {
     "srx_batch_configure": (_I32, [_H, _I32, _I32]),
     "srx_batch_autotune": (_I32, [_H, _U32, C.POINTER(BatchTuning), _SZ, C.POINTER(_SZ)]),
     "srx_batch_get_info": (_I32, [_H, C.POINTER(BatchInfo)]),
+    "srx_default_server_desc": (None, [C.POINTER(ServerDesc)]),
+    "srx_create_server": (_I32, [_H, C.POINTER(ServerDesc), C.POINTER(_H)]),
+    "srx_server_put_mesh": (_I32, [_H, C.c_uint64, _P, _U32]),
+    "srx_server_put_texture": (_I32, [_H, C.c_uint64, _P, _U32]),
+    "srx_server_submit": (_I32, [_H, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "srx_server_poll": (_I32, [_H, C.POINTER(Completion), _SZ, C.POINTER(_SZ), _I32]),
+    "srx_server_release": (_I32, [_H, C.c_uint64]),
+    "srx_server_now": (C.c_int64, [_H]),
+    "srx_server_get_stats": (_I32, [_H, C.POINTER(ServerStats)]),
 }
 
 
@@ -281,6 +315,77 @@ class Query(_Obj):
     def __init__(self, lib, handle, qid):
         super().__init__(lib, handle)
         self.id = qid
+
+
+class Server(_Obj):
+    """Request server: continuous batching of plain-pipeline render requests over resident meshes."""
+
+    def put_mesh(self, key, verts):
+        v = np.ascontiguousarray(verts, np.float32)
+        self.lib.call("srx_server_put_mesh", self.h, key, v.ctypes.data, v.size // 24)
+
+    def put_texture(self, key, rgba8):
+        t = np.ascontiguousarray(rgba8, np.uint8)
+        self.lib.call("srx_server_put_texture", self.h, key, t.ctypes.data, t.shape[0])
+
+    def submit(self, mesh_key, n_tris, width, height, samples=1, *, mesh=None, texture_key=0, texture=None,
+               clear=(0.0, 0.0, 0.0, 0.0), want_depth=False, arrival_ns=None) -> int:
+        q = Request()
+        q.mesh_key = mesh_key
+        keep = []
+        if mesh is not None:
+            m = np.ascontiguousarray(mesh, np.float32)
+            keep.append(m)
+            q.mesh_data = m.ctypes.data
+        q.n_tris = n_tris
+        q.texture_key = texture_key
+        if texture is not None:
+            t = np.ascontiguousarray(texture, np.uint8)
+            keep.append(t)
+            q.texture_data = t.ctypes.data
+            q.texture_size = t.shape[0]
+        q.width, q.height, q.samples, q.want_depth = width, height, samples, int(want_depth)
+        for i in range(4):
+            q.clear_color[i] = clear[i]
+        q.arrival_ns = self.now() if arrival_ns is None else arrival_ns
+        tk = C.c_uint64()
+        self.lib.call("srx_server_submit", self.h, C.byref(q), C.byref(tk))
+        return tk.value
+
+    def poll(self, capacity=256, wait_ms=0, copy=True):
+        """Finished requests as dicts; with copy, their images as numpy arrays (then released)."""
+        arr = (Completion * capacity)()
+        n = C.c_size_t()
+        self.lib.call("srx_server_poll", self.h, arr, capacity, C.byref(n), wait_ms)
+        out = []
+        for c in arr[: n.value]:
+            d = {"ticket": c.ticket, "status": c.status, "batch_jobs": c.batch_jobs, "arrival_ns": c.arrival_ns,
+                 "admitted_ns": c.admitted_ns, "done_ns": c.done_ns}
+            if copy:
+                if c.color:
+                    d["color"] = np.frombuffer(C.string_at(c.color, c.color_bytes), np.uint8).copy()
+                if c.depth:
+                    d["depth"] = np.frombuffer(C.string_at(c.depth, c.depth_bytes), np.uint8).copy()
+                self.release(c.ticket)
+            out.append(d)
+        return out
+
+    def release(self, ticket):
+        self.lib.call("srx_server_release", self.h, ticket)
+
+    def now(self) -> int:
+        return int(self.lib.dll.srx_server_now(self.h))
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        self.lib.call("srx_server_get_stats", self.h, C.byref(st))
+        return st
+
+
+def default_server_desc(lib: Lib) -> ServerDesc:
+    d = ServerDesc()
+    lib.dll.srx_default_server_desc(C.byref(d))
+    return d
 
 
 class Batch(_Obj):
@@ -476,6 +581,12 @@ class Renderer(_Obj):
     # batches: commands between batch_begin and batch_end are recorded, not executed
     def batch_begin(self):
         self.lib.call("srx_batch_begin", self.h)
+
+    def create_server(self, desc: ServerDesc = None) -> Server:
+        d = desc if desc is not None else default_server_desc(self.lib)
+        h = C.c_void_p()
+        self.lib.call("srx_create_server", self.h, C.byref(d), C.byref(h))
+        return Server(self.lib, h)
 
     def batch_end(self) -> Batch:
         h = C.c_void_p()
