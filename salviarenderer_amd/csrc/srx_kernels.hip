@@ -633,32 +633,146 @@ __global__ void clip_kernel(draw_params P, const float4* __restrict__ vbuf, floa
 // off: exclusive scan of the per-triangle record counts; a triangle owns [off[id], off[id+1]).
 // One thread per primitive, over its (at most 3) clipped triangles in fan order.  Records beyond
 // `capacity` are dropped; the host compares the total with it and emits again.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  int lane = __lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+// Records of one clipped triangle emitted by a whole wave: lanes take the triangle's 64x64 tiles in
+// for_each_gtile's order, count each tile's GPU tiles with the same tests, and write at their
+// wave-scanned offsets -- for triangles that reach many tiles, which one lane would walk serially.
+__device__ __forceinline__ void emit_wide(const draw_params& P, const float4* __restrict__ tri_rec, uint32_t id,
+                                          uint32_t first, uint32_t next, uint32_t capacity,
+                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int lane = __lane_id();
+  float e[9], bb[4];
+  read_tri_rec(tri_rec + tri_slot(P, id) * 5, e, bb);
+  const int tx = P.tiles_x, ty = P.tiles_y;
+  const int sx = min(fast_floori(std_max(0.0f, bb[0]) / TILE), tx);
+  const int sy = min(fast_floori(std_max(0.0f, bb[2]) / TILE), ty);
+  const int ex = min(fast_ceili(std_max(0.0f, bb[1]) / TILE) + 1, tx);
+  const int ey = min(fast_ceili(std_max(0.0f, bb[3]) / TILE) + 1, ty);
+  const bool small = (sx + 1 == ex) && (sy + 1 == ey);
+  bool mx[3], my[3];
+  float r2a[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    mx[k] = e[k * 3 + 0] > 0;
+    my[k] = e[k * 3 + 1] > 0;
+    const float stx = TILE * e[k * 3 + 0], sty = TILE * e[k * 3 + 1];
+    r2a[k] = -fabsf(stx) - fabsf(sty);
+  }
+  const int sub = TILE / P.gt, w = ex - sx;
+  const int R = (ex > sx && ey > sy) ? w * (ey - sy) : 0;
+  uint32_t base = first;
+  for (int t0 = 0; t0 < R; t0 += 64) {
+    const int t = t0 + lane;
+    uint32_t rows = 0, cols = 0, rmask = 0, cmask = 0;
+    bool full = false;
+    int x = 0, y = 0;
+    if (t < R) {
+      x = sx + t % w;
+      y = sy + t / w;
+      bool rej = false, acc = true;
+      if (!small) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const float ev = e[k * 3 + 2] - ((static_cast<float>(x + mx[k]) * TILE) * e[k * 3 + 0] +
+                                           (static_cast<float>(y + my[k]) * TILE) * e[k * 3 + 1]);
+          rej |= (0 < ev);
+          acc &= (r2a[k] >= ev);
+        }
+        full = acc;
+      }
+      if (!rej) {
+        const float xmin = bb[0] - static_cast<float>(x * TILE), xmax = bb[1] - static_cast<float>(x * TILE);
+        const float ymin = bb[2] - static_cast<float>(y * TILE), ymax = bb[3] - static_cast<float>(y * TILE);
+        for (int j = 0; j < sub; ++j) {
+          if (y * sub + j >= P.gtiles_y) break;
+          if (full || !(ymin >= static_cast<float>(j * P.gt + P.gt) || ymax < static_cast<float>(j * P.gt))) rmask |= 1u << j;
+        }
+        for (int i = 0; i < sub; ++i) {
+          if (x * sub + i >= P.gtiles_x) break;
+          if (full || !(xmin >= static_cast<float>(i * P.gt + P.gt) || xmax < static_cast<float>(i * P.gt))) cmask |= 1u << i;
+        }
+        rows = __popc(rmask);
+        cols = __popc(cmask);
+      }
+    }
+    const uint32_t cnt = rows * cols;
+    const uint32_t incl = wave_incl_scan(cnt);
+    uint32_t o = base + incl - cnt;
+    for (uint32_t rm = rmask; rm && cnt; rm &= rm - 1) {
+      const int j = __builtin_ctz(rm);
+      for (uint32_t cm = cmask; cm; cm &= cm - 1) {
+        const int i = __builtin_ctz(cm);
+        if (o < next && o < capacity) {
+          keys[o] = static_cast<uint32_t>((y * sub + j) * P.gtiles_x + x * sub + i);
+          vals[o] = (id << 1) | (full ? 1u : 0u);
+        }
+        ++o;
+      }
+    }
+    base += static_cast<uint32_t>(__shfl(static_cast<int>(incl), 63, 64));
+  }
+}
+
+constexpr uint32_t EMIT_WIDE = 64;  // records beyond which a triangle is emitted by its whole wave
+
 __device__ __forceinline__ void emit_body(const draw_params& P, const float4* __restrict__ tri_rec,
                                           const uint32_t* __restrict__ off, const uint32_t* __restrict__ total,
                                           uint32_t capacity, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                           uint32_t bx) {
   const uint32_t p = bx * blockDim.x + threadIdx.x;
   const uint32_t n = 3 * P.prim_count;
-  if (p >= P.prim_count) return;
-  const uint32_t id3 = 3 * p;
-  const uint32_t o0 = off[id3], o1 = off[id3 + 1], o2 = off[id3 + 2];
-  const uint32_t o3 = (id3 + 3 < n) ? off[id3 + 3] : *total;
+  uint32_t o[4] = {0, 0, 0, 0};
+  uint32_t wide = 0;  // fan triangles left to the wave
+  if (p < P.prim_count) {
+    const uint32_t id3 = 3 * p;
+    o[0] = off[id3];
+    o[1] = off[id3 + 1];
+    o[2] = off[id3 + 2];
+    o[3] = (id3 + 3 < n) ? off[id3 + 3] : *total;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const uint32_t id = id3 + k;
-    const uint32_t first = k == 0 ? o0 : (k == 1 ? o1 : o2);
-    const uint32_t next = k == 0 ? o1 : (k == 1 ? o2 : o3);
-    if (next == first) continue;
-    float e[9], bb[4];
-    read_tri_rec(tri_rec + tri_slot(P, id) * 5, e, bb);
-    uint32_t o = first;
-    for_each_gtile(P, e, bb, [&](uint32_t g, bool full) {
-      if (o < next && o < capacity) {  // the enumeration repeats the counting pass exactly
-        keys[o] = g;
-        vals[o] = (id << 1) | (full ? 1u : 0u);
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t id = id3 + k;
+      const uint32_t first = o[k], next = o[k + 1];
+      if (next == first) continue;
+      if (next - first > EMIT_WIDE) {
+        wide |= 1u << k;
+        continue;
       }
-      ++o;
-    });
+      float e[9], bb[4];
+      read_tri_rec(tri_rec + tri_slot(P, id) * 5, e, bb);
+      uint32_t q = first;
+      for_each_gtile(P, e, bb, [&](uint32_t g, bool full) {
+        if (q < next && q < capacity) {  // the enumeration repeats the counting pass exactly
+          keys[q] = g;
+          vals[q] = (id << 1) | (full ? 1u : 0u);
+        }
+        ++q;
+      });
+    }
+  }
+  // every lane of the wave takes part in its lanes' wide triangles, one after another
+  unsigned long long pending = __ballot(wide != 0);
+  while (pending) {
+    const int src = __ffsll(static_cast<long long>(pending)) - 1;
+    pending &= pending - 1;
+    const uint32_t ws = static_cast<uint32_t>(__shfl(static_cast<int>(wide), src, 64));
+    const uint32_t ps = static_cast<uint32_t>(__shfl(static_cast<int>(p), src, 64));
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (!(ws & (1u << k))) continue;
+      const uint32_t first = static_cast<uint32_t>(__shfl(static_cast<int>(o[k]), src, 64));
+      const uint32_t next = static_cast<uint32_t>(__shfl(static_cast<int>(o[k + 1]), src, 64));
+      emit_wide(P, tri_rec, 3 * ps + k, first, next, capacity, keys, vals);
+    }
   }
 }
 __global__ void __launch_bounds__(256) emit_kernel(draw_params P, const float4* __restrict__ tri_rec,
@@ -676,16 +790,6 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
 // ============================================================== scan (exclusive, u32)
 constexpr int SCAN_PER_THREAD = 16;
 constexpr int SCAN_BLOCK = 256 * SCAN_PER_THREAD;  // elements per workgroup
-
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-  int lane = __lane_id();
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    uint32_t t = __shfl_up(v, off, 64);
-    if (lane >= off) v += t;
-  }
-  return v;
-}
 
 // Exclusive scan of one SCAN_BLOCK-element block in place.  The block total goes to sums[block]
 // (when sums is given); a single-block launch also stores it to *total.

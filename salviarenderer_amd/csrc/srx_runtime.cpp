@@ -1314,8 +1314,9 @@ struct server_obj : object {
   }
 };
 
-uint64_t cache_key(uint64_t mesh, uint32_t w, uint32_t h) {
-  return mesh * 0x9E3779B97F4A7C15ull ^ (static_cast<uint64_t>(w) << 32 | h);
+// Records depend on the mesh and the target size (the sample count is kept apart too).
+uint64_t cache_key(const srx_request& q) {
+  return (q.mesh_key * 0x9E3779B97F4A7C15ull) ^ (static_cast<uint64_t>(q.width) << 36 | static_cast<uint64_t>(q.height) << 8 | q.samples);
 }
 
 // draw_params of a served request: the state build_draw derives for this pipeline (pass-through
@@ -1406,7 +1407,7 @@ job_layout layout_of(const srx_request& q, uint32_t records, uint32_t bin_items)
 }
 
 uint32_t estimate_records(server_obj* S, const srx_request& q, bool* exact) {
-  auto it = S->records_cache.find(cache_key(q.mesh_key, q.width, q.height));
+  auto it = S->records_cache.find(cache_key(q));
   *exact = it != S->records_cache.end();
   if (*exact) return it->second;
   const uint64_t T = ((q.width + 15) / 16) * ((q.height + 15) / 16);
@@ -1545,6 +1546,7 @@ int32_t launch_served_batch(server_obj* S, int si) {
   while (!S->queue.empty() && take.size() < S->desc.max_batch_jobs) {
     srv_job& q = S->queue.front();
     if (q.req.arrival_ns > now) break;
+    if (!q.exact) q.est_records = estimate_records(S, q.req, &q.exact);  // measured since submission?
     const uint32_t est = q.est_records;
     const job_layout L = layout_of(q.req, est, items);
     const uint64_t n = q.req.n_tris, T = ((q.req.width + 15) / 16) * ((q.req.height + 15) / 16);
@@ -1816,7 +1818,7 @@ void finish_batch(server_obj* S, srv_batch& b) {
     const uint32_t rec = b.counts_h[j];
     end += rec;
     const bool ok = end <= b.capacity && rec <= b.bucket_blocks[j] * items && !b.counts_h[J + j];
-    S->records_cache[cache_key(t.req.mesh_key, t.req.width, t.req.height)] = rec;
+    S->records_cache[cache_key(t.req)] = rec;
     if (!ok && !b.counts_h[J + j]) {
       S->host_arena.release(t.stage_off, t.stage_bytes);
       t.est_records = rec;
