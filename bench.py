@@ -160,8 +160,8 @@ def batched_jobs(lib, launches, plain_passes=1, autotune_reps=1, config=None):
             "one_by_one_frames_per_s": n / plain_s, "one_by_one_ms": plain_s * 1e3,
             "speedup": plain_s / batch_s, "status": status,
             "mtri_per_s": info.primitives / batch_s / 1e6,
-            "tuning": {"bin_items": info.bin_items, "vis_threads": info.vis_threads},
-            "sweep_ms": [[bi, th, round(ms, 3)] for bi, th, ms in sweep],
+            "tuning": {"tile": info.tile, "bin_items": info.bin_items, "vis_threads": info.vis_threads},
+            "sweep_ms": [[tl, bi, th, round(ms, 3)] for tl, bi, th, ms in sweep],
             "device_bytes": info.device_bytes, "records": info.records}
 
 
@@ -261,7 +261,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the secondary serving measurement")
     ap.add_argument("--serving-only", action="store_true", help="only the serving measurement")
     ap.add_argument("--serving-requests", type=int, default=10000)
-    ap.add_argument("--batch-config", default="", metavar="BIN_ITEMS,THREADS",
+    ap.add_argument("--batch-config", default="", metavar="TILE,BIN_ITEMS,THREADS",
                     help="configure the batch instead of autotuning it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel used to calibrate HBM byte counters, then exit")

@@ -339,13 +339,15 @@ int32_t srx_flush(srx_renderer r);
 typedef struct srx_batch_t* srx_batch;
 typedef struct srx_batch_info {
   uint32_t jobs, clears, resolves;
+  uint32_t tile;         /* GPU tile side (8, 16, 32) */
   uint32_t bin_items;    /* records binned per workgroup (2048, 4096, 8192) */
-  uint32_t vis_threads;  /* lanes per 16x16 tile in the visibility kernel (64, 128, 256) */
+  uint32_t vis_threads;  /* lanes per GPU tile in the visibility kernel (64, 128, 256) */
   uint32_t vis_launches; /* visibility launches (one per shader/sample-count variant) */
+  uint32_t pad;
   uint64_t primitives, records, device_bytes;
 } srx_batch_info;
 typedef struct srx_batch_tuning {
-  int32_t bin_items, vis_threads;
+  int32_t tile, bin_items, vis_threads;
   float ms; /* mean time of one launch */
 } srx_batch_tuning;
 int32_t srx_batch_begin(srx_renderer r);
@@ -355,9 +357,13 @@ int32_t srx_batch_launch(srx_batch b); /* queued on the renderer's stream */
 /* Waits for the renderer's stream; status bit 0: records beyond the prepared capacity were
  * dropped, bit 1: clipping overflow.  Zero after every complete launch. */
 int32_t srx_batch_status(srx_batch b, uint32_t* status);
-int32_t srx_batch_configure(srx_batch b, int32_t bin_items, int32_t vis_threads);
-/* Times `reps` launches of every (bin_items, vis_threads) pair, reports them in results (up to
- * capacity; *count = pairs tried) and leaves the batch configured with the fastest. */
+/* GPU tile side 8, 16 or 32 (8 and 32: single-sample jobs, and at most 8192 tiles per job), records
+ * per binning workgroup 2048, 4096 or 8192, lanes per tile 64, 128 or 256.  A new tile size
+ * repeats the counting pass once. */
+int32_t srx_batch_configure(srx_batch b, int32_t tile, int32_t bin_items, int32_t vis_threads);
+/* Times `reps` launches of every (tile, bin_items, vis_threads) the batch's jobs allow, reports
+ * them in results (up to capacity; *count = configurations tried) and leaves the batch
+ * configured with the fastest. */
 int32_t srx_batch_autotune(srx_batch b, uint32_t reps, srx_batch_tuning* results, size_t capacity, size_t* count);
 int32_t srx_batch_get_info(srx_batch b, srx_batch_info* info);
 

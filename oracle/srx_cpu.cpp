@@ -2035,7 +2035,8 @@ int32_t srx_batch_status(srx_batch bb, uint32_t* status) {
   *status = 0;
   return SRX_OK;
 }
-int32_t srx_batch_configure(srx_batch bb, int32_t bin_items, int32_t vis_threads) {
+int32_t srx_batch_configure(srx_batch bb, int32_t tile, int32_t bin_items, int32_t vis_threads) {
+  (void)tile;
   (void)bin_items;
   (void)vis_threads;
   return as<batch_obj>(bb, K_BATCH) ? SRX_OK : SRX_INVALID_PARAMETER;

@@ -89,13 +89,13 @@ class InternalStatistics(C.Structure):
 
 
 class BatchInfo(C.Structure):
-    _fields_ = [("jobs", C.c_uint32), ("clears", C.c_uint32), ("resolves", C.c_uint32), ("bin_items", C.c_uint32),
-                ("vis_threads", C.c_uint32), ("vis_launches", C.c_uint32), ("primitives", C.c_uint64),
-                ("records", C.c_uint64), ("device_bytes", C.c_uint64)]
+    _fields_ = [("jobs", C.c_uint32), ("clears", C.c_uint32), ("resolves", C.c_uint32), ("tile", C.c_uint32),
+                ("bin_items", C.c_uint32), ("vis_threads", C.c_uint32), ("vis_launches", C.c_uint32),
+                ("pad", C.c_uint32), ("primitives", C.c_uint64), ("records", C.c_uint64), ("device_bytes", C.c_uint64)]
 
 
 class BatchTuning(C.Structure):
-    _fields_ = [("bin_items", C.c_int32), ("vis_threads", C.c_int32), ("ms", C.c_float)]
+    _fields_ = [("tile", C.c_int32), ("bin_items", C.c_int32), ("vis_threads", C.c_int32), ("ms", C.c_float)]
 
 
 class ServerDesc(C.Structure):
@@ -188,7 +188,7 @@ PROTOTYPES = {
     "srx_batch_end": (_I32, [_H, C.POINTER(_H)]),
     "srx_batch_launch": (_I32, [_H]),
     "srx_batch_status": (_I32, [_H, C.POINTER(_U32)]),
-    "srx_batch_configure": (_I32, [_H, _I32, _I32]),
+    "srx_batch_configure": (_I32, [_H, _I32, _I32, _I32]),
     "srx_batch_autotune": (_I32, [_H, _U32, C.POINTER(BatchTuning), _SZ, C.POINTER(_SZ)]),
     "srx_batch_get_info": (_I32, [_H, C.POINTER(BatchInfo)]),
     "srx_default_server_desc": (None, [C.POINTER(ServerDesc)]),
@@ -399,15 +399,15 @@ class Batch(_Obj):
         self.lib.call("srx_batch_status", self.h, C.byref(st))
         return st.value
 
-    def configure(self, bin_items, vis_threads):
-        self.lib.call("srx_batch_configure", self.h, bin_items, vis_threads)
+    def configure(self, tile, bin_items, vis_threads):
+        self.lib.call("srx_batch_configure", self.h, tile, bin_items, vis_threads)
 
     def autotune(self, reps=3):
-        """Times every (bin_items, vis_threads) pair; the batch keeps the fastest."""
-        res = (BatchTuning * 16)()
+        """Times every (tile, bin_items, vis_threads) the jobs allow; the batch keeps the fastest."""
+        res = (BatchTuning * 32)()
         n = C.c_size_t()
-        self.lib.call("srx_batch_autotune", self.h, reps, res, 16, C.byref(n))
-        return [(r.bin_items, r.vis_threads, r.ms) for r in res[: min(n.value, 16)]]
+        self.lib.call("srx_batch_autotune", self.h, reps, res, 32, C.byref(n))
+        return [(r.tile, r.bin_items, r.vis_threads, r.ms) for r in res[: min(n.value, 32)]]
 
     def info(self) -> BatchInfo:
         d = BatchInfo()

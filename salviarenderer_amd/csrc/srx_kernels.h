@@ -95,9 +95,11 @@ void launch_bucket_batch(int pt, int phase, const batch_job* jobs, stage_map m, 
                          hipStream_t s);
 void launch_range_batch(const batch_job* jobs, stage_map m, uint32_t blocks, hipStream_t s);
 void launch_tile_order_batch(const batch_job* jobs, uint32_t njobs, hipStream_t s);
-// threads: 64, 128 or 256 lanes per 16x16 tile
-void launch_vis_batch(int na, int samples, bool shade, int threads, const batch_job* jobs, stage_map m, uint32_t blocks,
-                      hipStream_t s);
+// threads: 64, 128 or 256 lanes per GPU tile; gt: GPU tile side 8, 16 or 32 (8 and 32 for single-sample
+// draws).  False for a combination without a kernel.
+bool vis_batch_supported(int samples, bool shade, int threads, int gt);
+bool launch_vis_batch(int na, int samples, bool shade, int threads, int gt, const batch_job* jobs, stage_map m,
+                      uint32_t blocks, hipStream_t s);
 void launch_batch_check(const batch_job* jobs, uint32_t njobs, uint32_t capacity, int pt, uint32_t* status, hipStream_t s);
 // out[j]: records of job j; out[njobs + j]: its clip-overflow counter
 void launch_batch_counts(const batch_job* jobs, uint32_t njobs, uint32_t* out, hipStream_t s);

@@ -17,6 +17,8 @@
 #include "srx_kernels.h"
 #include "srx_sampler.h"
 
+#include <type_traits>
+
 namespace srx {
 
 __constant__ float c_ewa[256];
@@ -1895,13 +1897,19 @@ __device__ __forceinline__ void write_winner_depth(const draw_params& P, int mod
 #ifndef SRX_VIS_THREADS
 #define SRX_VIS_THREADS 256
 #endif
-// LDS shared by phase 1 (a chunk of NT records + their block work list) and phase 3 (shading items)
-__host__ __device__ constexpr int vis_union_bytes(int nt, int s, bool shade) {
-  return (nt * 64 + nt * 16 * 2) > (shade ? 256 * s * 8 : 0) ? (nt * 64 + nt * 16 * 2) : 256 * s * 8;
+// LDS shared by phase 1 (a chunk of NT records + their (record, block) work list) and phase 3
+// (shading items), for GT x GT GPU tiles
+__host__ __device__ constexpr int vis_union_bytes(int nt, int s, bool shade, int gt = 16) {
+  return (nt * 64 + nt * (gt / 4) * (gt / 4) * 2) > (shade ? gt * gt * s * 8 : 0) ? (nt * 64 + nt * (gt / 4) * (gt / 4) * 2)
+                                                                                   : gt * gt * s * 8;
 }
+// Static LDS of vis_body besides the union (keys, initial depths).
+__host__ __device__ constexpr int vis_static_bytes(int s, int gt) { return gt * gt * s * 12 + 256; }
 // NTC > 0: NTC lanes per workgroup, static LDS.  NTC == 0: blockDim.x lanes (64, 128 or 256) with
-// the phase-1/3 union in dynamic LDS of vis_union_bytes(blockDim.x, S, SHADE) bytes.
-template <int NA, int S, bool SHADE, int NTC>
+// the phase-1/3 union in dynamic LDS of vis_union_bytes(blockDim.x, S, SHADE, GT) bytes.
+// GT: GPU tile side, 8, 16 or 32 pixels: a quarter of a 16x16 stage-16 region, one region, or 2x2
+// regions of the reference's 64x64 tile; the 4x4 blocks are the reference's stage-4 blocks either way.
+template <int NA, int S, bool SHADE, int NTC, int GT>
 __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pass, const uint32_t* __restrict__ order,
                                          const uint4* __restrict__ entries, const uint4* __restrict__ merges,
                                          const uint32_t* __restrict__ counts, unsigned long long* __restrict__ partial,
@@ -1909,19 +1917,24 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
                                          const uint32_t* __restrict__ tend, const float4* __restrict__ tri_rec,
                                          const float4* __restrict__ tri_attr, const ps_samplers* __restrict__ SM,
                                          uint32_t bx) {
+  static_assert(GT == 8 || GT == 16 || GT == 32, "GPU tile side");
+  constexpr int PX = GT * GT;       // pixels per GPU tile
+  constexpr int NBS = GT / 4;       // 4x4 blocks per side
+  constexpr int RPS = GT > 16 ? GT / 16 : 1;  // stage-16 regions per side
+  using bmask_t = typename std::conditional<(GT > 16), unsigned long long, uint32_t>::type;
   const int NT = NTC > 0 ? NTC : static_cast<int>(blockDim.x);
-  constexpr int UNION_BYTES = NTC > 0 ? vis_union_bytes(NTC, S, SHADE) : 16;
-  __shared__ unsigned long long keys[256 * S];
-  __shared__ float d0s[256 * S];
+  constexpr int UNION_BYTES = NTC > 0 ? vis_union_bytes(NTC, S, SHADE, GT) : 16;
+  __shared__ unsigned long long keys[PX * S];
+  __shared__ float d0s[PX * S];
   __shared__ __align__(16) unsigned char lds_static[UNION_BYTES];
   extern __shared__ __align__(16) unsigned char lds_dynamic[];
   unsigned char* const lds_union = NTC > 0 ? lds_static : lds_dynamic;
   __shared__ uint32_t wsum[4];
-  __shared__ uint32_t hiz[16];
+  __shared__ uint32_t hiz[NBS * NBS];
   __shared__ uint32_t nitems;
   float4* recs = reinterpret_cast<float4*>(lds_union);                           // [NT][4]
-  uint16_t* work = reinterpret_cast<uint16_t*>(lds_union + NT * 64);             // [NT * 16]
-  uint2* items = reinterpret_cast<uint2*>(lds_union);                            // [256 * S]
+  uint16_t* work = reinterpret_cast<uint16_t*>(lds_union + NT * 64);             // [NT * NBS^2]
+  uint2* items = reinterpret_cast<uint2*>(lds_union);                            // [PX * S]
   // pass 0: one tile, or one part of a split tile; pass 1: merge and finish a split tile
   int gtile;
   uint32_t part = 0, parts = 1, slot = 0;
@@ -1942,7 +1955,7 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
     slot = mg.z;
   }
   const int gx = gtile % P.gtiles_x, gy = gtile / P.gtiles_x;
-  const int ox = gx * 16, oy = gy * 16;
+  const int ox = gx * GT, oy = gy * GT;
   const int t64x = ox & ~(TILE - 1), t64y = oy & ~(TILE - 1);
   const int tw = static_cast<int>(P.target_w), th = static_cast<int>(P.target_h);
   const int tid = threadIdx.x;
@@ -1950,18 +1963,18 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
   if (tid == 0) nitems = 0;
   if (pass == 1) {
     // the parts' keys, each started from the target's depth: their minimum is the tile's result
-    const unsigned long long* src = partial + static_cast<size_t>(slot) * (256 * S);
-    for (int i = tid; i < 256 * S; i += NT) {
+    const unsigned long long* src = partial + static_cast<size_t>(slot) * (PX * S);
+    for (int i = tid; i < PX * S; i += NT) {
       unsigned long long k = src[i];
       for (uint32_t q = 1; q < parts; ++q) {
-        const unsigned long long kq = src[static_cast<size_t>(q) * (256 * S) + i];
+        const unsigned long long kq = src[static_cast<size_t>(q) * (PX * S) + i];
         k = kq < k ? kq : k;
       }
       keys[i] = k;
     }
   } else {
-    for (int i = tid; i < 256 * S; i += NT) {
-      int px = ox + ((i / S) & 15), py = oy + ((i / S) >> 4), smp = i % S;
+    for (int i = tid; i < PX * S; i += NT) {
+      int px = ox + ((i / S) % GT), py = oy + ((i / S) / GT), smp = i % S;
       float d0 = 0.0f;
       bool blocked = false;
       if (has_ds && px < tw && py < th) {
@@ -1984,7 +1997,10 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
   const uint32_t t0 = pass == 1 ? tr1 : min(tr1, tr0 + part * VIS_PART);
   const uint32_t t1 = (pass == 1 || entries == nullptr) ? tr1 : min(tr1, t0 + VIS_PART);
   const uint32_t full_px = (S == 1) ? 1u : (S == 2 ? 3u : 15u);
-  const int jx = (ox - t64x) >> 4, jy = (oy - t64y) >> 4;
+  const int jx0 = (ox - t64x) >> 4, jy0 = (oy - t64y) >> 4;  // first stage-16 region of the tile
+  // an 8x8 tile covers blocks [qx, qx+2) x [qy, qy+2) of its region; larger tiles whole regions
+  const int qx = GT < 16 ? ((ox - t64x) & 15) >> 2 : 0, qy = GT < 16 ? ((oy - t64y) & 15) >> 2 : 0;
+  constexpr int CW = GT < 16 ? GT / 4 : 4;  // block columns (rows) of a region inside the tile
   const float tvx = static_cast<float>(t64x), tvy = static_cast<float>(t64y);
   // Depth culling from the second chunk on: hiz[b] is the largest key depth word of 4x4 block b's
   // samples after the previous chunks.  Keys only decrease, so a record whose smallest possible
@@ -1995,22 +2011,23 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
     __syncthreads();  // the previous chunk's work list is consumed (and the keys are initialised)
     const bool hiz_on = depth_keys && c0 > t0;
     if (hiz_on) {
-      // 64 consecutive pixels are one row of 4x4 blocks; lanes differing in bits 0, 1 (x) and 4, 5
-      // (y) share a block
-      for (int i = tid; i < 256; i += NT) {
+      // 16 consecutive lanes take one block's 16 pixels
+      for (int u = tid; u < PX; u += NT) {
+        const int b = u >> 4, q = u & 15;
+        const int pl = ((b / NBS) * 4 + (q >> 2)) * GT + (b % NBS) * 4 + (q & 3);
         uint32_t m = 0;
 #pragma unroll
-        for (int smp = 0; smp < S; ++smp) m = max(m, static_cast<uint32_t>(keys[i * S + smp] >> 32));
+        for (int smp = 0; smp < S; ++smp) m = max(m, static_cast<uint32_t>(keys[pl * S + smp] >> 32));
         m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), 1, 64)));
         m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), 2, 64)));
-        m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), 16, 64)));
-        m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), 32, 64)));
-        if ((i & 0x33) == 0) hiz[(i >> 6) * 4 + ((i >> 2) & 3)] = m;
+        m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), 4, 64)));
+        m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), 8, 64)));
+        if (q == 0) hiz[b] = m;
       }
       __syncthreads();
     }
     // phase 1a: one record per lane -> the 4x4 blocks of this GPU tile it reaches (full or partial)
-    uint32_t bm = 0, fm = 0;
+    bmask_t bm = 0, fm = 0;
     const uint32_t ri = c0 + tid;
     if (ri < t1) {
       const uint32_t rec = vals[ri];
@@ -2036,49 +2053,56 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
         s2x[k] = sx; s2y[k] = sy; ra2[k] = ra;
         ev2[k] = ez - part;
       }
-      bool region_full = (rec & 1) != 0;
-      bool keep = true;
-      if (!region_full) {
-        // stage 16 (subdivide_tile on the 64x64 tile)
-        bool rej = false, nacc = false;
+      const bool rec_full = (rec & 1) != 0;  // the record covers its whole 64x64 tile
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          float r = (s1x[k] * static_cast<float>(jx)) + (s1y[k] * static_cast<float>(jy));
-          float a = r + ra1[k];
-          rej |= r < ev1[k];
-          nacc |= a < ev1[k];
+      for (int rr = 0; rr < RPS * RPS; ++rr) {
+        const int jx = jx0 + rr % RPS, jy = jy0 + rr / RPS;
+        bool region_full = rec_full;
+        bool keep = true;
+        if (!region_full) {
+          // stage 16 (subdivide_tile on the 64x64 tile)
+          bool rej = false, nacc = false;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            float r = (s1x[k] * static_cast<float>(jx)) + (s1y[k] * static_cast<float>(jy));
+            float a = r + ra1[k];
+            rej |= r < ev1[k];
+            nacc |= a < ev1[k];
+          }
+          rej |= bx0 >= static_cast<float>(jx * 16 + 16);
+          rej |= bx1 < static_cast<float>(jx * 16);
+          rej |= by0 >= static_cast<float>(jy * 16 + 16);
+          rej |= by1 < static_cast<float>(jy * 16);
+          keep = !rej;
+          region_full = !nacc;
         }
-        rej |= bx0 >= static_cast<float>(jx * 16 + 16);
-        rej |= bx1 < static_cast<float>(jx * 16);
-        rej |= by0 >= static_cast<float>(jy * 16 + 16);
-        rej |= by1 < static_cast<float>(jy * 16);
-        keep = !rej;
-        region_full = !nacc;
-      }
-      if (keep) {
+        if (!keep) continue;
+        const int rox = t64x + jx * 16, roy = t64y + jy * 16;  // the region's origin
         float ev2q[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k)
-          ev2q[k] = ev2[k] - ((static_cast<float>(ox) * e[k * 3]) + (static_cast<float>(oy) * e[k * 3 + 1]));
-        // candidate 4x4 blocks: all of them for an accepted region, else those the stage-4
-        // bounding-box test keeps (bx0 < mxr + 4 and bx1 >= mxr, the same comparisons on the same floats)
+          ev2q[k] = ev2[k] - ((static_cast<float>(rox) * e[k * 3]) + (static_cast<float>(roy) * e[k * 3 + 1]));
+        // candidate 4x4 blocks of the region inside this tile: all of them for an accepted region,
+        // else those the stage-4 bounding-box test keeps (bx0 < mxr + 4 and bx1 >= mxr, the same
+        // comparisons on the same floats)
         uint32_t cols = 0, rows = 0;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const int mxr = jx * 16 + c * 4, myr = jy * 16 + c * 4;
-          if (region_full || !(bx0 >= static_cast<float>(mxr + 4) || bx1 < static_cast<float>(mxr))) cols |= 0x1111u << c;
-          if (region_full || !(by0 >= static_cast<float>(myr + 4) || by1 < static_cast<float>(myr))) rows |= 0xFu << (c * 4);
-          if (ox + c * 4 >= tw) cols &= ~(0x1111u << c);  // blocks outside the target
-          if (oy + c * 4 >= th) rows &= ~(0xFu << (c * 4));
+        for (int cc = 0; cc < CW; ++cc) {
+          const int cx = qx + cc, cy = qy + cc;
+          const int mxr = jx * 16 + cx * 4, myr = jy * 16 + cy * 4;
+          if (region_full || !(bx0 >= static_cast<float>(mxr + 4) || bx1 < static_cast<float>(mxr))) cols |= 0x1111u << cx;
+          if (region_full || !(by0 >= static_cast<float>(myr + 4) || by1 < static_cast<float>(myr))) rows |= 0xFu << (cy * 4);
+          if (rox + cx * 4 >= tw) cols &= ~(0x1111u << cx);  // blocks outside the target
+          if (roy + cy * 4 >= th) rows &= ~(0xFu << (cy * 4));
         }
         uint32_t blocks = cols & rows;
         while (blocks) {
-          const int b = __builtin_ctz(blocks);
+          const int b16 = __builtin_ctz(blocks);
           blocks &= blocks - 1;
+          const int ix4 = b16 & 3, iy4 = b16 >> 2;
           bool block_full = region_full;
           if (!block_full) {
             // stage 4 (subdivide_tile on the 16x16 region); the bounding-box half was applied above
-            const int ix4 = b & 3, iy4 = b >> 2;
             bool rej = false, nacc = false;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -2090,11 +2114,13 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
             if (rej) continue;
             block_full = !nacc;
           }
+          // the block's index in this tile
+          const int b = ((rr / RPS) * 4 + iy4 - qy) * NBS + (rr % RPS) * 4 + ix4 - qx;
           if (hiz_on) {
             // the depth plane's extreme over the block's sample positions [Bx, Bx+4) x [By, By+4),
             // widened by far more than the rounding of the per-sample evaluation
             const float z0 = d3.w, zx = d3.y, zy = d3.z, dzx = d4.x, dzy = d4.y;
-            const float ax = static_cast<float>(ox + (b & 3) * 4) - zx, ay = static_cast<float>(oy + (b >> 2) * 4) - zy;
+            const float ax = static_cast<float>(rox + ix4 * 4) - zx, ay = static_cast<float>(roy + iy4 * 4) - zy;
             const float tx0 = dzx * ax, tx1 = dzx * (ax + 4.0f), ty0 = dzy * ay, ty1 = dzy * (ay + 4.0f);
             const float margin = (fabsf(z0) + fmaxf(fabsf(tx0), fabsf(tx1)) + fmaxf(fabsf(ty0), fabsf(ty1))) * 1.52587890625e-05f;
             const float zext = far_is_min ? (z0 + fmaxf(tx0, tx1) + fmaxf(ty0, ty1)) + margin
@@ -2104,8 +2130,8 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
               if (zk > hiz[b]) continue;
             }
           }
-          bm |= 1u << b;
-          if (block_full) fm |= 1u << b;
+          bm |= static_cast<bmask_t>(1) << b;
+          if (block_full) fm |= static_cast<bmask_t>(1) << b;
         }
       }
       float4* rs = recs + tid * 4;
@@ -2115,7 +2141,7 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
       rs[3] = make_float4(d4.x, d4.y, __uint_as_float(id), 0.0f);  // dzx, dzy, id
     }
     // phase 1b: work list of (record, block) pairs, spread evenly over the lanes
-    const uint32_t cnt = __popc(bm);
+    const uint32_t cnt = GT > 16 ? static_cast<uint32_t>(__popcll(static_cast<unsigned long long>(bm))) : __popc(static_cast<uint32_t>(bm));
     const uint32_t incl = wave_incl_scan(cnt);
     if ((tid & 63) == 63) wsum[tid >> 6] = incl;
     __syncthreads();
@@ -2127,16 +2153,16 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
       total += wsum[w];
     }
     while (bm) {
-      const uint32_t b = __builtin_ctz(bm);
+      const uint32_t b = static_cast<uint32_t>(__builtin_ctzll(static_cast<unsigned long long>(bm)));
       bm &= bm - 1;
-      work[off++] = static_cast<uint16_t>((tid << 5) | (b << 1) | ((fm >> b) & 1u));
+      work[off++] = static_cast<uint16_t>((tid << 7) | (b << 1) | static_cast<uint32_t>((fm >> b) & 1u));
     }
     __syncthreads();
     // phase 1c: one 4x4 block per lane: per-sample coverage (draw_partial_tile, rasterizer.cpp:298-439),
     // quad depth (step_2d_unproj_pos_quad) and the key atomic-min
     for (uint32_t wi = tid; wi < total; wi += NT) {
       const uint32_t wk = work[wi];
-      const int j = static_cast<int>(wk >> 5), b = static_cast<int>((wk >> 1) & 15u);
+      const int j = static_cast<int>(wk >> 7), b = static_cast<int>((wk >> 1) & 63u);
       const bool block_full = (wk & 1u) != 0;
       const float4 r0 = recs[j * 4 + 0], r1 = recs[j * 4 + 1], r2 = recs[j * 4 + 2], r3 = recs[j * 4 + 3];
       const float e[9] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x};
@@ -2145,7 +2171,7 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
       float aa[S];
 #pragma unroll
       for (int s = 0; s < S; ++s) aa[s] = (S > 1) ? (P.sp[s][0] - 0.5f) * dzx + (P.sp[s][1] - 0.5f) * dzy : 0.0f;
-      const int Bx = ox + (b & 3) * 4, By = oy + (b >> 2) * 4;
+      const int Bx = ox + (b % NBS) * 4, By = oy + (b / NBS) * 4;
       float evp[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k)
@@ -2177,7 +2203,7 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
               if (!r) m |= 1u << s;
             }
           }
-          const int li = ((py - oy) * 16 + (px - ox)) * S;
+          const int li = ((py - oy) * GT + (px - ox)) * S;
 #pragma unroll
           for (int s = 0; s < S; ++s) {
             if (!(m & (1u << s))) continue;
@@ -2191,13 +2217,13 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
   }
   __syncthreads();
   if (pass == 0 && parts > 1) {  // one part of a split tile: leave the partial keys for the merge
-    unsigned long long* dst = partial + static_cast<size_t>(slot) * (256 * S);
-    for (int i = tid; i < 256 * S; i += NT) dst[i] = keys[i];
+    unsigned long long* dst = partial + static_cast<size_t>(slot) * (PX * S);
+    for (int i = tid; i < PX * S; i += NT) dst[i] = keys[i];
     return;
   }
   // phase 2: winners -> depth, shading items
-  for (int p = tid; p < 256; p += NT) {
-    const int px = ox + (p & 15), py = oy + (p >> 4);
+  for (int p = tid; p < PX; p += NT) {
+    const int px = ox + (p % GT), py = oy + (p / GT);
     if (px >= tw || py >= th) continue;
     uint32_t w[S];
     uint32_t left = 0;
@@ -2223,7 +2249,7 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
         for (int smp = 0; smp < S; ++smp)
           if (w[smp] == cur) m |= 1u << smp;
         left &= ~m;
-        items[atomicAdd(&nitems, 1u)] = make_uint2(cur, static_cast<uint32_t>(p) | (m << 8));
+        items[atomicAdd(&nitems, 1u)] = make_uint2(cur, static_cast<uint32_t>(p) | (m << 12));
       }
     }
   }
@@ -2233,9 +2259,9 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
   const uint32_t n = nitems;
   for (uint32_t j = tid; j < n; j += NT) {
     const uint2 item = items[j];
-    const uint32_t cur = item.x, p = item.y & 255u, m = item.y >> 8;
+    const uint32_t cur = item.x, p = item.y & 4095u, m = item.y >> 12;
     if (cur >= 3u * P.prim_count) continue;  // every winner id is below 3 * prim_count
-    const int px = ox + static_cast<int>(p & 15u), py = oy + static_cast<int>(p >> 4);
+    const int px = ox + static_cast<int>(p % GT), py = oy + static_cast<int>(p / GT);
     const int pi = (px & 1) | ((py & 1) << 1);
     attr_setup<NA> at;
     load_attr<NA>(P, tri_attr, cur, at);
@@ -2257,8 +2283,8 @@ __global__ void __launch_bounds__(SRX_VIS_THREADS, 4) vis_kernel(draw_params P, 
                                                    const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ tend,
                                                    const float4* __restrict__ tri_rec, const float4* __restrict__ tri_attr,
                                                    const ps_samplers* __restrict__ SM) {
-  vis_body<NA, S, SHADE, SRX_VIS_THREADS>(P, mode, pass, order, entries, merges, counts, partial, vals, tstart, tend,
-                                          tri_rec, tri_attr, SM, blockIdx.x);
+  vis_body<NA, S, SHADE, SRX_VIS_THREADS, 16>(P, mode, pass, order, entries, merges, counts, partial, vals, tstart, tend,
+                                              tri_rec, tri_attr, SM, blockIdx.x);
 }
 
 
@@ -2676,11 +2702,11 @@ __global__ void __launch_bounds__(1024) tile_order_batch_kernel(const batch_job*
   tile_order_body(J.B.tstart, J.B.tend, J.tiles, const_cast<uint32_t*>(J.B.order));
 }
 
-template <int NA, int S, bool SHADE>
+template <int NA, int S, bool SHADE, int GT>
 __global__ void __launch_bounds__(256, 4) vis_batch_kernel(const batch_job* __restrict__ jobs, stage_map m) {
   uint32_t lb, nb;
   const batch_job& J = jobs[stage_job(m, lb, nb)];
-  vis_body<NA, S, SHADE, 0>(J.P, J.vis_mode, 0, J.B.order, nullptr, nullptr, nullptr, nullptr, J.B.vals[1], J.B.tstart,
+  vis_body<NA, S, SHADE, 0, GT>(J.P, J.vis_mode, 0, J.B.order, nullptr, nullptr, nullptr, nullptr, J.B.vals[1], J.B.tstart,
                             J.B.tend, J.B.tri_rec, J.B.tri_attr, J.B.samplers, lb);
 }
 
@@ -2760,29 +2786,54 @@ void launch_range_batch(const batch_job* jobs, stage_map m, uint32_t blocks, hip
 void launch_tile_order_batch(const batch_job* jobs, uint32_t njobs, hipStream_t s) {
   if (njobs) hipLaunchKernelGGL(tile_order_batch_kernel, dim3(njobs), dim3(1024), 0, s, jobs);
 }
-void launch_vis_batch(int na, int samples, bool shade, int threads, const batch_job* jobs, stage_map m, uint32_t blocks,
-                      hipStream_t s) {
-  if (!blocks) return;
-  const size_t lds = static_cast<size_t>(vis_union_bytes(threads, samples, shade));
-#define SRX_VIS_B(NA, S, SH) \
-  hipLaunchKernelGGL((vis_batch_kernel<NA, S, SH>), dim3(blocks), dim3(threads), lds, s, jobs, m)
-#define SRX_VIS_BS(NA)                   \
-  switch (samples) {                     \
-  case 1: SRX_VIS_B(NA, 1, true); break; \
-  case 2: SRX_VIS_B(NA, 2, true); break; \
-  default: SRX_VIS_B(NA, 4, true); break; \
+// GPU tiles of 8 and 32 pixels are built for single-sample draws; every variant must fit 64 KB of LDS.
+bool vis_batch_supported(int samples, bool shade, int threads, int gt) {
+  if (!(threads == 64 || threads == 128 || threads == 256)) return false;
+  if (gt != 16 && !(gt == 8 || gt == 32)) return false;
+  if (gt != 16 && samples != 1) return false;
+  return vis_static_bytes(samples, gt) + vis_union_bytes(threads, samples, shade, gt) <= 65536;
+}
+bool launch_vis_batch(int na, int samples, bool shade, int threads, int gt, const batch_job* jobs, stage_map m,
+                      uint32_t blocks, hipStream_t s) {
+  if (!vis_batch_supported(samples, shade, threads, gt)) return false;
+  if (!blocks) return true;
+  const size_t lds = static_cast<size_t>(vis_union_bytes(threads, samples, shade, gt));
+#define SRX_VIS_B(NA, S, SH, GT) \
+  hipLaunchKernelGGL((vis_batch_kernel<NA, S, SH, GT>), dim3(blocks), dim3(threads), lds, s, jobs, m)
+#define SRX_VIS_BS(NA)                       \
+  switch (samples) {                         \
+  case 1: SRX_VIS_B(NA, 1, true, 16); break; \
+  case 2: SRX_VIS_B(NA, 2, true, 16); break; \
+  default: SRX_VIS_B(NA, 4, true, 16); break; \
   }
-  if (shade) {
+#define SRX_VIS_B8(NA) SRX_VIS_B(NA, 1, true, 8)
+#define SRX_VIS_B32(NA) SRX_VIS_B(NA, 1, true, 32)
+  if (gt == 8) {
+    if (shade) {
+      SRX_NA_DISPATCH(na, SRX_VIS_B8)
+    } else {
+      SRX_VIS_B(1, 1, false, 8);
+    }
+  } else if (gt == 32) {
+    if (shade) {
+      SRX_NA_DISPATCH(na, SRX_VIS_B32)
+    } else {
+      SRX_VIS_B(1, 1, false, 32);
+    }
+  } else if (shade) {
     SRX_NA_DISPATCH(na, SRX_VIS_BS)
   } else {
     switch (samples) {
-    case 1: SRX_VIS_B(1, 1, false); break;
-    case 2: SRX_VIS_B(1, 2, false); break;
-    default: SRX_VIS_B(1, 4, false); break;
+    case 1: SRX_VIS_B(1, 1, false, 16); break;
+    case 2: SRX_VIS_B(1, 2, false, 16); break;
+    default: SRX_VIS_B(1, 4, false, 16); break;
     }
   }
+#undef SRX_VIS_B32
+#undef SRX_VIS_B8
 #undef SRX_VIS_BS
 #undef SRX_VIS_B
+  return true;
 }
 void launch_batch_check(const batch_job* jobs, uint32_t njobs, uint32_t capacity, int pt, uint32_t* status, hipStream_t s) {
   if (njobs)

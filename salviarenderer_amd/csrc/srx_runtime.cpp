@@ -740,7 +740,8 @@ struct batch_obj : object {
   std::vector<std::pair<surface_obj*, uint32_t>> marks;  // recording hazards per surface
   int32_t error = SRX_OK;  // first recording error, reported by srx_batch_end
   // tuning
-  int bin_pt = 16, vis_threads = 256;
+  int tile = 16, bin_pt = 16, vis_threads = 256;
+  std::map<int, std::vector<uint32_t>> counted;  // records per draw, by GPU tile size
   // sizes
   std::vector<uint32_t> records;  // per draw, from the counting pass
   uint32_t capacity = 0;
@@ -902,6 +903,54 @@ int32_t record_resolve(renderer_obj* r, surface_obj* a, surface_obj* d) {
   return SRX_OK;
 }
 
+// GPU tiles of a draw's viewport for tiles of side gt (as build_draw derives them for 16).
+void gpu_tiles(const draw_params& P, int gt, int32_t* gx, int32_t* gy) {
+  const float cw = std::min(P.target_w, static_cast<float>(P.tiles_x * 64));
+  const float ch = std::min(P.target_h, static_cast<float>(P.tiles_y * 64));
+  *gx = static_cast<int32_t>(std::ceil(std::max(cw, 0.0f) / gt));
+  *gy = static_cast<int32_t>(std::ceil(std::max(ch, 0.0f) / gt));
+}
+
+// Sets every job to GPU tiles of side b->tile and, unless done before for that size, runs the
+// counting pass (vertex shader, setup, clipping, scan of the per-triangle record counts; one
+// host synchronisation) whose per-draw record counts size the binning.
+int32_t batch_count(batch_obj* b) {
+  hipStream_t s = b->r->stream;
+  const uint32_t J = static_cast<uint32_t>(b->draws.size());
+  for (uint32_t j = 0; j < J; ++j) {
+    batch_job& jb = b->jobs_h[j];
+    jb.P.gt = b->tile;
+    gpu_tiles(jb.P, b->tile, &jb.P.gtiles_x, &jb.P.gtiles_y);
+    jb.tiles = static_cast<uint32_t>(jb.P.gtiles_x * jb.P.gtiles_y);
+  }
+  if (b->counted.count(b->tile)) {
+    b->records = b->counted[b->tile];
+  } else {
+    RCHK(batch_alloc(b, b->tuned, &b->scan_tmp, scan_tmp_size(b->cnt_n)));
+    RCHK(batch_alloc(b, b->tuned, &b->hist_all, 1));
+    HIPCHK(hipMemcpyAsync(b->jobs_d, b->jobs_h.data(), sizeof(batch_job) * J, hipMemcpyHostToDevice, s));
+    const uint32_t* mf = b->maps_fixed;
+    launch_batch_reset(b->counters_all, J * CT_COUNT, b->cnt_all + (b->cnt_n - 1), b->hist_all, b->status_d, s);
+    launch_vs_batch(b->jobs_d, map_image::at(mf, b->m_vs), b->m_vs.blocks, s);
+    for (auto& e : b->m_setup) launch_setup_batch(e.first, b->jobs_d, map_image::at(mf, e.second), e.second.blocks, s);
+    for (auto& e : b->m_clip) launch_clip_batch(e.first, b->jobs_d, map_image::at(mf, e.second), e.second.blocks, s);
+    launch_exclusive_scan(b->cnt_all, b->cnt_n, b->scan_tmp, b->scan_total, s);
+    launch_batch_counts(b->jobs_d, J, b->counts_d, s);
+    HIPCHK(hipMemcpyAsync(b->status_h, b->counts_d, 2 * sizeof(uint32_t) * J, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipGetLastError());
+    for (uint32_t j = 0; j < J; ++j)
+      if (b->status_h[J + j]) return fail(SRX_OUT_OF_MEMORY, "clip buffer overflow");
+    b->records.assign(b->status_h, b->status_h + J);
+    b->counted[b->tile] = b->records;
+  }
+  uint64_t total = 0;
+  for (uint32_t j = 0; j < J; ++j) total += b->records[j];
+  if (total >= 0xFFFFFFFFull) return fail(SRX_E_UNSUPPORTED, "batch record count");
+  b->capacity = static_cast<uint32_t>(total);
+  return SRX_OK;
+}
+
 // Per-draw buffers, the job table, the fixed stage maps and the counting pass (vertex shader,
 // setup, clipping and the scan of the per-triangle record counts), whose per-job record counts size
 // the binning buffers.
@@ -920,7 +969,9 @@ int32_t batch_prepare(batch_obj* b) {
     n_clipv += 5ull * P.prim_count * regs;
     n_prims += P.prim_count;
     n_cnt += 3ull * P.prim_count;
-    n_tiles += static_cast<size_t>(P.gtiles_x) * P.gtiles_y;
+    int32_t gx8, gy8;  // tile arrays sized for the smallest GPU tiles
+    gpu_tiles(P, 8, &gx8, &gy8);
+    n_tiles += static_cast<size_t>(gx8) * gy8;
   }
   if (n_cnt + 1 >= 0xFFFFFFFFull) return fail(SRX_E_UNSUPPORTED, "batch primitive count");
   b->cnt_n = static_cast<uint32_t>(n_cnt + 1);
@@ -958,7 +1009,9 @@ int32_t batch_prepare(batch_obj* b) {
     jb.P = P;
     sm[j] = b->draws[j].SM;
     const size_t regs = static_cast<size_t>(P.nattr) + 1;
-    const uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
+    int32_t gx8, gy8;
+    gpu_tiles(P, 8, &gx8, &gy8);
+    const uint32_t T = static_cast<uint32_t>(gx8 * gy8);
     draw_buffers& B = jb.B;
     B.vbuf = vbuf + o_vbuf;
     B.tri_rec = tri_rec + o_rec;
@@ -976,7 +1029,7 @@ int32_t batch_prepare(batch_obj* b) {
     B.tend = tiles3 + n_tiles + o_tiles;
     B.order = tiles3 + 2 * n_tiles + o_tiles;
     jb.vis_mode = vis_mode(P);
-    jb.tiles = T;
+    jb.tiles = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
     o_vbuf += (static_cast<size_t>(P.nverts) + 1) * regs;
     o_rec += 15ull * P.prim_count;
     o_attr += 9ull * P.prim_count * regs;
@@ -1031,29 +1084,8 @@ int32_t batch_prepare(batch_obj* b) {
   RCHK(batch_alloc(b, F, &b->resolves_d, ri.size()));
   if (!fi.empty()) HIPCHK(hipMemcpyAsync(b->fills_d, fi.data(), fi.size() * sizeof(fill_item), hipMemcpyHostToDevice, s));
   if (!ri.empty()) HIPCHK(hipMemcpyAsync(b->resolves_d, ri.data(), ri.size() * sizeof(resolve_item), hipMemcpyHostToDevice, s));
-  // the counting pass's scan needs its temporaries; binning sizes them again
-  RCHK(batch_alloc(b, b->tuned, &b->scan_tmp, scan_tmp_size(b->cnt_n)));
-  RCHK(batch_alloc(b, b->tuned, &b->hist_all, 1));
-  HIPCHK(hipMemcpyAsync(b->jobs_d, b->jobs_h.data(), sizeof(batch_job) * J, hipMemcpyHostToDevice, s));
-  const uint32_t* mf = b->maps_fixed;
-  launch_batch_reset(b->counters_all, J * CT_COUNT, b->cnt_all + (b->cnt_n - 1), b->hist_all, b->status_d, s);
-  launch_vs_batch(b->jobs_d, map_image::at(mf, b->m_vs), b->m_vs.blocks, s);
-  for (auto& e : b->m_setup) launch_setup_batch(e.first, b->jobs_d, map_image::at(mf, e.second), e.second.blocks, s);
-  for (auto& e : b->m_clip) launch_clip_batch(e.first, b->jobs_d, map_image::at(mf, e.second), e.second.blocks, s);
-  launch_exclusive_scan(b->cnt_all, b->cnt_n, b->scan_tmp, b->scan_total, s);
-  launch_batch_counts(b->jobs_d, J, b->counts_d, s);
-  HIPCHK(hipMemcpyAsync(b->status_h, b->counts_d, 2 * sizeof(uint32_t) * J, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  HIPCHK(hipGetLastError());
-  b->records.assign(b->status_h, b->status_h + J);
-  uint64_t total = 0;
-  for (uint32_t j = 0; j < J; ++j) {
-    if (b->status_h[J + j]) return fail(SRX_OUT_OF_MEMORY, "clip buffer overflow");
-    total += b->records[j];
-  }
-  if (total >= 0xFFFFFFFFull) return fail(SRX_E_UNSUPPORTED, "batch record count");
-  b->capacity = static_cast<uint32_t>(total);
-  return SRX_OK;
+  b->tile = 16;  // the recorded draws' GPU tiles
+  return batch_count(b);
 }
 
 // Every stage of the batch on stream s (captured into the batch's graph).
@@ -1075,18 +1107,29 @@ void batch_enqueue(batch_obj* b, hipStream_t s) {
   launch_range_batch(b->jobs_d, map_image::at(mt, b->m_range), b->m_range.blocks, s);
   launch_tile_order_batch(b->jobs_d, J, s);
   for (auto& c : b->m_vis)
-    launch_vis_batch(c.na, c.samples, c.shade, b->vis_threads, b->jobs_d, map_image::at(mt, c.m), c.m.blocks, s);
+    launch_vis_batch(c.na, c.samples, c.shade, b->vis_threads, b->tile, b->jobs_d, map_image::at(mt, c.m), c.m.blocks, s);
   launch_resolve_batch(b->resolves_d, map_image::at(mf, b->m_resolve), b->m_resolve.blocks, s);
   launch_batch_check(b->jobs_d, J, b->capacity, b->bin_pt, b->status_d, s);
 }
 
 // Binning buffers and maps for the tuning (bin_pt, vis_threads), then the graph.
-int32_t batch_configure(batch_obj* b, int bin_pt, int vis_threads) {
-  if (!batch_bin_items_valid(bin_pt) || !(vis_threads == 64 || vis_threads == 128 || vis_threads == 256))
+int32_t batch_configure(batch_obj* b, int tile, int bin_pt, int vis_threads) {
+  if (!batch_bin_items_valid(bin_pt) || !(vis_threads == 64 || vis_threads == 128 || vis_threads == 256) ||
+      !(tile == 8 || tile == 16 || tile == 32))
     return fail(SRX_INVALID_PARAMETER, "batch tuning");
+  for (auto& d : b->draws) {
+    int32_t gx, gy;
+    gpu_tiles(d.P, tile, &gx, &gy);
+    if (!vis_batch_supported(d.P.samples, needs_shading(d.P), vis_threads, tile) ||
+        static_cast<uint64_t>(gx) * gy > BATCH_MAX_TILES)
+      return fail(SRX_E_UNSUPPORTED, "GPU tile size not available for every job of the batch");
+  }
   renderer_obj* r = b->r;
   hipStream_t s = r->stream;
   HIPCHK(hipStreamSynchronize(s));
+  b->free_tuned();
+  b->tile = tile;
+  RCHK(batch_count(b));
   b->free_tuned();
   b->bin_pt = bin_pt;
   b->vis_threads = vis_threads;
@@ -1770,7 +1813,7 @@ int32_t launch_served_batch(server_obj* S, int si) {
   launch_range_batch(jobs_d, map_image::at(mf, m_range), m_range.blocks, s);
   launch_tile_order_batch(jobs_d, J, s);
   for (auto& v : m_vis)
-    launch_vis_batch(1, v.first, true, S->desc.vis_threads, jobs_d, map_image::at(mf, v.second), v.second.blocks, s);
+    launch_vis_batch(1, v.first, true, S->desc.vis_threads, 16, jobs_d, map_image::at(mf, v.second), v.second.blocks, s);
   launch_resolve_batch(res_d, map_image::at(mf, m_res), m_res.blocks, s);
   launch_batch_counts(jobs_d, J, counts_d, s);
   HIPCHK(hipGetLastError());
@@ -2369,7 +2412,7 @@ int32_t srx_batch_end(srx_renderer rr, srx_batch* out) {
   b->r = r;
   b->owns_renderer = true;
   int32_t rc = batch_prepare(b);
-  if (rc == SRX_OK) rc = batch_configure(b, b->bin_pt, b->vis_threads);
+  if (rc == SRX_OK) rc = batch_configure(b, b->tile, b->bin_pt, b->vis_threads);
   if (rc != SRX_OK) {
     release(b);
     return rc;
@@ -2391,10 +2434,10 @@ int32_t srx_batch_status(srx_batch bb, uint32_t* status) {
   *status = b->status_h[0];
   return SRX_OK;
 }
-int32_t srx_batch_configure(srx_batch bb, int32_t bin_items, int32_t vis_threads) {
+int32_t srx_batch_configure(srx_batch bb, int32_t tile, int32_t bin_items, int32_t vis_threads) {
   auto* b = as<batch_obj>(bb, K_BATCH);
   if (!b || bin_items % 256) return SRX_INVALID_PARAMETER;
-  return batch_configure(b, bin_items / 256, vis_threads);
+  return batch_configure(b, tile, bin_items / 256, vis_threads);
 }
 int32_t srx_batch_autotune(srx_batch bb, uint32_t reps, srx_batch_tuning* results, size_t capacity, size_t* count) {
   auto* b = as<batch_obj>(bb, K_BATCH);
@@ -2405,31 +2448,39 @@ int32_t srx_batch_autotune(srx_batch bb, uint32_t reps, srx_batch_tuning* result
   HIPCHK(hipEventCreate(&e1));
   size_t n = 0;
   float best = 0.0f;
-  int best_pt = 16, best_th = 256;
+  int best_tile = b->tile, best_pt = b->bin_pt, best_th = b->vis_threads;
   int32_t rc = SRX_OK;
-  for (int pt : {8, 16, 32}) {
-    for (int th : {64, 128, 256}) {
-      rc = batch_configure(b, pt, th);
+  for (int tile : {8, 16, 32}) {
+    for (int pt : {8, 16, 32}) {
+      for (int th : {64, 128, 256}) {
+        rc = batch_configure(b, tile, pt, th);
+        if (rc == SRX_E_UNSUPPORTED) {  // a tile size some job cannot take: not a candidate
+          rc = SRX_OK;
+          continue;
+        }
+        if (rc != SRX_OK) break;
+        hipError_t e = hipGraphLaunch(b->exec, s);  // warm-up
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        for (uint32_t i = 0; i < reps && e == hipSuccess; ++i) e = hipGraphLaunch(b->exec, s);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+          rc = hip_fail(e, "srx_batch_autotune");
+          break;
+        }
+        ms /= static_cast<float>(reps);
+        if (results && n < capacity) results[n] = srx_batch_tuning{tile, pt * 256, th, ms};
+        ++n;
+        if (n == 1 || ms < best) {
+          best = ms;
+          best_tile = tile;
+          best_pt = pt;
+          best_th = th;
+        }
+      }
       if (rc != SRX_OK) break;
-      hipError_t e = hipGraphLaunch(b->exec, s);  // warm-up
-      if (e == hipSuccess) e = hipEventRecord(e0, s);
-      for (uint32_t i = 0; i < reps && e == hipSuccess; ++i) e = hipGraphLaunch(b->exec, s);
-      if (e == hipSuccess) e = hipEventRecord(e1, s);
-      if (e == hipSuccess) e = hipEventSynchronize(e1);
-      float ms = 0.0f;
-      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-      if (e != hipSuccess) {
-        rc = hip_fail(e, "srx_batch_autotune");
-        break;
-      }
-      ms /= static_cast<float>(reps);
-      if (results && n < capacity) results[n] = srx_batch_tuning{pt * 256, th, ms};
-      ++n;
-      if (n == 1 || ms < best) {
-        best = ms;
-        best_pt = pt;
-        best_th = th;
-      }
     }
     if (rc != SRX_OK) break;
   }
@@ -2437,7 +2488,7 @@ int32_t srx_batch_autotune(srx_batch bb, uint32_t reps, srx_batch_tuning* result
   (void)hipEventDestroy(e1);
   if (count) *count = n;
   if (rc != SRX_OK) return rc;
-  return batch_configure(b, best_pt, best_th);
+  return batch_configure(b, best_tile, best_pt, best_th);
 }
 int32_t srx_batch_get_info(srx_batch bb, srx_batch_info* info) {
   auto* b = as<batch_obj>(bb, K_BATCH);
@@ -2445,6 +2496,7 @@ int32_t srx_batch_get_info(srx_batch bb, srx_batch_info* info) {
   info->jobs = static_cast<uint32_t>(b->draws.size());
   info->clears = static_cast<uint32_t>(b->fills.size());
   info->resolves = static_cast<uint32_t>(b->resolves.size());
+  info->tile = static_cast<uint32_t>(b->tile);
   info->bin_items = static_cast<uint32_t>(b->bin_pt * 256);
   info->vis_threads = static_cast<uint32_t>(b->vis_threads);
   info->vis_launches = static_cast<uint32_t>(b->m_vis.size());

@@ -18,6 +18,7 @@ SPECS = [
     S.JobSpec(0x5A11B005, 16, 16, 50, 64, 1),
     S.JobSpec(0x5A11B006, 640, 360, 20000, 128, 4),
     S.JobSpec(0x5A11B007, 512, 512, 8000, 64, 1),
+    S.JobSpec(0x5A11B008, 200, 130, 3000, 64, 1),
 ]
 
 
@@ -112,13 +113,17 @@ def test_batch_equals_one_by_one_and_oracle(hip_lib, cpu_lib):
     assert_same(outputs(jobs), ref)
 
 
+SINGLE = [s for s in SPECS if s.samples == 1]  # 8x8 and 32x32 tiles take single-sample jobs
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("bin_items,threads", [(2048, 64), (4096, 128), (8192, 256), (2048, 256)])
-def test_batch_tunings_agree(hip_lib, bin_items, threads):
-    r, jobs, ref = one_by_one(hip_lib)
+@pytest.mark.parametrize("tile,bin_items,threads", [(16, 2048, 64), (16, 4096, 128), (16, 8192, 256), (16, 2048, 256),
+                                                    (8, 2048, 64), (8, 8192, 256), (32, 4096, 64), (32, 8192, 256)])
+def test_batch_tunings_agree(hip_lib, tile, bin_items, threads):
+    r, jobs, ref = one_by_one(hip_lib, SPECS if tile == 16 else SINGLE)
     b = jobs.record()
-    b.configure(bin_items, threads)
-    assert (b.info().bin_items, b.info().vis_threads) == (bin_items, threads)
+    b.configure(tile, bin_items, threads)
+    assert (b.info().tile, b.info().bin_items, b.info().vis_threads) == (tile, bin_items, threads)
     scramble(jobs)
     b.launch()
     r.flush()
@@ -131,9 +136,22 @@ def test_batch_autotune_keeps_output(hip_lib):
     r, jobs, ref = one_by_one(hip_lib, SPECS[:3])
     b = jobs.record()
     sweep = b.autotune(reps=1)
-    assert len(sweep) == 9 and all(ms > 0 for _, _, ms in sweep)
-    best = min(sweep, key=lambda t: t[2])
-    assert (b.info().bin_items, b.info().vis_threads) == best[:2]
+    # SPECS[1] has 4 samples: only 16x16 tiles apply
+    assert len(sweep) == 9 and all(ms > 0 for *_, ms in sweep) and {t[0] for t in sweep} == {16}
+    best = min(sweep, key=lambda t: t[3])
+    assert (b.info().tile, b.info().bin_items, b.info().vis_threads) == best[:3]
+    scramble(jobs)
+    b.launch()
+    r.flush()
+    assert_same(outputs(jobs), ref)
+
+
+@pytest.mark.gpu
+def test_batch_autotune_sweeps_tile_sizes(hip_lib):
+    r, jobs, ref = one_by_one(hip_lib, SINGLE[:3])
+    b = jobs.record()
+    sweep = b.autotune(reps=1)
+    assert {t[0] for t in sweep} == {8, 16, 32} and len(sweep) == 27
     scramble(jobs)
     b.launch()
     r.flush()
