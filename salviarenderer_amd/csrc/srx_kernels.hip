@@ -305,104 +305,145 @@ __device__ __forceinline__ void load_clipped_triangle(const draw_params& P, cons
 // ============================================================== binning geometry
 // Visits, in order, the GPU tiles that may hold pixels of a triangle, with the reference's
 // 64x64 tile-level decision (threaded_dispatch_primitive, rasterizer.cpp:775-862).
-template <class F>
-__device__ __forceinline__ void for_each_gtile(const draw_params& P, const float* e, const float* bb, F&& f) {
-  int tx = P.tiles_x, ty = P.tiles_y;
-  int sx = min(fast_floori(std_max(0.0f, bb[0]) / TILE), tx);
-  int sy = min(fast_floori(std_max(0.0f, bb[2]) / TILE), ty);
-  int ex = min(fast_ceili(std_max(0.0f, bb[1]) / TILE) + 1, tx);
-  int ey = min(fast_ceili(std_max(0.0f, bb[3]) / TILE) + 1, ty);
-  bool small = (sx + 1 == ex) && (sy + 1 == ey);
+// Binning of a triangle to GPU tiles.  The reference bins by 64x64 tile (rasterizer.cpp:947-994,
+// the tile's corner test rejects or fully accepts it); a GPU tile of a partially covered 64x64 tile
+// gets a record when its rows and columns meet the triangle's bounding box and one of its 16x16
+// stage-16 regions passes the rasteriser's first subdivision test (subdivide_tile on the 64x64
+// tile, rasterizer.cpp:662-675, the same float expressions as the visibility and raster kernels):
+// a region that test rejects produces no fragment, so its record could change nothing.
+// Larger triangles skip the region test: their partially covered 64x64 tiles are few next to the
+// inner ones, and the test would lengthen the per-triangle counting loop.
+constexpr int REGION_TEST_TILES = 16;
+struct tri_bin {
+  int sx, sy, ex, ey;  // 64x64 tiles [sx, ex) x [sy, ey)
+  bool small;          // one 64x64 tile: the corner test is skipped (rasterizer.cpp:957)
+  bool regions;        // at most REGION_TEST_TILES 64x64 tiles: GPU tiles are region-tested
   bool mx[3], my[3];
-  float r2a[3];
+  float r2a[3], s1x[3], s1y[3];
+};
+__device__ __forceinline__ void bin_setup(const draw_params& P, const float* e, const float* bb, tri_bin& t) {
+  const int tx = P.tiles_x, ty = P.tiles_y;
+  t.sx = min(fast_floori(std_max(0.0f, bb[0]) / TILE), tx);
+  t.sy = min(fast_floori(std_max(0.0f, bb[2]) / TILE), ty);
+  t.ex = min(fast_ceili(std_max(0.0f, bb[1]) / TILE) + 1, tx);
+  t.ey = min(fast_ceili(std_max(0.0f, bb[3]) / TILE) + 1, ty);
+  t.small = (t.sx + 1 == t.ex) && (t.sy + 1 == t.ey);
+  t.regions = (t.ex - t.sx) * (t.ey - t.sy) <= REGION_TEST_TILES;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    mx[k] = e[k * 3 + 0] > 0;
-    my[k] = e[k * 3 + 1] > 0;
-    float stx = TILE * e[k * 3 + 0], sty = TILE * e[k * 3 + 1];
-    r2a[k] = -fabsf(stx) - fabsf(sty);
+    const float ex = e[k * 3 + 0], ey = e[k * 3 + 1];
+    t.mx[k] = ex > 0;
+    t.my[k] = ey > 0;
+    float stx = TILE * ex, sty = TILE * ey;
+    t.r2a[k] = -fabsf(stx) - fabsf(sty);
+    stx *= 0.25f;
+    sty *= 0.25f;
+    t.s1x[k] = stx;
+    t.s1y[k] = sty;
   }
-  int sub = TILE / P.gt;
-  for (int y = sy; y < ey; ++y)
-    for (int x = sx; x < ex; ++x) {
-      bool full = false;
-      if (!small) {
-        bool rej = false, acc = true;
+}
+// The GPU tiles of 64x64 tile (x, y) that get a record, bit j * sub + i for GPU tile (i, j) of it
+// (row-major, the order records are written in); full: the tile is fully accepted.
+__device__ __forceinline__ uint64_t tile_gmask(const draw_params& P, const float* e, const float* bb, const tri_bin& t,
+                                               int x, int y, bool& full) {
+  full = false;
+  if (!t.small) {
+    bool rej = false, acc = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float ev = e[k * 3 + 2] - ((static_cast<float>(x + t.mx[k]) * TILE) * e[k * 3 + 0] +
+                                 (static_cast<float>(y + t.my[k]) * TILE) * e[k * 3 + 1]);
+      rej |= (0 < ev);
+      acc &= (t.r2a[k] >= ev);
+    }
+    if (rej) return 0;
+    full = acc;
+  }
+  const int sub = TILE / P.gt;
+  const float xmin = bb[0] - static_cast<float>(x * TILE), xmax = bb[1] - static_cast<float>(x * TILE);
+  const float ymin = bb[2] - static_cast<float>(y * TILE), ymax = bb[3] - static_cast<float>(y * TILE);
+  uint32_t rows = 0, cols = 0;
+  for (int j = 0; j < sub; ++j) {
+    if (y * sub + j >= P.gtiles_y) break;
+    if (full || !(ymin >= static_cast<float>(j * P.gt + P.gt) || ymax < static_cast<float>(j * P.gt))) rows |= 1u << j;
+  }
+  for (int i = 0; i < sub; ++i) {
+    if (x * sub + i >= P.gtiles_x) break;
+    if (full || !(xmin >= static_cast<float>(i * P.gt + P.gt) || xmax < static_cast<float>(i * P.gt))) cols |= 1u << i;
+  }
+  if (!rows || !cols) return 0;
+  uint64_t all = 0;  // every GPU tile in the bounding-box rows and columns
+  for (uint32_t rm = rows; rm; rm &= rm - 1) all |= static_cast<uint64_t>(cols) << (__builtin_ctz(rm) * sub);
+  if (full || !t.regions) return all;
+  // stage-16 regions the first subdivision rejects (bit ry * 4 + rx)
+  uint32_t rrej = 0;
+  {
+    const float tvx = static_cast<float>(x * TILE), tvy = static_cast<float>(y * TILE);
+    float ev1[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float ex = e[k * 3], ey = e[k * 3 + 1], ez = e[k * 3 + 2];
+      float part = static_cast<float>(t.mx[k] * TILE) * ex + static_cast<float>(t.my[k] * TILE) * ey;
+      part *= 0.25f;
+      ev1[k] = (ez - part) - ((tvx * ex) + (tvy * ey));
+    }
+    // regions inside the bounding-box rows and columns only
+    const int r0 = (__builtin_ctz(rows) * P.gt) >> 4, r1 = ((31 - __builtin_clz(rows)) * P.gt + P.gt - 1) >> 4;
+    const int c0 = (__builtin_ctz(cols) * P.gt) >> 4, c1 = ((31 - __builtin_clz(cols)) * P.gt + P.gt - 1) >> 4;
+    for (int ry = r0; ry <= r1; ++ry)
+      for (int rx = c0; rx <= c1; ++rx) {
+        bool rej = false;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          float ev = e[k * 3 + 2] - ((static_cast<float>(x + mx[k]) * TILE) * e[k * 3 + 0] +
-                                     (static_cast<float>(y + my[k]) * TILE) * e[k * 3 + 1]);
-          rej |= (0 < ev);
-          acc &= (r2a[k] >= ev);
+          const float r = (t.s1x[k] * static_cast<float>(rx)) + (t.s1y[k] * static_cast<float>(ry));
+          rej |= r < ev1[k];
         }
-        if (rej) continue;
-        full = acc;
+        if (rej) rrej |= 1u << (ry * 4 + rx);
       }
-      float xmin = bb[0] - static_cast<float>(x * TILE), xmax = bb[1] - static_cast<float>(x * TILE);
-      float ymin = bb[2] - static_cast<float>(y * TILE), ymax = bb[3] - static_cast<float>(y * TILE);
-      for (int j = 0; j < sub; ++j) {
-        int gy = y * sub + j;
-        if (gy >= P.gtiles_y) break;
-        if (!full && (ymin >= static_cast<float>(j * P.gt + P.gt) || ymax < static_cast<float>(j * P.gt))) continue;
-        for (int i = 0; i < sub; ++i) {
-          int gx = x * sub + i;
-          if (gx >= P.gtiles_x) break;
-          if (!full && (xmin >= static_cast<float>(i * P.gt + P.gt) || xmax < static_cast<float>(i * P.gt))) continue;
-          f(static_cast<uint32_t>(gy * P.gtiles_x + gx), full);
-        }
+  }
+  if (!rrej) return all;
+  if (P.gt == 16) return all & ~static_cast<uint64_t>(rrej);  // GPU tile (i, j) is region (i, j)
+  uint64_t m = 0;
+  for (uint32_t rm = rows; rm; rm &= rm - 1) {
+    const int j = __builtin_ctz(rm);
+    const int ry0 = (j * P.gt) >> 4, ry1 = (j * P.gt + P.gt - 1) >> 4;
+    for (uint32_t cm = cols; cm; cm &= cm - 1) {
+      const int i = __builtin_ctz(cm);
+      const int rx0 = (i * P.gt) >> 4, rx1 = (i * P.gt + P.gt - 1) >> 4;
+      bool any = false;  // a region of the GPU tile the first subdivision keeps
+      for (int ry = ry0; ry <= ry1; ++ry)
+        for (int rx = rx0; rx <= rx1; ++rx) any |= !((rrej >> (ry * 4 + rx)) & 1u);
+      if (any) m |= 1ull << (j * sub + i);
+    }
+  }
+  return m;
+}
+
+template <class F>
+__device__ __forceinline__ void for_each_gtile(const draw_params& P, const float* e, const float* bb, F&& f) {
+  tri_bin t;
+  bin_setup(P, e, bb, t);
+  const int sub = TILE / P.gt;
+  for (int y = t.sy; y < t.ey; ++y)
+    for (int x = t.sx; x < t.ex; ++x) {
+      bool full;
+      for (uint64_t m = tile_gmask(P, e, bb, t, x, y, full); m; m &= m - 1) {
+        const int b = __builtin_ctzll(m);
+        f(static_cast<uint32_t>((y * sub + b / sub) * P.gtiles_x + x * sub + b % sub), full);
       }
     }
 }
 
-// The number of GPU tiles for_each_gtile visits, with each 64x64 tile's GPU tiles counted as
-// (rows in the bounding box) x (columns in the bounding box) instead of one by one.
+// The number of GPU tiles for_each_gtile visits.
 __device__ __forceinline__ uint32_t count_gtiles(const draw_params& P, const float* e, const float* bb) {
-  int tx = P.tiles_x, ty = P.tiles_y;
-  int sx = min(fast_floori(std_max(0.0f, bb[0]) / TILE), tx);
-  int sy = min(fast_floori(std_max(0.0f, bb[2]) / TILE), ty);
-  int ex = min(fast_ceili(std_max(0.0f, bb[1]) / TILE) + 1, tx);
-  int ey = min(fast_ceili(std_max(0.0f, bb[3]) / TILE) + 1, ty);
-  bool small = (sx + 1 == ex) && (sy + 1 == ey);
-  bool mx[3], my[3];
-  float r2a[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    mx[k] = e[k * 3 + 0] > 0;
-    my[k] = e[k * 3 + 1] > 0;
-    float stx = TILE * e[k * 3 + 0], sty = TILE * e[k * 3 + 1];
-    r2a[k] = -fabsf(stx) - fabsf(sty);
-  }
-  const int sub = TILE / P.gt;
+  tri_bin t;
+  bin_setup(P, e, bb, t);
   uint32_t n = 0;
-  for (int y = sy; y < ey; ++y) {
-    const float ymin = bb[2] - static_cast<float>(y * TILE), ymax = bb[3] - static_cast<float>(y * TILE);
-    for (int x = sx; x < ex; ++x) {
-      bool full = false;
-      if (!small) {
-        bool rej = false, acc = true;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          float ev = e[k * 3 + 2] - ((static_cast<float>(x + mx[k]) * TILE) * e[k * 3 + 0] +
-                                     (static_cast<float>(y + my[k]) * TILE) * e[k * 3 + 1]);
-          rej |= (0 < ev);
-          acc &= (r2a[k] >= ev);
-        }
-        if (rej) continue;
-        full = acc;
-      }
-      const float xmin = bb[0] - static_cast<float>(x * TILE), xmax = bb[1] - static_cast<float>(x * TILE);
-      uint32_t rows = 0, cols = 0;
-      for (int j = 0; j < sub; ++j) {
-        if (y * sub + j >= P.gtiles_y) break;
-        if (full || !(ymin >= static_cast<float>(j * P.gt + P.gt) || ymax < static_cast<float>(j * P.gt))) ++rows;
-      }
-      for (int i = 0; i < sub; ++i) {
-        if (x * sub + i >= P.gtiles_x) break;
-        if (full || !(xmin >= static_cast<float>(i * P.gt + P.gt) || xmax < static_cast<float>(i * P.gt))) ++cols;
-      }
-      n += rows * cols;
+  for (int y = t.sy; y < t.ey; ++y)
+    for (int x = t.sx; x < t.ex; ++x) {
+      bool full;
+      n += static_cast<uint32_t>(__popcll(tile_gmask(P, e, bb, t, x, y, full)));
     }
-  }
   return n;
 }
 
@@ -654,71 +695,31 @@ __device__ __forceinline__ void emit_wide(const draw_params& P, const float4* __
   const int lane = __lane_id();
   float e[9], bb[4];
   read_tri_rec(tri_rec + tri_slot(P, id) * 5, e, bb);
-  const int tx = P.tiles_x, ty = P.tiles_y;
-  const int sx = min(fast_floori(std_max(0.0f, bb[0]) / TILE), tx);
-  const int sy = min(fast_floori(std_max(0.0f, bb[2]) / TILE), ty);
-  const int ex = min(fast_ceili(std_max(0.0f, bb[1]) / TILE) + 1, tx);
-  const int ey = min(fast_ceili(std_max(0.0f, bb[3]) / TILE) + 1, ty);
-  const bool small = (sx + 1 == ex) && (sy + 1 == ey);
-  bool mx[3], my[3];
-  float r2a[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    mx[k] = e[k * 3 + 0] > 0;
-    my[k] = e[k * 3 + 1] > 0;
-    const float stx = TILE * e[k * 3 + 0], sty = TILE * e[k * 3 + 1];
-    r2a[k] = -fabsf(stx) - fabsf(sty);
-  }
-  const int sub = TILE / P.gt, w = ex - sx;
-  const int R = (ex > sx && ey > sy) ? w * (ey - sy) : 0;
+  tri_bin t;
+  bin_setup(P, e, bb, t);
+  const int sub = TILE / P.gt, w = t.ex - t.sx;
+  const int R = (t.ex > t.sx && t.ey > t.sy) ? w * (t.ey - t.sy) : 0;
   uint32_t base = first;
   for (int t0 = 0; t0 < R; t0 += 64) {
-    const int t = t0 + lane;
-    uint32_t rows = 0, cols = 0, rmask = 0, cmask = 0;
+    const int tt = t0 + lane;
+    uint64_t m = 0;
     bool full = false;
     int x = 0, y = 0;
-    if (t < R) {
-      x = sx + t % w;
-      y = sy + t / w;
-      bool rej = false, acc = true;
-      if (!small) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const float ev = e[k * 3 + 2] - ((static_cast<float>(x + mx[k]) * TILE) * e[k * 3 + 0] +
-                                           (static_cast<float>(y + my[k]) * TILE) * e[k * 3 + 1]);
-          rej |= (0 < ev);
-          acc &= (r2a[k] >= ev);
-        }
-        full = acc;
-      }
-      if (!rej) {
-        const float xmin = bb[0] - static_cast<float>(x * TILE), xmax = bb[1] - static_cast<float>(x * TILE);
-        const float ymin = bb[2] - static_cast<float>(y * TILE), ymax = bb[3] - static_cast<float>(y * TILE);
-        for (int j = 0; j < sub; ++j) {
-          if (y * sub + j >= P.gtiles_y) break;
-          if (full || !(ymin >= static_cast<float>(j * P.gt + P.gt) || ymax < static_cast<float>(j * P.gt))) rmask |= 1u << j;
-        }
-        for (int i = 0; i < sub; ++i) {
-          if (x * sub + i >= P.gtiles_x) break;
-          if (full || !(xmin >= static_cast<float>(i * P.gt + P.gt) || xmax < static_cast<float>(i * P.gt))) cmask |= 1u << i;
-        }
-        rows = __popc(rmask);
-        cols = __popc(cmask);
-      }
+    if (tt < R) {
+      x = t.sx + tt % w;
+      y = t.sy + tt / w;
+      m = tile_gmask(P, e, bb, t, x, y, full);
     }
-    const uint32_t cnt = rows * cols;
+    const uint32_t cnt = static_cast<uint32_t>(__popcll(m));
     const uint32_t incl = wave_incl_scan(cnt);
     uint32_t o = base + incl - cnt;
-    for (uint32_t rm = rmask; rm && cnt; rm &= rm - 1) {
-      const int j = __builtin_ctz(rm);
-      for (uint32_t cm = cmask; cm; cm &= cm - 1) {
-        const int i = __builtin_ctz(cm);
-        if (o < next && o < capacity) {
-          keys[o] = static_cast<uint32_t>((y * sub + j) * P.gtiles_x + x * sub + i);
-          vals[o] = (id << 1) | (full ? 1u : 0u);
-        }
-        ++o;
+    for (; m; m &= m - 1) {
+      const int b = __builtin_ctzll(m);
+      if (o < next && o < capacity) {
+        keys[o] = static_cast<uint32_t>((y * sub + b / sub) * P.gtiles_x + x * sub + b % sub);
+        vals[o] = (id << 1) | (full ? 1u : 0u);
       }
+      ++o;
     }
     base += static_cast<uint32_t>(__shfl(static_cast<int>(incl), 63, 64));
   }
