@@ -1304,26 +1304,67 @@ __device__ __forceinline__ float4 quad_reg(const float4 (&q)[4][NA + 1], int i, 
   return sel4(i == 0, q[0][r], sel4(i == 1, q[1][r], sel4(i == 2, q[2][r], q[3][r])));
 }
 
+// Interpolated registers of single pixels of a 2x2 quad, computed only where read: pixel i's value
+// is interp_quad's chain for it (p00 = v0 + (ddx*dx + ddy*dy), +ddx for the right column, then
+// +ddy for the bottom row), perspective-divided by its own 1/w, so each value equals interp_quad's.
+template <int NA>
+struct quad_lazy {
+  const attr_setup<NA>& t;
+  const draw_params& P;
+  float dx, dy;
+  __device__ __forceinline__ quad_lazy(const attr_setup<NA>& at, const draw_params& p, int px0, int py0)
+      : t(at), P(p), dx((0.5f + static_cast<float>(px0)) - at.v0[0].x), dy((0.5f + static_cast<float>(py0)) - at.v0[0].y) {}
+  __device__ __forceinline__ static float chain(float v, float a, float b, float dx, float dy, int i) {
+    float c = v + ((a * dx) + (b * dy));
+    c = (i & 1) ? c + a : c;
+    return (i & 2) ? c + b : c;
+  }
+  __device__ __forceinline__ float inv_w(int i) const {
+    return 1.0f / chain(t.v0[0].w, t.dx[0].w, t.dy[0].w, dx, dy, i);
+  }
+  // register r (1..NA) of pixel i, components [0, nc)
+  __device__ __forceinline__ float4 reg(int r, int i, float iw, int nc = 4) const {
+    const float4 v = t.v0[r], a = t.dx[r], b = t.dy[r];
+    const uint32_t mod = P.attr_mod[r - 1];
+    float4 o;
+    if (mod & AM_NOINTERPOLATION) {
+      o = v;
+    } else {
+      o.x = chain(v.x, a.x, b.x, dx, dy, i);
+      o.y = nc > 1 ? chain(v.y, a.y, b.y, dx, dy, i) : 0.0f;
+      o.z = nc > 2 ? chain(v.z, a.z, b.z, dx, dy, i) : 0.0f;
+      o.w = nc > 3 ? chain(v.w, a.w, b.w, dx, dy, i) : 0.0f;
+    }
+    if (!(mod & AM_NOPERSPECTIVE)) o = make_float4(o.x * iw, o.y * iw, o.z * iw, o.w * iw);
+    return o;
+  }
+};
+
 // The pixel shader's colour for quad pixel pi only (the deferred shading paths need one pixel of
 // the quad; the other three supply derivatives).  Same arithmetic as run_ps.
 template <int NA>
 __device__ __forceinline__ float4 run_ps_pixel(const draw_params& P, const ps_samplers* __restrict__ SM,
-                                               const float4 (&q)[4][NA + 1], int pi) {
+                                               const quad_lazy<NA>& q, int pi) {
   switch (P.ps_id) {
   case PS_GOURAUD:
-    return quad_reg<NA>(q, pi, 1);
+    return q.reg(1, pi, q.inv_w(pi));
   case PS_CONSTANT:
     return make_float4(P.ps_consts[0], P.ps_consts[1], P.ps_consts[2], P.ps_consts[3]);
   case PS_LAMBERT_TEX:
     if constexpr (NA >= 2) {
       // SASL tex2D: ddx per row (p1-p0, p3-p2), ddy per column (p2-p0, p3-p1)
-      // (sasl/src/codegen/cgs_simd.cpp:275-312), then sample_2d_grad (sampler_api.cpp:13-18)
-      const bool row1 = (pi & 2) != 0, col1 = (pi & 1) != 0;
-      const float4 r0 = sel4(row1, q[2][2], q[0][2]), r1 = sel4(row1, q[3][2], q[1][2]);
-      const float4 c0 = sel4(col1, q[1][2], q[0][2]), c1 = sel4(col1, q[3][2], q[2][2]);
+      // (sasl/src/codegen/cgs_simd.cpp:275-312), then sample_2d_grad (sampler_api.cpp:13-18);
+      // uv at pi and its row and column neighbours
+      const int pr = pi ^ 1, pc = pi ^ 2;
+      const float iw = q.inv_w(pi), iwr = q.inv_w(pr), iwc = q.inv_w(pc);
+      const float4 uv = q.reg(2, pi, iw, 2), uvr = q.reg(2, pr, iwr, 2), uvc = q.reg(2, pc, iwc, 2);
+      const bool col1 = (pi & 1) != 0, row1 = (pi & 2) != 0;
+      // row: (left, right) = col1 ? (neighbour, pi) : (pi, neighbour); column likewise
+      const float4 r0 = sel4(col1, uvr, uv), r1 = sel4(col1, uv, uvr);
+      const float4 c0 = sel4(row1, uvc, uv), c1 = sel4(row1, uv, uvc);
       const float dxu = r1.x - r0.x, dxv = r1.y - r0.y;
       const float dyu = c1.x - c0.x, dyv = c1.y - c0.y;
-      const float4 uv = quad_reg<NA>(q, pi, 2), nrm = quad_reg<NA>(q, pi, 1);
+      const float4 nrm = q.reg(1, pi, iw, 3);
       float4 t = make_float4(0, 0, 0, 0);
       if (P.has_sampler[0]) {
         const sampler_desc_dev& sp = SM->s[0];
@@ -1341,25 +1382,27 @@ __device__ __forceinline__ float4 run_ps_pixel(const draw_params& P, const ps_sa
     // cpp tex2d: one LOD per quad from p1 - p0 and p2 - p0 (cpp_pixel_shader.cpp:13-31)
     if (!P.has_sampler[0]) return make_float4(0, 0, 0, 1.0f);
     const sampler_desc_dev& sp = SM->s[0];
-    float dxu = q[1][1].x - q[0][1].x, dxv = q[1][1].y - q[0][1].y;
-    float dyu = q[2][1].x - q[0][1].x, dyv = q[2][1].y - q[0][1].y;
+    const float4 u0 = q.reg(1, 0, q.inv_w(0), 2), u1 = q.reg(1, 1, q.inv_w(1), 2), u2 = q.reg(1, 2, q.inv_w(2), 2);
+    float dxu = u1.x - u0.x, dxv = u1.y - u0.y;
+    float dyu = u2.x - u0.x, dyv = u2.y - u0.y;
     float lod = (sp.mip_filter == FILTER_ANISO && sp.max_aniso > 1) ? calc_aniso(sp, dxu, dxv, dyu, dyv, 0.0f).lod
                                                                     : calc_lod(sp, dxu, dxv, dyu, dyv, 0.0f);
-    const float4 uv = quad_reg<NA>(q, pi, 1);
+    const float4 uv = q.reg(1, pi, q.inv_w(pi), 2);
     float4 t = sample_impl(sp, uv.x, uv.y, lod, nullptr);
     t.w = 1.0f;
     return t;
   }
   case PS_SHADOW_PCF:
-    if constexpr (NA >= 2) return shadow_pcf(P, SM, quad_reg<NA>(q, pi, 1), quad_reg<NA>(q, pi, 2));
+    if constexpr (NA >= 2) {
+      const float iw = q.inv_w(pi);
+      return shadow_pcf(P, SM, q.reg(1, pi, iw), q.reg(2, pi, iw));
+    }
     return make_float4(0, 0, 0, 0);
   default:
     return make_float4(0, 0, 0, 0);
   }
 }
 
-// step_2d_unproj_pos_quad + step_2d_unproj_attr_n_quad (shader.cpp:257-367) for the quad whose
-// top-left pixel is (px0, py0)
 template <int NA>
 __device__ __forceinline__ void interp_quad(const attr_setup<NA>& t, int px0, int py0, const draw_params& P,
                                             float4 (&q)[4][NA + 1]) {
@@ -2266,8 +2309,7 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
     const int pi = (px & 1) | ((py & 1) << 1);
     attr_setup<NA> at;
     load_attr<NA>(P, tri_attr, cur, at);
-    float4 q[4][NA + 1];
-    interp_quad<NA>(at, px & ~1, py & ~1, P, q);
+    const quad_lazy<NA> q(at, P, px & ~1, py & ~1);
     const float4 c = run_ps_pixel<NA>(P, SM, q, pi);
 #pragma unroll
     for (int smp = 0; smp < S; ++smp)
@@ -2314,8 +2356,7 @@ __global__ void __launch_bounds__(256) shade_kernel(draw_params P, const ps_samp
     if (cur >= 3u * P.prim_count) break;  // NO_WINNER (every id is below 3 * prim_count)
     attr_setup<NA> at;
     load_attr<NA>(P, tri_attr, cur, at);
-    float4 q[4][NA + 1];
-    interp_quad<NA>(at, px0, py0, P, q);
+    const quad_lazy<NA> q(at, P, px0, py0);
     const float4 c = run_ps_pixel<NA>(P, SM, q, pi);
 #pragma unroll
     for (int s = 0; s < S; ++s)
