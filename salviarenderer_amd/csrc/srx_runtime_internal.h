@@ -1,0 +1,421 @@
+// Internals of the HIP backend's host runtime shared by srx_runtime.cpp (renderer, resources, draws),
+// srx_batch.cpp (recorded batches) and srx_server.cpp (request server).
+#pragma once
+
+#include "../../include/srx.h"
+#include "srx_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace srx_rt {
+using namespace srx;
+
+
+inline thread_local int32_t g_last_error = 0;
+inline thread_local std::string g_last_error_str;
+
+inline int32_t fail(int32_t code, const char* msg) {
+  g_last_error = 0;
+  g_last_error_str = msg;
+  return code;
+}
+inline int32_t hip_fail(hipError_t e, const char* where) {
+  g_last_error = static_cast<int32_t>(e);
+  g_last_error_str = std::string(where) + ": " + hipGetErrorString(e);
+  return SRX_E_HIP;
+}
+#define HIPCHK(expr)                                 \
+  do {                                               \
+    hipError_t _e = (expr);                          \
+    if (_e != hipSuccess) return hip_fail(_e, #expr); \
+  } while (0)
+
+// ---------------------------------------------------------------- objects
+enum obj_kind { K_RENDERER = 1, K_BUFFER, K_TEXTURE, K_SURFACE, K_SAMPLER, K_LAYOUT, K_RASTER, K_DS, K_QUERY, K_BATCH, K_SERVER };
+
+struct object {
+  std::atomic<int> refs{1};
+  int kind;
+  explicit object(int k) : kind(k) {}
+  virtual ~object() = default;
+};
+inline void retain(object* o) {
+  if (o) o->refs.fetch_add(1);
+}
+inline void release(object* o) {
+  if (o && o->refs.fetch_sub(1) == 1) delete o;
+}
+template <class T>
+struct ref {
+  T* p = nullptr;
+  ref() = default;
+  explicit ref(T* q) : p(q) { retain(p); }
+  ref(const ref& o) : p(o.p) { retain(p); }
+  ref& operator=(const ref& o) {
+    if (this != &o) {
+      retain(o.p);
+      release(p);
+      p = o.p;
+    }
+    return *this;
+  }
+  ~ref() { release(p); }
+  T* operator->() const { return p; }
+  T* get() const { return p; }
+  explicit operator bool() const { return p != nullptr; }
+};
+
+inline int pf_bytes(int pf) {
+  switch (pf) {
+  case PF_RGBA32F: return 16;
+  case PF_RGB32F: return 12;
+  case PF_RG32F: return 8;
+  default: return 4;
+  }
+}
+
+struct buffer_obj : object {
+  uint8_t* d = nullptr;
+  size_t size = 0;
+  buffer_obj() : object(K_BUFFER) {}
+  ~buffer_obj() override {
+    if (d) (void)hipFree(d);
+  }
+};
+
+struct surface_obj : object {
+  int w, h, samples, pf;
+  uint8_t* d = nullptr;
+  size_t bytes = 0;
+  surface_obj(int w_, int h_, int s_, int pf_) : object(K_SURFACE), w(w_), h(h_), samples(s_), pf(pf_) {}
+  ~surface_obj() override {
+    if (d) (void)hipFree(d);
+  }
+  surface_desc desc() const { return surface_desc{d, w, h, samples, pf}; }
+};
+
+inline int32_t make_surface(int w, int h, int s, int pf, surface_obj** out) {
+  auto* o = new surface_obj(w, h, s, pf);
+  o->bytes = static_cast<size_t>(w) * h * s * pf_bytes(pf);
+  hipError_t e = hipMalloc(&o->d, o->bytes ? o->bytes : 1);
+  if (e != hipSuccess) {
+    delete o;
+    return hip_fail(e, "hipMalloc(surface)");
+  }
+  e = hipMemset(o->d, 0, o->bytes);
+  if (e != hipSuccess) {
+    delete o;
+    return hip_fail(e, "hipMemset(surface)");
+  }
+  *out = o;
+  return SRX_OK;
+}
+
+struct texture_obj : object {
+  int pf, samples;
+  int max_lod = 0, min_lod = 0;
+  uint32_t size[3];
+  std::vector<ref<surface_obj>> surfs;
+  texture_obj() : object(K_TEXTURE) {}
+};
+
+struct sampler_obj : object {
+  srx_sampler_desc desc;
+  ref<texture_obj> tex;
+  sampler_obj() : object(K_SAMPLER) {}
+};
+
+struct layout_elem {
+  std::string sem;
+  uint32_t index, fmt, slot, offset;
+};
+struct layout_obj : object {
+  std::vector<layout_elem> elems;
+  layout_obj() : object(K_LAYOUT) {}
+};
+struct raster_obj : object {
+  srx_raster_desc desc;
+  raster_obj() : object(K_RASTER) {}
+};
+struct ds_obj : object {
+  srx_depth_stencil_desc desc;
+  ds_obj() : object(K_DS) {}
+};
+// pipeline_profiles slots (async_object.h:187-196): gather_vtx, vtx_proc, clipping, compact_clip,
+// vp_trans, tri_dispatch, ras; slot 7 counts the profiled draws.  Slots 8 and 9 split ras into
+// its visibility/in-order raster kernel and the deferred shading kernel (returned when the caller's
+// buffer holds them).
+enum { PROF_GATHER = 0, PROF_VTX, PROF_CLIP, PROF_COMPACT, PROF_VP, PROF_DISPATCH, PROF_RAS, PROF_DRAWS,
+       PROF_RAS_VIS, PROF_RAS_SHADE, PROF_COUNT };
+
+struct query_obj : object {
+  uint32_t id;
+  bool running = false, has_data = false;
+  uint64_t begin_snap[ST_COUNT] = {};
+  srx_pipeline_statistics stats{};
+  srx_internal_statistics istats{};
+  uint64_t prof[PROF_COUNT] = {};
+  query_obj() : object(K_QUERY) {}
+};
+
+// HIP events bracketing the stages of one profiled draw, on the renderer's stream:
+// [0,1) vertex shader, [1,2) setup + clipping (viewport transform fused), [2,3) compaction scan,
+// [3,4) record emission + tile sort, [4,5) visibility or in-order raster, [5,6) deferred shading.
+struct draw_events {
+  hipEvent_t ev[7];
+};
+
+inline std::string lower(const char* s) {
+  std::string r(s ? s : "");
+  for (auto& c : r)
+    if ('A' <= c && c <= 'Z') c = static_cast<char>(c - 'A' + 'a');
+  return r;
+}
+// semantic_value name normalisation (shader/constants.h:52-83)
+inline std::string canon_sem(const std::string& l) {
+  if (l == "position" || l == "sv_position") return "position";
+  if (l == "color" || l == "sv_target") return "color";
+  return l;
+}
+
+struct vs_binding {
+  const char* sem;
+  uint32_t index;
+  int reg;
+};
+inline int vs_bindings(int id, vs_binding* out) {
+  switch (id) {
+  case SRX_VS_PASSTHROUGH_COLOR:
+  case SRX_VS_WVP_COLOR:
+  case SRX_VS_WVP_COLOR_LIGHT:
+    out[0] = {"position", 0, 0};
+    out[1] = {"color", 0, 1};
+    return 2;
+  case SRX_VS_WVP_NORMAL_UV:
+    out[0] = {"position", 0, 0};
+    out[1] = {"normal", 0, 1};
+    out[2] = {"texcoord", 0, 2};
+    return 3;
+  }
+  return -1;
+}
+inline int vs_output_count(int id) {
+  switch (id) {
+  case SRX_VS_PASSTHROUGH_COLOR:
+  case SRX_VS_WVP_COLOR: return 1;
+  case SRX_VS_WVP_NORMAL_UV:
+  case SRX_VS_WVP_COLOR_LIGHT: return 2;
+  }
+  return 0;
+}
+inline uint32_t fmt_bytes(uint32_t fmt) {
+  switch (fmt) {
+  case SRX_FMT_R32_FLOAT: return 4;
+  case SRX_FMT_R32G32_FLOAT: return 8;
+  case SRX_FMT_R32G32B32_FLOAT: return 12;
+  case SRX_FMT_R32G32B32A32_FLOAT:
+  case SRX_FMT_R32G32B32A32_SINT: return 16;
+  }
+  return 0;
+}
+
+// Blender's EWA weights (exp(a r) - exp(a)) / (1 - exp(a)), a = -2, r = i/255, printed with six
+// significant digits -- the values of the reference's table (sampler.cpp:618-677).
+inline void make_ewa(float* t) {
+  for (int i = 0; i < 256; ++i) {
+    double r = i / 255.0;
+    double w = (std::exp(-2.0 * r) - std::exp(-2.0)) / (1.0 - std::exp(-2.0));
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%g", w);
+    t[i] = std::strtof(buf, nullptr);
+  }
+}
+
+// ---------------------------------------------------------------- renderer
+struct vb_slot {
+  ref<buffer_obj> buf;
+  size_t stride = 0, offset = 0;
+};
+
+struct render_state {
+  vb_slot vbs[SRX_MAX_INPUT_SLOTS];
+  ref<buffer_obj> ib;
+  uint32_t index_fmt = SRX_FMT_R16_UINT;
+  ref<layout_obj> layout;
+  uint32_t topo = SRX_TOPOLOGY_TRIANGLE_LIST;
+  int vs_id = 0, ps_id = 0, blend_id = SRX_BLEND_OFF;
+  float vs_consts[32] = {}, ps_consts[16] = {};
+  ref<sampler_obj> samplers[SRX_MAX_SAMPLER_SLOTS];
+  srx_raster_desc raster;
+  srx_depth_stencil_desc ds;
+  int32_t stencil_ref = 0;
+  std::vector<ref<surface_obj>> color_targets;
+  ref<surface_obj> ds_target;
+  srx_viewport target_vp{0, 0, 0, 0, 0, 0};
+  size_t target_samples = 0;
+  srx_viewport vp{0, 0, 0, 0, 0, 1};
+};
+
+template <class T>
+struct dev_array {
+  T* p = nullptr;
+  size_t cap = 0;
+};
+
+struct batch_obj;
+
+struct renderer_obj : object {
+  render_state st;
+  batch_obj* recording = nullptr;  // srx_batch_begin .. srx_batch_end: commands are recorded
+  hipStream_t stream = nullptr;
+  std::vector<ref<query_obj>> active;
+  // per-draw device scratch, grown on demand
+  dev_array<float4> vbuf, tri_rec, tri_attr, clip_verts;
+  dev_array<ps_samplers> samplers_d;
+  ps_samplers* samplers_h = nullptr;  // pinned staging of the per-draw sampler table
+  dev_array<uint32_t> cnt, keys0, keys1, vals0, vals1, tstart, tend, hist, scan_tmp, clip_hdr, counters, vis;
+  dev_array<uint32_t> order;  // GPU tile scheduling order
+  dev_array<uint4> vis_entries, vis_merges;  // visibility work table
+  dev_array<uint32_t> vis_counts;
+  dev_array<unsigned long long> vis_partial;
+  dev_array<uint8_t> clip_flag;
+  dev_array<int32_t> clip_slot;
+  dev_array<unsigned long long> partial;
+  unsigned long long* stats_d = nullptr;
+  uint32_t* pinned = nullptr;  // host staging for counter read-back
+  uint32_t last_records = 0;   // binned records of the previous draw (sizes the next emission)
+  uint64_t host_stats[ST_COUNT] = {};
+  std::vector<draw_events> pending_prof;  // recorded while a pipeline_profiles query runs
+  std::vector<hipEvent_t> event_pool;
+  bool profiling() const {
+    for (auto& q : active)
+      if (q->id == SRX_QUERY_PIPELINE_PROFILES) return true;
+    return false;
+  }
+  hipEvent_t take_event() {
+    if (!event_pool.empty()) {
+      hipEvent_t e = event_pool.back();
+      event_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+  renderer_obj() : object(K_RENDERER) {
+    srx_default_raster_desc(&st.raster);
+    srx_default_depth_stencil_desc(&st.ds);
+  }
+  ~renderer_obj() override;
+};
+
+template <class T>
+int32_t ensure(renderer_obj* r, dev_array<T>& a, size_t n) {
+  if (n <= a.cap && a.p) return SRX_OK;
+  size_t cap = std::max<size_t>(n + n / 2, 1024);
+  if (a.p) {
+    HIPCHK(hipStreamSynchronize(r->stream));
+    HIPCHK(hipFree(a.p));
+    a.p = nullptr;
+    a.cap = 0;
+  }
+  HIPCHK(hipMalloc(&a.p, cap * sizeof(T)));
+  a.cap = cap;
+  return SRX_OK;
+}
+
+inline std::once_flag g_ewa_once;
+inline int32_t g_ewa_status = SRX_OK;
+
+inline int gpu_tile_size() {
+  static int g = [] {
+    const char* e = std::getenv("SRX_GPU_TILE");
+    int v = e ? std::atoi(e) : 16;
+    return (v == 32) ? 32 : 16;
+  }();
+  return g;
+}
+
+inline void snapshot_stats(renderer_obj* r, uint64_t* out) {
+  unsigned long long dev[ST_COUNT] = {};
+  (void)hipStreamSynchronize(r->stream);
+  (void)hipMemcpy(dev, r->stats_d, sizeof dev, hipMemcpyDeviceToHost);
+  for (int i = 0; i < ST_COUNT; ++i) out[i] = r->host_stats[i] + dev[i];
+}
+
+// Reads the draw's device counters (waits for the queued work).
+inline int32_t read_counters(renderer_obj* r, uint32_t* out) {
+  HIPCHK(hipMemcpyAsync(r->pinned, r->counters.p, CT_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, r->stream));
+  HIPCHK(hipStreamSynchronize(r->stream));
+  for (int i = 0; i < CT_COUNT; ++i) out[i] = r->pinned[i];
+  return SRX_OK;
+}
+
+#define RCHK(expr)                     \
+  do {                                 \
+    int32_t _rc = (expr);              \
+    if (_rc != SRX_OK) return _rc;     \
+  } while (0)
+
+template <class T>
+T* as(void* h, int kind) {
+  auto* o = static_cast<object*>(h);
+  return (o && o->kind == kind) ? static_cast<T*>(o) : nullptr;
+}
+
+// One device image of stage maps: per stage, job[] then range[] (uint2, 8-byte aligned).
+struct map_image {
+  std::vector<uint32_t> words;
+  struct entry {
+    size_t job_off, range_off;
+    uint32_t blocks;
+  };
+  // order: jobs in launch order; blocks[j]: workgroups of job j (0: none)
+  entry add(const std::vector<uint32_t>& blocks, const std::vector<uint32_t>& order) {
+    std::vector<uint32_t> first(blocks.size(), 0);
+    uint32_t total = 0;
+    for (uint32_t j : order) {
+      first[j] = total;
+      total += blocks[j];
+    }
+    entry e;
+    if (words.size() & 1) words.push_back(0);
+    e.range_off = words.size();
+    for (size_t j = 0; j < blocks.size(); ++j) {
+      words.push_back(first[j]);
+      words.push_back(blocks[j]);
+    }
+    e.job_off = words.size();
+    for (uint32_t j : order)
+      for (uint32_t b = 0; b < blocks[j]; ++b) words.push_back(j);
+    e.blocks = total;
+    return e;
+  }
+  static stage_map at(const uint32_t* base, const entry& e) {
+    return stage_map{base + e.job_off, reinterpret_cast<const uint2*>(base + e.range_off)};
+  }
+};
+
+
+// ---------------------------------------------------------------- across translation units
+// srx_runtime.cpp
+int32_t build_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t primcnt, int32_t basevert, draw_params& P,
+                   ps_samplers& SM, bool& skip);
+// srx_batch.cpp: commands recorded between srx_batch_begin and srx_batch_end
+int32_t record_draw(renderer_obj* r, const draw_params& P, const ps_samplers& SM);
+int32_t record_fill(renderer_obj* r, surface_obj* o, float4 v);
+int32_t record_resolve(renderer_obj* r, surface_obj* a, surface_obj* d);
+int32_t batch_refuse(renderer_obj* r, int32_t code, const char* msg);  // the first recording error sticks
+void drop_recording(renderer_obj* r);
+
+}  // namespace srx_rt
