@@ -219,10 +219,29 @@ def serving(lib, n_requests=10000, rates=(0.25, 0.5, 0.75, 1.0), seed=0x5A11A005
         sweep.append(res)
         print(f"serving: load {f} -> {res['frames_per_s']:.0f} frames/s p50 {res['p50_ms']:.2f} ms "
               f"p99 {res['p99_ms']:.2f} ms", file=sys.stderr, flush=True)
+    # spot check: a few served images against the same jobs rendered one by one through the API
+    small = sorted(range(len(specs)), key=lambda j: specs[j].width * specs[j].height * specs[j].n_tris)[:4]
+    checked = mismatches = 0
+    for j in small:
+        for smp in (1, 4):
+            sp = specs[j]
+            t = srv.submit(sp.seed, sp.n_tris, sp.width, sp.height, smp, texture_key=sp.seed)
+            got = None
+            while got is None:
+                for c in srv.poll(16, wait_ms=50):
+                    if c["ticket"] == t:
+                        got = c["color"]
+            ref = scenes.BatchedJobs(lib.renderer(), [scenes.JobSpec(sp.seed, sp.width, sp.height, sp.n_tris,
+                                                                     sp.tex_size, smp)], with_textures=False)
+            ref.render_all()
+            ref.r.flush()
+            checked += 1
+            mismatches += int(not np.array_equal(got, ref.output(0).ravel()))
     st = srv.stats()
     return {"workload": f"{n_requests} requests per rate from the 256 batched jobs, 1x/4x MSAA at random, "
                         "Poisson arrivals; meshes and textures resident (LRU pool cap 200 GiB)",
-            "capacity": cap, "sweep": sweep, "batches": st.batches, "retries": st.retries,
+            "capacity": cap, "sweep": sweep, "checked_against_plain": checked, "mismatches": mismatches,
+            "batches": st.batches, "retries": st.retries,
             "arena_peak_bytes": st.arena_peak_bytes, "pool_bytes": st.pool_bytes}
 
 

@@ -1299,10 +1299,7 @@ __device__ __forceinline__ float4 shadow_pcf(const draw_params& P, const ps_samp
 
 // Register r of quad pixel i (0..3) by selects (a dynamically indexed register array would go to
 // scratch memory).
-template <int NA>
-__device__ __forceinline__ float4 quad_reg(const float4 (&q)[4][NA + 1], int i, int r) {
-  return sel4(i == 0, q[0][r], sel4(i == 1, q[1][r], sel4(i == 2, q[2][r], q[3][r])));
-}
+
 
 // Interpolated registers of single pixels of a 2x2 quad, computed only where read: pixel i's value
 // is interp_quad's chain for it (p00 = v0 + (ddx*dx + ddy*dy), +ddx for the right column, then
