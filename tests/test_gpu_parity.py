@@ -115,8 +115,9 @@ def render_multipass(lib, stats, **kw):
 
 
 @pytest.mark.parametrize("kw", [dict(width=320, height=180, samples=4, shadow_size=256, n_tris=3000),
-                                dict(width=640, height=360, samples=2, shadow_size=512, n_tris=20000)],
-                         ids=["small", "medium"])
+                                dict(width=640, height=360, samples=2, shadow_size=512, n_tris=20000),
+                                dict()],
+                         ids=["small", "medium", "full_config"])
 def test_multipass_shadow_blend_stencil_mirror(hip_lib, cpu_lib, kw):
     """Shadow map (depth only, split tiles) -> PCF-shaded opaque pass -> translucent blend pass ->
     stencil mirror mark -> mirrored redraw, bit-exact on both GPU paths."""
