@@ -254,7 +254,7 @@ int32_t batch_prepare(batch_obj* b) {
   float4 *vbuf, *tri_rec, *tri_attr, *clip_verts;
   uint8_t* clip_flag;
   int32_t* clip_slot;
-  uint32_t *clip_hdr, *tiles3;
+  uint32_t *clip_hdr, *tiles3, *wide;
   ps_samplers* samplers;
   auto& F = b->fixed;
   RCHK(batch_alloc(b, F, &vbuf, n_vbuf));
@@ -264,6 +264,7 @@ int32_t batch_prepare(batch_obj* b) {
   RCHK(batch_alloc(b, F, &clip_flag, n_prims));
   RCHK(batch_alloc(b, F, &clip_slot, n_prims));
   RCHK(batch_alloc(b, F, &clip_hdr, n_prims));
+  RCHK(batch_alloc(b, F, &wide, n_prims));
   RCHK(batch_alloc(b, F, &tiles3, 3 * n_tiles));
   RCHK(batch_alloc(b, F, &samplers, J));
   RCHK(batch_alloc(b, F, &b->cnt_all, b->cnt_n));
@@ -296,6 +297,7 @@ int32_t batch_prepare(batch_obj* b) {
     B.clip_flag = clip_flag + o_prims;
     B.clip_slot = clip_slot + o_prims;
     B.clip_hdr = clip_hdr + o_prims;
+    B.wide_list = wide + o_prims;
     B.clip_capacity = P.prim_count;
     B.cnt = b->cnt_all + o_cnt;
     B.counters = b->counters_all + static_cast<size_t>(j) * CT_COUNT;
@@ -315,7 +317,7 @@ int32_t batch_prepare(batch_obj* b) {
     o_cnt += 3ull * P.prim_count;
     vs_blocks[j] = (P.nverts + 1 + 255) / 256;
     setup_blocks[j] = (P.prim_count + 255) / 256;
-    clip_blocks[j] = std::min<uint32_t>((P.prim_count + 63) / 64, 16);
+    clip_blocks[j] = std::min<uint32_t>((P.prim_count + 63) / 64, 128);
   }
   HIPCHK(hipMemcpyAsync(samplers, sm.data(), sizeof(ps_samplers) * J, hipMemcpyHostToDevice, s));
 

@@ -18,7 +18,8 @@ enum stat_slot {
   ST_PS_INVOCATIONS, ST_BACKEND_PIXELS, ST_COUNT
 };
 // Setup counters
-enum counter_slot { CT_CLIP_PRIMS = 0, CT_CLIP_SLOTS, CT_CLIP_OVERFLOW, CT_TOTAL, CT_COUNT };
+// CT_WIDE: triangles whose records the clip launch counts by wave (their ids in wide_list)
+enum counter_slot { CT_CLIP_PRIMS = 0, CT_CLIP_SLOTS, CT_CLIP_OVERFLOW, CT_TOTAL, CT_WIDE, CT_COUNT };
 
 // Per-draw device buffers (owned by the renderer, grown on demand).
 struct draw_buffers {
@@ -33,6 +34,7 @@ struct draw_buffers {
   float4* clip_verts;  // clip_capacity * 5 * (nattr+1), screen space
   uint32_t* clip_hdr;  // clip_capacity
   uint32_t clip_capacity;
+  uint32_t* wide_list; // prims: triangles over many 64x64 tiles, counted by the clip launch
   uint32_t* keys[2];   // records: GPU tile id
   uint32_t* vals[2];   // records: triangle id << 1 | tile-level accept
   const uint32_t* order;  // GPU tiles, most records first (workgroup launch order)

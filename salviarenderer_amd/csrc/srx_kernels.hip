@@ -434,6 +434,35 @@ __device__ __forceinline__ void for_each_gtile(const draw_params& P, const float
     }
 }
 
+// 64x64 tiles in a triangle's bounding box (as bin_setup clamps it)
+__device__ __forceinline__ int bin_span(const draw_params& P, const float* bb) {
+  const int sx = min(fast_floori(std_max(0.0f, bb[0]) / TILE), P.tiles_x);
+  const int sy = min(fast_floori(std_max(0.0f, bb[2]) / TILE), P.tiles_y);
+  const int ex = min(fast_ceili(std_max(0.0f, bb[1]) / TILE) + 1, P.tiles_x);
+  const int ey = min(fast_ceili(std_max(0.0f, bb[3]) / TILE) + 1, P.tiles_y);
+  return (ex > sx && ey > sy) ? (ex - sx) * (ey - sy) : 0;
+}
+// Triangles over more than this many 64x64 tiles are counted by a whole wave (clip launch), not by
+// setup's one lane.
+constexpr int COUNT_WIDE = 32;
+
+// count_gtiles by a whole wave (every lane calls it with the same triangle): lanes take the 64x64
+// tiles, the wave sums their GPU tiles.
+__device__ __forceinline__ uint32_t count_gtiles_wave(const draw_params& P, const float* e, const float* bb) {
+  tri_bin t;
+  bin_setup(P, e, bb, t);
+  const int w = t.ex - t.sx;
+  const int R = (t.ex > t.sx && t.ey > t.sy) ? w * (t.ey - t.sy) : 0;
+  uint32_t n = 0;
+  for (int tt = __lane_id(); tt < R; tt += 64) {
+    bool full;
+    n += static_cast<uint32_t>(__popcll(tile_gmask(P, e, bb, t, t.sx + tt % w, t.sy + tt / w, full)));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) n += static_cast<uint32_t>(__shfl_xor(static_cast<int>(n), off, 64));
+  return n;
+}
+
 // The number of GPU tiles for_each_gtile visits.
 __device__ __forceinline__ uint32_t count_gtiles(const draw_params& P, const float* e, const float* bb) {
   tri_bin t;
@@ -508,7 +537,8 @@ template <int NA>
 __device__ __forceinline__ void setup_body(const draw_params& P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
                                            float4* __restrict__ tri_attr, uint32_t* __restrict__ cnt,
                                            uint8_t* __restrict__ clip_flag, uint32_t* __restrict__ counters,
-                                           unsigned long long* __restrict__ partial, uint32_t bx) {
+                                           uint32_t* __restrict__ wide_list, unsigned long long* __restrict__ partial,
+                                           uint32_t bx) {
   uint32_t p = bx * blockDim.x + threadIdx.x;
   unsigned long long cprims = 0;
   if (p < P.prim_count) {
@@ -546,7 +576,10 @@ __device__ __forceinline__ void setup_body(const draw_params& P, const float4* _
           for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) e[i * 3 + j] = t.e[i][j];
           for (int i = 0; i < 4; ++i) bb[i] = t.bb[i];
-          cnt[3 * p] = count_gtiles(P, e, bb);
+          if (bin_span(P, bb) > COUNT_WIDE)
+            wide_list[atomicAdd(&counters[CT_WIDE], 1u)] = p;  // counted in the clip launch
+          else
+            cnt[3 * p] = count_gtiles(P, e, bb);
         }
       }
     }
@@ -557,8 +590,9 @@ template <int NA>
 __global__ void __launch_bounds__(256) setup_kernel(draw_params P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
                                                      float4* __restrict__ tri_attr,
                                                      uint32_t* __restrict__ cnt, uint8_t* __restrict__ clip_flag,
-                                                     uint32_t* __restrict__ counters, unsigned long long* __restrict__ partial) {
-  setup_body<NA>(P, vbuf, tri_rec, tri_attr, cnt, clip_flag, counters, partial, blockIdx.x);
+                                                     uint32_t* __restrict__ counters, uint32_t* __restrict__ wide_list,
+                                                     unsigned long long* __restrict__ partial) {
+  setup_body<NA>(P, vbuf, tri_rec, tri_attr, cnt, clip_flag, counters, wide_list, partial, blockIdx.x);
 }
 
 // ============================================================== clipping (slow path)
@@ -657,8 +691,19 @@ __device__ __forceinline__ void clip_body(const draw_params& P, const float4* __
                                           const uint8_t* __restrict__ clip_flag, int32_t* __restrict__ clip_slot,
                                           float4* __restrict__ clip_verts, uint32_t* __restrict__ clip_hdr, uint32_t capacity,
                                           uint32_t* __restrict__ counters, unsigned long long* __restrict__ stats,
-                                          uint32_t bx, uint32_t gx) {
-  if (counters[CT_CLIP_PRIMS] == 0) return;  // setup flagged nothing
+                                          const uint32_t* __restrict__ wide_list, uint32_t bx, uint32_t gx) {
+  const uint32_t nclip = counters[CT_CLIP_PRIMS], nwide = counters[CT_WIDE];
+  if (nclip == 0 && nwide == 0) return;  // setup flagged nothing
+  // triangles over many 64x64 tiles: one wave counts each one's records
+  const uint32_t wave = (bx * blockDim.x + threadIdx.x) >> 6, waves = (gx * blockDim.x) >> 6;
+  for (uint32_t w = wave; w < nwide; w += waves) {
+    const uint32_t p = wide_list[w];
+    float e[9], bb[4];
+    read_tri_rec(tri_rec + static_cast<size_t>(p) * 5, e, bb);  // fan 0 of primitive p
+    const uint32_t n = count_gtiles_wave(P, e, bb);
+    if (__lane_id() == 0) cnt[3 * p] = n;
+  }
+  if (nclip == 0) return;
   for (uint32_t p = bx * blockDim.x + threadIdx.x; p < P.prim_count; p += gx * blockDim.x)
     if (clip_flag[p]) clip_primitive<NA>(P, p, vbuf, tri_rec, tri_attr, cnt, clip_slot, clip_verts, clip_hdr, capacity,
                                          counters, stats);
@@ -667,9 +712,10 @@ template <int NA>
 __global__ void clip_kernel(draw_params P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
                             float4* __restrict__ tri_attr, uint32_t* __restrict__ cnt, const uint8_t* __restrict__ clip_flag, int32_t* __restrict__ clip_slot,
                             float4* __restrict__ clip_verts, uint32_t* __restrict__ clip_hdr, uint32_t capacity,
-                            uint32_t* __restrict__ counters, unsigned long long* __restrict__ stats) {
+                            uint32_t* __restrict__ counters, unsigned long long* __restrict__ stats,
+                            const uint32_t* __restrict__ wide_list) {
   clip_body<NA>(P, vbuf, tri_rec, tri_attr, cnt, clip_flag, clip_slot, clip_verts, clip_hdr, capacity, counters, stats,
-                blockIdx.x, gridDim.x);
+                wide_list, blockIdx.x, gridDim.x);
 }
 
 // ============================================================== record emission
@@ -1296,10 +1342,6 @@ __device__ __forceinline__ float4 shadow_pcf(const draw_params& P, const ps_samp
   const float k = P.ps_consts[0] + (1.0f - P.ps_consts[0]) * lit;
   return make_float4(a.x * k, a.y * k, a.z * k, a.w);
 }
-
-// Register r of quad pixel i (0..3) by selects (a dynamically indexed register array would go to
-// scratch memory).
-
 
 // Interpolated registers of single pixels of a 2x2 quad, computed only where read: pixel i's value
 // is interp_quad's chain for it (p00 = v0 + (ddx*dx + ddy*dy), +ddx for the right column, then
@@ -2385,7 +2427,8 @@ void launch_setup(const draw_params& P, const draw_buffers& B, hipStream_t s) {
   if (P.prim_count == 0) return;
   dim3 grid((P.prim_count + 255) / 256);
 #define SRX_SETUP(NA) \
-  hipLaunchKernelGGL(setup_kernel<NA>, grid, dim3(256), 0, s, P, B.vbuf, B.tri_rec, B.tri_attr, B.cnt, B.clip_flag, B.counters, B.partial)
+  hipLaunchKernelGGL(setup_kernel<NA>, grid, dim3(256), 0, s, P, B.vbuf, B.tri_rec, B.tri_attr, B.cnt, B.clip_flag, B.counters, \
+                     B.wide_list, B.partial)
   SRX_NA_DISPATCH(P.nattr, SRX_SETUP)
 #undef SRX_SETUP
   if (P.count_stats)
@@ -2397,7 +2440,7 @@ void launch_clip(const draw_params& P, const draw_buffers& B, hipStream_t s) {
   dim3 grid(std::min<uint32_t>((P.prim_count + 63) / 64, 1024));
 #define SRX_CLIP(NA)                                                                                                     \
   hipLaunchKernelGGL(clip_kernel<NA>, grid, dim3(64), 0, s, P, B.vbuf, B.tri_rec, B.tri_attr, B.cnt, B.clip_flag, B.clip_slot, \
-                     B.clip_verts, B.clip_hdr, B.clip_capacity, B.counters, B.stats)
+                     B.clip_verts, B.clip_hdr, B.clip_capacity, B.counters, B.stats, B.wide_list)
   SRX_NA_DISPATCH(P.nattr, SRX_CLIP)
 #undef SRX_CLIP
 }
@@ -2681,7 +2724,8 @@ template <int NA>
 __global__ void __launch_bounds__(256) setup_batch_kernel(const batch_job* __restrict__ jobs, stage_map m) {
   uint32_t lb, nb;
   const batch_job& J = jobs[stage_job(m, lb, nb)];
-  setup_body<NA>(J.P, J.B.vbuf, J.B.tri_rec, J.B.tri_attr, J.B.cnt, J.B.clip_flag, J.B.counters, J.B.partial, lb);
+  setup_body<NA>(J.P, J.B.vbuf, J.B.tri_rec, J.B.tri_attr, J.B.cnt, J.B.clip_flag, J.B.counters, J.B.wide_list, J.B.partial,
+                 lb);
 }
 
 template <int NA>
@@ -2689,7 +2733,7 @@ __global__ void clip_batch_kernel(const batch_job* __restrict__ jobs, stage_map 
   uint32_t lb, nb;
   const batch_job& J = jobs[stage_job(m, lb, nb)];
   clip_body<NA>(J.P, J.B.vbuf, J.B.tri_rec, J.B.tri_attr, J.B.cnt, J.B.clip_flag, J.B.clip_slot, J.B.clip_verts,
-                J.B.clip_hdr, J.B.clip_capacity, J.B.counters, J.B.stats, lb, nb);
+                J.B.clip_hdr, J.B.clip_capacity, J.B.counters, J.B.stats, J.B.wide_list, lb, nb);
 }
 
 // The job's counts are followed by the next job's first scanned count (or the batch total): the

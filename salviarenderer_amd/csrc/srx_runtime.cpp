@@ -191,6 +191,7 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   RCHK(ensure(r, r->cnt, P3));
   RCHK(ensure(r, r->clip_flag, primcnt));
   RCHK(ensure(r, r->clip_slot, primcnt));
+  RCHK(ensure(r, r->wide, primcnt));
   RCHK(ensure(r, r->scan_tmp, scan_tmp_size(P3)));
   uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
   RCHK(ensure(r, r->partial, std::max<size_t>(T, (primcnt + 255) / 256) + 1));
@@ -206,6 +207,7 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   B.cnt = r->cnt.p;
   B.clip_flag = r->clip_flag.p;
   B.clip_slot = r->clip_slot.p;
+  B.wide_list = r->wide.p;
   B.counters = r->counters.p;
   B.stats = r->stats_d;
   B.partial = r->partial.p;
@@ -319,7 +321,7 @@ renderer_obj::~renderer_obj() {
   };
   fr(vbuf); fr(tri_rec); fr(tri_attr); fr(clip_verts); fr(samplers_d); fr(vis);
   if (samplers_h) (void)hipHostFree(samplers_h); fr(cnt); fr(keys0); fr(keys1); fr(vals0); fr(vals1);
-  fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot); fr(partial); fr(order); fr(vis_entries); fr(vis_merges); fr(vis_counts); fr(vis_partial);
+  fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot); fr(partial); fr(order); fr(wide); fr(vis_entries); fr(vis_merges); fr(vis_counts); fr(vis_partial);
   if (stats_d) (void)hipFree(stats_d);
   if (pinned) (void)hipHostFree(pinned);
   for (auto& d : pending_prof)

@@ -285,6 +285,7 @@ struct renderer_obj : object {
   ps_samplers* samplers_h = nullptr;  // pinned staging of the per-draw sampler table
   dev_array<uint32_t> cnt, keys0, keys1, vals0, vals1, tstart, tend, hist, scan_tmp, clip_hdr, counters, vis;
   dev_array<uint32_t> order;  // GPU tile scheduling order
+  dev_array<uint32_t> wide;   // triangles counted by wave
   dev_array<uint4> vis_entries, vis_merges;  // visibility work table
   dev_array<uint32_t> vis_counts;
   dev_array<unsigned long long> vis_partial;

@@ -194,7 +194,7 @@ void serve_params(draw_params& P, const uint8_t* mesh, uint32_t n, uint32_t w, u
 
 // Device bytes of one job with `records` bin entries and its share of the batch-wide arrays.
 struct job_layout {
-  size_t color, ds, resolved, vbuf, tri_rec, tri_attr, clip_verts, clip_flag, clip_slot, clip_hdr, tiles, hist, total;
+  size_t color, ds, resolved, vbuf, tri_rec, tri_attr, clip_verts, clip_flag, clip_slot, clip_hdr, wide, tiles, hist, total;
 };
 job_layout layout_of(const srx_request& q, uint32_t records, uint32_t bin_items) {
   const size_t px = static_cast<size_t>(q.width) * q.height, n = q.n_tris, T = ((q.width + 15) / 16) * ((q.height + 15) / 16);
@@ -216,6 +216,7 @@ job_layout layout_of(const srx_request& q, uint32_t records, uint32_t bin_items)
   L.clip_flag = put(n);
   L.clip_slot = put(4 * n);
   L.clip_hdr = put(4 * n);
+  L.wide = put(4 * n);
   L.tiles = put(12 * T);
   L.hist = put(4 * T * nb);
   L.total = o;
@@ -325,14 +326,14 @@ uint64_t job_map_words(const srx_request& q, uint32_t est, uint32_t items) {
   const uint64_t nb = std::max<uint64_t>((est + items - 1) / items, 1);
   const uint64_t fill_c = (std::max<uint64_t>(px * q.samples * 4 / 16, 1) + 255) / 256;
   const uint64_t fill_d = (std::max<uint64_t>(px * q.samples * 8 / 16, 1) + 255) / 256;
-  return 2 * 12 + (3 * n + 1 + 255) / 256 + (n + 255) / 256 + std::min<uint64_t>((n + 63) / 64, 16) + nb +
+  return 2 * 12 + (3 * n + 1 + 255) / 256 + (n + 255) / 256 + std::min<uint64_t>((n + 63) / 64, 128) + nb +
          (T + 255) / 256 + T + fill_c + fill_d + (q.samples > 1 ? (px + 255) / 256 : 0);
 }
 
 // Device bytes a batch needs besides its jobs' own ranges.
 size_t batch_shared_bytes(size_t jobs, uint64_t cnt_n, uint64_t capacity, uint64_t hist_n, uint64_t map_words) {
   const size_t scan = std::max(scan_tmp_size(static_cast<uint32_t>(cnt_n)), scan_tmp_size(static_cast<uint32_t>(hist_n)));
-  return range_arena::round(4 * cnt_n) + range_arena::round(16 * jobs) + range_arena::round(8 * jobs + 16) +
+  return range_arena::round(4 * cnt_n) + range_arena::round(4 * CT_COUNT * jobs) + range_arena::round(8 * jobs + 16) +
          3 * range_arena::round(4 * capacity) + range_arena::round(4 * hist_n) + range_arena::round(4 * scan) +
          range_arena::round(sizeof(batch_job) * jobs) + range_arena::round(sizeof(ps_samplers) * jobs) +
          range_arena::round(4 * (map_words + 16)) + range_arena::round(sizeof(fill_item) * 2 * jobs) +
@@ -459,6 +460,7 @@ int32_t launch_served_batch(server_obj* S, int si) {
     B.clip_flag = base + L.clip_flag;
     B.clip_slot = reinterpret_cast<int32_t*>(base + L.clip_slot);
     B.clip_hdr = reinterpret_cast<uint32_t*>(base + L.clip_hdr);
+    B.wide_list = reinterpret_cast<uint32_t*>(base + L.wide);
     B.clip_capacity = q.n_tris;
     uint32_t* t3 = reinterpret_cast<uint32_t*>(base + L.tiles);
     B.tstart = t3;
@@ -481,7 +483,7 @@ int32_t launch_served_batch(server_obj* S, int si) {
   }
   // batch-wide arrays
   uint32_t* cnt_all = reinterpret_cast<uint32_t*>(take_dev(4 * cnt_n));
-  uint32_t* counters_all = reinterpret_cast<uint32_t*>(take_dev(16 * J));
+  uint32_t* counters_all = reinterpret_cast<uint32_t*>(take_dev(4 * CT_COUNT * J));
   uint32_t* counts_d = reinterpret_cast<uint32_t*>(take_dev(8 * J + 16));
   uint32_t* keys0 = reinterpret_cast<uint32_t*>(take_dev(4 * cap));
   uint32_t* vals0 = reinterpret_cast<uint32_t*>(take_dev(4 * cap));
@@ -496,7 +498,7 @@ int32_t launch_served_batch(server_obj* S, int si) {
   for (uint32_t j = 0; j < J; ++j) {
     batch_job& jb = jobs[j];
     jb.B.cnt = cnt_all + co;
-    jb.B.counters = counters_all + 4 * j;
+    jb.B.counters = counters_all + CT_COUNT * j;
     jb.B.samplers = sm_d + j;
     jb.B.keys[0] = keys0;
     jb.B.vals[0] = vals0;
@@ -506,7 +508,7 @@ int32_t launch_served_batch(server_obj* S, int si) {
     ho += static_cast<uint64_t>(jb.tiles) * jb.bucket_blocks;
     vs_b[j] = (jb.P.nverts + 1 + 255) / 256;
     setup_b[j] = (jb.P.prim_count + 255) / 256;
-    clip_b[j] = std::min<uint32_t>((jb.P.prim_count + 63) / 64, 16);
+    clip_b[j] = std::min<uint32_t>((jb.P.prim_count + 63) / 64, 128);
     bucket_b[j] = jb.bucket_blocks;
     range_b[j] = (jb.tiles + 255) / 256;
     order[j] = j;
@@ -572,7 +574,7 @@ int32_t launch_served_batch(server_obj* S, int si) {
   if (!resolves.empty()) HIPCHK(hipMemcpyAsync(res_d, h_res, sizeof(resolve_item) * resolves.size(), hipMemcpyHostToDevice, s));
   // the pipeline of batch_enqueue, launched directly (every served batch differs, so no graph)
   const uint32_t* mf = maps_d;
-  launch_batch_reset(counters_all, 4 * J, cnt_all + (cnt_n - 1), hist_all + (hist_n - 1), counts_d + 2 * J, s);
+  launch_batch_reset(counters_all, CT_COUNT * J, cnt_all + (cnt_n - 1), hist_all + (hist_n - 1), counts_d + 2 * J, s);
   launch_fill_batch(fills_d, map_image::at(mf, m_fill), m_fill.blocks, s);
   launch_vs_batch(jobs_d, map_image::at(mf, m_vs), m_vs.blocks, s);
   launch_setup_batch(1, jobs_d, map_image::at(mf, m_setup), m_setup.blocks, s);
