@@ -927,8 +927,9 @@ __global__ void __launch_bounds__(256) radix_hist_kernel(const uint32_t* __restr
   hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Each wave ranks its 1024 consecutive items in 16 rounds of 64; equal digits inside a round are
-// grouped with ballots, so ranks follow input order (stability).
+// Each wave ranks its 1024 consecutive items in 16 rounds of 64; a lane finds the lanes sharing its
+// digit with eight ballots (one per digit bit), so ranks follow input order (stability) at a cost
+// independent of how many digits a round holds.
 __global__ void __launch_bounds__(256) radix_scatter_kernel(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                              uint32_t n, int shift, uint32_t nblocks,
                                                              const uint32_t* __restrict__ hist_scanned,
@@ -947,19 +948,19 @@ __global__ void __launch_bounds__(256) radix_scatter_kernel(const uint32_t* __re
     key[r] = valid ? kin[i] : 0u;
     val[r] = valid ? vin[i] : 0u;
     uint32_t d = (key[r] >> shift) & 255u;
-    uint64_t active = __ballot(valid);
-    rank[r] = 0;
-    while (active) {
-      int leader = __builtin_ctzll(active);
-      uint32_t dl = __shfl(d, leader, 64);
-      uint64_t m = __ballot(valid && d == dl);
-      uint32_t before = wcnt[wave][dl];
-      if (valid && d == dl) rank[r] = before + static_cast<uint32_t>(__popcll(m & lt));
-      __builtin_amdgcn_wave_barrier();
-      if (lane == leader) wcnt[wave][dl] = before + static_cast<uint32_t>(__popcll(m));
-      __builtin_amdgcn_wave_barrier();
-      active &= ~m;
+    // m: the valid lanes holding this lane's digit, from one ballot per digit bit
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = ((d >> b) & 1u) != 0;
+      const uint64_t bal = __ballot(bit);
+      m &= bit ? bal : ~bal;
     }
+    const uint32_t before = valid ? wcnt[wave][d] : 0u;
+    rank[r] = before + static_cast<uint32_t>(__popcll(m & lt));
+    __builtin_amdgcn_wave_barrier();
+    if (valid && (m & lt) == 0) wcnt[wave][d] = before + static_cast<uint32_t>(__popcll(m));  // lowest lane of the digit
+    __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
   {
