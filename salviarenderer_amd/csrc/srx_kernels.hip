@@ -1260,69 +1260,6 @@ __device__ __forceinline__ void load_attr(const draw_params& P, const float4* __
   }
 }
 
-// Pixel shader on a 2x2 quad (cpp_pixel_shader::execute, cpp_pixel_shader.cpp:62-74); colour is
-// produced for the pixels in `need`.  The sample shaders never discard.
-template <int NA>
-__device__ __forceinline__ void run_ps(const draw_params& P, const ps_samplers* __restrict__ SM,
-                                       const float4 (&q)[4][NA + 1], uint32_t need, float4 (&col)[4]) {
-  switch (P.ps_id) {
-  case PS_GOURAUD:
-#pragma unroll
-    for (int i = 0; i < 4; ++i) col[i] = q[i][1];
-    return;
-  case PS_CONSTANT:
-#pragma unroll
-    for (int i = 0; i < 4; ++i) col[i] = make_float4(P.ps_consts[0], P.ps_consts[1], P.ps_consts[2], P.ps_consts[3]);
-    return;
-  case PS_LAMBERT_TEX:
-    if constexpr (NA >= 2) {
-      // SASL tex2D: ddx per row (p1-p0, p3-p2), ddy per column (p2-p0, p3-p1)
-      // (sasl/src/codegen/cgs_simd.cpp:275-312), then sample_2d_grad (sampler_api.cpp:13-18)
-      for (int i = 0; i < 4; ++i) {
-        if (!(need & (1u << i))) continue;
-        int row = i & 2, cl = i & 1;
-        float dxu = q[row + 1][2].x - q[row][2].x, dxv = q[row + 1][2].y - q[row][2].y;
-        float dyu = q[cl + 2][2].x - q[cl][2].x, dyv = q[cl + 2][2].y - q[cl][2].y;
-        float4 t = P.has_sampler[0] ? sample_2d_grad(SM->s[0], q[i][2].x, q[i][2].y, dxu, dxv, dyu, dyv, 0.0f)
-                                    : make_float4(0, 0, 0, 0);
-        float nx, ny, nz;
-        normalize3(q[i][1].x, q[i][1].y, q[i][1].z, nx, ny, nz);
-        float d = nx * P.ps_consts[0] + ny * P.ps_consts[1] + nz * P.ps_consts[2];
-        float k = std_clamp(d, 0.0f, 1.0f);
-        col[i] = make_float4(t.x * k, t.y * k, t.z * k, 1.0f);
-      }
-    }
-    return;
-  case PS_TEX_CPP: {
-    // cpp tex2d: one LOD per quad from p1 - p0 and p2 - p0 (cpp_pixel_shader.cpp:13-31)
-    float lod = 0.0f;
-    if (P.has_sampler[0]) {
-      const sampler_desc_dev& sp = SM->s[0];
-      float dxu = q[1][1].x - q[0][1].x, dxv = q[1][1].y - q[0][1].y;
-      float dyu = q[2][1].x - q[0][1].x, dyv = q[2][1].y - q[0][1].y;
-      lod = (sp.mip_filter == FILTER_ANISO && sp.max_aniso > 1) ? calc_aniso(sp, dxu, dxv, dyu, dyv, 0.0f).lod
-                                                                : calc_lod(sp, dxu, dxv, dyu, dyv, 0.0f);
-    }
-    for (int i = 0; i < 4; ++i) {
-      if (!(need & (1u << i))) continue;
-      float4 t = P.has_sampler[0] ? sample_impl(SM->s[0], q[i][1].x, q[i][1].y, lod, nullptr) : make_float4(0, 0, 0, 0);
-      t.w = 1.0f;
-      col[i] = t;
-    }
-    return;
-  }
-  case PS_SHADOW_PCF:
-    if constexpr (NA >= 2) {
-      for (int i = 0; i < 4; ++i)
-        col[i] = (need & (1u << i)) ? shadow_pcf(P, SM, q[i][1], q[i][2]) : make_float4(0, 0, 0, 0);
-    }
-    return;
-  default:
-#pragma unroll
-    for (int i = 0; i < 4; ++i) col[i] = make_float4(0, 0, 0, 0);
-  }
-}
-
 // 3x3 percentage-closer shadow lookup of one pixel (attr0 colour, attr1 light clip position): the
 // tap layout of StandardShadowMap.cpp:86-106, each tap a tex2dlod at lod 0, lit when the light-space
 // depth minus the bias does not exceed it.
@@ -1345,8 +1282,9 @@ __device__ __forceinline__ float4 shadow_pcf(const draw_params& P, const ps_samp
 }
 
 // Interpolated registers of single pixels of a 2x2 quad, computed only where read: pixel i's value
-// is interp_quad's chain for it (p00 = v0 + (ddx*dx + ddy*dy), +ddx for the right column, then
-// +ddy for the bottom row), perspective-divided by its own 1/w, so each value equals interp_quad's.
+// follows the quad's chain (p00 = v0 + (ddx*dx + ddy*dy), +ddx for the right column, then +ddy for
+// the bottom row), perspective-divided by its own 1/w, so every pixel gets the value a whole-quad
+// interpolation would give it.
 template <int NA>
 struct quad_lazy {
   const attr_setup<NA>& t;
@@ -1380,8 +1318,8 @@ struct quad_lazy {
   }
 };
 
-// The pixel shader's colour for quad pixel pi only (the deferred shading paths need one pixel of
-// the quad; the other three supply derivatives).  Same arithmetic as run_ps.
+// Pixel shader (cpp_pixel_shader::execute, cpp_pixel_shader.cpp:62-74) colour of quad pixel pi
+// only; the other three pixels supply derivatives.  The sample shaders never discard.
 template <int NA>
 __device__ __forceinline__ float4 run_ps_pixel(const draw_params& P, const ps_samplers* __restrict__ SM,
                                                const quad_lazy<NA>& q, int pi) {
@@ -1440,48 +1378,6 @@ __device__ __forceinline__ float4 run_ps_pixel(const draw_params& P, const ps_sa
     return make_float4(0, 0, 0, 0);
   default:
     return make_float4(0, 0, 0, 0);
-  }
-}
-
-template <int NA>
-__device__ __forceinline__ void interp_quad(const attr_setup<NA>& t, int px0, int py0, const draw_params& P,
-                                            float4 (&q)[4][NA + 1]) {
-  const float dx = (0.5f + static_cast<float>(px0)) - t.v0[0].x;
-  const float dy = (0.5f + static_cast<float>(py0)) - t.v0[0].y;
-  {
-    const float4 v = t.v0[0], a = t.dx[0], b = t.dy[0];
-    float4 p00 = make_float4(v.x + ((a.x * dx) + (b.x * dy)), v.y + ((a.y * dx) + (b.y * dy)),
-                             v.z + ((a.z * dx) + (b.z * dy)), v.w + ((a.w * dx) + (b.w * dy)));
-    float4 p01 = make_float4(p00.x + a.x, p00.y + a.y, p00.z + a.z, p00.w + a.w);
-    q[0][0] = p00;
-    q[1][0] = p01;
-    q[2][0] = make_float4(p00.x + b.x, p00.y + b.y, p00.z + b.z, p00.w + b.w);
-    q[3][0] = make_float4(p01.x + b.x, p01.y + b.y, p01.z + b.z, p01.w + b.w);
-  }
-  float iw[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) iw[i] = 1.0f / q[i][0].w;
-#pragma unroll
-  for (int r = 1; r <= NA; ++r) {
-    const float4 v = t.v0[r], a = t.dx[r], b = t.dy[r];
-    float4 i00, i01, i10, i11;
-    if (P.attr_mod[r - 1] & AM_NOINTERPOLATION) {
-      i00 = i01 = i10 = i11 = v;
-    } else {
-      i00 = make_float4(v.x + ((a.x * dx) + (b.x * dy)), v.y + ((a.y * dx) + (b.y * dy)),
-                        v.z + ((a.z * dx) + (b.z * dy)), v.w + ((a.w * dx) + (b.w * dy)));
-      i01 = make_float4(i00.x + a.x, i00.y + a.y, i00.z + a.z, i00.w + a.w);
-      i10 = make_float4(i00.x + b.x, i00.y + b.y, i00.z + b.z, i00.w + b.w);
-      i11 = make_float4(i01.x + b.x, i01.y + b.y, i01.z + b.z, i01.w + b.w);
-    }
-    if (P.attr_mod[r - 1] & AM_NOPERSPECTIVE) {
-      q[0][r] = i00; q[1][r] = i01; q[2][r] = i10; q[3][r] = i11;
-    } else {
-      q[0][r] = make_float4(i00.x * iw[0], i00.y * iw[0], i00.z * iw[0], i00.w * iw[0]);
-      q[1][r] = make_float4(i01.x * iw[1], i01.y * iw[1], i01.z * iw[1], i01.w * iw[1]);
-      q[2][r] = make_float4(i10.x * iw[2], i10.y * iw[2], i10.z * iw[2], i10.w * iw[2]);
-      q[3][r] = make_float4(i11.x * iw[3], i11.y * iw[3], i11.z * iw[3], i11.w * iw[3]);
-    }
   }
 }
 
@@ -1552,6 +1448,25 @@ __device__ __forceinline__ void blend_store(const draw_params& P, int x, int y, 
 }
 
 constexpr uint32_t NO_WINNER = 0xFFFFFFFFu;
+
+// Pixel shader and backend for one triangle on a 2x2 quad: `pm` holds the samples to store, bits
+// 4i..4i+S-1 for pixel i.  Interpolants are formed per pixel where read (quad_lazy), so only one
+// pixel's registers are live at a time; the colour of each pixel is the quad shader's.
+template <int NA, int S>
+__device__ __forceinline__ void shade_store(const draw_params& P, const ps_samplers* __restrict__ SM,
+                                         const float4* __restrict__ tri_attr, uint32_t id, int px0, int py0, uint32_t pm) {
+  attr_setup<NA> at;
+  load_attr<NA>(P, tri_attr, id, at);
+  const quad_lazy<NA> q(at, P, px0, py0);
+#pragma unroll 1
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t m = (pm >> (4 * i)) & 15u;
+    if (!m) continue;
+    const float4 c = run_ps_pixel<NA>(P, SM, q, i);
+    for (int s = 0; s < S; ++s)
+      if (m & (1u << s)) blend_store(P, px0 + (i & 1), py0 + (i >> 1), s, c);
+  }
+}
 
 // One workgroup per GPU tile, one thread per 2x2 quad.  VIS: opaque early-Z draws -- record for every
 // sample the last triangle that passed (the colour it ends with) into `vis`; the pixel shader runs
@@ -1787,20 +1702,8 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, const ps_sam
               for (int s = 0; s < S; ++s)
                 if (tested[i] & (1u << s)) winner[i][s] = id;
           } else {
-            attr_setup<NA> at;
-            load_attr<NA>(P, tri_attr, id, at);
-            float4 q[4][NA + 1];
-            interp_quad<NA>(at, px0, py0, P, q);
-            // colours only for the pixels that store one (interpolants, not neighbours' colours,
-            // supply the shaders' derivatives)
-            float4 col[4];
-            run_ps<NA>(P, SM, q, (tested[0] ? 1u : 0u) | (tested[1] ? 2u : 0u) | (tested[2] ? 4u : 0u) | (tested[3] ? 8u : 0u),
-                       col);
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-              for (int s = 0; s < S; ++s)
-                if (tested[i] & (1u << s)) blend_store(P, px0 + (i & 1), py0 + (i >> 1), s, col[i]);
+            shade_store<NA, S>(P, SM, tri_attr, id, px0, py0,
+                               tested[0] | (tested[1] << 4) | (tested[2] << 8) | (tested[3] << 12));
           }
         } else if constexpr (!VIS) {
           // late depth/stencil (render_sample, framebuffer.cpp:445-479); writes only when both pass.
@@ -1823,13 +1726,8 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, const ps_sam
               if (dp && sp) passed[i] |= 1u << s;
             }
           if ((passed[0] | passed[1] | passed[2] | passed[3]) == 0) continue;
-          attr_setup<NA> at;
-          load_attr<NA>(P, tri_attr, id, at);
-          float4 q[4][NA + 1];
-          interp_quad<NA>(at, px0, py0, P, q);
-          float4 col[4];
-          run_ps<NA>(P, SM, q, (passed[0] ? 1u : 0u) | (passed[1] ? 2u : 0u) | (passed[2] ? 4u : 0u) | (passed[3] ? 8u : 0u),
-                     col);
+          shade_store<NA, S>(P, SM, tri_attr, id, px0, py0,
+                             passed[0] | (passed[1] << 4) | (passed[2] << 8) | (passed[3] << 12));
 #pragma unroll
           for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -1838,7 +1736,6 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, const ps_sam
               float nd = (S > 1) ? (z[i] + L.aa[s]) : z[i];
               uint32_t os = P.stencil_enable ? (stencil[i][s] & P.read_mask) : 0u;
               uint32_t ns = P.stencil_enable ? stencil_op(P.st_pass_op[face], P.stencil_ref, os) : 0u;
-              blend_store(P, px0 + (i & 1), py0 + (i >> 1), s, col[i]);
               if (P.write_depth) depth[i][s] = nd;
               if (P.stencil_enable) stencil[i][s] = ns & P.write_mask;
             }
