@@ -611,8 +611,8 @@ __device__ __forceinline__ void lerp_vtx(vtx<NA>& o, const vtx<NA>& a, const vtx
 
 // clip_triangle_to_poly_general + clip_solid_triangle (clipper.cpp:43-228) for primitive p
 template <int NA>
-__device__ void clip_primitive(const draw_params& P, uint32_t p, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
-                               float4* __restrict__ tri_attr, uint32_t* __restrict__ cnt, int32_t* __restrict__ clip_slot,
+__device__ uint32_t clip_primitive(const draw_params& P, uint32_t p, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
+                                   float4* __restrict__ tri_attr, int32_t* __restrict__ clip_slot,
                                float4* __restrict__ clip_verts, uint32_t* __restrict__ clip_hdr, uint32_t capacity,
                                uint32_t* __restrict__ counters, unsigned long long* __restrict__ stats) {
   vtx<NA> cv[2][5];
@@ -646,18 +646,18 @@ __device__ void clip_primitive(const draw_params& P, uint32_t p, const float4* _
       float ax = px[2] - px[0], ay = py[2] - py[0], bx = px[1] - px[0], by = py[1] - py[0];
       float area = ax * by - ay * bx;
       front = area > 0.0f;
-      if (cull_fn(P, area)) return;
+      if (cull_fn(P, area)) return 0;
     }
     src ^= 1;
     dst ^= 1;
   }
   uint32_t n = nc[src];
-  if (n < 3) return;
+  if (n < 3) return 0;
   atomicAdd(&stats[ST_CPRIMITIVES], static_cast<unsigned long long>(n - 2));
   uint32_t slot = atomicAdd(&counters[CT_CLIP_SLOTS], 1u);
   if (slot >= capacity) {
     atomicAdd(&counters[CT_CLIP_OVERFLOW], 1u);
-    return;
+    return 0;
   }
   clip_slot[p] = static_cast<int32_t>(slot);
   clip_hdr[slot] = n | (front ? 0x100u : 0u);
@@ -666,6 +666,7 @@ __device__ void clip_primitive(const draw_params& P, uint32_t p, const float4* _
     viewport_project<NA>(cv[src][i], P);  // once per polygon vertex, as for the unique vertices
     for (int r = 0; r <= NA; ++r) out[i * (NA + 1) + r] = cv[src][i].r[r];
   }
+  uint32_t fans = 0;
   for (uint32_t t = 1; t + 1 < n; ++t) {
     const vtx<NA>& a = cv[src][0];
     const vtx<NA>& b = front ? cv[src][t] : cv[src][t + 1];
@@ -676,12 +677,9 @@ __device__ void clip_primitive(const draw_params& P, uint32_t p, const float4* _
     if (!ts.valid) continue;
     write_tri_rec<NA>(tri_rec + tri_slot(P, id) * 5, ts);
     write_tri_attr<NA>(tri_attr + tri_slot(P, id) * 3 * (NA + 1), ts);
-    float e[9], bb[4];
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) e[i * 3 + j] = ts.e[i][j];
-    for (int i = 0; i < 4; ++i) bb[i] = ts.bb[i];
-    cnt[id] = count_gtiles(P, e, bb);
+    fans |= 1u << (t - 1);
   }
+  return fans;
 }
 
 // Clipping launch: a bounded grid strides over the primitives setup flagged (none, usually).
@@ -704,9 +702,29 @@ __device__ __forceinline__ void clip_body(const draw_params& P, const float4* __
     if (__lane_id() == 0) cnt[3 * p] = n;
   }
   if (nclip == 0) return;
-  for (uint32_t p = bx * blockDim.x + threadIdx.x; p < P.prim_count; p += gx * blockDim.x)
-    if (clip_flag[p]) clip_primitive<NA>(P, p, vbuf, tri_rec, tri_attr, cnt, clip_slot, clip_verts, clip_hdr, capacity,
-                                         counters, stats);
+  // lanes clip the flagged primitives among the wave's 64; the wave then counts the records of the
+  // resulting triangles together (clipped triangles are few and often reach across the target)
+  const int lane = __lane_id();
+  for (uint32_t b = wave * 64; b < P.prim_count; b += waves * 64) {
+    const uint32_t p = b + static_cast<uint32_t>(lane);
+    uint32_t fans = 0;  // bit k: fan triangle 3p + k was set up
+    if (p < P.prim_count && clip_flag[p])
+      fans = clip_primitive<NA>(P, p, vbuf, tri_rec, tri_attr, clip_slot, clip_verts, clip_hdr, capacity, counters, stats);
+    __threadfence();  // the fans' records are read back by every lane of the wave
+    for (unsigned long long m = __ballot(fans != 0); m; m &= m - 1) {
+      const int src = __builtin_ctzll(m);
+      const uint32_t fs = static_cast<uint32_t>(__shfl(static_cast<int>(fans), src, 64));
+      const uint32_t ps = b + static_cast<uint32_t>(src);
+      for (int k = 0; k < 3; ++k) {
+        if (!(fs & (1u << k))) continue;
+        const uint32_t id = 3 * ps + k;
+        float e[9], bb[4];
+        read_tri_rec(tri_rec + tri_slot(P, id) * 5, e, bb);
+        const uint32_t n = count_gtiles_wave(P, e, bb);
+        if (lane == 0) cnt[id] = n;
+      }
+    }
+  }
 }
 template <int NA>
 __global__ void clip_kernel(draw_params P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
@@ -771,7 +789,9 @@ __device__ __forceinline__ void emit_wide(const draw_params& P, const float4* __
   }
 }
 
-constexpr uint32_t EMIT_WIDE = 64;  // records beyond which a triangle is emitted by its whole wave
+// Records beyond which a triangle is emitted by its whole wave (so is any triangle over more than
+// COUNT_WIDE 64x64 tiles).
+constexpr uint32_t EMIT_WIDE = 64;
 
 __device__ __forceinline__ void emit_body(const draw_params& P, const float4* __restrict__ tri_rec,
                                           const uint32_t* __restrict__ off, const uint32_t* __restrict__ total,
@@ -798,6 +818,10 @@ __device__ __forceinline__ void emit_body(const draw_params& P, const float4* __
       }
       float e[9], bb[4];
       read_tri_rec(tri_rec + tri_slot(P, id) * 5, e, bb);
+      if (bin_span(P, bb) > COUNT_WIDE) {  // few records over a large box: the walk is over the box
+        wide |= 1u << k;
+        continue;
+      }
       uint32_t q = first;
       for_each_gtile(P, e, bb, [&](uint32_t g, bool full) {
         if (q < next && q < capacity) {  // the enumeration repeats the counting pass exactly
