@@ -862,6 +862,7 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
 
 // ============================================================== scan (exclusive, u32)
 constexpr int SCAN_PER_THREAD = 16;
+static_assert(SCAN_PER_THREAD % 4 == 0, "scan blocks move as 16-byte words");
 constexpr int SCAN_BLOCK = 256 * SCAN_PER_THREAD;  // elements per workgroup
 
 // Exclusive scan of one SCAN_BLOCK-element block in place.  The block total goes to sums[block]
@@ -869,13 +870,26 @@ constexpr int SCAN_BLOCK = 256 * SCAN_PER_THREAD;  // elements per workgroup
 __global__ void __launch_bounds__(256) scan_block_kernel(uint32_t* data, uint32_t n, uint32_t* sums, uint32_t* total) {
   __shared__ uint32_t wsum[4];
   const uint32_t base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_PER_THREAD;
+  // a full block of 16-byte-aligned data moves as four 16-byte words per lane
+  const bool vec = (blockIdx.x + 1) * static_cast<uint32_t>(SCAN_BLOCK) <= n && (reinterpret_cast<uintptr_t>(data) & 15) == 0;
   uint32_t v[SCAN_PER_THREAD];
   uint32_t t = 0;
+  if (vec) {
+    const uint4* d4 = reinterpret_cast<const uint4*>(data + base);
 #pragma unroll
-  for (int i = 0; i < SCAN_PER_THREAD; ++i) {
-    v[i] = (base + i < n) ? data[base + i] : 0u;
-    t += v[i];
+    for (int i = 0; i < SCAN_PER_THREAD / 4; ++i) {
+      const uint4 q = d4[i];
+      v[4 * i] = q.x;
+      v[4 * i + 1] = q.y;
+      v[4 * i + 2] = q.z;
+      v[4 * i + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < SCAN_PER_THREAD; ++i) v[i] = (base + i < n) ? data[base + i] : 0u;
   }
+#pragma unroll
+  for (int i = 0; i < SCAN_PER_THREAD; ++i) t += v[i];
   const uint32_t incl = wave_incl_scan(t);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (lane == 63) wsum[wave] = incl;
@@ -886,8 +900,17 @@ __global__ void __launch_bounds__(256) scan_block_kernel(uint32_t* data, uint32_
 #pragma unroll
   for (int i = 0; i < SCAN_PER_THREAD; ++i) {
     const uint32_t x = v[i];
-    if (base + i < n) data[base + i] = run;
+    v[i] = run;
     run += x;
+  }
+  if (vec) {
+    uint4* d4 = reinterpret_cast<uint4*>(data + base);
+#pragma unroll
+    for (int i = 0; i < SCAN_PER_THREAD / 4; ++i) d4[i] = make_uint4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < SCAN_PER_THREAD; ++i)
+      if (base + i < n) data[base + i] = v[i];
   }
   if (threadIdx.x == 255) {
     const uint32_t block_total = wprefix + incl;
