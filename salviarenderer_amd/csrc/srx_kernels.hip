@@ -1059,6 +1059,14 @@ constexpr int BUCKET_ITEMS = 256 * BUCKET_PER_THREAD;  // records per workgroup
 constexpr uint32_t BUCKET_MAX_TILES = 8192;
 
 // PT records per thread: 256 * PT records per workgroup (the batched path tunes PT)
+// Column of workgroup b in a tile's row of the bucket histogram: the workgroups of one XCD
+// (b mod 8, as workgroups are dealt to the XCDs) take adjacent columns, so each histogram line is
+// written from one XCD's L2.  Any order of the workgroups within a tile is valid here (the tile's
+// records carry no order on this path); the map is a bijection on [0, nblocks).
+__device__ __forceinline__ uint32_t bucket_column(uint32_t b, uint32_t nblocks) {
+  const uint32_t q = nblocks / 8, r = nblocks % 8, x = b % 8;
+  return x * q + min(x, r) + b / 8;
+}
 template <int PT>
 __device__ __forceinline__ void bucket_hist_body(const uint32_t* __restrict__ keys, uint32_t n, uint32_t T,
                                                  uint32_t nblocks, uint32_t* __restrict__ hist, uint32_t bx) {
@@ -1076,7 +1084,8 @@ __device__ __forceinline__ void bucket_hist_body(const uint32_t* __restrict__ ke
   for (int j = 0; j < PT; ++j)
     if (k[j] != 0xFFFFFFFFu) atomicAdd(&cnt[k[j]], 1u);
   __syncthreads();
-  for (uint32_t d = threadIdx.x; d < T; d += 256) hist[d * nblocks + bx] = cnt[d];
+  const uint32_t col = bucket_column(bx, nblocks);
+  for (uint32_t d = threadIdx.x; d < T; d += 256) hist[d * nblocks + col] = cnt[d];
 }
 __global__ void __launch_bounds__(256) bucket_hist_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t T,
                                                           uint32_t nblocks, uint32_t* __restrict__ hist) {
@@ -1106,7 +1115,7 @@ __device__ __forceinline__ void bucket_scatter_body(const uint32_t* __restrict__
     if (k[j] != 0xFFFFFFFFu) off[k[j]] = 0u;  // mark the tiles present
   __syncthreads();
   for (uint32_t d = threadIdx.x; d < T; d += 256)
-    if (off[d] == 0u) off[d] = hist_scanned[d * nblocks + bx];
+    if (off[d] == 0u) off[d] = hist_scanned[d * nblocks + bucket_column(bx, nblocks)];
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < PT; ++j)
