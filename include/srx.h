@@ -340,7 +340,7 @@ typedef struct srx_batch_t* srx_batch;
 typedef struct srx_batch_info {
   uint32_t jobs, clears, resolves;
   uint32_t tile;         /* GPU tile side (8, 16, 32) */
-  uint32_t bin_items;    /* records binned per workgroup (2048, 4096, 8192) */
+  uint32_t bin_items;    /* records binned per workgroup (2048, 4096, 8192, 16384) */
   uint32_t vis_threads;  /* lanes per GPU tile in the visibility kernel (64, 128, 256) */
   uint32_t vis_launches; /* visibility launches (one per shader/sample-count variant) */
   uint32_t pad;
@@ -358,7 +358,7 @@ int32_t srx_batch_launch(srx_batch b); /* queued on the renderer's stream */
  * dropped, bit 1: clipping overflow.  Zero after every complete launch. */
 int32_t srx_batch_status(srx_batch b, uint32_t* status);
 /* GPU tile side 8, 16 or 32 (8 and 32: single-sample jobs, and at most 8192 tiles per job), records
- * per binning workgroup 2048, 4096 or 8192, lanes per tile 64, 128 or 256.  A new tile size
+ * per binning workgroup 2048, 4096, 8192 or 16384, lanes per tile 64, 128 or 256.  A new tile size
  * repeats the counting pass once. */
 int32_t srx_batch_configure(srx_batch b, int32_t tile, int32_t bin_items, int32_t vis_threads);
 /* Times `reps` launches of every (tile, bin_items, vis_threads) the batch's jobs allow, reports

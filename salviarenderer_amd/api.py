@@ -404,10 +404,10 @@ class Batch(_Obj):
 
     def autotune(self, reps=3):
         """Times every (tile, bin_items, vis_threads) the jobs allow; the batch keeps the fastest."""
-        res = (BatchTuning * 32)()
+        res = (BatchTuning * 64)()
         n = C.c_size_t()
-        self.lib.call("srx_batch_autotune", self.h, reps, res, 32, C.byref(n))
-        return [(r.tile, r.bin_items, r.vis_threads, r.ms) for r in res[: min(n.value, 32)]]
+        self.lib.call("srx_batch_autotune", self.h, reps, res, 64, C.byref(n))
+        return [(r.tile, r.bin_items, r.vis_threads, r.ms) for r in res[: min(n.value, 64)]]
 
     def info(self) -> BatchInfo:
         d = BatchInfo()

@@ -550,7 +550,7 @@ int32_t srx_batch_autotune(srx_batch bb, uint32_t reps, srx_batch_tuning* result
   int best_tile = b->tile, best_pt = b->bin_pt, best_th = b->vis_threads;
   int32_t rc = SRX_OK;
   for (int tile : {8, 16, 32}) {
-    for (int pt : {8, 16, 32}) {
+    for (int pt : {8, 16, 32, 64}) {
       for (int th : {64, 128, 256}) {
         rc = batch_configure(b, tile, pt, th);
         if (rc == SRX_E_UNSUPPORTED) {  // a tile size some job cannot take: not a candidate

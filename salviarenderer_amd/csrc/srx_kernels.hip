@@ -2801,7 +2801,7 @@ void launch_clip_batch(int na, const batch_job* jobs, stage_map m, uint32_t bloc
 void launch_emit_batch(const batch_job* jobs, stage_map m, uint32_t blocks, uint32_t capacity, hipStream_t s) {
   if (blocks) hipLaunchKernelGGL(emit_batch_kernel, dim3(blocks), dim3(256), 0, s, jobs, m, capacity);
 }
-bool batch_bin_items_valid(int pt) { return pt == 8 || pt == 16 || pt == 32; }
+bool batch_bin_items_valid(int pt) { return pt == 8 || pt == 16 || pt == 32 || pt == 64; }
 void launch_bucket_batch(int pt, int phase, const batch_job* jobs, stage_map m, uint32_t blocks, uint32_t capacity,
                          hipStream_t s) {
   if (!blocks) return;
@@ -2813,6 +2813,7 @@ void launch_bucket_batch(int pt, int phase, const batch_job* jobs, stage_map m, 
   switch (pt) {
   case 8: SRX_BUCKET_B(8); break;
   case 32: SRX_BUCKET_B(32); break;
+  case 64: SRX_BUCKET_B(64); break;
   default: SRX_BUCKET_B(16); break;
   }
 #undef SRX_BUCKET_B

@@ -118,7 +118,8 @@ SINGLE = [s for s in SPECS if s.samples == 1]  # 8x8 and 32x32 tiles take single
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("tile,bin_items,threads", [(16, 2048, 64), (16, 4096, 128), (16, 8192, 256), (16, 2048, 256),
-                                                    (8, 2048, 64), (8, 8192, 256), (32, 4096, 64), (32, 8192, 256)])
+                                                    (16, 16384, 64), (8, 2048, 64), (8, 8192, 256), (8, 16384, 128),
+                                                    (32, 4096, 64), (32, 8192, 256)])
 def test_batch_tunings_agree(hip_lib, tile, bin_items, threads):
     r, jobs, ref = one_by_one(hip_lib, SPECS if tile == 16 else SINGLE)
     b = jobs.record()
@@ -137,7 +138,7 @@ def test_batch_autotune_keeps_output(hip_lib):
     b = jobs.record()
     sweep = b.autotune(reps=1)
     # SPECS[1] has 4 samples: only 16x16 tiles apply
-    assert len(sweep) == 9 and all(ms > 0 for *_, ms in sweep) and {t[0] for t in sweep} == {16}
+    assert len(sweep) == 12 and all(ms > 0 for *_, ms in sweep) and {t[0] for t in sweep} == {16}
     best = min(sweep, key=lambda t: t[3])
     assert (b.info().tile, b.info().bin_items, b.info().vis_threads) == best[:3]
     scramble(jobs)
@@ -151,7 +152,7 @@ def test_batch_autotune_sweeps_tile_sizes(hip_lib):
     r, jobs, ref = one_by_one(hip_lib, SINGLE[:3])
     b = jobs.record()
     sweep = b.autotune(reps=1)
-    assert {t[0] for t in sweep} == {8, 16, 32} and len(sweep) == 27
+    assert {t[0] for t in sweep} == {8, 16, 32} and len(sweep) == 36
     scramble(jobs)
     b.launch()
     r.flush()
