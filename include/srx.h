@@ -86,7 +86,10 @@ enum srx_stencil_op {
   SRX_SOP_DECR_SAT = 5, SRX_SOP_INVERT = 6, SRX_SOP_INCR_WRAP = 7, SRX_SOP_DECR_WRAP = 8
 };
 enum srx_clear_flag { SRX_CLEAR_DEPTH = 1, SRX_CLEAR_STENCIL = 2 };
-enum srx_map_mode { SRX_MAP_READ = 1, SRX_MAP_WRITE = 2, SRX_MAP_READ_WRITE = 3, SRX_MAP_WRITE_DISCARD = 4 };
+/* map_mode (constants.h:11-18) */
+enum srx_map_mode {
+  SRX_MAP_READ = 1, SRX_MAP_WRITE = 2, SRX_MAP_READ_WRITE = 3, SRX_MAP_WRITE_DISCARD = 4, SRX_MAP_WRITE_NO_OVERWRITE = 5
+};
 
 /* async_object_ids (constants.h:166-175) */
 enum srx_query_id { SRX_QUERY_PIPELINE_STATISTICS = 3, SRX_QUERY_INTERNAL_STATISTICS = 5, SRX_QUERY_PIPELINE_PROFILES = 6 };
@@ -306,12 +309,50 @@ int32_t srx_set_depth_stencil_state(srx_renderer r, srx_ds_state dss, int32_t st
 int32_t srx_set_render_targets(srx_renderer r, size_t color_target_count,
                                const srx_surface* color_targets, srx_surface ds_target);
 int32_t srx_set_viewport(srx_renderer r, const srx_viewport* vp);
+/* set_vs_sampler(name, sampler) (renderer.h:80) by slot.  No built-in vertex shader samples a
+   texture; the binding is kept (and validated) for the renderer state. */
+int32_t srx_set_vs_sampler(srx_renderer r, uint32_t slot, srx_sampler s);
+
+/* ---- state get (renderer.h:104-113; renderer_impl.cpp:61-157) ---- */
+/* get_index_buffer: retained handle (release it), NULL when none is bound */
+int32_t srx_get_index_buffer(srx_renderer r, srx_buffer* out);
+uint32_t srx_get_index_format(srx_renderer r);       /* default SRX_FMT_R16_UINT */
+uint32_t srx_get_primitive_topology(srx_renderer r); /* default SRX_TOPOLOGY_TRIANGLE_LIST */
+int32_t srx_get_vertex_shader(srx_renderer r);       /* srx_vertex_shader_id, 0 when none */
+/* get_rasterizer_state: retained handle; a fresh renderer holds the default state (cull back) */
+int32_t srx_get_rasterizer_state(srx_renderer r, srx_raster_state* out);
+int32_t srx_get_pixel_shader(srx_renderer r);        /* srx_pixel_shader_id, 0 when none */
+int32_t srx_get_blend_shader(srx_renderer r);        /* srx_blend_shader_id */
 int32_t srx_get_viewport(srx_renderer r, srx_viewport* vp);
+/* renderer_impl::get_depth_stencil_state / get_stencil_ref: retained handle and the reference value */
+int32_t srx_get_depth_stencil_state(srx_renderer r, srx_ds_state* out, int32_t* stencil_ref);
+/* The descriptors behind state objects (raster_state::get_desc, depth_stencil_state::get_desc). */
+int32_t srx_raster_state_desc(srx_raster_state rs, srx_raster_desc* d);
+int32_t srx_depth_stencil_state_desc(srx_ds_state ds, srx_depth_stencil_desc* d);
+
+/* ---- map / unmap (renderer.h:61-63; resource_manager.cpp:7-113) ----
+ * One resource is mapped per renderer at a time (a second map fails).  Every mode waits for the
+ * renderer's queued work and points data at host memory holding the resource's current bytes;
+ * srx_unmap writes them back for every mode but SRX_MAP_READ.  Buffers: row_pitch = depth_pitch =
+ * size (buffer.cpp:25); surfaces: row_pitch = width * samples * texel bytes, depth_pitch =
+ * row_pitch * height (surface.cpp:111-112).  data stays valid until srx_unmap. */
+typedef struct srx_mapped_resource {
+  void* data;
+  uint32_t row_pitch;
+  uint32_t depth_pitch;
+} srx_mapped_resource;
+int32_t srx_map_buffer(srx_renderer r, srx_mapped_resource* mapped, srx_buffer b, int32_t mode);
+int32_t srx_map_surface(srx_renderer r, srx_mapped_resource* mapped, srx_surface s, int32_t mode);
+int32_t srx_unmap(srx_renderer r);
 
 /* ---- render operations (renderer.h:117-129) ---- */
+/* begin/end do not wait for the GPU: they queue the counters' read-back on the renderer's stream. */
 int32_t srx_begin(srx_renderer r, srx_query q);
 int32_t srx_end(srx_renderer r, srx_query q);
-/* data: srx_pipeline_statistics / srx_internal_statistics / uint64_t[8] pipeline profiles in ns
+/* async_object::get (async_object.h:62-81): SRX_ASYNC_ERROR while the query runs or before its
+   first end; with do_not_wait, SRX_ASYNC_TIMEOUT until the GPU has finished the work queued before
+   srx_end, else it waits; then SRX_ASYNC_READY with
+   data: srx_pipeline_statistics / srx_internal_statistics / uint64_t[8] pipeline profiles in ns
    (gather_vtx, vtx_proc, clipping, compact_clip, vp_trans, tri_dispatch, ras, profiled draws);
    a uint64_t[10] buffer also receives ras split into visibility (8) and deferred shading (9). */
 int32_t srx_get_data(srx_renderer r, srx_query q, void* data, size_t size, int32_t do_not_wait);

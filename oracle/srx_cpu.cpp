@@ -500,8 +500,11 @@ struct render_state {
   int vs_id = 0, ps_id = 0, blend_id = SRX_BLEND_OFF;
   std::vector<float> vs_consts, ps_consts;
   ref<sampler_obj> samplers[SRX_MAX_SAMPLER_SLOTS];
+  ref<sampler_obj> vs_samplers[SRX_MAX_SAMPLER_SLOTS];
   srx_raster_desc raster;
   srx_depth_stencil_desc ds;
+  ref<raster_obj> raster_state;  // renderer_impl.cpp:276-277 creates the defaults
+  ref<ds_obj> ds_state;
   int32_t stencil_ref = 0;
   std::vector<ref<surface_obj>> color_targets;
   ref<surface_obj> ds_target;
@@ -528,9 +531,22 @@ struct renderer_obj : object {
   render_state st;
   std::vector<ref<query_obj>> active;
   batch_obj* recording = nullptr;  // srx_batch_begin .. srx_batch_end
+  // resource_manager (resource_manager.cpp:7-113): the mapped resource and map_read's copy
+  int32_t map_mode = 0;
+  ref<buffer_obj> mapped_buf;
+  ref<surface_obj> mapped_surf;
+  std::vector<uint8_t> map_copy;
   renderer_obj() : object(K_RENDERER) {
     srx_default_raster_desc(&st.raster);
     srx_default_depth_stencil_desc(&st.ds);
+    auto* rs = new raster_obj();
+    rs->desc = st.raster;
+    st.raster_state = ref<raster_obj>(rs);
+    release(rs);
+    auto* ds = new ds_obj();
+    ds->desc = st.ds;
+    st.ds_state = ref<ds_obj>(ds);
+    release(ds);
   }
   ~renderer_obj() override;
 };
@@ -1669,6 +1685,7 @@ int32_t srx_set_rasterizer_state(srx_renderer rr, srx_raster_state rs) {
   auto* o = as<raster_obj>(rs, K_RASTER);
   if (!r || !o) return SRX_INVALID_PARAMETER;
   r->st.raster = o->desc;
+  r->st.raster_state = ref<raster_obj>(o);
   return SRX_OK;
 }
 int32_t srx_set_depth_stencil_state(srx_renderer rr, srx_ds_state d, int32_t ref_) {
@@ -1676,6 +1693,7 @@ int32_t srx_set_depth_stencil_state(srx_renderer rr, srx_ds_state d, int32_t ref
   auto* o = as<ds_obj>(d, K_DS);
   if (!r || !o) return SRX_INVALID_PARAMETER;
   r->st.ds = o->desc;
+  r->st.ds_state = ref<ds_obj>(o);
   r->st.stencil_ref = ref_;
   return SRX_OK;
 }
@@ -1732,6 +1750,114 @@ int32_t srx_get_viewport(srx_renderer rr, srx_viewport* vp) {
   return SRX_OK;
 }
 
+int32_t srx_set_vs_sampler(srx_renderer rr, uint32_t slot, srx_sampler s) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || slot >= SRX_MAX_SAMPLER_SLOTS) return SRX_INVALID_PARAMETER;
+  r->st.vs_samplers[slot] = ref<sampler_obj>(as<sampler_obj>(s, K_SAMPLER));
+  return SRX_OK;
+}
+// renderer_impl.cpp:61-157
+int32_t srx_get_index_buffer(srx_renderer rr, srx_buffer* out) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !out) return SRX_INVALID_PARAMETER;
+  retain(r->st.ib.get());
+  *out = reinterpret_cast<srx_buffer>(static_cast<object*>(r->st.ib.get()));
+  return SRX_OK;
+}
+uint32_t srx_get_index_format(srx_renderer rr) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  return r ? r->st.index_fmt : 0;
+}
+uint32_t srx_get_primitive_topology(srx_renderer rr) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  return r ? r->st.topo : 0;
+}
+int32_t srx_get_vertex_shader(srx_renderer rr) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  return r ? r->st.vs_id : 0;
+}
+int32_t srx_get_pixel_shader(srx_renderer rr) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  return r ? r->st.ps_id : 0;
+}
+int32_t srx_get_blend_shader(srx_renderer rr) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  return r ? r->st.blend_id : -1;
+}
+int32_t srx_get_rasterizer_state(srx_renderer rr, srx_raster_state* out) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !out) return SRX_INVALID_PARAMETER;
+  retain(r->st.raster_state.get());
+  *out = reinterpret_cast<srx_raster_state>(static_cast<object*>(r->st.raster_state.get()));
+  return SRX_OK;
+}
+int32_t srx_get_depth_stencil_state(srx_renderer rr, srx_ds_state* out, int32_t* stencil_ref) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !out) return SRX_INVALID_PARAMETER;
+  retain(r->st.ds_state.get());
+  *out = reinterpret_cast<srx_ds_state>(static_cast<object*>(r->st.ds_state.get()));
+  if (stencil_ref) *stencil_ref = r->st.stencil_ref;
+  return SRX_OK;
+}
+int32_t srx_raster_state_desc(srx_raster_state rs, srx_raster_desc* d) {
+  auto* o = as<raster_obj>(rs, K_RASTER);
+  if (!o || !d) return SRX_INVALID_PARAMETER;
+  *d = o->desc;
+  return SRX_OK;
+}
+int32_t srx_depth_stencil_state_desc(srx_ds_state ds, srx_depth_stencil_desc* d) {
+  auto* o = as<ds_obj>(ds, K_DS);
+  if (!o || !d) return SRX_INVALID_PARAMETER;
+  *d = o->desc;
+  return SRX_OK;
+}
+// resource_manager::map_impl / buffer::map / surface::map (resource_manager.cpp:7-56,
+// buffer.cpp:9-26, surface.cpp:94-116): map_read hands out a copy, the other modes the resource's
+// own bytes; one mapping at a time.
+static int32_t map_bytes(renderer_obj* r, srx_mapped_resource* m, std::vector<uint8_t>& bytes, int32_t mode,
+                         uint32_t row_pitch, uint32_t depth_pitch) {
+  if (mode < SRX_MAP_READ || mode > SRX_MAP_WRITE_NO_OVERWRITE) return SRX_FAILED;
+  if (r->map_mode != 0) return SRX_FAILED;
+  if (mode == SRX_MAP_READ) {
+    r->map_copy = bytes;
+    m->data = r->map_copy.data();
+  } else {
+    m->data = bytes.data();
+  }
+  m->row_pitch = row_pitch;
+  m->depth_pitch = depth_pitch;
+  r->map_mode = mode;
+  return SRX_OK;
+}
+int32_t srx_map_buffer(srx_renderer rr, srx_mapped_resource* m, srx_buffer bb, int32_t mode) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  auto* b = as<buffer_obj>(bb, K_BUFFER);
+  if (!r || !m || !b) return SRX_INVALID_PARAMETER;
+  const uint32_t pitch = static_cast<uint32_t>(b->data.size());
+  const int32_t rc = map_bytes(r, m, b->data, mode, pitch, pitch);
+  if (rc == SRX_OK) r->mapped_buf = ref<buffer_obj>(b);
+  return rc;
+}
+int32_t srx_map_surface(srx_renderer rr, srx_mapped_resource* m, srx_surface ss, int32_t mode) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  auto* o = as<surface_obj>(ss, K_SURFACE);
+  if (!r || !m || !o) return SRX_INVALID_PARAMETER;
+  const uint32_t pitch = static_cast<uint32_t>(static_cast<size_t>(o->w) * o->samples * o->esize());
+  const int32_t rc = map_bytes(r, m, o->data, mode, pitch, pitch * static_cast<uint32_t>(o->h));
+  if (rc == SRX_OK) r->mapped_surf = ref<surface_obj>(o);
+  return rc;
+}
+int32_t srx_unmap(srx_renderer rr) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r) return SRX_INVALID_PARAMETER;
+  if (r->map_mode == 0) return SRX_FAILED;
+  r->map_mode = 0;
+  r->mapped_buf = ref<buffer_obj>();
+  r->mapped_surf = ref<surface_obj>();
+  r->map_copy.clear();
+  return SRX_OK;
+}
+
 int32_t srx_begin(srx_renderer rr, srx_query qq) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);
   auto* q = as<query_obj>(qq, K_QUERY);
@@ -1759,11 +1885,12 @@ int32_t srx_end(srx_renderer rr, srx_query qq) {
     }
   return SRX_OK;
 }
+// The oracle executes every command when it is issued, so a finished query is always ready.
 int32_t srx_get_data(srx_renderer rr, srx_query qq, void* data, size_t size, int32_t do_not_wait) {
   (void)do_not_wait;
   auto* q = as<query_obj>(qq, K_QUERY);
   if (!as<renderer_obj>(rr, K_RENDERER) || !q || !data) return SRX_ASYNC_ERROR;
-  if (!q->has_data) return SRX_ASYNC_ERROR;
+  if (!q->has_data || q->running) return SRX_ASYNC_ERROR;
   if (q->id == SRX_QUERY_PIPELINE_STATISTICS && size >= sizeof(q->stats)) std::memcpy(data, &q->stats, sizeof(q->stats));
   else if (q->id == SRX_QUERY_INTERNAL_STATISTICS && size >= sizeof(q->istats)) std::memcpy(data, &q->istats, sizeof(q->istats));
   else if (q->id == SRX_QUERY_PIPELINE_PROFILES && size >= sizeof(q->prof)) std::memcpy(data, q->prof, sizeof(q->prof));
@@ -1964,7 +2091,9 @@ int32_t srx_server_poll(srx_server ss, srx_completion* out, size_t capacity, siz
   while (n < capacity && !S->unreported.empty()) {
     const uint64_t tk = S->unreported.front();
     S->unreported.pop_front();
-    const served& o = S->finished[tk];
+    auto it = S->finished.find(tk);
+    if (it == S->finished.end()) continue;  // released before it was reported
+    const served& o = it->second;
     srx_completion& c = out[n++];
     c = srx_completion{};
     c.ticket = tk;
@@ -1984,6 +2113,7 @@ int32_t srx_server_poll(srx_server ss, srx_completion* out, size_t capacity, siz
 int32_t srx_server_release(srx_server ss, uint64_t ticket) {
   auto* S = as<server_obj>(ss, K_SERVER);
   if (!S || !S->finished.erase(ticket)) return SRX_INVALID_PARAMETER;
+  S->unreported.erase(std::remove(S->unreported.begin(), S->unreported.end(), ticket), S->unreported.end());
   return SRX_OK;
 }
 int64_t srx_server_now(srx_server ss) {

@@ -31,6 +31,8 @@ VS_PASSTHROUGH_COLOR, VS_WVP_NORMAL_UV, VS_WVP_COLOR, VS_WVP_COLOR_LIGHT = 1, 2,
 PS_GOURAUD, PS_LAMBERT_TEX, PS_TEX_CPP, PS_CONSTANT, PS_SHADOW_PCF = 1, 2, 3, 4, 5
 BLEND_OFF, BLEND_ON, BLEND_DISABLED = 0, 1, 2
 RENDERER_ASYNC, RENDERER_BENCHMARK = 0, 1
+ASYNC_ERROR, ASYNC_TIMEOUT, ASYNC_READY = 0, 1, 2
+MAP_READ, MAP_WRITE, MAP_READ_WRITE, MAP_WRITE_DISCARD, MAP_WRITE_NO_OVERWRITE = 1, 2, 3, 4, 5
 
 
 class SrxError(RuntimeError):
@@ -76,6 +78,10 @@ class InputElementDesc(C.Structure):
     _fields_ = [("semantic_name", C.c_char_p), ("semantic_index", C.c_uint32), ("data_format", C.c_uint32),
                 ("input_slot", C.c_uint32), ("aligned_byte_offset", C.c_uint32), ("slot_class", C.c_uint32),
                 ("instance_data_step_rate", C.c_uint32)]
+
+
+class MappedResource(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("row_pitch", C.c_uint32), ("depth_pitch", C.c_uint32)]
 
 
 class PipelineStatistics(C.Structure):
@@ -176,6 +182,20 @@ PROTOTYPES = {
     "srx_set_render_targets": (_I32, [_H, _SZ, C.POINTER(_H), _H]),
     "srx_set_viewport": (_I32, [_H, C.POINTER(Viewport)]),
     "srx_get_viewport": (_I32, [_H, C.POINTER(Viewport)]),
+    "srx_set_vs_sampler": (_I32, [_H, _U32, _H]),
+    "srx_get_index_buffer": (_I32, [_H, C.POINTER(_H)]),
+    "srx_get_index_format": (_U32, [_H]),
+    "srx_get_primitive_topology": (_U32, [_H]),
+    "srx_get_vertex_shader": (_I32, [_H]),
+    "srx_get_rasterizer_state": (_I32, [_H, C.POINTER(_H)]),
+    "srx_get_pixel_shader": (_I32, [_H]),
+    "srx_get_blend_shader": (_I32, [_H]),
+    "srx_get_depth_stencil_state": (_I32, [_H, C.POINTER(_H), C.POINTER(_I32)]),
+    "srx_raster_state_desc": (_I32, [_H, C.POINTER(RasterDesc)]),
+    "srx_depth_stencil_state_desc": (_I32, [_H, C.POINTER(DepthStencilDesc)]),
+    "srx_map_buffer": (_I32, [_H, C.POINTER(MappedResource), _H, _I32]),
+    "srx_map_surface": (_I32, [_H, C.POINTER(MappedResource), _H, _I32]),
+    "srx_unmap": (_I32, [_H]),
     "srx_begin": (_I32, [_H, _H]),
     "srx_end": (_I32, [_H, _H]),
     "srx_get_data": (_I32, [_H, _H, _P, _SZ, _I32]),
@@ -558,6 +578,70 @@ class Renderer(_Obj):
             d = (C.c_uint64 * 10)()
         st = self.lib.dll.srx_get_data(self.h, q.h, C.byref(d), C.sizeof(d), int(do_not_wait))
         return st, d
+
+    # map / unmap (renderer.h:61-63)
+    def map(self, res, mode) -> MappedResource:
+        """Maps a Buffer or Surface; returns the mapped resource (data, row_pitch, depth_pitch)."""
+        m = MappedResource()
+        fn = "srx_map_buffer" if isinstance(res, Buffer) else "srx_map_surface"
+        self.lib.call(fn, self.h, C.byref(m), res.h, mode)
+        self._keep["mapped"] = res
+        return m
+
+    def unmap(self):
+        self.lib.call("srx_unmap", self.h)
+        self._keep.pop("mapped", None)
+
+    def set_vs_sampler(self, slot, samp):
+        self._keep[f"vs_sampler{slot}"] = samp
+        self.lib.call("srx_set_vs_sampler", self.h, slot, samp.h if samp else None)
+
+    # state get (renderer.h:104-113)
+    def get_index_buffer(self):
+        h = C.c_void_p()
+        self.lib.call("srx_get_index_buffer", self.h, C.byref(h))
+        return Buffer(self.lib, h) if h.value else None
+
+    def get_index_format(self):
+        return self.lib.dll.srx_get_index_format(self.h)
+
+    def get_primitive_topology(self):
+        return self.lib.dll.srx_get_primitive_topology(self.h)
+
+    def get_vertex_shader(self):
+        return self.lib.dll.srx_get_vertex_shader(self.h)
+
+    def get_pixel_shader(self):
+        return self.lib.dll.srx_get_pixel_shader(self.h)
+
+    def get_blend_shader(self):
+        return self.lib.dll.srx_get_blend_shader(self.h)
+
+    def get_rasterizer_state(self) -> RasterDesc:
+        h = C.c_void_p()
+        self.lib.call("srx_get_rasterizer_state", self.h, C.byref(h))
+        d = RasterDesc()
+        try:
+            self.lib.call("srx_raster_state_desc", h, C.byref(d))
+        finally:
+            self.lib.dll.srx_release(h)
+        return d
+
+    def get_depth_stencil_state(self):
+        """(DepthStencilDesc, stencil_ref) of the bound state."""
+        h, ref = C.c_void_p(), C.c_int32()
+        self.lib.call("srx_get_depth_stencil_state", self.h, C.byref(h), C.byref(ref))
+        d = DepthStencilDesc()
+        try:
+            self.lib.call("srx_depth_stencil_state_desc", h, C.byref(d))
+        finally:
+            self.lib.dll.srx_release(h)
+        return d, ref.value
+
+    def get_viewport(self) -> Viewport:
+        vp = Viewport()
+        self.lib.call("srx_get_viewport", self.h, C.byref(vp))
+        return vp
 
     def draw(self, start, primcnt):
         self.lib.call("srx_draw", self.h, start, primcnt)

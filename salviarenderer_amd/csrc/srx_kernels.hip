@@ -869,9 +869,10 @@ constexpr int SCAN_BLOCK = 256 * SCAN_PER_THREAD;  // elements per workgroup
 // (when sums is given); a single-block launch also stores it to *total.
 __global__ void __launch_bounds__(256) scan_block_kernel(uint32_t* data, uint32_t n, uint32_t* sums, uint32_t* total) {
   __shared__ uint32_t wsum[4];
-  const uint32_t base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_PER_THREAD;
+  // 64-bit positions: n may come within one block of 2^32
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * SCAN_BLOCK + threadIdx.x * SCAN_PER_THREAD;
   // a full block of 16-byte-aligned data moves as four 16-byte words per lane
-  const bool vec = (blockIdx.x + 1) * static_cast<uint32_t>(SCAN_BLOCK) <= n && (reinterpret_cast<uintptr_t>(data) & 15) == 0;
+  const bool vec = (static_cast<uint64_t>(blockIdx.x) + 1) * SCAN_BLOCK <= n && (reinterpret_cast<uintptr_t>(data) & 15) == 0;
   uint32_t v[SCAN_PER_THREAD];
   uint32_t t = 0;
   if (vec) {
@@ -921,19 +922,19 @@ __global__ void __launch_bounds__(256) scan_block_kernel(uint32_t* data, uint32_
 
 __global__ void __launch_bounds__(256) scan_add_kernel(uint32_t* data, uint32_t n, const uint32_t* sums) {
   const uint32_t add = sums[blockIdx.x];
-  const uint32_t b0 = blockIdx.x * SCAN_BLOCK;
+  const uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * SCAN_BLOCK;
 #pragma unroll
   for (int k = 0; k < SCAN_PER_THREAD; ++k) {
-    const uint32_t i = b0 + k * 256 + threadIdx.x;
+    const uint64_t i = b0 + k * 256 + threadIdx.x;
     if (i < n) data[i] += add;
   }
 }
 
 size_t scan_tmp_size(uint32_t n) {
   size_t total = 0;
-  uint32_t m = n;
+  uint64_t m = n;
   while (true) {
-    uint32_t nb = (m + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    const uint64_t nb = (m + SCAN_BLOCK - 1) / SCAN_BLOCK;
     if (nb <= 1) break;
     total += nb;
     m = nb;
@@ -944,7 +945,7 @@ size_t scan_tmp_size(uint32_t n) {
 // Reduce-then-scan: blocks scan in place, the block totals are scanned the same way (one block up
 // to SCAN_BLOCK^2 elements), then added back.  The top-level single block yields the total.
 static void scan_rec(uint32_t* data, uint32_t n, uint32_t* tmp, uint32_t* total_dev, hipStream_t s) {
-  uint32_t nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  const uint32_t nb = static_cast<uint32_t>((static_cast<uint64_t>(n) + SCAN_BLOCK - 1) / SCAN_BLOCK);
   if (nb <= 1) {
     hipLaunchKernelGGL(scan_block_kernel, dim3(1), dim3(256), 0, s, data, n, nullptr, total_dev);
     return;

@@ -163,13 +163,60 @@ int32_t build_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t primc
   return SRX_OK;
 }
 
-int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t primcnt, int32_t basevert) {
+// Device scratch of one draw: the renderer's grow-only arrays, or (fixed != null) one caller-free
+// region carved front to back at 256-byte alignment, which refuses what does not fit.
+struct draw_scratch {
+  renderer_obj* r;
+  const fixed_scratch* fixed;
+  size_t used = 0;
+  template <class T>
+  int32_t take(dev_array<T>& a, size_t n, T*& p) {
+    if (!fixed) {
+      RCHK(ensure(r, a, n));
+      p = a.p;
+      return SRX_OK;
+    }
+    const size_t off = (used + 255) & ~static_cast<size_t>(255);
+    if (off > fixed->bytes || n > (fixed->bytes - off) / sizeof(T))
+      return fail(SRX_OUT_OF_MEMORY, "the draw's scratch exceeds the no-allocation region");
+    p = reinterpret_cast<T*>(fixed->base + off);
+    used = off + n * sizeof(T);
+    return SRX_OK;
+  }
+};
+
+static size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+
+// Bytes the record-sized buffers take for `records` binned records (keys/values twice, the sort
+// histogram and its scan space, the tile order and the visibility work table), in take() order.
+static size_t record_scratch_bytes(uint32_t P3, uint32_t T, int samples, bool vis_tables, uint64_t records) {
+  const uint32_t n = static_cast<uint32_t>(std::min<uint64_t>(records, 0xFFFFFFFFu));
+  const size_t rn = std::max<uint32_t>(n, 1);
+  const uint32_t nblocks = (std::max(n, T) + 4095) / 4096;
+  const size_t hist_n = std::max<size_t>(static_cast<size_t>(256) * std::max<uint32_t>(nblocks, 1), bucket_hist_size(T, n));
+  size_t b = 2 * align256(rn * 4) + 2 * align256(rn * 4) + align256(hist_n * 4) +
+             align256(std::max(scan_tmp_size(P3), scan_tmp_size(static_cast<uint32_t>(hist_n))) * 4) +
+             align256(std::max<uint32_t>(T, 1) * 4);
+  if (vis_tables) {
+    uint32_t ne, nm, nsl;
+    vis_table_bounds(T, n, &ne, &nm, &nsl);
+    b += align256(static_cast<size_t>(ne) * 16) + align256(static_cast<size_t>(nm) * 16) + align256(16) +
+         align256(static_cast<size_t>(nsl) * 256 * samples * 8);
+  }
+  return b + 256;
+}
+
+int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t primcnt, int32_t basevert,
+                     const fixed_scratch* fixed) {
   draw_params P;
   ps_samplers SM;
   bool skip = false;
   RCHK(build_draw(r, indexed, start, primcnt, basevert, P, SM, skip));
   if (skip) return SRX_OK;
-  if (r->recording) return record_draw(r, P, SM);
+  if (r->recording) {
+    if (fixed) return batch_refuse(r, SRX_E_UNSUPPORTED, "a batch records allocating draws only");
+    return record_draw(r, P, SM);
+  }
   const int nout = P.nattr;
   const int samples = P.samples;
   hipStream_t s = r->stream;
@@ -181,41 +228,66 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   hs_vs += P.nverts;
   if (primcnt == 0) return SRX_OK;
 
-  RCHK(ensure(r, r->vbuf, (static_cast<size_t>(P.nverts) + 1) * (nout + 1)));
-  RCHK(ensure(r, r->tri_rec, static_cast<size_t>(P3) * 5));
-  RCHK(ensure(r, r->tri_attr, static_cast<size_t>(P3) * 3 * (nout + 1)));
-  bool vis_pass = (P.early_z || P.vis_stencil) && (P.blend_id == SRX_BLEND_OFF || P.blend_id == SRX_BLEND_DISABLED);
-  if (vis_pass)
-    RCHK(ensure(r, r->vis, static_cast<size_t>(std::max(P.target_w, 1.0f)) * static_cast<size_t>(std::max(P.target_h, 1.0f)) * samples));
-  RCHK(ensure(r, r->samplers_d, 1));
-  RCHK(ensure(r, r->cnt, P3));
-  RCHK(ensure(r, r->clip_flag, primcnt));
-  RCHK(ensure(r, r->clip_slot, primcnt));
-  RCHK(ensure(r, r->wide, primcnt));
-  RCHK(ensure(r, r->scan_tmp, scan_tmp_size(P3)));
-  uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
-  RCHK(ensure(r, r->partial, std::max<size_t>(T, (primcnt + 255) / 256) + 1));
-  RCHK(ensure(r, r->tstart, std::max<uint32_t>(T, 1)));
-  RCHK(ensure(r, r->tend, std::max<uint32_t>(T, 1)));
-
+  draw_scratch X{r, fixed};
   draw_buffers B{};
-  B.vbuf = r->vbuf.p;
-  B.tri_rec = r->tri_rec.p;
-  B.tri_attr = r->tri_attr.p;
-  B.vis = r->vis.p;
-  B.samplers = r->samplers_d.p;
-  B.cnt = r->cnt.p;
-  B.clip_flag = r->clip_flag.p;
-  B.clip_slot = r->clip_slot.p;
-  B.wide_list = r->wide.p;
+  uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
+  bool vis_pass = (P.early_z || P.vis_stencil) && (P.blend_id == SRX_BLEND_OFF || P.blend_id == SRX_BLEND_DISABLED);
+  RCHK(X.take(r->vbuf, (static_cast<size_t>(P.nverts) + 1) * (nout + 1), B.vbuf));
+  RCHK(X.take(r->tri_rec, static_cast<size_t>(P3) * 5, B.tri_rec));
+  RCHK(X.take(r->tri_attr, static_cast<size_t>(P3) * 3 * (nout + 1), B.tri_attr));
+  if (vis_pass)
+    RCHK(X.take(r->vis, static_cast<size_t>(std::max(P.target_w, 1.0f)) * static_cast<size_t>(std::max(P.target_h, 1.0f)) * samples,
+                B.vis));
+  ps_samplers* samplers_d = nullptr;
+  RCHK(X.take(r->samplers_d, 1, samplers_d));
+  B.samplers = samplers_d;
+  RCHK(X.take(r->cnt, P3, B.cnt));
+  RCHK(X.take(r->clip_flag, primcnt, B.clip_flag));
+  RCHK(X.take(r->clip_slot, primcnt, B.clip_slot));
+  RCHK(X.take(r->wide, primcnt, B.wide_list));
+  RCHK(X.take(r->scan_tmp, scan_tmp_size(P3), B.scan_tmp));
+  RCHK(X.take(r->partial, std::max<size_t>(T, (primcnt + 255) / 256) + 1, B.partial));
+  RCHK(X.take(r->tstart, std::max<uint32_t>(T, 1), B.tstart));
+  RCHK(X.take(r->tend, std::max<uint32_t>(T, 1), B.tend));
+  // clipping capacity for every primitive, so no read-back decides whether clipping runs
+  RCHK(X.take(r->clip_verts, static_cast<size_t>(primcnt) * 5 * (nout + 1), B.clip_verts));
+  RCHK(X.take(r->clip_hdr, primcnt, B.clip_hdr));
+  B.clip_capacity = primcnt;
   B.counters = r->counters.p;
   B.stats = r->stats_d;
-  B.partial = r->partial.p;
-  B.scan_tmp = r->scan_tmp.p;
-  B.tstart = r->tstart.p;
-  B.tend = r->tend.p;
 
-  bool prof = r->profiling();
+  // statistics queries need the in-order walk (ps_invocations counts early-Z passes as they happen)
+  const bool counting = P.count_stats != 0;
+  const bool vis_tables = vis_pass && !counting;
+  // Records are emitted before their count is read back, so the read-back overlaps the emission.
+  // Allocating draws emit into a grow-only buffer sized from the previous draw and repeat the
+  // emission when it was too small; a fixed region gets the largest record capacity whose
+  // record-sized buffers all fit, and a draw with more records is refused.
+  uint32_t capacity;
+  if (!fixed) {
+    const uint32_t guess = std::max<uint32_t>(std::max<uint32_t>(r->last_records + r->last_records / 4, P3 + T), 1u << 16);
+    if (r->keys0.cap < guess) {
+      RCHK(ensure(r, r->keys0, guess));
+      RCHK(ensure(r, r->vals0, guess));
+    }
+    B.keys[0] = r->keys0.p;
+    B.vals[0] = r->vals0.p;
+    capacity = static_cast<uint32_t>(std::min<size_t>(r->keys0.cap, r->vals0.cap));
+  } else {
+    const size_t room = fixed->bytes - std::min(fixed->bytes, align256(X.used));
+    uint64_t lo = 0, hi = 0xFFFFFFFEu;  // largest capacity whose buffers fit
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi + 1) / 2;
+      if (record_scratch_bytes(P3, T, samples, vis_tables, mid) <= room) lo = mid;
+      else hi = mid - 1;
+    }
+    if (lo == 0) return fail(SRX_OUT_OF_MEMORY, "the draw's scratch exceeds the no-allocation region");
+    capacity = static_cast<uint32_t>(lo);
+    RCHK(X.take(r->keys0, capacity, B.keys[0]));
+    RCHK(X.take(r->vals0, capacity, B.vals[0]));
+  }
+
+  bool prof = r->profiling() && !fixed;  // event creation may allocate: no-allocation draws are not profiled
   draw_events de{};
   if (prof)
     for (auto& e : de.ev) {
@@ -227,57 +299,42 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   };
   // sampler table: staged in pinned memory; the stream syncs below before the staging is reused
   *r->samplers_h = SM;
-  HIPCHK(hipMemcpyAsync(r->samplers_d.p, r->samplers_h, sizeof(ps_samplers), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(samplers_d, r->samplers_h, sizeof(ps_samplers), hipMemcpyHostToDevice, s));
   launch_fill_u32(B.counters, CT_COUNT, 0u, s);
   mark(0);
   launch_vs(P, B.vbuf, s);
   mark(1);
-  // clipping capacity for every primitive, so no read-back decides whether clipping runs
-  RCHK(ensure(r, r->clip_verts, static_cast<size_t>(primcnt) * 5 * (nout + 1)));
-  RCHK(ensure(r, r->clip_hdr, primcnt));
-  B.clip_verts = r->clip_verts.p;
-  B.clip_hdr = r->clip_hdr.p;
-  B.clip_capacity = primcnt;
   launch_setup(P, B, s);
   launch_clip(P, B, s);  // returns at once on the device when setup flagged no primitive
   mark(2);
   launch_exclusive_scan(B.cnt, P3, B.scan_tmp, B.counters + CT_TOTAL, s);
   mark(3);
-  // Records are emitted into a grow-only buffer before their count is read back, so the read-back
-  // overlaps the emission; a buffer found too small is grown and the emission repeated.
-  const uint32_t guess = std::max<uint32_t>(std::max<uint32_t>(r->last_records + r->last_records / 4, P3 + T), 1u << 16);
-  if (r->keys0.cap < guess) {
-    RCHK(ensure(r, r->keys0, guess));
-    RCHK(ensure(r, r->vals0, guess));
-  }
-  B.keys[0] = r->keys0.p;
-  B.vals[0] = r->vals0.p;
-  launch_emit(P, B, static_cast<uint32_t>(std::min<size_t>(r->keys0.cap, r->vals0.cap)), s);
+  launch_emit(P, B, capacity, s);
   uint32_t ctr[CT_COUNT];
   RCHK(read_counters(r, ctr));
   if (ctr[CT_CLIP_OVERFLOW]) return fail(SRX_OUT_OF_MEMORY, "clip buffer overflow");
   const uint32_t total = ctr[CT_TOTAL];
-  r->last_records = total;
-  if (total > std::min<size_t>(r->keys0.cap, r->vals0.cap)) {
-    RCHK(ensure(r, r->keys0, total));
-    RCHK(ensure(r, r->vals0, total));
-    B.keys[0] = r->keys0.p;
-    B.vals[0] = r->vals0.p;
-    launch_emit(P, B, total, s);
+  if (fixed && total > capacity)
+    return fail(SRX_OUT_OF_MEMORY, "the draw's binned records exceed the no-allocation region");
+  if (!fixed) {
+    r->last_records = total;
+    if (total > capacity) {
+      RCHK(ensure(r, r->keys0, total));
+      RCHK(ensure(r, r->vals0, total));
+      B.keys[0] = r->keys0.p;
+      B.vals[0] = r->vals0.p;
+      launch_emit(P, B, total, s);
+    }
   }
-  // statistics queries need the in-order walk (ps_invocations counts early-Z passes as they happen)
-  const bool counting = P.count_stats != 0;
-  const bool order_free = vis_pass && !counting && vis_applicable(P);  // tile lists need no primitive order
-  uint32_t nblocks = (std::max(total, T) + 4095) / 4096;
-  const size_t hist_n = std::max<size_t>(static_cast<size_t>(256) * std::max<uint32_t>(nblocks, 1), bucket_hist_size(T, total));
-  RCHK(ensure(r, r->keys1, std::max<uint32_t>(total, 1)));
-  RCHK(ensure(r, r->vals1, std::max<uint32_t>(total, 1)));
-  RCHK(ensure(r, r->hist, hist_n));
-  RCHK(ensure(r, r->scan_tmp, std::max(scan_tmp_size(P3), scan_tmp_size(static_cast<uint32_t>(hist_n)))));
-  B.keys[1] = r->keys1.p;
-  B.vals[1] = r->vals1.p;
-  B.hist = r->hist.p;
-  B.scan_tmp = r->scan_tmp.p;
+  const bool order_free = vis_tables && vis_applicable(P);  // tile lists need no primitive order
+  // sized for the capacity on a fixed region (the layout the capacity was chosen for), else for the count
+  const uint32_t sized = fixed ? capacity : total;
+  uint32_t nblocks = (std::max(sized, T) + 4095) / 4096;
+  const size_t hist_n = std::max<size_t>(static_cast<size_t>(256) * std::max<uint32_t>(nblocks, 1), bucket_hist_size(T, sized));
+  RCHK(X.take(r->keys1, std::max<uint32_t>(sized, 1), B.keys[1]));
+  RCHK(X.take(r->vals1, std::max<uint32_t>(sized, 1), B.vals[1]));
+  RCHK(X.take(r->hist, hist_n, B.hist));
+  RCHK(X.take(r->scan_tmp, std::max(scan_tmp_size(P3), scan_tmp_size(static_cast<uint32_t>(hist_n))), B.scan_tmp));
   int which = 1;
   if (!(order_free && launch_bucket_sort(P, B, total, s))) {
     int bits = 1;
@@ -285,23 +342,20 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
     which = launch_sort(B, total, bits, s);
     launch_tile_ranges(P, B, which, total, s);
   }
-  RCHK(ensure(r, r->order, std::max<uint32_t>(T, 1)));
-  launch_tile_order(P, B, r->order.p, s);
-  B.order = r->order.p;
+  uint32_t* order = nullptr;
+  RCHK(X.take(r->order, std::max<uint32_t>(T, 1), order));
+  launch_tile_order(P, B, order, s);
+  B.order = order;
   mark(4);
-  if (vis_pass && !counting) {
+  if (vis_tables) {
     uint32_t ne, nm, nsl;
-    vis_table_bounds(T, total, &ne, &nm, &nsl);
-    RCHK(ensure(r, r->vis_entries, ne));
-    RCHK(ensure(r, r->vis_merges, nm));
-    RCHK(ensure(r, r->vis_counts, 4));
-    RCHK(ensure(r, r->vis_partial, static_cast<size_t>(nsl) * 256 * samples));
-    B.vis_entries = r->vis_entries.p;
-    B.vis_merges = r->vis_merges.p;
-    B.vis_counts = r->vis_counts.p;
-    B.vis_partial = r->vis_partial.p;
+    vis_table_bounds(T, sized, &ne, &nm, &nsl);
+    RCHK(X.take(r->vis_entries, ne, B.vis_entries));
+    RCHK(X.take(r->vis_merges, nm, B.vis_merges));
+    RCHK(X.take(r->vis_counts, 4, B.vis_counts));
+    RCHK(X.take(r->vis_partial, static_cast<size_t>(nsl) * 256 * samples, B.vis_partial));
   }
-  const bool fused = vis_pass && !counting && launch_vis(P, B, which, needs_shading(P), total, s);
+  const bool fused = vis_tables && launch_vis(P, B, which, needs_shading(P), total, s);
   if (!fused) launch_raster(P, B, which, s);
   mark(5);
   // the in-order kernel defers shading (to shade_kernel) only for early-Z draws
@@ -324,6 +378,7 @@ renderer_obj::~renderer_obj() {
   fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot); fr(partial); fr(order); fr(wide); fr(vis_entries); fr(vis_merges); fr(vis_counts); fr(vis_partial);
   if (stats_d) (void)hipFree(stats_d);
   if (pinned) (void)hipHostFree(pinned);
+  if (map_host) (void)hipHostFree(map_host);
   for (auto& d : pending_prof)
     for (auto e : d.ev)
       if (e) (void)hipEventDestroy(e);
@@ -332,29 +387,38 @@ renderer_obj::~renderer_obj() {
 }
 
 
-// Folds the recorded stage events into the running pipeline_profiles queries (ns).
-int32_t collect_profiles(renderer_obj* r) {
-  if (r->pending_prof.empty()) return SRX_OK;
-  HIPCHK(hipStreamSynchronize(r->stream));
-  uint64_t add[PROF_COUNT] = {};
-  for (auto& d : r->pending_prof) {
+// A finished query's result (its `done` event has completed): counters as end - begin, and the
+// stage times of its profiled draws in ns (their events go back to the renderer's pool).
+int32_t resolve_query(renderer_obj* r, query_obj* q) {
+  if (q->resolved) return SRX_OK;
+  uint64_t d[ST_COUNT];
+  for (int i = 0; i < ST_COUNT; ++i)
+    d[i] = (q->host_end[i] + q->dev_snap[ST_COUNT + i]) - (q->host_begin[i] + q->dev_snap[i]);
+  q->stats = srx_pipeline_statistics{};
+  q->stats.ia_vertices = d[ST_IA_VERTICES];
+  q->stats.ia_primitives = d[ST_IA_PRIMITIVES];
+  q->stats.vs_invocations = d[ST_VS_INVOCATIONS];
+  q->stats.cinvocations = d[ST_CINVOCATIONS];
+  q->stats.cprimitives = d[ST_CPRIMITIVES];
+  q->stats.ps_invocations = d[ST_PS_INVOCATIONS];
+  q->istats.backend_input_pixels = d[ST_BACKEND_PIXELS];
+  for (auto& v : q->prof) v = 0;
+  for (auto& de : q->draws) {
     float ms[6];
-    for (int i = 0; i < 6; ++i) HIPCHK(hipEventElapsedTime(&ms[i], d.ev[i], d.ev[i + 1]));
+    for (int i = 0; i < 6; ++i) HIPCHK(hipEventElapsedTime(&ms[i], de.ev[i], de.ev[i + 1]));
     auto ns = [](float m) { return static_cast<uint64_t>(static_cast<double>(m) * 1.0e6); };
-    add[PROF_VTX] += ns(ms[0]);
-    add[PROF_CLIP] += ns(ms[1]);
-    add[PROF_COMPACT] += ns(ms[2]);
-    add[PROF_DISPATCH] += ns(ms[3]);
-    add[PROF_RAS] += ns(ms[4]) + ns(ms[5]);
-    add[PROF_RAS_VIS] += ns(ms[4]);
-    add[PROF_RAS_SHADE] += ns(ms[5]);
-    add[PROF_DRAWS] += 1;
-    for (auto e : d.ev) r->event_pool.push_back(e);
+    q->prof[PROF_VTX] += ns(ms[0]);
+    q->prof[PROF_CLIP] += ns(ms[1]);
+    q->prof[PROF_COMPACT] += ns(ms[2]);
+    q->prof[PROF_DISPATCH] += ns(ms[3]);
+    q->prof[PROF_RAS] += ns(ms[4]) + ns(ms[5]);
+    q->prof[PROF_RAS_VIS] += ns(ms[4]);
+    q->prof[PROF_RAS_SHADE] += ns(ms[5]);
+    q->prof[PROF_DRAWS] += 1;
+    for (auto e : de.ev) r->event_pool.push_back(e);
   }
-  r->pending_prof.clear();
-  for (auto& q : r->active)
-    if (q->id == SRX_QUERY_PIPELINE_PROFILES)
-      for (int i = 0; i < PROF_COUNT; ++i) q->prof[i] += add[i];
+  q->draws.clear();
+  q->resolved = true;
   return SRX_OK;
 }
 
@@ -396,7 +460,18 @@ int32_t srx_create_renderer(uint32_t kind, srx_renderer* out) {
   });
   if (g_ewa_status != SRX_OK) return g_ewa_status;
   auto* r = new renderer_obj();
-  hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamDefault);
+  {  // renderer_impl::renderer_impl (renderer_impl.cpp:269-284): default raster and depth/stencil states
+    auto* rs = new raster_obj();
+    rs->desc = r->st.raster;
+    r->st.raster_state = ref<raster_obj>(rs);
+    release(rs);
+    auto* ds = new ds_obj();
+    ds->desc = r->st.ds;
+    r->st.ds_state = ref<ds_obj>(ds);
+    release(ds);
+  }
+  hipError_t e = hipGetDevice(&r->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&r->stream, hipStreamDefault);
   if (e == hipSuccess) e = hipMalloc(&r->counters.p, CT_COUNT * sizeof(uint32_t));
   if (e == hipSuccess) r->counters.cap = CT_COUNT;
   if (e == hipSuccess) e = hipMalloc(&r->stats_d, ST_COUNT * sizeof(unsigned long long));
@@ -651,6 +726,7 @@ int32_t srx_set_rasterizer_state(srx_renderer rr, srx_raster_state rs) {
   auto* o = as<raster_obj>(rs, K_RASTER);
   if (!r || !o) return SRX_INVALID_PARAMETER;
   r->st.raster = o->desc;
+  r->st.raster_state = ref<raster_obj>(o);
   return SRX_OK;
 }
 int32_t srx_set_depth_stencil_state(srx_renderer rr, srx_ds_state d, int32_t ref_) {
@@ -658,6 +734,7 @@ int32_t srx_set_depth_stencil_state(srx_renderer rr, srx_ds_state d, int32_t ref
   auto* o = as<ds_obj>(d, K_DS);
   if (!r || !o) return SRX_INVALID_PARAMETER;
   r->st.ds = o->desc;
+  r->st.ds_state = ref<ds_obj>(o);
   r->st.stencil_ref = ref_;
   return SRX_OK;
 }
@@ -720,31 +797,32 @@ int32_t srx_begin(srx_renderer rr, srx_query qq) {
   if (q->running) return SRX_FAILED;
   for (auto& a : r->active)
     if (a->id == q->id) return SRX_FAILED;
-  RCHK(collect_profiles(r));
+  if (r->recording) return batch_refuse(r, SRX_E_UNSUPPORTED, "queries cannot run inside a batch");
+  if (!q->done) HIPCHK(hipEventCreateWithFlags(&q->done, hipEventDefault));
+  if (!q->dev_snap) HIPCHK(hipHostMalloc(&q->dev_snap, 2 * ST_COUNT * sizeof(unsigned long long), hipHostMallocDefault));
+  if (q->has_data && !q->resolved) HIPCHK(hipEventSynchronize(q->done));  // the previous end's copies
+  for (auto& d : q->draws)
+    for (auto e : d.ev) r->event_pool.push_back(e);
+  q->draws.clear();
   q->running = true;
   q->has_data = false;
-  snapshot_stats(r, q->begin_snap);
-  for (auto& v : q->prof) v = 0;
+  q->resolved = false;
+  std::memcpy(q->host_begin, r->host_stats, sizeof q->host_begin);
+  HIPCHK(hipMemcpyAsync(q->dev_snap, r->stats_d, ST_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost, r->stream));
   r->active.emplace_back(q);
   return SRX_OK;
 }
+// Queues the counters' read-back and the completion event; does not wait (async_renderer.cpp:17-110).
 int32_t srx_end(srx_renderer rr, srx_query qq) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);
   auto* q = as<query_obj>(qq, K_QUERY);
   if (!r || !q || !q->running) return SRX_FAILED;
-  RCHK(collect_profiles(r));
-  uint64_t now[ST_COUNT];
-  snapshot_stats(r, now);
-  uint64_t d[ST_COUNT];
-  for (int i = 0; i < ST_COUNT; ++i) d[i] = now[i] - q->begin_snap[i];
-  q->stats = srx_pipeline_statistics{};
-  q->stats.ia_vertices = d[ST_IA_VERTICES];
-  q->stats.ia_primitives = d[ST_IA_PRIMITIVES];
-  q->stats.vs_invocations = d[ST_VS_INVOCATIONS];
-  q->stats.cinvocations = d[ST_CINVOCATIONS];
-  q->stats.cprimitives = d[ST_CPRIMITIVES];
-  q->stats.ps_invocations = d[ST_PS_INVOCATIONS];
-  q->istats.backend_input_pixels = d[ST_BACKEND_PIXELS];
+  std::memcpy(q->host_end, r->host_stats, sizeof q->host_end);
+  HIPCHK(hipMemcpyAsync(q->dev_snap + ST_COUNT, r->stats_d, ST_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        r->stream));
+  HIPCHK(hipEventRecord(q->done, r->stream));
+  if (q->id == SRX_QUERY_PIPELINE_PROFILES) q->draws = std::move(r->pending_prof);
+  r->pending_prof.clear();
   q->running = false;
   q->has_data = true;
   for (size_t i = 0; i < r->active.size(); ++i)
@@ -754,10 +832,25 @@ int32_t srx_end(srx_renderer rr, srx_query qq) {
     }
   return SRX_OK;
 }
+// async_object::get (async_object.h:62-81): error while running or before the first end; timeout
+// while the GPU has not reached the end (do_not_wait), else waits; then ready.
 int32_t srx_get_data(srx_renderer rr, srx_query qq, void* data, size_t size, int32_t do_not_wait) {
-  (void)do_not_wait;
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
   auto* q = as<query_obj>(qq, K_QUERY);
-  if (!as<renderer_obj>(rr, K_RENDERER) || !q || !data || !q->has_data) return SRX_ASYNC_ERROR;
+  if (!r || !q || !data || !q->has_data || q->running) return SRX_ASYNC_ERROR;
+  if (!q->resolved) {
+    if (do_not_wait) {
+      const hipError_t e = hipEventQuery(q->done);
+      if (e == hipErrorNotReady) return SRX_ASYNC_TIMEOUT;
+      if (e != hipSuccess) {
+        hip_fail(e, "hipEventQuery");
+        return SRX_ASYNC_ERROR;
+      }
+    } else if (hipEventSynchronize(q->done) != hipSuccess) {
+      return SRX_ASYNC_ERROR;
+    }
+    if (resolve_query(r, q) != SRX_OK) return SRX_ASYNC_ERROR;
+  }
   if (q->id == SRX_QUERY_PIPELINE_STATISTICS && size >= sizeof(q->stats)) std::memcpy(data, &q->stats, sizeof(q->stats));
   else if (q->id == SRX_QUERY_INTERNAL_STATISTICS && size >= sizeof(q->istats)) std::memcpy(data, &q->istats, sizeof(q->istats));
   else if (q->id == SRX_QUERY_PIPELINE_PROFILES && size >= 8 * sizeof(uint64_t))
@@ -765,16 +858,144 @@ int32_t srx_get_data(srx_renderer rr, srx_query qq, void* data, size_t size, int
   else return SRX_ASYNC_ERROR;
   return SRX_ASYNC_READY;
 }
+
+// ---- map / unmap (renderer.h:61-63; resource_manager.cpp:7-113, buffer.cpp:9-29, surface.cpp:94-121)
+// The mapped bytes live in pinned host memory.  Every mode first finishes the renderer's queued
+// work and copies the resource down, so the host sees its current bytes whatever it reads or leaves
+// unwritten (the reference hands out the resource's own memory for every mode but map_read); unmap
+// copies them back for every mode that may write.
+static int32_t map_common(renderer_obj* r, srx_mapped_resource* m, uint8_t* dev, size_t bytes, int32_t mode,
+                          uint32_t row_pitch, uint32_t depth_pitch) {
+  if (mode < SRX_MAP_READ || mode > SRX_MAP_WRITE_NO_OVERWRITE) return SRX_FAILED;  // map_mode_none or unknown
+  if (r->map_mode != 0) return SRX_FAILED;                                          // one mapping at a time
+  if (r->map_cap < bytes) {
+    if (r->map_host) HIPCHK(hipHostFree(r->map_host));
+    r->map_host = nullptr;
+    r->map_cap = 0;
+    HIPCHK(hipHostMalloc(&r->map_host, std::max<size_t>(bytes, 4096), hipHostMallocDefault));
+    r->map_cap = std::max<size_t>(bytes, 4096);
+  }
+  HIPCHK(hipStreamSynchronize(r->stream));
+  if (bytes) {
+    HIPCHK(hipMemcpyAsync(r->map_host, dev, bytes, hipMemcpyDeviceToHost, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+  }
+  m->data = r->map_host;
+  m->row_pitch = row_pitch;
+  m->depth_pitch = depth_pitch;
+  r->map_mode = mode;
+  return SRX_OK;
+}
+int32_t srx_map_buffer(srx_renderer rr, srx_mapped_resource* m, srx_buffer bb, int32_t mode) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  auto* b = as<buffer_obj>(bb, K_BUFFER);
+  if (!r || !m || !b) return SRX_INVALID_PARAMETER;
+  const uint32_t pitch = static_cast<uint32_t>(b->size);
+  RCHK(map_common(r, m, b->d, b->size, mode, pitch, pitch));
+  r->mapped_buf = ref<buffer_obj>(b);
+  return SRX_OK;
+}
+int32_t srx_map_surface(srx_renderer rr, srx_mapped_resource* m, srx_surface ss, int32_t mode) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  auto* o = as<surface_obj>(ss, K_SURFACE);
+  if (!r || !m || !o) return SRX_INVALID_PARAMETER;
+  const uint32_t pitch = static_cast<uint32_t>(static_cast<size_t>(o->w) * o->samples * pf_bytes(o->pf));
+  RCHK(map_common(r, m, o->d, o->bytes, mode, pitch, pitch * static_cast<uint32_t>(o->h)));
+  r->mapped_surf = ref<surface_obj>(o);
+  return SRX_OK;
+}
+int32_t srx_unmap(srx_renderer rr) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r) return SRX_INVALID_PARAMETER;
+  if (r->map_mode == 0) return SRX_FAILED;
+  uint8_t* dev = r->mapped_buf ? r->mapped_buf->d : r->mapped_surf->d;
+  const size_t bytes = r->mapped_buf ? r->mapped_buf->size : r->mapped_surf->bytes;
+  const int32_t mode = r->map_mode;
+  r->map_mode = 0;
+  r->mapped_buf = ref<buffer_obj>();
+  r->mapped_surf = ref<surface_obj>();
+  if (mode != SRX_MAP_READ && bytes) {
+    HIPCHK(hipMemcpyAsync(dev, r->map_host, bytes, hipMemcpyHostToDevice, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));  // the staging may be mapped again at once
+  }
+  return SRX_OK;
+}
+
+// ---- state get (renderer.h:104-113, renderer_impl.cpp:61-157)
+int32_t srx_get_index_buffer(srx_renderer rr, srx_buffer* out) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !out) return SRX_INVALID_PARAMETER;
+  retain(r->st.ib.get());
+  *out = reinterpret_cast<srx_buffer>(static_cast<object*>(r->st.ib.get()));
+  return SRX_OK;
+}
+uint32_t srx_get_index_format(srx_renderer rr) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  return r ? r->st.index_fmt : 0;
+}
+uint32_t srx_get_primitive_topology(srx_renderer rr) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  return r ? r->st.topo : 0;
+}
+int32_t srx_get_vertex_shader(srx_renderer rr) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  return r ? r->st.vs_id : 0;
+}
+int32_t srx_get_pixel_shader(srx_renderer rr) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  return r ? r->st.ps_id : 0;
+}
+int32_t srx_get_blend_shader(srx_renderer rr) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  return r ? r->st.blend_id : -1;
+}
+int32_t srx_get_rasterizer_state(srx_renderer rr, srx_raster_state* out) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !out) return SRX_INVALID_PARAMETER;
+  retain(r->st.raster_state.get());
+  *out = reinterpret_cast<srx_raster_state>(static_cast<object*>(r->st.raster_state.get()));
+  return SRX_OK;
+}
+int32_t srx_get_depth_stencil_state(srx_renderer rr, srx_ds_state* out, int32_t* stencil_ref) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !out) return SRX_INVALID_PARAMETER;
+  retain(r->st.ds_state.get());
+  *out = reinterpret_cast<srx_ds_state>(static_cast<object*>(r->st.ds_state.get()));
+  if (stencil_ref) *stencil_ref = r->st.stencil_ref;
+  return SRX_OK;
+}
+int32_t srx_raster_state_desc(srx_raster_state rs, srx_raster_desc* d) {
+  auto* o = as<raster_obj>(rs, K_RASTER);
+  if (!o || !d) return SRX_INVALID_PARAMETER;
+  *d = o->desc;
+  return SRX_OK;
+}
+int32_t srx_depth_stencil_state_desc(srx_ds_state ds, srx_depth_stencil_desc* d) {
+  auto* o = as<ds_obj>(ds, K_DS);
+  if (!o || !d) return SRX_INVALID_PARAMETER;
+  *d = o->desc;
+  return SRX_OK;
+}
+// set_vs_sampler (renderer_impl.cpp:336-339): bound by slot; kept for vertex shaders that sample
+int32_t srx_set_vs_sampler(srx_renderer rr, uint32_t slot, srx_sampler s) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || slot >= SRX_MAX_SAMPLER_SLOTS) return SRX_INVALID_PARAMETER;
+  r->st.vs_samplers[slot] = ref<sampler_obj>(as<sampler_obj>(s, K_SAMPLER));
+  return SRX_OK;
+}
 int32_t srx_draw(srx_renderer rr, size_t start, size_t primcnt) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);
   if (!r) return SRX_INVALID_PARAMETER;
+  if (primcnt > MAX_DRAW_PRIMITIVES) return fail(SRX_E_UNSUPPORTED, "more than 1,431,650,304 primitives in one draw");
   (void)start;  // the index fetcher ignores the start of non-indexed draws (index_fetcher.cpp:103-107)
-  return execute_draw(r, false, 0, static_cast<uint32_t>(primcnt), 0);
+  return execute_draw(r, false, 0, static_cast<uint32_t>(primcnt), 0, nullptr);
 }
 int32_t srx_draw_index(srx_renderer rr, size_t start, size_t primcnt, int32_t basevert) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);
   if (!r) return SRX_INVALID_PARAMETER;
-  return execute_draw(r, true, static_cast<uint32_t>(start), static_cast<uint32_t>(primcnt), basevert);
+  if (primcnt > MAX_DRAW_PRIMITIVES) return fail(SRX_E_UNSUPPORTED, "more than 1,431,650,304 primitives in one draw");
+  if (start > 0xFFFFFFFFu) return fail(SRX_E_UNSUPPORTED, "start index beyond 2^32");
+  return execute_draw(r, true, static_cast<uint32_t>(start), static_cast<uint32_t>(primcnt), basevert, nullptr);
 }
 int32_t srx_clear_color(srx_renderer rr, srx_surface s, const float* c) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);

@@ -158,21 +158,35 @@ struct ds_obj : object {
 enum { PROF_GATHER = 0, PROF_VTX, PROF_CLIP, PROF_COMPACT, PROF_VP, PROF_DISPATCH, PROF_RAS, PROF_DRAWS,
        PROF_RAS_VIS, PROF_RAS_SHADE, PROF_COUNT };
 
-struct query_obj : object {
-  uint32_t id;
-  bool running = false, has_data = false;
-  uint64_t begin_snap[ST_COUNT] = {};
-  srx_pipeline_statistics stats{};
-  srx_internal_statistics istats{};
-  uint64_t prof[PROF_COUNT] = {};
-  query_obj() : object(K_QUERY) {}
-};
-
 // HIP events bracketing the stages of one profiled draw, on the renderer's stream:
 // [0,1) vertex shader, [1,2) setup + clipping (viewport transform fused), [2,3) compaction scan,
 // [3,4) record emission + tile sort, [4,5) visibility or in-order raster, [5,6) deferred shading.
 struct draw_events {
   hipEvent_t ev[7];
+};
+
+// An async_object (async_object.h:17-233).  begin/end enqueue copies of the device counters into
+// the query's pinned slots and end records `done`, so neither waits; get_data polls or waits for
+// `done` (async_object::get, :62-81) and then folds host counters, device counters and the stage
+// events of the profiled draws into the result once.
+struct query_obj : object {
+  uint32_t id;
+  bool running = false, has_data = false, resolved = false;
+  hipEvent_t done = nullptr;
+  unsigned long long* dev_snap = nullptr;  // pinned: [0, ST_COUNT) at begin, [ST_COUNT, 2*ST_COUNT) at end
+  uint64_t host_begin[ST_COUNT] = {}, host_end[ST_COUNT] = {};
+  std::vector<draw_events> draws;  // pipeline_profiles: the draws between begin and end
+  srx_pipeline_statistics stats{};
+  srx_internal_statistics istats{};
+  uint64_t prof[PROF_COUNT] = {};
+  query_obj() : object(K_QUERY) {}
+  ~query_obj() override {
+    for (auto& d : draws)
+      for (auto e : d.ev)
+        if (e) (void)hipEventDestroy(e);
+    if (done) (void)hipEventDestroy(done);
+    if (dev_snap) (void)hipHostFree(dev_snap);
+  }
 };
 
 inline std::string lower(const char* s) {
@@ -256,8 +270,11 @@ struct render_state {
   int vs_id = 0, ps_id = 0, blend_id = SRX_BLEND_OFF;
   float vs_consts[32] = {}, ps_consts[16] = {};
   ref<sampler_obj> samplers[SRX_MAX_SAMPLER_SLOTS];
+  ref<sampler_obj> vs_samplers[SRX_MAX_SAMPLER_SLOTS];  // set_vs_sampler: no built-in vertex shader samples
   srx_raster_desc raster;
   srx_depth_stencil_desc ds;
+  ref<raster_obj> raster_state;  // the objects behind raster / ds, for the getters
+  ref<ds_obj> ds_state;
   int32_t stencil_ref = 0;
   std::vector<ref<surface_obj>> color_targets;
   ref<surface_obj> ds_target;
@@ -277,6 +294,7 @@ struct batch_obj;
 struct renderer_obj : object {
   render_state st;
   batch_obj* recording = nullptr;  // srx_batch_begin .. srx_batch_end: commands are recorded
+  int device = 0;                  // current at creation: the stream's device
   hipStream_t stream = nullptr;
   std::vector<ref<query_obj>> active;
   // per-draw device scratch, grown on demand
@@ -296,6 +314,12 @@ struct renderer_obj : object {
   uint32_t* pinned = nullptr;  // host staging for counter read-back
   uint32_t last_records = 0;   // binned records of the previous draw (sizes the next emission)
   uint64_t host_stats[ST_COUNT] = {};
+  // map/unmap (resource_manager.cpp:7-113): one mapped resource at a time, staged in pinned memory
+  int32_t map_mode = 0;
+  ref<buffer_obj> mapped_buf;
+  ref<surface_obj> mapped_surf;
+  uint8_t* map_host = nullptr;
+  size_t map_cap = 0;
   std::vector<draw_events> pending_prof;  // recorded while a pipeline_profiles query runs
   std::vector<hipEvent_t> event_pool;
   bool profiling() const {
@@ -345,13 +369,6 @@ inline int gpu_tile_size() {
     return (v == 32) ? 32 : 16;
   }();
   return g;
-}
-
-inline void snapshot_stats(renderer_obj* r, uint64_t* out) {
-  unsigned long long dev[ST_COUNT] = {};
-  (void)hipStreamSynchronize(r->stream);
-  (void)hipMemcpy(dev, r->stats_d, sizeof dev, hipMemcpyDeviceToHost);
-  for (int i = 0; i < ST_COUNT; ++i) out[i] = r->host_stats[i] + dev[i];
 }
 
 // Reads the draw's device counters (waits for the queued work).
@@ -409,7 +426,16 @@ struct map_image {
 
 
 // ---------------------------------------------------------------- across translation units
+// A caller-free device region that a draw carves its scratch from instead of allocating.
+struct fixed_scratch {
+  uint8_t* base;
+  size_t bytes;
+};
 // srx_runtime.cpp
+// One draw on the renderer's stream; fixed == nullptr: scratch from the renderer's grow-only arrays.
+int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t primcnt, int32_t basevert,
+                     const fixed_scratch* fixed);
+constexpr size_t MAX_DRAW_PRIMITIVES = 0x55554000u;  // clipped-triangle ids (3 per primitive) are u32
 int32_t build_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t primcnt, int32_t basevert, draw_params& P,
                    ps_samplers& SM, bool& skip);
 // srx_batch.cpp: commands recorded between srx_batch_begin and srx_batch_end

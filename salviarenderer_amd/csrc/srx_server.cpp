@@ -819,7 +819,9 @@ int32_t srx_server_poll(srx_server ss, srx_completion* out, size_t capacity, siz
   while (n < capacity && !S->unreported.empty()) {
     const uint64_t tk = S->unreported.front();
     S->unreported.erase(S->unreported.begin());
-    const srv_job& t = S->finished[tk];
+    auto it = S->finished.find(tk);
+    if (it == S->finished.end()) continue;  // released before it was reported
+    const srv_job& t = it->second;
     srx_completion& c = out[n++];
     c.ticket = tk;
     c.status = t.status;
@@ -844,6 +846,7 @@ int32_t srx_server_release(srx_server ss, uint64_t ticket) {
   if (it == S->finished.end()) return SRX_INVALID_PARAMETER;
   if (it->second.stage_bytes) S->host_arena.release(it->second.stage_off, it->second.stage_bytes);
   S->finished.erase(it);
+  S->unreported.erase(std::remove(S->unreported.begin(), S->unreported.end(), ticket), S->unreported.end());
   S->cv.notify_all();
   return SRX_OK;
 }

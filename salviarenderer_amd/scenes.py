@@ -9,7 +9,10 @@ Parameters the configurations leave open are fixed here once (seeds, camera, lig
                  indices (522,242 triangles); 2048^2 RGBA8 uniform-noise texture (seed 0x5A11A002)
                  with the full box-filtered mip chain; min/mag linear, mip anisotropic x16, wrap;
                  Lambert with L = normalize(1,1,-1); depth LESS; cull none; camera at
-                 (0, 0.8, -0.6) looking at (0, 0.5, 0), vertical fov pi/2, near 0.1, far 100.
+                 (0, 1.5, -1.5) looking at the origin, vertical fov pi/2, near 0.1, far 100 (the
+                 grid then covers ~11% of the frame at 45 degrees: ~0.45 pixels per triangle).
+                 The "close" camera, (0, 0.8, -0.6) looking at (0, 0.5, 0), fills ~80% of the frame
+                 with a flatter view (~4 pixels per triangle, more shading, less anisotropy).
   multipass    : shadow map + shaded, blended and mirrored passes over 200,000 random triangles
                  (seed 0x5A11A003; setup_multipass below has the exact recipe).
 Random numbers come from splitmix64; a float in [0,1) is (u >> 40) * 2^-24.
@@ -111,15 +114,18 @@ def perspective_fov_lh(fovy, aspect, zn, zf):
     return m
 
 
-# Camera of the textured-mesh scene: the grid covers ~79% of a 16:9 frame and stays clear of
-# the near plane (clip w >= 0.43), so no triangle takes the clipper's slow path.
-MESH_EYE = (0.0, 0.8, -0.6)
-MESH_TARGET = (0.0, 0.5, 0.0)
+# Cameras of the textured-mesh scene (eye, target).  Both keep the grid clear of the near plane
+# (clip w >= 1.40 and >= 0.43), so no triangle takes the clipper's slow path.
+MESH_CAMERAS = {
+    "oblique": ((0.0, 1.5, -1.5), (0.0, 0.0, 0.0)),  # the configuration's camera: ~11% of the frame
+    "close": ((0.0, 0.8, -0.6), (0.0, 0.5, 0.0)),    # ~80% of the frame
+}
 MESH_FOVY = math.pi / 2
 
 
-def mesh_wvp(width: int, height: int) -> np.ndarray:
-    v = look_at_lh(MESH_EYE, MESH_TARGET, (0.0, 1.0, 0.0))
+def mesh_wvp(width: int, height: int, camera: str = "oblique") -> np.ndarray:
+    eye, target = MESH_CAMERAS[camera]
+    v = look_at_lh(eye, target, (0.0, 1.0, 0.0))
     p = perspective_fov_lh(MESH_FOVY, width / height, 0.1, 100.0)
     return (v @ p).astype(np.float32)  # row-vector convention: clip = [x y z 1] @ wvp
 
@@ -197,7 +203,7 @@ def setup_plain(r: A.Renderer, width=512, height=512, n_tris=2000, samples=1, se
 
 
 def setup_mesh(r: A.Renderer, width=1920, height=1080, samples=4, grid=512, tex_size=2048,
-               max_aniso=16, mip_filter=A.FILTER_ANISOTROPIC) -> Frame:
+               max_aniso=16, mip_filter=A.FILTER_ANISOTROPIC, camera="oblique") -> Frame:
     lib = r.lib
     color, ds, resolved, keep = _targets(r, width, height, samples)
     verts, idx = grid_mesh(grid)
@@ -210,7 +216,7 @@ def setup_mesh(r: A.Renderer, width=1920, height=1080, samples=4, grid=512, tex_
     r.set_vertex_buffers(0, [vb], [32])
     r.set_index_buffer(ib, A.FMT_R32_UINT)
     r.set_primitive_topology(A.TOPOLOGY_TRIANGLE_LIST)
-    r.set_vertex_shader(A.VS_WVP_NORMAL_UV, mesh_wvp(width, height).ravel())
+    r.set_vertex_shader(A.VS_WVP_NORMAL_UV, mesh_wvp(width, height, camera).ravel())
     r.set_pixel_shader(A.PS_LAMBERT_TEX, np.array([*LIGHT_DIR, 0.0], np.float32))
     tex = r.create_tex2d(tex_size, tex_size, 1, A.PF_RGBA8)
     tex.surface(0).write(noise_texture(tex_size))
@@ -288,7 +294,7 @@ def reflect_matrix(plane) -> np.ndarray:
 
 
 class MultipassFrame:
-    """One frame: shadow pass (depth only, from the light) -> opaque shaded pass with a 3x3 PCF
+    """One frame: shadow pass over all triangles (depth only, from the light) -> opaque shaded pass with a 3x3 PCF
     shadow lookup -> translucent triangles back-to-front (blend on, no depth write) -> mirror quad at
     depth 0.95 marking stencil 1 (StencilMirror's stencil-replace state with depth ALWAYS, so the
     mirror's depth replaces the scene's; colour writes disabled) -> the opaque triangles reflected
@@ -349,10 +355,12 @@ class MultipassFrame:
         self.pcf = np.array([0.3, 0.002, 1.0 / shadow_size, 0.0], np.float32)
         self.shadow_size = shadow_size
 
-    def _bind_geometry(self, vb, wvp, light):
+    def _bind_geometry(self, vb, wvp, light, first_tri=0):
         r = self.r
         r.set_input_layout(self.layout)
-        r.set_vertex_buffers(0, [vb], [28])
+        # a non-indexed draw ignores its start (index_fetcher.cpp:58-62,103-105), so a sub-range of
+        # the vertex buffer is selected by the binding's byte offset
+        r.set_vertex_buffers(0, [vb], [28], [first_tri * 3 * 28])
         r.set_primitive_topology(A.TOPOLOGY_TRIANGLE_LIST)
         r.set_vertex_shader(A.VS_WVP_COLOR_LIGHT, np.concatenate([wvp.ravel(), light.ravel()]).astype(np.float32))
         r.set_rasterizer_state(self.rs)
@@ -367,7 +375,7 @@ class MultipassFrame:
         r.set_pixel_shader(A.PS_GOURAUD)
         r.set_blend_shader(A.BLEND_OFF)
         r.set_depth_stencil_state(self.dss_normal, 0)
-        r.draw(0, self.n_opaque)
+        r.draw(0, self.n_tris)  # every triangle casts a shadow, translucent ones included
         # (b) shaded opaque, then translucent back-to-front
         r.set_render_targets([self.color], self.ds)
         r.set_viewport(0, 0, self.width, self.height, 0.0, 1.0)
@@ -378,9 +386,10 @@ class MultipassFrame:
         r.set_pixel_shader(A.PS_SHADOW_PCF, self.pcf)
         r.set_ps_sampler(0, self.samp)
         r.draw(0, self.n_opaque)
+        self._bind_geometry(self.vb, self.I, self.L, first_tri=self.n_opaque)
         r.set_blend_shader(A.BLEND_ON)
         r.set_depth_stencil_state(self.dss_translucent, 0)
-        r.draw(self.n_opaque * 3, self.n_trans)
+        r.draw(0, self.n_trans)
         # (c) mirror quad: stencil 1, colour writes disabled
         self._bind_geometry(self.mvb, self.I, self.L)
         r.set_pixel_shader(A.PS_CONSTANT, np.array([1, 1, 1, 1], np.float32))

@@ -24,6 +24,8 @@ SCENES = {
     "plain_small_512": (S.setup_plain, dict(width=512, height=512, n_tris=2000)),
     "plain_msaa4_cull_back": (S.setup_plain, dict(width=256, height=192, n_tris=600, samples=4, cull=A.CULL_BACK, seed=7)),
     "textured_mesh_320x176": (S.setup_mesh, dict(width=320, height=176, samples=4, grid=48, tex_size=256)),
+    "textured_mesh_320x176_close": (S.setup_mesh, dict(width=320, height=176, samples=4, grid=48, tex_size=256,
+                                                       camera="close")),
     "multipass_320x180": (S.MultipassFrame, dict(width=320, height=180, samples=4, shadow_size=256, n_tris=3000)),
 }
 

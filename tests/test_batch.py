@@ -216,15 +216,20 @@ def test_batch_rejects_dependent_commands(hip_lib):
 
 
 @pytest.mark.gpu
-def test_batched_full_config_equals_one_by_one(hip_lib):
+@pytest.mark.parametrize("config", [None, (16, 16384, 64)], ids=["default", "bench_tuned"])
+def test_batched_full_config_equals_one_by_one(hip_lib, config):
     """The 256-job batched configuration at full size (37.4M triangles): every job's colour and
-    depth/stencil bytes from one batch launch equal its one-by-one render."""
+    depth/stencil bytes from one batch launch equal its one-by-one render, with the default tuning
+    and with the tuning the benchmark's autotuner picks (16x16 tiles, 16,384 records per binning
+    workgroup, 64 lanes per tile)."""
     r = hip_lib.renderer()
     jobs = S.BatchedJobs(r, S.batch_job_specs(), with_textures=False)  # textures only occupy memory
     jobs.render_all()
     r.flush()
     ref = [(jb["color"].read().copy(), jb["ds"].read().copy()) for jb in jobs.jobs]
     b = jobs.record()
+    if config is not None:
+        b.configure(*config)
     for jb in jobs.jobs:  # other bytes in every target, so a skipped job cannot pass
         r.clear_color(jb["color"], (1.0, 0.0, 1.0, 1.0))
         r.clear_depth_stencil(jb["ds"], A.CLEAR_DEPTH | A.CLEAR_STENCIL, 0.5, 7)

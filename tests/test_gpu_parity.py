@@ -82,14 +82,17 @@ def test_plain_msaa2(hip_lib, cpu_lib):
     compare(render(hip_lib, S.setup_plain, **kw), render(cpu_lib, S.setup_plain, **kw))
 
 
-def test_textured_mesh_small(hip_lib, cpu_lib):
-    kw = dict(width=320, height=176, samples=4, grid=48, tex_size=256)
+@pytest.mark.parametrize("camera", sorted(S.MESH_CAMERAS))
+def test_textured_mesh_small(hip_lib, cpu_lib, camera):
+    kw = dict(width=320, height=176, samples=4, grid=48, tex_size=256, camera=camera)
     compare(render(hip_lib, S.setup_mesh, **kw), render(cpu_lib, S.setup_mesh, **kw), color_exact=False)
 
 
-def test_textured_mesh_full_config(hip_lib, cpu_lib):
+@pytest.mark.parametrize("camera", sorted(S.MESH_CAMERAS))
+def test_textured_mesh_full_config(hip_lib, cpu_lib, camera):
     """1920x1080 4xMSAA, 522,242 triangles, 2048^2 texture, 16x anisotropic."""
-    compare(render(hip_lib, S.setup_mesh), render(cpu_lib, S.setup_mesh), color_exact=False)
+    compare(render(hip_lib, S.setup_mesh, camera=camera), render(cpu_lib, S.setup_mesh, camera=camera),
+            color_exact=False)
 
 
 def render_multipass(lib, stats, **kw):
@@ -114,6 +117,20 @@ def render_multipass(lib, stats, **kw):
     return out
 
 
+def assert_multipass_not_vacuous(out):
+    """Every pass of the frame must leave a mark: blended samples (alpha < 255: only the
+    translucent pass writes such alpha over the opaque clear), stencil 1 under the mirror, and a
+    shadow map that is not all cleared."""
+    alpha = out["color"].reshape(-1, 4)[:, 3]
+    frac = float((alpha < 255).mean())
+    print(f"\n  blended colour samples: {frac:.3f}", flush=True)
+    assert frac > 0.15, frac
+    stencil = np.ascontiguousarray(out["ds"]).view(np.uint32).reshape(-1, 2)[:, 1]
+    assert 0.02 < float((stencil == 1).mean()) < 0.5
+    shadow_depth = np.ascontiguousarray(out["shadow"]).view(np.float32).reshape(-1, 2)[:, 0]
+    assert float((shadow_depth < 1.0).mean()) > 0.3
+
+
 @pytest.mark.parametrize("kw", [dict(width=320, height=180, samples=4, shadow_size=256, n_tris=3000),
                                 dict(width=640, height=360, samples=2, shadow_size=512, n_tris=20000),
                                 dict()],
@@ -122,6 +139,7 @@ def test_multipass_shadow_blend_stencil_mirror(hip_lib, cpu_lib, kw):
     """Shadow map (depth only, split tiles) -> PCF-shaded opaque pass -> translucent blend pass ->
     stencil mirror mark -> mirrored redraw, bit-exact on both GPU paths."""
     want = render_multipass(cpu_lib, True, **kw)
+    assert_multipass_not_vacuous(want)
     for stats in (False, True):
         got = render_multipass(hip_lib, stats, **kw)
         for k in ("shadow", "ds", "color", "resolved"):

@@ -164,3 +164,33 @@ def test_server_admission_by_footprint(hip_lib):
     assert drain(srv, 1)[t]["status"] == A.OUT_OF_MEMORY
     t = submit(srv, 0, 1)
     assert drain(srv, 1)[t]["status"] == 0
+
+
+def _release_before_poll(lib):
+    r = lib.renderer()
+    srv = r.create_server(A.default_server_desc(lib))
+    a = submit(srv, 4, 1)
+    b = submit(srv, 0, 1)
+    # wait (capacity 0 reports nothing) until both are done, then release one unreported ticket
+    import time
+
+    deadline = time.time() + 60
+    while srv.stats().jobs < 2 and time.time() < deadline:
+        srv.poll(0, wait_ms=50)
+    srv.release(a)
+    got = drain(srv, 1)
+    assert set(got) == {b} and got[b]["status"] == 0 and "color" in got[b]
+    assert srv.poll(16, wait_ms=200) == []  # the released ticket is never reported
+    with pytest.raises(A.SrxError):
+        srv.release(a)
+
+
+def test_oracle_server_release_before_poll(cpu_lib):
+    """A finished ticket released before poll reports it is gone: poll never reports it."""
+    _release_before_poll(cpu_lib)
+
+
+@pytest.mark.gpu
+def test_server_release_before_poll(hip_lib):
+    _release_before_poll(hip_lib)
+
