@@ -363,6 +363,19 @@ int32_t srx_clear_depth_stencil(srx_renderer r, srx_surface ds_target, uint32_t 
                                 uint32_t stencil);
 int32_t srx_flush(srx_renderer r);
 
+/* ---- a draw that allocates nothing ----
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops and harnesses that forbid hipMalloc): srx_draw with every buffer
+ * beyond its inputs and outputs carved from one fixed region of SRX_NOALLOC_SCRATCH_BYTES per GPU,
+ * which the library holds until the process ends.  Call it with the renderer's device current.
+ * A draw whose scratch does not fit returns SRX_OUT_OF_MEMORY and writes no target (about 100k
+ * primitives over a 1024x1024 1-sample target fit; the binned-record count decides).  It is not
+ * profiled by a pipeline_profiles query and cannot be recorded into a batch.  Every call on one
+ * GPU shares the region: calls from renderers on different streams must not overlap (one
+ * renderer's calls are ordered by its stream). */
+#define SRX_NOALLOC_SCRATCH_BYTES (64u << 20)
+int32_t srx_draw_noalloc(srx_renderer r, size_t startpos, size_t primcnt);
+
 /* ---- batches of independent render jobs ----
  * No counterpart in the reference, whose renderer executes one command at a time
  * (render_core.cpp:30-61).  Between srx_batch_begin and srx_batch_end the renderer records

@@ -1918,6 +1918,14 @@ int32_t srx_draw(srx_renderer rr, size_t start, size_t primcnt) {
   (void)start;
   return execute_draw(r, false, 0, static_cast<uint32_t>(primcnt), 0);
 }
+// The oracle has no device memory to avoid allocating: the same draw, refused inside a batch.
+int32_t srx_draw_noalloc(srx_renderer rr, size_t start, size_t primcnt) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r) return SRX_INVALID_PARAMETER;
+  if (r->recording) return SRX_E_UNSUPPORTED;
+  (void)start;
+  return execute_draw(r, false, 0, static_cast<uint32_t>(primcnt), 0);
+}
 int32_t srx_draw_index(srx_renderer rr, size_t start, size_t primcnt, int32_t basevert) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);
   if (!r) return SRX_INVALID_PARAMETER;

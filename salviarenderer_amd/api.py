@@ -201,6 +201,7 @@ PROTOTYPES = {
     "srx_get_data": (_I32, [_H, _H, _P, _SZ, _I32]),
     "srx_draw": (_I32, [_H, _SZ, _SZ]),
     "srx_draw_index": (_I32, [_H, _SZ, _SZ, _I32]),
+    "srx_draw_noalloc": (_I32, [_H, _SZ, _SZ]),
     "srx_clear_color": (_I32, [_H, _H, C.POINTER(C.c_float)]),
     "srx_clear_depth_stencil": (_I32, [_H, _H, _U32, C.c_float, _U32]),
     "srx_flush": (_I32, [_H]),
@@ -645,6 +646,10 @@ class Renderer(_Obj):
 
     def draw(self, start, primcnt):
         self.lib.call("srx_draw", self.h, start, primcnt)
+
+    def draw_noalloc(self, start, primcnt):
+        """draw() with every scratch buffer in the library's fixed per-GPU region (no allocation)."""
+        self.lib.call("srx_draw_noalloc", self.h, start, primcnt)
 
     def draw_index(self, start, primcnt, basevert=0):
         self.lib.call("srx_draw_index", self.h, start, primcnt, basevert)
