@@ -192,7 +192,29 @@ enum srx_vertex_shader_id {
   /* in: POSITION0, COLOR0.  out: pos = mul(P, wvp); attr0 = COLOR0; attr1 = mul(P, light_wvp), the
      light's clip-space position for a shadow-map lookup (StandardShadowMap.cpp draw_cpp_vs).
      consts: float wvp[16], float light_wvp[16] */
-  SRX_VS_WVP_COLOR_LIGHT = 4
+  SRX_VS_WVP_COLOR_LIGHT = 4,
+  /* ColorizedTriangle's vs_main (samples/ColorizedTriangle/ColorizedTriangle.cpp:29-53).
+     in: POSITION0, NORMAL0 (float4).  out: pos = mul(pos, wvp); attr0 = norm;
+     attr1..3 = lightPos0..2 - pos.  consts: float wvp[16], lightPos0[4], lightPos1[4], lightPos2[4] */
+  SRX_VS_COLORIZED = 5,
+  /* ComplexMesh's vs_mesh (samples/ComplexMesh/ComplexMesh.cpp:185-222).  in: POSITION0, NORMAL0.
+     pos_es = transform(pos, wv); out: pos = transform(pos_es, proj); attr0 = (light - pos_es.xyz, 1);
+     attr1 = (eye - pos_es.xyz, 1); attr2 = transform33(normal, wv).
+     consts: float wv[16], proj[16], light_pos[4] (xyz used), eye_pos[4] (xyz used) */
+  SRX_VS_MESH = 6,
+  /* StandardShadowMap's Draw.savs (resources/ssm/Draw.savs).  in: POSITION0, NORMAL0, TEXCOORD0.
+     out: pos = mul(pos, cameraWvp); attr0 = tex; attr1 = norm; attr2 = lightPos - pos;
+     attr3 = cameraPos - pos; attr4 = mul(pos, lightWvp).
+     consts: float cameraPos[4], cameraWvp[16], lightPos[4], lightWvp[16] */
+  SRX_VS_SSM_DRAW = 7,
+  /* StencilMirror's cup_vs_code (samples/StencilMirror/StencilMirror.cpp:33-57).  in: POSITION0,
+     NORMAL0, TEXCOORD0.  out: pos = mul(pos, wvp); attr0 = tex; attr1 = norm; attr2 = lightPos - pos;
+     attr3 = eyePos - pos.  consts: float wvp[16], eyePos[4], lightPos[4] */
+  SRX_VS_CUP = 8,
+  /* TextureAndBlending's vs_box (samples/TextureAndBlending/TextureAndBlending.cpp:60-86).
+     in: POSITION0, TEXCOORD0.  out: pos = transform(pos, wvp); attr0 = pos; attr1 = tex.
+     consts: float wvp[16] */
+  SRX_VS_BOX = 9
 };
 enum srx_pixel_shader_id {
   /* color0 = attr0 (Gouraud). consts: none */
@@ -213,7 +235,25 @@ enum srx_pixel_shader_id {
      l.z - consts[1] <= tap; lit = count / 9; k = consts[0] + (1 - consts[0]) * lit;
      color0 = (attr0.r * k, attr0.g * k, attr0.b * k, attr0.a).
      consts: float ambient, bias, spacing, unused */
-  SRX_PS_SHADOW_PCF = 5
+  SRX_PS_SHADOW_PCF = 5,
+  /* ColorizedTriangle's ps (ColorizedTriangle.cpp:55-92): three point lights with 1/d^2 falloff,
+     attr0 = normal, attr1..3 = light vectors; colour clamped to [0, 1], alpha 1.  consts: none */
+  SRX_PS_COLORIZED = 6,
+  /* ComplexMesh's ps_mesh (ComplexMesh.cpp:227-246): n.l * (0.8 + 3.5 pow(max(n.h, 0), 5) * 0.4)
+     with l, e, n = normalize3(attr0, attr1, attr2).xyz.  consts: none */
+  SRX_PS_MESH = 7,
+  /* StandardShadowMap's draw_cpp_ps (StandardShadowMap.cpp:62-142): exponential shadow map with a
+     3x3 Gaussian of tex2dlod(s0) taps around attr4's projection (sampler 0 = depth map; none bound:
+     occlusion 0), texture tex2d(s1, attr0) (none bound: white), diffuse + pow(specular, shininess).
+     consts: float ambient[4], diffuse[4], specular[4]; int32 shininess (its bits in float slot 12) */
+  SRX_PS_SSM = 8,
+  /* StencilMirror's cup_ps (StencilMirror.cpp:66-110): tex2d(s0, attr0) (none bound: white) times
+     ambient + diffuse * clamp(l.n) + specular * clamp(reflect(l, n).e); alpha 1.
+     consts: float ambient[4], diffuse[4], specular[4] */
+  SRX_PS_CUP = 9,
+  /* TextureAndBlending's ps_box (TextureAndBlending.cpp:96-114): tex2d(s0, attr1) with alpha 0.5.
+     consts: none */
+  SRX_PS_BOX = 10
 };
 enum srx_blend_shader_id {
   SRX_BLEND_OFF = 0,      /* ts_blend_off: target = src (TextureAndBlending.cpp:182-193) */

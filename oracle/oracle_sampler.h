@@ -185,6 +185,35 @@ inline void border_point_2d(float u, float v, int W, int H, int& ix, int& iy) {
   iy = out[1];
 }
 
+// sampler.cpp:347-390: texel indices of the bilinear footprint, -1 at or beyond the size (and
+// -1 below 0, where the coordinate is clamped to -0.5 texel)
+inline coord2 border_linear_2d(float u, float v, int W, int H) {
+  coord2 c;
+  float cc[2] = {u, v};
+  int szi[2] = {W, H};
+  int lo[2], hi[2];
+  float fr[2];
+  for (int k = 0; k < 2; ++k) {
+    float sz = static_cast<float>(szi[k]);
+    float f = cc[k] * sz;
+    f = sse_max(f, -0.5f);
+    f = sse_min(f, sz - (-0.5f));
+    f = f + (-0.5f);
+    float ip = sse_floor(f);
+    fr[k] = f - ip;
+    int i = cvtt(ip);
+    lo[k] = i >= szi[k] ? -1 : i;
+    hi[k] = i + 1 >= szi[k] ? -1 : i + 1;
+  }
+  c.x0 = lo[0];
+  c.x1 = hi[0];
+  c.tx = fr[0];
+  c.y0 = lo[1];
+  c.y1 = hi[1];
+  c.ty = fr[1];
+  return c;
+}
+
 // ---- 1-D addressers for mixed modes (sampler.cpp:20-25,127-130,205-210,300-304,394-412) ----
 inline float do_coordf(int mode, float coord, int size) {
   switch (mode) {
@@ -229,8 +258,11 @@ inline vec4 filter_point(const surf_view& s, const sampler_params& p, float x, f
   return s.texel(ix, iy, sample);
 }
 
-// Linear filtering with border addressing reads texel -1 in the reference (undefined); the
-// oracle clamps such reads to the border colour and reports them as unsupported upstream.
+// Border addressing hands index -1 to the bilinear fetch (sampler.cpp:347-390, 436-447), which
+// reads texel offset ((y*w + x)*samples + s)*elem in size_t arithmetic (surface.cpp:277-295): x = -1
+// is the last texel of row y - 1, while y = -1, or x = -1 on row 0, lies before the surface.  The
+// first is followed; the second (undefined in the reference) reads the nearest texel instead,
+// which gives the reference's result whenever that corner's weight is zero.
 inline vec4 filter_linear(const surf_view& s, const sampler_params& p, float x, float y, int sample) {
   coord2 c;
   if (p.addr_u == p.addr_v) {
@@ -238,10 +270,9 @@ inline vec4 filter_linear(const surf_view& s, const sampler_params& p, float x, 
     case ADDR_WRAP: c = wrap_linear_2d(x, y, s.w, s.h); break;
     case ADDR_MIRROR: c = mirror_linear_2d(x, y, s.w, s.h); break;
     case ADDR_CLAMP: c = clamp_linear_2d(x, y, s.w, s.h); break;
-    default: return p.border;
+    default: c = border_linear_2d(x, y, s.w, s.h); break;
     }
   } else {
-    if (p.addr_u == ADDR_BORDER || p.addr_v == ADDR_BORDER) return p.border;
     float ox = do_coordf(p.addr_u, x, s.w);
     int ipx = fast_floori(ox);
     c.x0 = do_coordi_point_1d(p.addr_u, ipx, s.w);
@@ -252,6 +283,18 @@ inline vec4 filter_linear(const surf_view& s, const sampler_params& p, float x, 
     c.y0 = do_coordi_point_1d(p.addr_v, ipy, s.h);
     c.y1 = do_coordi_point_1d(p.addr_v, ipy + 1, s.h);
     c.ty = oy - static_cast<float>(ipy);
+  }
+  if (p.addr_u == ADDR_BORDER || p.addr_v == ADDR_BORDER) {
+    // the four corners, each moved to the texel its linear offset names
+    int xs[4] = {c.x0, c.x1, c.x0, c.x1}, ys[4] = {c.y0, c.y0, c.y1, c.y1};
+    for (int k = 0; k < 4; ++k) {
+      if (xs[k] < 0) {
+        xs[k] = s.w - 1;
+        ys[k] -= 1;
+      }
+    }
+    return lerp_2d(s.pf, s.addr(xs[0], ys[0], sample), s.addr(xs[1], ys[1], sample), s.addr(xs[2], ys[2], sample),
+                   s.addr(xs[3], ys[3], sample), c.tx, c.ty);
   }
   return s.texel_lerp(c.x0, c.y0, c.x1, c.y1, c.tx, c.ty, sample);
 }

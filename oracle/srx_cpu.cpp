@@ -343,6 +343,7 @@ struct vs_binding {
   int reg;
 };
 // Register map of each built-in vertex shader (bind_semantic calls of the cpp shaders).
+// (the SASL shaders: their input structs in declaration order)
 int vs_bindings(int id, vs_binding* out) {
   switch (id) {
   case SRX_VS_PASSTHROUGH_COLOR:
@@ -352,10 +353,21 @@ int vs_bindings(int id, vs_binding* out) {
     out[1] = {"color", 0, 1};
     return 2;
   case SRX_VS_WVP_NORMAL_UV:
+  case SRX_VS_SSM_DRAW:
+  case SRX_VS_CUP:
     out[0] = {"position", 0, 0};
     out[1] = {"normal", 0, 1};
     out[2] = {"texcoord", 0, 2};
     return 3;
+  case SRX_VS_COLORIZED:
+  case SRX_VS_MESH:
+    out[0] = {"position", 0, 0};
+    out[1] = {"normal", 0, 1};
+    return 2;
+  case SRX_VS_BOX:  // TextureAndBlending.cpp:70-71
+    out[0] = {"position", 0, 0};
+    out[1] = {"texcoord", 0, 1};
+    return 2;
   }
   return -1;
 }
@@ -364,7 +376,35 @@ int vs_output_count(int id) {
   case SRX_VS_PASSTHROUGH_COLOR:
   case SRX_VS_WVP_COLOR: return 1;
   case SRX_VS_WVP_NORMAL_UV:
-  case SRX_VS_WVP_COLOR_LIGHT: return 2;
+  case SRX_VS_WVP_COLOR_LIGHT:
+  case SRX_VS_BOX: return 2;
+  case SRX_VS_MESH: return 3;
+  case SRX_VS_COLORIZED:
+  case SRX_VS_CUP: return 4;
+  case SRX_VS_SSM_DRAW: return 5;
+  }
+  return 0;
+}
+size_t vs_const_bytes(int id) {
+  switch (id) {
+  case SRX_VS_WVP_NORMAL_UV:
+  case SRX_VS_WVP_COLOR:
+  case SRX_VS_BOX: return 64;
+  case SRX_VS_CUP: return 96;
+  case SRX_VS_COLORIZED: return 112;
+  case SRX_VS_WVP_COLOR_LIGHT: return 128;
+  case SRX_VS_MESH:
+  case SRX_VS_SSM_DRAW: return 160;
+  }
+  return 0;
+}
+size_t ps_const_bytes(int id) {
+  switch (id) {
+  case SRX_PS_LAMBERT_TEX:
+  case SRX_PS_CONSTANT:
+  case SRX_PS_SHADOW_PCF: return 16;
+  case SRX_PS_CUP: return 48;
+  case SRX_PS_SSM: return 52;
   }
   return 0;
 }
@@ -374,7 +414,54 @@ vec4 transform_rowvec(vec4 v, const float* m) {
   for (int i = 0; i < 4; ++i) o[i] = v[0] * m[0 * 4 + i] + v[1] * m[1 * 4 + i] + v[2] * m[2 * 4 + i] + v[3] * m[3 * 4 + i];
   return o;
 }
+// eflib::transform33 (math.cpp:184-197): dot_prod3 with the first three columns, w = 0
+vec4 transform33_rowvec(vec4 v, const float* m) {
+  vec4 o;
+  for (int i = 0; i < 3; ++i) o[i] = v[0] * m[0 * 4 + i] + v[1] * m[1 * 4 + i] + v[2] * m[2 * 4 + i];
+  o[3] = 0.0f;
+  return o;
+}
+vec4 sub4(const float* c, const vec4& v) { return v4(c[0] - v[0], c[1] - v[1], c[2] - v[2], c[3] - v[3]); }
+
 void run_vs(int id, const float* consts, const vec4* in, vso& out) {
+  const float* k = consts;
+  switch (id) {
+  case SRX_VS_COLORIZED:  // ColorizedTriangle.cpp:44-52 (vs_main in SASL)
+    out.r[0] = transform_rowvec(in[0], k);
+    out.r[1] = in[1];
+    out.r[2] = sub4(k + 16, in[0]);
+    out.r[3] = sub4(k + 20, in[0]);
+    out.r[4] = sub4(k + 24, in[0]);
+    return;
+  case SRX_VS_MESH: {  // ComplexMesh.cpp:202-211: consts wv, proj, light_pos, eye_pos
+    vec4 es = transform_rowvec(in[0], k);
+    out.r[0] = transform_rowvec(es, k + 16);
+    out.r[1] = v4(k[32] - es[0], k[33] - es[1], k[34] - es[2], 1.0f);
+    out.r[2] = v4(k[36] - es[0], k[37] - es[1], k[38] - es[2], 1.0f);
+    out.r[3] = transform33_rowvec(in[1], k);
+    return;
+  }
+  case SRX_VS_SSM_DRAW:  // resources/ssm/Draw.savs: consts cameraPos, cameraWvp, lightPos, lightWvp
+    out.r[0] = transform_rowvec(in[0], k + 4);
+    out.r[1] = in[2];
+    out.r[2] = in[1];
+    out.r[3] = sub4(k + 20, in[0]);
+    out.r[4] = sub4(k, in[0]);
+    out.r[5] = transform_rowvec(in[0], k + 24);
+    return;
+  case SRX_VS_CUP:  // StencilMirror.cpp:47-55: consts wvp, eyePos, lightPos
+    out.r[0] = transform_rowvec(in[0], k);
+    out.r[1] = in[2];
+    out.r[2] = in[1];
+    out.r[3] = sub4(k + 20, in[0]);
+    out.r[4] = sub4(k + 16, in[0]);
+    return;
+  case SRX_VS_BOX:  // TextureAndBlending.cpp:76-81
+    out.r[0] = transform_rowvec(in[0], k);
+    out.r[1] = in[0];
+    out.r[2] = in[1];
+    return;
+  }
   switch (id) {
   case SRX_VS_PASSTHROUGH_COLOR:
     out.r[0] = in[0];
@@ -437,6 +524,130 @@ vec4 shadow_pcf(const ps_context& c, const vso& v) {
   return v4(a[0] * k, a[1] * k, a[2] * k, a[3]);
 }
 
+// cpp_pixel_shader::tex2d(s, reg) (cpp_pixel_shader.cpp:13-31) on the four pixels of a quad: one LOD
+// from the register's p1 - p0 and p2 - p0 (calc_lod_2d), then sample_2d_lod; no sampler bound:
+// white (the shaders that test for one) .
+void tex2d_cpp(const ps_context& c, int slot, const vso* q, int reg, vec4* out) {
+  if (!c.has_tex[slot]) {
+    for (int i = 0; i < 4; ++i) out[i] = v4(1, 1, 1, 1);
+    return;
+  }
+  float lod = calc_lod_2d(c.tex[slot], c.sp[slot], q[1].r[reg][0] - q[0].r[reg][0], q[1].r[reg][1] - q[0].r[reg][1],
+                          q[2].r[reg][0] - q[0].r[reg][0], q[2].r[reg][1] - q[0].r[reg][1]);
+  for (int i = 0; i < 4; ++i) out[i] = sample_impl(c.tex[slot], c.sp[slot], q[i].r[reg][0], q[i].r[reg][1], lod, nullptr);
+}
+// eflib vector helpers on .xyz: dot_prod3, length, normalize3, reflect3 (math.cpp:17-41, 85-87;
+// vector_generic.h:110-118).  Square roots are correctly rounded in float and in double alike.
+inline float dot3(const vec4& a, const vec4& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline float length3(const vec4& a) {
+  float t = 0.0f;
+  t += a[0] * a[0];
+  t += a[1] * a[1];
+  t += a[2] * a[2];
+  return std::sqrt(t);
+}
+inline vec4 normalize3v(const vec4& v) {
+  float o[3];
+  normalize3(v.v, o);
+  return v4(o[0], o[1], o[2], 0.0f);
+}
+inline vec4 reflect3(const vec4& i, const vec4& n) {
+  float k = 2.0f * dot3(i, n);
+  return v4(i[0] - n[0] * k, i[1] - n[1] * k, i[2] - n[2] * k, 0.0f);
+}
+
+// ColorizedTriangle.cpp:55-92: attr0 normal, attr1..3 light vectors; 1/d^2 falloff, clamped
+vec4 ps_colorized(const vso& v) {
+  const vec4& n = v.r[1];
+  float inv[3];
+  vec4 nl[3];
+  for (int i = 0; i < 3; ++i) {
+    const vec4& l = v.r[2 + i];
+    inv[i] = 1.0f / length3(l);
+    nl[i] = v4(l[0] * inv[i], l[1] * inv[i], l[2] * inv[i], 0.0f);
+  }
+  vec4 nn = normalize3v(n);
+  float refl[3];
+  for (int i = 0; i < 3; ++i) refl[i] = dot3(nn, nl[i]);
+  static const float col[3][3] = {{0.7f, 0.1f, 0.3f}, {0.1f, 0.3f, 0.7f}, {0.3f, 0.7f, 0.1f}};
+  vec4 o = v4(0, 0, 0, 1.0f);
+  for (int k = 0; k < 3; ++k) {
+    float a = ((col[0][k] * refl[0]) * inv[0]) * inv[0];
+    float b = ((col[1][k] * refl[1]) * inv[1]) * inv[1];
+    float d = ((col[2][k] * refl[2]) * inv[2]) * inv[2];
+    o[k] = std_clamp((a + b) + d, 0.0f, 1.0f);
+  }
+  return o;
+}
+// ComplexMesh.cpp:227-246 (std::pow(float, float))
+vec4 ps_mesh(const vso& v) {
+  vec4 l = normalize3v(v.r[1]), e = normalize3v(v.r[2]), n = normalize3v(v.r[3]);
+  float n_dot_l = dot3(n, l);
+  float roughness = 5;
+  vec4 h = normalize3v(v4(l[0] + e[0], l[1] + e[1], l[2] + e[2], 0.0f));
+  float spec = (roughness + 2) / 2 * std::pow(std_max(dot3(h, n), 0.0f), roughness);
+  float clr = n_dot_l * (0.8f + spec * 0.4f);
+  return v4(clr, clr, clr, 1.0f);
+}
+// StandardShadowMap.cpp:62-142: exponential shadow map over 3x3 tex2dlod taps (std::exp and std::log
+// on float), white without a texture, pow(float, int) evaluated in double (C++11 <cmath>)
+vec4 ps_ssm(const ps_context& c, const vso& v, const vec4& tex) {
+  static const float esm_constant = 25000.0f;
+  static const float gaussian_weights[9] = {0.027681f, 0.111014f, 0.027681f, 0.111014f, 0.445213f,
+                                            0.111014f, 0.027681f, 0.111014f, 0.027681f};
+  float occlusion = 0.0f;
+  if (c.has_tex[0]) {
+    const vec4& l4 = v.r[5];
+    float lis[3] = {l4[0] / l4[3], l4[1] / l4[3], l4[2] / l4[3]};
+    float cx = (lis[0] + 1.0f) * 0.5f, cy = (1.0f - (lis[1] + 1.0f) * 0.5f);
+    float o = 1 / 512.0f;
+    const float ox[9] = {-o, 0.0f, +o, -o, 0.0f, +o, -o, 0.0f, +o};
+    const float oy[9] = {-o, -o, -o, 0.0f, 0.0f, 0.0f, +o, +o, +o};
+    float sd[9];
+    for (int i = 0; i < 9; ++i) sd[i] = sample_impl(c.tex[0], c.sp[0], cx + ox[i], cy + oy[i], 0.0f, nullptr)[0];
+    float occluder = 0.0f;
+    for (int i = 1; i < 9; ++i) occluder += gaussian_weights[i] * std::exp(esm_constant * (sd[i] - sd[0]));
+    occluder += gaussian_weights[0];
+    occluder = std::log(occluder);
+    occluder += esm_constant * sd[0];
+    occlusion = std_clamp(std::exp(occluder - esm_constant * lis[2]), 0.0f, 1.0f);
+  }
+  vec4 n = normalize3v(v.r[2]), ld = normalize3v(v.r[3]), ed = normalize3v(v.r[4]);
+  float idiff = std_clamp(dot3(ld, n), 0.0f, 1.0f);
+  vec4 r = reflect3(ld, n);
+  float ispec = std_clamp(dot3(v4(-r[0], -r[1], -r[2], 0.0f), ed), 0.0f, 1.0f);
+  int32_t shininess;
+  std::memcpy(&shininess, &c.consts[12], 4);
+  float p = static_cast<float>(std::pow(static_cast<double>(ispec), shininess));
+  vec4 out;
+  for (int k = 0; k < 3; ++k)
+    out[k] = tex[k] * (c.consts[k] + (c.consts[4 + k] * idiff + c.consts[8 + k] * p) * occlusion);
+  out[3] = 1.0f;
+  return out;
+}
+// StencilMirror.cpp:66-110
+vec4 ps_cup(const ps_context& c, const vso& v, const vec4& tex) {
+  vec4 n = normalize3v(v.r[2]), ld = normalize3v(v.r[3]), ed = normalize3v(v.r[4]);
+  float idiff = std_clamp(dot3(ld, n), 0.0f, 1.0f);
+  float ispec = std_clamp(dot3(reflect3(ld, n), ed), 0.0f, 1.0f);
+  vec4 out;
+  for (int k = 0; k < 3; ++k)
+    out[k] = tex[k] * ((c.consts[k] + c.consts[4 + k] * idiff) + c.consts[8 + k] * ispec);
+  out[3] = 1.0f;
+  return out;
+}
+
+// ddx/ddy of one value over a 2x2 quad (pixels 0 1 / 2 3) in a SASL pixel shader
+// (sasl/src/codegen/cgs_simd.cpp:267-312): ddx pairs the columns of each row (p1 - p0 for pixels
+// 0 and 1, p3 - p2 for 2 and 3), ddy the rows of each column (p2 - p0 for 0 and 2, p3 - p1 for 1
+// and 3).  The cpp shaders use p1 - p0 and p2 - p0 for the whole quad instead.
+void sasl_quad_derivatives(const float* p, float* ddx, float* ddy) {
+  ddx[0] = ddx[1] = p[1] - p[0];
+  ddx[2] = ddx[3] = p[3] - p[2];
+  ddy[0] = ddy[2] = p[2] - p[0];
+  ddy[1] = ddy[3] = p[3] - p[1];
+}
+
 // Runs the pixel shader on the 4 pixels of a quad (cpp_pixel_shader::execute,
 // cpp_pixel_shader.cpp:62-74).  Returns the per-pixel pass mask (the sample shaders never discard).
 void run_ps_quad(const ps_context& c, const vso* q, vec4* color) {
@@ -448,13 +659,17 @@ void run_ps_quad(const ps_context& c, const vso* q, vec4* color) {
     for (int i = 0; i < 4; ++i) color[i] = v4(c.consts[0], c.consts[1], c.consts[2], c.consts[3]);
     break;
   case SRX_PS_LAMBERT_TEX: {
-    // tex2D in a SASL pixel shader: ddx per row (p1-p0, p3-p2), ddy per column (p2-p0, p3-p1)
-    // (sasl/src/codegen/cgs_simd.cpp:275-312), then sample_2d_grad (sampler_api.cpp:13-18).
+    // tex2D in a SASL pixel shader: the quad's derivatives (sasl_quad_derivatives), then
+    // sample_2d_grad (sampler_api.cpp:13-18).
+    float u[4], v[4], dxu[4], dyu[4], dxv[4], dyv[4];
     for (int i = 0; i < 4; ++i) {
-      int row = i & 2, col = i & 1;
-      float dxu = q[row + 1].r[2][0] - q[row].r[2][0], dxv = q[row + 1].r[2][1] - q[row].r[2][1];
-      float dyu = q[col + 2].r[2][0] - q[col].r[2][0], dyv = q[col + 2].r[2][1] - q[col].r[2][1];
-      vec4 t = c.has_tex[0] ? sample_2d_grad(c.tex[0], c.sp[0], q[i].r[2][0], q[i].r[2][1], dxu, dxv, dyu, dyv, 0.0f)
+      u[i] = q[i].r[2][0];
+      v[i] = q[i].r[2][1];
+    }
+    sasl_quad_derivatives(u, dxu, dyu);
+    sasl_quad_derivatives(v, dxv, dyv);
+    for (int i = 0; i < 4; ++i) {
+      vec4 t = c.has_tex[0] ? sample_2d_grad(c.tex[0], c.sp[0], u[i], v[i], dxu[i], dxv[i], dyu[i], dyv[i], 0.0f)
                             : v4(0, 0, 0, 0);
       float n[3];
       normalize3(q[i].r[1].v, n);
@@ -475,6 +690,29 @@ void run_ps_quad(const ps_context& c, const vso* q, vec4* color) {
       t[3] = 1.0f;
       color[i] = t;
     }
+    break;
+  }
+  case SRX_PS_COLORIZED:
+    for (int i = 0; i < 4; ++i) color[i] = ps_colorized(q[i]);
+    break;
+  case SRX_PS_MESH:
+    for (int i = 0; i < 4; ++i) color[i] = ps_mesh(q[i]);
+    break;
+  case SRX_PS_SSM: {
+    vec4 tex[4];
+    tex2d_cpp(c, 1, q, 1, tex);
+    for (int i = 0; i < 4; ++i) color[i] = ps_ssm(c, q[i], tex[i]);
+    break;
+  }
+  case SRX_PS_CUP: {
+    vec4 tex[4];
+    tex2d_cpp(c, 0, q, 1, tex);
+    for (int i = 0; i < 4; ++i) color[i] = ps_cup(c, q[i], tex[i]);
+    break;
+  }
+  case SRX_PS_BOX: {  // TextureAndBlending.cpp:96-114: tex2d(s0, attr1), alpha 0.5
+    tex2d_cpp(c, 0, q, 2, color);
+    for (int i = 0; i < 4; ++i) color[i][3] = 0.5f;
     break;
   }
   case SRX_PS_SHADOW_PCF:
@@ -1652,17 +1890,16 @@ int32_t srx_set_primitive_topology(srx_renderer rr, uint32_t t) {
 int32_t srx_set_vertex_shader(srx_renderer rr, int32_t id, const void* consts, size_t size) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);
   if (!r || vs_output_count(id) <= 0) return SRX_INVALID_PARAMETER;
-  if ((id == SRX_VS_WVP_NORMAL_UV || id == SRX_VS_WVP_COLOR) && size < 64) return SRX_INVALID_PARAMETER;
+  if (size < vs_const_bytes(id) || (vs_const_bytes(id) && !consts)) return SRX_INVALID_PARAMETER;
   r->st.vs_id = id;
-  if (id == SRX_VS_WVP_COLOR_LIGHT && size < 128) return SRX_INVALID_PARAMETER;
-  r->st.vs_consts.assign(32, 0.0f);
-  if (consts && size) std::memcpy(r->st.vs_consts.data(), consts, std::min<size_t>(size, 128));
+  r->st.vs_consts.assign(48, 0.0f);
+  if (consts && size) std::memcpy(r->st.vs_consts.data(), consts, std::min<size_t>(size, 48 * sizeof(float)));
   return SRX_OK;
 }
 int32_t srx_set_pixel_shader(srx_renderer rr, int32_t id, const void* consts, size_t size) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);
-  if (!r || id < SRX_PS_GOURAUD || id > SRX_PS_SHADOW_PCF) return SRX_INVALID_PARAMETER;
-  if ((id == SRX_PS_LAMBERT_TEX || id == SRX_PS_CONSTANT || id == SRX_PS_SHADOW_PCF) && size < 16) return SRX_INVALID_PARAMETER;
+  if (!r || id < SRX_PS_GOURAUD || id > SRX_PS_BOX) return SRX_INVALID_PARAMETER;
+  if (size < ps_const_bytes(id) || (ps_const_bytes(id) && !consts)) return SRX_INVALID_PARAMETER;
   r->st.ps_id = id;
   r->st.ps_consts.assign(16, 0.0f);
   if (consts && size) std::memcpy(r->st.ps_consts.data(), consts, std::min<size_t>(size, 64));
@@ -2214,6 +2451,54 @@ float srxo_test_fast_round(float v) { return fast_round(v); }
 float srxo_test_ewa(int32_t i) { return ewa_table[i & 255]; }
 // Triangle setup of three screen-space positions (x, y, z, w each): edges[9], bbox[4], v0 index,
 // returns 0 for a degenerate triangle.
+// The SASL derivatives of a 4x4 pixel package (row-major, 4 per line) taken quad by quad.
+void srxo_test_sasl_ddx_ddy(const float* in16, float* ddx16, float* ddy16) {
+  for (int qy = 0; qy < 4; qy += 2)
+    for (int qx = 0; qx < 4; qx += 2) {
+      const int idx[4] = {qy * 4 + qx, qy * 4 + qx + 1, (qy + 1) * 4 + qx, (qy + 1) * 4 + qx + 1};
+      float p[4], dx[4], dy[4];
+      for (int i = 0; i < 4; ++i) p[i] = in16[idx[i]];
+      sasl_quad_derivatives(p, dx, dy);
+      for (int i = 0; i < 4; ++i) {
+        ddx16[idx[i]] = dx[i];
+        ddy16[idx[i]] = dy[i];
+      }
+    }
+}
+// The sampler of a sampler object at explicit coordinates (sampler.cpp:680-873):
+// mode 0: sample_2d_grad(u, v, a=ddx.u, b=ddx.v, c=ddy.u, d=ddy.v, bias); mode 1: sample_impl at level
+// a with no anisotropic information.
+int32_t srxo_test_sample(srx_sampler ss, int32_t mode, float u, float v, float a, float b, float c, float d,
+                         float bias, float* out4) {
+  auto* so = as<sampler_obj>(ss, K_SAMPLER);
+  if (!so || !so->tex || !out4) return SRX_INVALID_PARAMETER;
+  const texture_view t = so->tex->view();
+  const sampler_params p = so->params();
+  const vec4 r = mode == 0 ? sample_2d_grad(t, p, u, v, a, b, c, d, bias) : sample_impl(t, p, u, v, a, nullptr);
+  for (int k = 0; k < 4; ++k) out4[k] = r[k];
+  return SRX_OK;
+}
+// calc_lod (sampler.cpp:521-603) and calc_anisotropic_info (:875-958) of a sampler object's texture:
+// out = lod, then (anisotropic) probe count, weight_D, delta u, delta v.
+int32_t srxo_test_lod(srx_sampler ss, int32_t aniso, float dxu, float dxv, float dyu, float dyv, float bias, float* out5) {
+  auto* so = as<sampler_obj>(ss, K_SAMPLER);
+  if (!so || !so->tex || !out5) return SRX_INVALID_PARAMETER;
+  const texture_view t = so->tex->view();
+  const sampler_params p = so->params();
+  const vec4 ddx = v4(dxu, dxv, 0, 0), ddy = v4(dyu, dyv, 0, 0);
+  if (aniso) {
+    const aniso_info a = calc_aniso(t, p, ddx, ddy, bias);
+    out5[0] = a.lod;
+    out5[1] = a.probe_count;
+    out5[2] = a.weight_D;
+    out5[3] = a.delta_uv[0];
+    out5[4] = a.delta_uv[1];
+  } else {
+    out5[0] = calc_lod(t, p, ddx, ddy, bias);
+    out5[1] = out5[2] = out5[3] = out5[4] = 0.0f;
+  }
+  return SRX_OK;
+}
 int32_t srxo_test_triangle_setup(const float* pos12, float* edges9, float* bbox4, int32_t* v0_index) {
   vso v[3];
   for (int i = 0; i < 3; ++i) {

@@ -28,7 +28,9 @@ SOP_KEEP, SOP_ZERO, SOP_REPLACE, SOP_INCR_SAT, SOP_DECR_SAT, SOP_INVERT, SOP_INC
 CLEAR_DEPTH, CLEAR_STENCIL = 1, 2
 QUERY_PIPELINE_STATISTICS, QUERY_INTERNAL_STATISTICS, QUERY_PIPELINE_PROFILES = 3, 5, 6
 VS_PASSTHROUGH_COLOR, VS_WVP_NORMAL_UV, VS_WVP_COLOR, VS_WVP_COLOR_LIGHT = 1, 2, 3, 4
+VS_COLORIZED, VS_MESH, VS_SSM_DRAW, VS_CUP, VS_BOX = 5, 6, 7, 8, 9
 PS_GOURAUD, PS_LAMBERT_TEX, PS_TEX_CPP, PS_CONSTANT, PS_SHADOW_PCF = 1, 2, 3, 4, 5
+PS_COLORIZED, PS_MESH, PS_SSM, PS_CUP, PS_BOX = 6, 7, 8, 9, 10
 BLEND_OFF, BLEND_ON, BLEND_DISABLED = 0, 1, 2
 RENDERER_ASYNC, RENDERER_BENCHMARK = 0, 1
 ASYNC_ERROR, ASYNC_TIMEOUT, ASYNC_READY = 0, 1, 2
@@ -540,7 +542,7 @@ class Renderer(_Obj):
 
     def set_ps_sampler(self, slot, samp):
         self._keep[f"samp{slot}"] = samp
-        self.lib.call("srx_set_ps_sampler", self.h, slot, samp.h)
+        self.lib.call("srx_set_ps_sampler", self.h, slot, samp.h if samp is not None else None)
 
     def set_blend_shader(self, bid):
         self.lib.call("srx_set_blend_shader", self.h, bid)

@@ -123,6 +123,7 @@ int32_t record_draw(renderer_obj* r, const draw_params& P, const ps_samplers& SM
       !vis_applicable(P))
     return b->refuse(SRX_E_UNSUPPORTED, "a batched draw must be opaque with an order-independent depth test");
   if (T > BATCH_MAX_TILES) return b->refuse(SRX_E_UNSUPPORTED, "batched draw target larger than 8192 16x16 tiles");
+  if (P.nattr > 2) return b->refuse(SRX_E_UNSUPPORTED, "batched draws use vertex shaders with at most 2 attributes");
   if (3ull * P.prim_count >= 0x40000000ull) return b->refuse(SRX_E_UNSUPPORTED, "batched draw primitive count");
   surface_obj* ts[2] = {r->st.color_targets.empty() ? nullptr : r->st.color_targets[0].get(), r->st.ds_target.get()};
   for (surface_obj* t : ts) {

@@ -16,6 +16,7 @@ constexpr int MAX_REGS = MAX_ATTRS + 1;
 constexpr int MAX_LEVELS = 16;
 constexpr int MAX_SAMPLERS = 4;
 constexpr int MAX_INPUTS = 8;
+constexpr int VS_CONSTS = 48;  // floats of a vertex shader's constant blob
 constexpr int TILE = 64;  // reference tile size (rasterizer.cpp:32); baked into the arithmetic
 
 enum { PF_RGBA32F = 0, PF_RGB32F = 1, PF_BGRA8 = 2, PF_RGBA8 = 3, PF_R32F = 4, PF_RG32F = 5, PF_R32I = 6 };
@@ -52,7 +53,7 @@ struct draw_params {
   // vertex stage
   int32_t vs_id, nin, nattr;
   uint32_t attr_mod[MAX_ATTRS];
-  float vs_consts[32];
+  float vs_consts[VS_CONSTS];
   input_elem_dev in[MAX_INPUTS];
   uint32_t nverts;  // vertex-shader outputs stored (one more slot holds an all-zero input)
   // index fetch

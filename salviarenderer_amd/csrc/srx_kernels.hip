@@ -23,8 +23,10 @@ namespace srx {
 
 __constant__ float c_ewa[256];
 
-enum { VS_PASSTHROUGH_COLOR = 1, VS_WVP_NORMAL_UV = 2, VS_WVP_COLOR = 3, VS_WVP_COLOR_LIGHT = 4 };
-enum { PS_GOURAUD = 1, PS_LAMBERT_TEX = 2, PS_TEX_CPP = 3, PS_CONSTANT = 4, PS_SHADOW_PCF = 5 };
+enum { VS_PASSTHROUGH_COLOR = 1, VS_WVP_NORMAL_UV = 2, VS_WVP_COLOR = 3, VS_WVP_COLOR_LIGHT = 4, VS_COLORIZED = 5,
+       VS_MESH = 6, VS_SSM_DRAW = 7, VS_CUP = 8, VS_BOX = 9 };
+enum { PS_GOURAUD = 1, PS_LAMBERT_TEX = 2, PS_TEX_CPP = 3, PS_CONSTANT = 4, PS_SHADOW_PCF = 5, PS_COLORIZED = 6,
+       PS_MESH = 7, PS_SSM = 8, PS_CUP = 9, PS_BOX = 10 };
 enum { BLEND_OFF = 0, BLEND_ON = 1, BLEND_DISABLED = 2 };
 enum { AM_NOINTERPOLATION = 4, AM_NOPERSPECTIVE = 8 };
 enum { TOPO_TRIANGLE_LIST = 2, TOPO_TRIANGLE_STRIP = 4 };
@@ -44,6 +46,18 @@ __device__ __forceinline__ float4 transform_rowvec(float4 v, const float* m) {
 #pragma unroll
   for (int i = 0; i < 4; ++i) o[i] = v.x * m[0 * 4 + i] + v.y * m[1 * 4 + i] + v.z * m[2 * 4 + i] + v.w * m[3 * 4 + i];
   return make_float4(o[0], o[1], o[2], o[3]);
+}
+
+// eflib::transform33(out, v, m): out[i] = dot_prod3(v.xyz, column i.xyz), out.w = 0 (math.cpp:184-197)
+__device__ __forceinline__ float4 transform33_rowvec(float4 v, const float* m) {
+  float o[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) o[i] = v.x * m[0 * 4 + i] + v.y * m[1 * 4 + i] + v.z * m[2 * 4 + i];
+  return make_float4(o[0], o[1], o[2], 0.0f);
+}
+// a float4 constant minus a register, component by component
+__device__ __forceinline__ float4 sub4(const float* c, float4 v) {
+  return make_float4(c[0] - v.x, c[1] - v.y, c[2] - v.z, c[3] - v.w);
 }
 
 __device__ __forceinline__ float4 fetch_elem(const input_elem_dev& e, uint32_t v) {
@@ -81,30 +95,68 @@ __device__ __forceinline__ void vs_body(const draw_params& P, float4* __restrict
       for (int r = 0; r < MAX_INPUTS; ++r)  // register-resident: no dynamically indexed array
         if (P.in[i].reg == r) in[r] = x;
     }
-  float4 out[3];
+  float4 out[MAX_REGS];
+#pragma unroll
+  for (int r = 0; r < MAX_REGS; ++r) out[r] = make_float4(0, 0, 0, 0);
+  const float* k = P.vs_consts;
   switch (P.vs_id) {
   case VS_PASSTHROUGH_COLOR:
     out[0] = in[0];
     out[1] = in[1];
     break;
   case VS_WVP_COLOR:
-    out[0] = transform_rowvec(in[0], P.vs_consts);
+    out[0] = transform_rowvec(in[0], k);
     out[1] = in[1];
     break;
   case VS_WVP_COLOR_LIGHT:  // light-space position for the shadow lookup (StandardShadowMap.cpp)
-    out[0] = transform_rowvec(in[0], P.vs_consts);
+    out[0] = transform_rowvec(in[0], k);
     out[1] = in[1];
-    out[2] = transform_rowvec(in[0], P.vs_consts + 16);
+    out[2] = transform_rowvec(in[0], k + 16);
     break;
-  default:
-    out[0] = transform_rowvec(in[0], P.vs_consts);
+  case VS_COLORIZED:  // ColorizedTriangle.cpp:44-52
+    out[0] = transform_rowvec(in[0], k);
+    out[1] = in[1];
+    out[2] = sub4(k + 16, in[0]);
+    out[3] = sub4(k + 20, in[0]);
+    out[4] = sub4(k + 24, in[0]);
+    break;
+  case VS_MESH: {  // ComplexMesh.cpp:202-211
+    const float4 es = transform_rowvec(in[0], k);
+    out[0] = transform_rowvec(es, k + 16);
+    out[1] = make_float4(k[32] - es.x, k[33] - es.y, k[34] - es.z, 1.0f);
+    out[2] = make_float4(k[36] - es.x, k[37] - es.y, k[38] - es.z, 1.0f);
+    out[3] = transform33_rowvec(in[1], k);
+    break;
+  }
+  case VS_SSM_DRAW:  // resources/ssm/Draw.savs: consts cameraPos, cameraWvp, lightPos, lightWvp
+    out[0] = transform_rowvec(in[0], k + 4);
+    out[1] = in[2];
+    out[2] = in[1];
+    out[3] = sub4(k + 20, in[0]);
+    out[4] = sub4(k, in[0]);
+    out[5] = transform_rowvec(in[0], k + 24);
+    break;
+  case VS_CUP:  // StencilMirror.cpp:47-55: consts wvp, eyePos, lightPos
+    out[0] = transform_rowvec(in[0], k);
+    out[1] = in[2];
+    out[2] = in[1];
+    out[3] = sub4(k + 20, in[0]);
+    out[4] = sub4(k + 16, in[0]);
+    break;
+  case VS_BOX:  // TextureAndBlending.cpp:76-81
+    out[0] = transform_rowvec(in[0], k);
+    out[1] = in[0];
+    out[2] = in[1];
+    break;
+  default:  // VS_WVP_NORMAL_UV
+    out[0] = transform_rowvec(in[0], k);
     out[1] = in[1];
     out[2] = in[2];
     break;
   }
   size_t base = static_cast<size_t>(v) * (P.nattr + 1);
 #pragma unroll
-  for (int r = 0; r < 3; ++r)
+  for (int r = 0; r < MAX_REGS; ++r)
     if (r <= P.nattr) vbuf[base + r] = out[r];
 }
 __global__ void vs_kernel(draw_params P, float4* __restrict__ vbuf) { vs_body(P, vbuf, blockIdx.x); }
@@ -1375,6 +1427,143 @@ struct quad_lazy {
   }
 };
 
+// cpp_pixel_shader::tex2d(s, reg) (cpp_pixel_shader.cpp:13-31): one LOD per quad from the register's
+// p1 - p0 and p2 - p0 (calc_lod_2d, sampler.cpp:834-852), then sample_2d_lod at pixel pi
+template <int NA>
+__device__ __forceinline__ float4 tex2d_cpp(const sampler_desc_dev& sp, const quad_lazy<NA>& q, int reg, int pi) {
+  const float4 u0 = q.reg(reg, 0, q.inv_w(0), 2), u1 = q.reg(reg, 1, q.inv_w(1), 2), u2 = q.reg(reg, 2, q.inv_w(2), 2);
+  const float dxu = u1.x - u0.x, dxv = u1.y - u0.y;
+  const float dyu = u2.x - u0.x, dyv = u2.y - u0.y;
+  const float lod = (sp.mip_filter == FILTER_ANISO && sp.max_aniso > 1) ? calc_aniso(sp, dxu, dxv, dyu, dyv, 0.0f).lod
+                                                                        : calc_lod(sp, dxu, dxv, dyu, dyv, 0.0f);
+  const float4 uv = q.reg(reg, pi, q.inv_w(pi), 2);
+  return sample_impl(sp, uv.x, uv.y, lod, nullptr);
+}
+// eflib vector helpers on .xyz (vector_generic.h:110-118, math.cpp:39-41,85-87)
+__device__ __forceinline__ float dot3(float4 a, float4 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float length3(float4 a) {
+  float t = 0.0f;
+  t += a.x * a.x;
+  t += a.y * a.y;
+  t += a.z * a.z;
+  return sqrtf(t);
+}
+__device__ __forceinline__ float4 normalize3v(float4 v) {
+  float x, y, z;
+  normalize3(v.x, v.y, v.z, x, y, z);
+  return make_float4(x, y, z, 0.0f);
+}
+__device__ __forceinline__ float4 reflect3(float4 i, float4 n) {
+  const float k = 2.0f * dot3(i, n);
+  return make_float4(i.x - n.x * k, i.y - n.y * k, i.z - n.z * k, 0.0f);
+}
+
+// ColorizedTriangle.cpp:55-92: three lights, attr0 normal, attr1..3 light vectors
+template <int NA>
+__device__ __forceinline__ float4 ps_colorized(const quad_lazy<NA>& q, int pi) {
+  const float iw = q.inv_w(pi);
+  const float4 n = q.reg(1, pi, iw, 3);
+  float4 ld[3];
+  float inv[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    ld[i] = q.reg(2 + i, pi, iw, 3);
+    inv[i] = 1.0f / length3(ld[i]);
+  }
+  const float4 nn = normalize3v(n);
+  float refl[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) refl[i] = dot3(nn, make_float4(ld[i].x * inv[i], ld[i].y * inv[i], ld[i].z * inv[i], 0.0f));
+  const float c[3][3] = {{0.7f, 0.1f, 0.3f}, {0.1f, 0.3f, 0.7f}, {0.3f, 0.7f, 0.1f}};
+  float o[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float a = ((c[0][k] * refl[0]) * inv[0]) * inv[0];
+    float b = ((c[1][k] * refl[1]) * inv[1]) * inv[1];
+    float d = ((c[2][k] * refl[2]) * inv[2]) * inv[2];
+    o[k] = std_clamp((a + b) + d, 0.0f, 1.0f);
+  }
+  return make_float4(o[0], o[1], o[2], 1.0f);
+}
+
+// ComplexMesh.cpp:227-246
+template <int NA>
+__device__ __forceinline__ float4 ps_mesh(const quad_lazy<NA>& q, int pi) {
+  const float iw = q.inv_w(pi);
+  const float4 l = normalize3v(q.reg(1, pi, iw, 3)), e = normalize3v(q.reg(2, pi, iw, 3)), n = normalize3v(q.reg(3, pi, iw, 3));
+  const float n_dot_l = dot3(n, l);
+  const float4 h = normalize3v(make_float4(l.x + e.x, l.y + e.y, l.z + e.z, 0.0f));
+  const float spec = 3.5f * powf(std_max(dot3(h, n), 0.0f), 5.0f);  // (roughness + 2) / 2 with roughness 5
+  const float clr = n_dot_l * (0.8f + spec * 0.4f);
+  return make_float4(clr, clr, clr, 1.0f);
+}
+
+// StandardShadowMap.cpp:62-142 (exponential shadow map, esm_constant 25000, 3x3 Gaussian weights)
+template <int NA>
+__device__ __forceinline__ float4 ps_ssm(const draw_params& P, const ps_samplers* __restrict__ SM, const quad_lazy<NA>& q,
+                                         int pi) {
+  const float iw = q.inv_w(pi);
+  float occlusion = 0.0f;
+  if (P.has_sampler[0]) {
+    const float4 l4 = q.reg(5, pi, iw);
+    const float lx = l4.x / l4.w, ly = l4.y / l4.w, lz = l4.z / l4.w;
+    const float cx = (lx + 1.0f) * 0.5f, cy = 1.0f - (ly + 1.0f) * 0.5f;
+    const float o = 1 / 512.0f;
+    const float ox[9] = {-o, 0.0f, +o, -o, 0.0f, +o, -o, 0.0f, +o};
+    const float oy[9] = {-o, -o, -o, 0.0f, 0.0f, 0.0f, +o, +o, +o};
+    const float gw[9] = {0.027681f, 0.111014f, 0.027681f, 0.111014f, 0.445213f, 0.111014f, 0.027681f, 0.111014f, 0.027681f};
+    float sd[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) sd[i] = sample_lod0(SM->s[0], cx + ox[i], cy + oy[i]).x;
+    const float esm = 25000.0f;
+    float occ = 0.0f;
+#pragma unroll
+    for (int i = 1; i < 9; ++i) occ += gw[i] * expf(esm * (sd[i] - sd[0]));
+    occ += gw[0];
+    occ = logf(occ);
+    occ += esm * sd[0];
+    occlusion = std_clamp(expf(occ - esm * lz), 0.0f, 1.0f);
+  }
+  float4 tex = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+  if (P.has_sampler[1]) tex = tex2d_cpp(SM->s[1], q, 1, pi);
+  const float4 n = normalize3v(q.reg(2, pi, iw, 3)), ld = normalize3v(q.reg(3, pi, iw, 3)), ed = normalize3v(q.reg(4, pi, iw, 3));
+  const float idiff = std_clamp(dot3(ld, n), 0.0f, 1.0f);
+  const float4 r = reflect3(ld, n);
+  const float ispec = std_clamp(dot3(make_float4(-r.x, -r.y, -r.z, 0.0f), ed), 0.0f, 1.0f);
+  // pow(float, int) promotes to double (C++11 <cmath>), then the vec4 product rounds it to float
+  const float p = static_cast<float>(pow(static_cast<double>(ispec), static_cast<double>(__float_as_int(P.ps_consts[12]))));
+  const float* amb = P.ps_consts;
+  const float* dif = P.ps_consts + 4;
+  const float* spc = P.ps_consts + 8;
+  float4 out;
+  out.x = tex.x * (amb[0] + (dif[0] * idiff + spc[0] * p) * occlusion);
+  out.y = tex.y * (amb[1] + (dif[1] * idiff + spc[1] * p) * occlusion);
+  out.z = tex.z * (amb[2] + (dif[2] * idiff + spc[2] * p) * occlusion);
+  out.w = 1.0f;
+  return out;
+}
+
+// StencilMirror.cpp:66-110
+template <int NA>
+__device__ __forceinline__ float4 ps_cup(const draw_params& P, const ps_samplers* __restrict__ SM, const quad_lazy<NA>& q,
+                                         int pi) {
+  const float iw = q.inv_w(pi);
+  float4 tex = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+  if (P.has_sampler[0]) tex = tex2d_cpp(SM->s[0], q, 1, pi);
+  const float4 n = normalize3v(q.reg(2, pi, iw, 3)), ld = normalize3v(q.reg(3, pi, iw, 3)), ed = normalize3v(q.reg(4, pi, iw, 3));
+  const float idiff = std_clamp(dot3(ld, n), 0.0f, 1.0f);
+  const float ispec = std_clamp(dot3(reflect3(ld, n), ed), 0.0f, 1.0f);
+  const float* amb = P.ps_consts;
+  const float* dif = P.ps_consts + 4;
+  const float* spc = P.ps_consts + 8;
+  float4 out;
+  out.x = tex.x * ((amb[0] + dif[0] * idiff) + spc[0] * ispec);
+  out.y = tex.y * ((amb[1] + dif[1] * idiff) + spc[1] * ispec);
+  out.z = tex.z * ((amb[2] + dif[2] * idiff) + spc[2] * ispec);
+  out.w = 1.0f;
+  return out;
+}
+
 // Pixel shader (cpp_pixel_shader::execute, cpp_pixel_shader.cpp:62-74) colour of quad pixel pi
 // only; the other three pixels supply derivatives.  The sample shaders never discard.
 template <int NA>
@@ -1414,18 +1603,32 @@ __device__ __forceinline__ float4 run_ps_pixel(const draw_params& P, const ps_sa
     }
     return make_float4(0, 0, 0, 0);
   case PS_TEX_CPP: {
-    // cpp tex2d: one LOD per quad from p1 - p0 and p2 - p0 (cpp_pixel_shader.cpp:13-31)
+    // AnisotropicFilter / TextureAndBlending ps_plane: tex2d(s0, attr0) with alpha 1
     if (!P.has_sampler[0]) return make_float4(0, 0, 0, 1.0f);
-    const sampler_desc_dev& sp = SM->s[0];
-    const float4 u0 = q.reg(1, 0, q.inv_w(0), 2), u1 = q.reg(1, 1, q.inv_w(1), 2), u2 = q.reg(1, 2, q.inv_w(2), 2);
-    float dxu = u1.x - u0.x, dxv = u1.y - u0.y;
-    float dyu = u2.x - u0.x, dyv = u2.y - u0.y;
-    float lod = (sp.mip_filter == FILTER_ANISO && sp.max_aniso > 1) ? calc_aniso(sp, dxu, dxv, dyu, dyv, 0.0f).lod
-                                                                    : calc_lod(sp, dxu, dxv, dyu, dyv, 0.0f);
-    const float4 uv = q.reg(1, pi, q.inv_w(pi), 2);
-    float4 t = sample_impl(sp, uv.x, uv.y, lod, nullptr);
+    float4 t = tex2d_cpp(SM->s[0], q, 1, pi);
     t.w = 1.0f;
     return t;
+  }
+  case PS_COLORIZED:
+    if constexpr (NA >= 4) return ps_colorized(q, pi);
+    return make_float4(0, 0, 0, 0);
+  case PS_MESH:
+    if constexpr (NA >= 3) return ps_mesh(q, pi);
+    return make_float4(0, 0, 0, 0);
+  case PS_SSM:
+    if constexpr (NA >= 5) return ps_ssm(P, SM, q, pi);
+    return make_float4(0, 0, 0, 0);
+  case PS_CUP:
+    if constexpr (NA >= 4) return ps_cup(P, SM, q, pi);
+    return make_float4(0, 0, 0, 0);
+  case PS_BOX: {  // TextureAndBlending.cpp:96-114
+    if constexpr (NA >= 2) {
+      if (!P.has_sampler[0]) return make_float4(1.0f, 1.0f, 1.0f, 0.5f);
+      float4 t = tex2d_cpp(SM->s[0], q, 2, pi);
+      t.w = 0.5f;
+      return t;
+    }
+    return make_float4(0, 0, 0, 0);
   }
   case PS_SHADOW_PCF:
     if constexpr (NA >= 2) {
@@ -2371,11 +2574,20 @@ void launch_vs(const draw_params& P, float4* vbuf, hipStream_t s) {
   hipLaunchKernelGGL(vs_kernel, dim3((n + 255) / 256), dim3(256), 0, s, P, vbuf);
 }
 
+// Kernels are instantiated per attribute count: the vertex and setup buffers are strided by it.
 #define SRX_NA_DISPATCH(NAV, CALL) \
   switch (NAV) {                   \
   case 1: CALL(1); break;          \
   case 2: CALL(2); break;          \
-  default: CALL(3); break;         \
+  case 3: CALL(3); break;          \
+  case 4: CALL(4); break;          \
+  default: CALL(5); break;         \
+  }
+// Batched draws are refused above two attributes when they are recorded (batch_max_attrs).
+#define SRX_NA_DISPATCH_BATCH(NAV, CALL) \
+  switch (NAV) {                         \
+  case 1: CALL(1); break;                \
+  default: CALL(2); break;               \
   }
 
 void launch_setup(const draw_params& P, const draw_buffers& B, hipStream_t s) {
@@ -2790,13 +3002,13 @@ void launch_vs_batch(const batch_job* jobs, stage_map m, uint32_t blocks, hipStr
 void launch_setup_batch(int na, const batch_job* jobs, stage_map m, uint32_t blocks, hipStream_t s) {
   if (!blocks) return;
 #define SRX_SETUP_B(NA) hipLaunchKernelGGL(setup_batch_kernel<NA>, dim3(blocks), dim3(256), 0, s, jobs, m)
-  SRX_NA_DISPATCH(na, SRX_SETUP_B)
+  SRX_NA_DISPATCH_BATCH(na, SRX_SETUP_B)
 #undef SRX_SETUP_B
 }
 void launch_clip_batch(int na, const batch_job* jobs, stage_map m, uint32_t blocks, hipStream_t s) {
   if (!blocks) return;
 #define SRX_CLIP_B(NA) hipLaunchKernelGGL(clip_batch_kernel<NA>, dim3(blocks), dim3(64), 0, s, jobs, m)
-  SRX_NA_DISPATCH(na, SRX_CLIP_B)
+  SRX_NA_DISPATCH_BATCH(na, SRX_CLIP_B)
 #undef SRX_CLIP_B
 }
 void launch_emit_batch(const batch_job* jobs, stage_map m, uint32_t blocks, uint32_t capacity, hipStream_t s) {
@@ -2849,18 +3061,18 @@ bool launch_vis_batch(int na, int samples, bool shade, int threads, int gt, cons
 #define SRX_VIS_B32(NA) SRX_VIS_B(NA, 1, true, 32)
   if (gt == 8) {
     if (shade) {
-      SRX_NA_DISPATCH(na, SRX_VIS_B8)
+      SRX_NA_DISPATCH_BATCH(na, SRX_VIS_B8)
     } else {
       SRX_VIS_B(1, 1, false, 8);
     }
   } else if (gt == 32) {
     if (shade) {
-      SRX_NA_DISPATCH(na, SRX_VIS_B32)
+      SRX_NA_DISPATCH_BATCH(na, SRX_VIS_B32)
     } else {
       SRX_VIS_B(1, 1, false, 32);
     }
   } else if (shade) {
-    SRX_NA_DISPATCH(na, SRX_VIS_BS)
+    SRX_NA_DISPATCH_BATCH(na, SRX_VIS_BS)
   } else {
     switch (samples) {
     case 1: SRX_VIS_B(1, 1, false, 16); break;

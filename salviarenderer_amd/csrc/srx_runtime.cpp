@@ -693,8 +693,7 @@ int32_t srx_set_primitive_topology(srx_renderer rr, uint32_t t) {
 int32_t srx_set_vertex_shader(srx_renderer rr, int32_t id, const void* consts, size_t size) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);
   if (!r || vs_output_count(id) <= 0) return SRX_INVALID_PARAMETER;
-  if ((id == SRX_VS_WVP_NORMAL_UV || id == SRX_VS_WVP_COLOR) && size < 64) return SRX_INVALID_PARAMETER;
-  if (id == SRX_VS_WVP_COLOR_LIGHT && size < 128) return SRX_INVALID_PARAMETER;
+  if (size < vs_const_bytes(id) || (vs_const_bytes(id) && !consts)) return SRX_INVALID_PARAMETER;
   r->st.vs_id = id;
   std::memset(r->st.vs_consts, 0, sizeof r->st.vs_consts);
   if (consts && size) std::memcpy(r->st.vs_consts, consts, std::min<size_t>(size, sizeof r->st.vs_consts));
@@ -702,8 +701,8 @@ int32_t srx_set_vertex_shader(srx_renderer rr, int32_t id, const void* consts, s
 }
 int32_t srx_set_pixel_shader(srx_renderer rr, int32_t id, const void* consts, size_t size) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);
-  if (!r || id < SRX_PS_GOURAUD || id > SRX_PS_SHADOW_PCF) return SRX_INVALID_PARAMETER;
-  if ((id == SRX_PS_LAMBERT_TEX || id == SRX_PS_CONSTANT || id == SRX_PS_SHADOW_PCF) && size < 16) return SRX_INVALID_PARAMETER;
+  if (!r || id < SRX_PS_GOURAUD || id > SRX_PS_BOX) return SRX_INVALID_PARAMETER;
+  if (size < ps_const_bytes(id) || (ps_const_bytes(id) && !consts)) return SRX_INVALID_PARAMETER;
   r->st.ps_id = id;
   std::memset(r->st.ps_consts, 0, sizeof r->st.ps_consts);
   if (consts && size) std::memcpy(r->st.ps_consts, consts, std::min<size_t>(size, 64));

@@ -207,6 +207,8 @@ struct vs_binding {
   uint32_t index;
   int reg;
 };
+// Input registers of each built-in vertex shader (bind_semantic of the cpp shaders; the SASL
+// shaders' input structs in declaration order).
 inline int vs_bindings(int id, vs_binding* out) {
   switch (id) {
   case SRX_VS_PASSTHROUGH_COLOR:
@@ -216,10 +218,21 @@ inline int vs_bindings(int id, vs_binding* out) {
     out[1] = {"color", 0, 1};
     return 2;
   case SRX_VS_WVP_NORMAL_UV:
+  case SRX_VS_SSM_DRAW:
+  case SRX_VS_CUP:
     out[0] = {"position", 0, 0};
     out[1] = {"normal", 0, 1};
     out[2] = {"texcoord", 0, 2};
     return 3;
+  case SRX_VS_COLORIZED:
+  case SRX_VS_MESH:
+    out[0] = {"position", 0, 0};
+    out[1] = {"normal", 0, 1};
+    return 2;
+  case SRX_VS_BOX:
+    out[0] = {"position", 0, 0};
+    out[1] = {"texcoord", 0, 1};
+    return 2;
   }
   return -1;
 }
@@ -228,7 +241,36 @@ inline int vs_output_count(int id) {
   case SRX_VS_PASSTHROUGH_COLOR:
   case SRX_VS_WVP_COLOR: return 1;
   case SRX_VS_WVP_NORMAL_UV:
-  case SRX_VS_WVP_COLOR_LIGHT: return 2;
+  case SRX_VS_WVP_COLOR_LIGHT:
+  case SRX_VS_BOX: return 2;
+  case SRX_VS_MESH: return 3;
+  case SRX_VS_COLORIZED:
+  case SRX_VS_CUP: return 4;
+  case SRX_VS_SSM_DRAW: return 5;
+  }
+  return 0;
+}
+// Bytes of the constant blob each shader reads (srx.h lists the layouts).
+inline size_t vs_const_bytes(int id) {
+  switch (id) {
+  case SRX_VS_WVP_NORMAL_UV:
+  case SRX_VS_WVP_COLOR:
+  case SRX_VS_BOX: return 64;
+  case SRX_VS_CUP: return 96;
+  case SRX_VS_COLORIZED: return 112;
+  case SRX_VS_WVP_COLOR_LIGHT: return 128;
+  case SRX_VS_MESH:
+  case SRX_VS_SSM_DRAW: return 160;
+  }
+  return 0;
+}
+inline size_t ps_const_bytes(int id) {
+  switch (id) {
+  case SRX_PS_LAMBERT_TEX:
+  case SRX_PS_CONSTANT:
+  case SRX_PS_SHADOW_PCF: return 16;
+  case SRX_PS_CUP: return 48;
+  case SRX_PS_SSM: return 52;
   }
   return 0;
 }
@@ -268,7 +310,7 @@ struct render_state {
   ref<layout_obj> layout;
   uint32_t topo = SRX_TOPOLOGY_TRIANGLE_LIST;
   int vs_id = 0, ps_id = 0, blend_id = SRX_BLEND_OFF;
-  float vs_consts[32] = {}, ps_consts[16] = {};
+  float vs_consts[VS_CONSTS] = {}, ps_consts[16] = {};
   ref<sampler_obj> samplers[SRX_MAX_SAMPLER_SLOTS];
   ref<sampler_obj> vs_samplers[SRX_MAX_SAMPLER_SLOTS];  // set_vs_sampler: no built-in vertex shader samples
   srx_raster_desc raster;

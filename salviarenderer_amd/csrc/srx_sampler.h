@@ -27,13 +27,12 @@ __device__ __forceinline__ const uint8_t* texel_addr(const surface_desc& s, int 
   return s.data + (static_cast<size_t>(y) * s.w + x) * static_cast<size_t>(s.samples) * pf_size(s.pf);
 }
 
-// bilinear blend of the 2x2 footprint (colors.h:372-488 per format)
-__device__ __forceinline__ float4 lerp_2d(const surface_desc& s, const coord2& c) {
-  const uint8_t* a0 = texel_addr(s, c.x0, c.y0);
-  const uint8_t* a1 = texel_addr(s, c.x1, c.y0);
-  const uint8_t* a2 = texel_addr(s, c.x0, c.y1);
-  const uint8_t* a3 = texel_addr(s, c.x1, c.y1);
-  float tx = c.tx, ty = c.ty;
+// bilinear blend of four texels (colors.h:372-488 per format): corners 0 1 on the upper row, 2 3 below
+__device__ __forceinline__ float4 lerp_2d4(const surface_desc& s, const int* xs, const int* ys, float tx, float ty) {
+  const uint8_t* a0 = texel_addr(s, xs[0], ys[0]);
+  const uint8_t* a1 = texel_addr(s, xs[1], ys[1]);
+  const uint8_t* a2 = texel_addr(s, xs[2], ys[2]);
+  const uint8_t* a3 = texel_addr(s, xs[3], ys[3]);
   switch (s.pf) {
   case PF_RGBA8:
   case PF_BGRA8: {
@@ -96,6 +95,11 @@ __device__ __forceinline__ float4 lerp_2d(const surface_desc& s, const coord2& c
   }
   }
   return make_float4(0, 0, 0, 0);
+}
+// bilinear blend of the 2x2 footprint
+__device__ __forceinline__ float4 lerp_2d(const surface_desc& s, const coord2& c) {
+  const int xs[4] = {c.x0, c.x1, c.x0, c.x1}, ys[4] = {c.y0, c.y0, c.y1, c.y1};
+  return lerp_2d4(s, xs, ys, c.tx, c.ty);
 }
 
 // m / size, correctly rounded.  For a power-of-two size the quotient is m * 2^-k exactly (both are the
@@ -173,6 +177,20 @@ __device__ __forceinline__ void border_point_axis(float c, int size, int& out) {
   out = ip >= size ? -1 : ip;
 }
 
+// sampler.cpp:347-390: the bilinear footprint's indices, -1 at or beyond the size (and below 0)
+__device__ __forceinline__ void border_linear_axis(float c, int size, int& lo, int& hi, float& frac) {
+  float sz = static_cast<float>(size);
+  float f = c * sz;
+  f = sse_max(f, -0.5f);
+  f = sse_min(f, sz - (-0.5f));
+  f = f + (-0.5f);
+  float ip = sse_floor(f);
+  frac = f - ip;
+  int i = cvtt(ip);
+  lo = i >= size ? -1 : i;
+  hi = i + 1 >= size ? -1 : i + 1;
+}
+
 // ---- 1-D addressers for mixed modes (sampler.cpp:20-25,127-130,205-210,300-304,394-412) ----
 __device__ __forceinline__ float do_coordf(int mode, float coord, int size) {
   float sz = static_cast<float>(size);
@@ -213,8 +231,11 @@ __device__ __forceinline__ float4 filter_point(const surface_desc& s, const samp
   return load_texel(s.pf, texel_addr(s, ix, iy));
 }
 
-// Linear filtering with border addressing reads texel -1 in the reference (undefined); here such
-// footprints return the border colour.
+// Border addressing hands index -1 to the bilinear fetch (sampler.cpp:347-390, 436-447), which
+// reads texel offset ((y*w + x)*samples + s)*elem in size_t arithmetic (surface.cpp:277-295): x = -1
+// is the last texel of row y - 1, while y = -1, or x = -1 on row 0, lies before the surface.  The
+// first is followed; the second (undefined in the reference) reads the nearest texel instead
+// (texel_addr clamps), which gives the reference's result whenever that corner's weight is zero.
 __device__ __forceinline__ float4 filter_linear(const surface_desc& s, const sampler_desc_dev& p, float x, float y) {
   coord2 c;
   if (p.addr_u == p.addr_v) {
@@ -222,11 +243,9 @@ __device__ __forceinline__ float4 filter_linear(const surface_desc& s, const sam
     case ADDR_WRAP: wrap_linear_axis(x, s.w, c.x0, c.x1, c.tx); wrap_linear_axis(y, s.h, c.y0, c.y1, c.ty); break;
     case ADDR_MIRROR: mirror_linear_axis(x, s.w, c.x0, c.x1, c.tx); mirror_linear_axis(y, s.h, c.y0, c.y1, c.ty); break;
     case ADDR_CLAMP: clamp_linear_axis(x, s.w, c.x0, c.x1, c.tx); clamp_linear_axis(y, s.h, c.y0, c.y1, c.ty); break;
-    default: return make_float4(p.border[0], p.border[1], p.border[2], p.border[3]);
+    default: border_linear_axis(x, s.w, c.x0, c.x1, c.tx); border_linear_axis(y, s.h, c.y0, c.y1, c.ty); break;
     }
   } else {
-    if (p.addr_u == ADDR_BORDER || p.addr_v == ADDR_BORDER)
-      return make_float4(p.border[0], p.border[1], p.border[2], p.border[3]);
     float ox = do_coordf(p.addr_u, x, s.w);
     int ipx = fast_floori(ox);
     c.x0 = point_1d(p.addr_u, ipx, s.w);
@@ -237,6 +256,17 @@ __device__ __forceinline__ float4 filter_linear(const surface_desc& s, const sam
     c.y0 = point_1d(p.addr_v, ipy, s.h);
     c.y1 = point_1d(p.addr_v, ipy + 1, s.h);
     c.ty = oy - static_cast<float>(ipy);
+  }
+  if (p.addr_u == ADDR_BORDER || p.addr_v == ADDR_BORDER) {
+    int xs[4] = {c.x0, c.x1, c.x0, c.x1}, ys[4] = {c.y0, c.y0, c.y1, c.y1};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (xs[k] < 0) {
+        xs[k] = s.w - 1;
+        ys[k] -= 1;
+      }
+    }
+    return lerp_2d4(s, xs, ys, c.tx, c.ty);
   }
   return lerp_2d(s, c);
 }

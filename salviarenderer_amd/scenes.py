@@ -543,3 +543,166 @@ class BatchedJobs:
         px = sp.width * sp.height
         tex = sum(max(sp.tex_size >> l, 1) ** 2 for l in range(int(math.log2(sp.tex_size)) + 1)) * 4
         return sp.n_tris * 96 + px * sp.samples * 12 + (px * 4 if sp.samples > 1 else 0) + tex
+
+
+# ---- the reference's sample shaders --------------------------------------------------------
+# One scene per sample shader pair of the reference (ColorizedTriangle, ComplexMesh,
+# StandardShadowMap, StencilMirror's cup, TextureAndBlending's box), drawn over the displaced grid
+# of the textured-mesh scene (positions, normals, uv) seen by the close camera.  Lights, materials
+# and textures are this module's choices; the shaders are the reference's.
+SAMPLE_SHADERS = ("colorized", "mesh", "ssm", "cup", "box")
+
+
+def _grid_layout(r, vs_id, with_uv=True):
+    elems = [("POSITION", 0, A.FMT_R32G32B32_FLOAT, 0, 0), ("NORMAL", 0, A.FMT_R32G32B32_FLOAT, 0, 12)]
+    if with_uv:
+        elems.append(("TEXCOORD", 0, A.FMT_R32G32_FLOAT, 0, 24))
+    return r.create_input_layout(elems, vs_id)
+
+
+def _linear_sampler(r, tex, addr=A.ADDRESS_WRAP):
+    sd = A.default_sampler_desc(r.lib)
+    sd.min_filter = sd.mag_filter = sd.mip_filter = A.FILTER_LINEAR
+    sd.addr_mode_u = sd.addr_mode_v = sd.addr_mode_w = addr
+    return r.create_sampler(sd, tex)
+
+
+class SampleShaderFrame:
+    """One frame of a sample shader scene: clear, (ssm: shadow map pass,) draw, (MSAA: resolve)."""
+
+    def __init__(self, r: A.Renderer, name, width=320, height=240, samples=1, fmt=A.PF_RGBA8, grid=48, tex_size=64):
+        if name not in SAMPLE_SHADERS:
+            raise ValueError(name)
+        lib = r.lib
+        self.r, self.name, self.width, self.height, self.samples = r, name, width, height, samples
+        ct = r.create_tex2d(width, height, samples, fmt)
+        dt = r.create_tex2d(width, height, samples, A.PF_RG32F)
+        self.color, self.ds = ct.surface(0), dt.surface(0)
+        self.keep = [ct, dt]
+        self.resolved = None
+        if samples > 1:
+            rt = r.create_tex2d(width, height, 1, fmt)
+            self.resolved = rt.surface(0)
+            self.keep.append(rt)
+        verts, idx = grid_mesh(grid)
+        self.n_tris = idx.size // 3
+        self.vb = r.create_buffer_from(verts)
+        self.ib = r.create_buffer_from(idx)
+        self.tex = r.create_tex2d(tex_size, tex_size, 1, A.PF_RGBA8)
+        self.tex.surface(0).write(noise_texture(tex_size, SEED_TEXTURE + 7))
+        self.tex.gen_mipmap(A.FILTER_LINEAR, True)
+        rd = A.default_raster_desc(lib)
+        rd.cull_mode = A.CULL_NONE
+        self.rs = r.create_raster_state(rd)
+        self.dss = r.create_depth_stencil_state(A.default_depth_stencil_desc(lib))
+        eye, target = MESH_CAMERAS["close"]
+        self.view = look_at_lh(eye, target, (0.0, 1.0, 0.0))
+        self.proj = perspective_fov_lh(MESH_FOVY, width / height, 0.1, 100.0)
+        self.wvp = (self.view @ self.proj).astype(np.float32)
+        self.clear_rgba = (0.2, 0.2, 0.5, 1.0)  # StandardShadowMap.cpp:245
+        self.pre = None
+        f32 = np.float32
+        if name == "colorized":  # ColorizedTriangle: three lights in front of the surface
+            lights = np.array([[0.6, 0.6, -0.5, 1.0], [-0.7, 0.1, -0.4, 1.0], [0.0, -0.6, -0.6, 1.0]], f32)
+            self.vs = (A.VS_COLORIZED, np.concatenate([self.wvp.ravel(), lights.ravel()]).astype(f32))
+            self.ps = (A.PS_COLORIZED, None)
+            self.layout = _grid_layout(r, A.VS_COLORIZED, with_uv=False)
+            self.samplers = {}
+            self.blend = A.BLEND_OFF
+        elif name == "mesh":  # ComplexMesh: eye-space Blinn-Phong, light and eye in eye space
+            consts = np.zeros(40, f32)
+            consts[0:16] = self.view.astype(f32).ravel()
+            consts[16:32] = self.proj.astype(f32).ravel()
+            consts[32:35] = (2.0, 2.0, -1.0)
+            consts[36:39] = (0.0, 0.0, 0.0)
+            self.vs = (A.VS_MESH, consts)
+            self.ps = (A.PS_MESH, None)
+            self.layout = _grid_layout(r, A.VS_MESH, with_uv=False)
+            self.samplers = {}
+            self.blend = A.BLEND_OFF
+        elif name == "cup":  # StencilMirror's cup: textured Phong, object-space eye and light
+            consts = np.zeros(24, f32)
+            consts[0:16] = self.wvp.ravel()
+            consts[16:20] = (*MESH_CAMERAS["close"][0], 1.0)
+            consts[20:24] = (1.5, 1.5, -1.5, 1.0)
+            self.vs = (A.VS_CUP, consts)
+            mat = np.array([0.1, 0.1, 0.1, 1.0, 0.7, 0.6, 0.5, 1.0, 0.4, 0.4, 0.4, 1.0], f32)
+            self.ps = (A.PS_CUP, mat)
+            self.layout = _grid_layout(r, A.VS_CUP)
+            self.samplers = {0: _linear_sampler(r, self.tex)}
+            self.blend = A.BLEND_OFF
+        elif name == "box":  # TextureAndBlending's box: tex2d(attr1) at alpha 0.5 blended over the clear
+            self.vs = (A.VS_BOX, self.wvp.ravel())
+            self.ps = (A.PS_BOX, None)
+            self.layout = r.create_input_layout([("POSITION", 0, A.FMT_R32G32B32_FLOAT, 0, 0),
+                                                 ("TEXCOORD", 0, A.FMT_R32G32_FLOAT, 0, 24)], A.VS_BOX)
+            self.samplers = {0: _linear_sampler(r, self.tex)}
+            self.blend = A.BLEND_ON
+        else:  # ssm: StandardShadowMap -- a depth-only pass from the light, then the ESM shader
+            light_eye = np.array([1.2, 1.6, -1.4])
+            lv = look_at_lh(light_eye, (0.0, 0.3, 0.0), (0.0, 1.0, 0.0))
+            lp = perspective_fov_lh(math.pi / 2.5, 1.0, 0.1, 10.0)
+            self.light_wvp = (lv @ lp).astype(f32)
+            self.sm_size = 256
+            smt = r.create_tex2d(self.sm_size, self.sm_size, 1, A.PF_RG32F)
+            self.shadow = smt.surface(0)
+            self.keep.append(smt)
+            sd = A.default_sampler_desc(lib)  # StandardShadowMap.cpp:173-182
+            sd.min_filter = sd.mag_filter = sd.mip_filter = A.FILTER_POINT
+            sd.addr_mode_u = sd.addr_mode_v = sd.addr_mode_w = A.ADDRESS_BORDER
+            for i, c in enumerate((1.0, 0.0, 0.0, 0.0)):
+                sd.border_color[i] = c
+            self.sm_sampler = r.create_sampler(sd, smt)
+            consts = np.zeros(40, f32)
+            consts[0:4] = (*MESH_CAMERAS["close"][0], 1.0)
+            consts[4:20] = self.wvp.ravel()
+            consts[20:24] = (*light_eye, 1.0)
+            consts[24:40] = self.light_wvp.ravel()
+            self.vs = (A.VS_SSM_DRAW, consts)
+            mat = np.zeros(13, f32)  # StandardShadowMap.cpp:256-259
+            mat[0:12] = (0.1, 0.1, 0.1, 0.1, 0.8, 0.8, 0.8, 0.1, 0.4, 0.4, 0.4, 0.1)
+            mat[12] = np.array([32], np.int32).view(f32)[0]
+            self.ps = (A.PS_SSM, mat)
+            self.layout = _grid_layout(r, A.VS_SSM_DRAW)
+            self.samplers = {0: self.sm_sampler, 1: _linear_sampler(r, self.tex, A.ADDRESS_CLAMP)}
+            self.blend = A.BLEND_OFF
+            self.pre = self._shadow_pass
+
+    def _bind(self, vs, ps):
+        r = self.r
+        r.set_input_layout(self.layout)
+        r.set_vertex_buffers(0, [self.vb], [32])
+        r.set_index_buffer(self.ib, A.FMT_R32_UINT)
+        r.set_primitive_topology(A.TOPOLOGY_TRIANGLE_LIST)
+        r.set_vertex_shader(*vs)
+        r.set_pixel_shader(*ps)
+        r.set_rasterizer_state(self.rs)
+        r.set_depth_stencil_state(self.dss, 0)
+
+    def _shadow_pass(self):
+        r = self.r
+        r.set_render_targets([], self.shadow)
+        r.set_viewport(0, 0, self.sm_size, self.sm_size, 0.0, 1.0)
+        r.clear_depth_stencil(self.shadow, A.CLEAR_DEPTH | A.CLEAR_STENCIL, 1.0, 0)
+        consts = np.array(self.vs[1], np.float32).copy()
+        consts[4:20] = self.light_wvp.ravel()  # the draw shader seen from the light (GenSM.savs)
+        self._bind((A.VS_SSM_DRAW, consts), (A.PS_CONSTANT, np.zeros(4, np.float32)))
+        r.set_blend_shader(A.BLEND_OFF)
+        r.draw_index(0, self.n_tris, 0)
+
+    def render(self, clear=True, resolve=True):
+        r = self.r
+        if self.pre is not None:
+            self.pre()
+        r.set_render_targets([self.color], self.ds)
+        r.set_viewport(0, 0, self.width, self.height, 0.0, 1.0)
+        if clear:
+            r.clear_color(self.color, self.clear_rgba)
+            r.clear_depth_stencil(self.ds, A.CLEAR_DEPTH | A.CLEAR_STENCIL, 1.0, 0)
+        self._bind(self.vs, self.ps)
+        for slot in range(4):
+            r.set_ps_sampler(slot, self.samplers.get(slot))
+        r.set_blend_shader(self.blend)
+        r.draw_index(0, self.n_tris, 0)
+        if resolve and self.resolved is not None:
+            r.resolve(self.color, self.resolved)
