@@ -2063,32 +2063,6 @@ __device__ __forceinline__ unsigned long long initial_key(int mode, float d0) {
   return (static_cast<unsigned long long>(zk) << 32) | lo;
 }
 
-// key of triangle `id` at depth z, or ~0 when it cannot pass
-__device__ __forceinline__ unsigned long long sample_key(int mode, const draw_params& P, float z, float d0, uint32_t id) {
-  switch (mode) {
-  case VIS_LESS:
-  case VIS_GREATER: {
-    if (z != z) return ~0ull;
-    uint32_t zk = ord_depth(z);
-    if (mode == VIS_GREATER) zk = ~zk;
-    return (static_cast<unsigned long long>(zk) << 32) | (id + 1u);
-  }
-  case VIS_LEQUAL:
-  case VIS_GEQUAL: {
-    if (z != z) return ~0ull;
-    uint32_t zk = ord_depth(z);
-    if (mode == VIS_GEQUAL) zk = ~zk;
-    return (static_cast<unsigned long long>(zk) << 32) | (0xFFFFFFFEu - id);
-  }
-  case VIS_LATEST: {
-    float od = P.read_depth ? d0 : 0.0f;
-    bool pass = P.depth_enable ? cmpf(P.depth_func, z, od) : true;
-    return pass ? static_cast<unsigned long long>(0xFFFFFFFEu - id) : ~0ull;
-  }
-  default: return ~0ull;
-  }
-}
-
 __device__ __forceinline__ uint32_t key_winner(int mode, unsigned long long k) {
   uint32_t lo = static_cast<uint32_t>(k);
   if (mode == VIS_LESS || mode == VIS_GREATER) return lo == 0u ? NO_WINNER : lo - 1u;
@@ -2124,6 +2098,108 @@ __device__ __forceinline__ void write_winner_depth(const draw_params& P, int mod
   float nd = zp;
   if (S > 1) nd = ((P.sp[smp][0] - 0.5f) * d4.x + (P.sp[smp][1] - 0.5f) * d4.y) + zp;
   *reinterpret_cast<float*>(sample_addr(P.ds, px, py, smp)) = nd;
+}
+
+// cmpf as a truth table over the comparison's outcome: bit 0 less, 1 equal, 2 greater, 3 unordered.
+__device__ __forceinline__ uint32_t cmpf_table(int f) {
+  // nibble f: never, less, equal, less_equal, greater, not_equal, greater_equal, always
+  return (f >= 0 && f < 8) ? (0xF6D43210u >> (4 * f)) & 0xFu : 0xFu;
+}
+__device__ __forceinline__ bool cmpf_by_table(uint32_t table, float a, float b) {
+  const uint32_t o = (a < b) ? 0u : (a == b) ? 1u : (a > b) ? 2u : 3u;
+  return ((table >> o) & 1u) != 0u;
+}
+
+// Phase 1c of vis_body over the chunk's work list: items [0, np) are (record, 4x4 block) pairs the
+// rasteriser tests sample by sample (draw_partial_tile, rasterizer.cpp:298-439), items [np, nt)
+// blocks it accepts whole (draw_full_tile, :285-296).  Sample by sample, the edge products of the
+// block's four columns and four rows are formed once and shared by its 16 pixels -- the same float
+// operations as the reference's per-sample test ((sx + fx) * ex) + ((sy + fy) * ey) < e_value --
+// and every covered sample's key goes to the LDS atomic-min.  Depth per sample is the quad chain of
+// step_2d_unproj_pos_quad (p00 = v0 + (ddx*dx + ddy*dy), +ddx to the right, +ddy below) plus the
+// sample offset term of rasterize_triangle (rasterizer.cpp:678-687).
+// Keys: DEPTH (the LESS, LEQUAL, GREATER, GEQUAL modes with depth writes) = the sample depth's
+// order-preserving bits (complemented when the greater depth wins) over the record's order term,
+// and a NaN depth passes nothing; otherwise (VIS_LATEST) the sample passes against the target's
+// initial depth (always without a depth test) and the key is the order term alone.  The per-sample
+// predicates are combined with bitwise operators so that each sample costs one masked atomic and
+// no branch.
+template <int S, int GT, bool DEPTH>
+__device__ __forceinline__ void vis_items(const draw_params& P, int mode, const float4* __restrict__ recs,
+                                          const uint16_t* __restrict__ work, uint32_t np, uint32_t nt,
+                                          unsigned long long* __restrict__ keys, const float* __restrict__ d0s,
+                                          int ox, int oy, int tw, int th, int tid, int NT) {
+  constexpr int NBS = GT / 4;
+  const uint32_t flip = (mode == VIS_GREATER || mode == VIS_GEQUAL) ? 0xFFFFFFFFu : 0u;
+  const bool later_wins = mode == VIS_LEQUAL || mode == VIS_GEQUAL || mode == VIS_LATEST;
+  const uint32_t table = P.depth_enable ? cmpf_table(P.depth_func) : 0xFu;
+  const bool read_depth = P.read_depth != 0;
+  for (uint32_t wi = tid; wi < nt; wi += NT) {
+    const uint32_t wk = work[wi];
+    const int j = static_cast<int>(wk >> 7), b = static_cast<int>((wk >> 1) & 63u);
+    const float4 r0 = recs[j * 4 + 0], r1 = recs[j * 4 + 1], r2 = recs[j * 4 + 2], r3 = recs[j * 4 + 3];
+    const float e[9] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x};
+    const float z0 = r2.y, zx = r2.z, zy = r2.w, dzx = r3.x, dzy = r3.y;
+    const uint32_t id = __float_as_uint(r3.z);
+    const uint32_t lo = later_wins ? 0xFFFFFFFEu - id : id + 1u;
+    const bool full = wi >= np;
+    float aa[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) aa[s] = (S > 1) ? (P.sp[s][0] - 0.5f) * dzx + (P.sp[s][1] - 0.5f) * dzy : 0.0f;
+    const int bc = b % NBS, br = b / NBS;
+    const int Bx = ox + bc * 4, By = oy + br * 4;
+    const int lbase = (br * 4 * GT + bc * 4) * S;  // the block's first sample in `keys`
+    const int bw = min(4, tw - Bx), bh = min(4, th - By);  // columns and rows inside the target
+    // pixel depths of the block (row r, column c at [r * 4 + c]), quad by quad
+    float zp[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float dx = (0.5f + static_cast<float>(Bx + (q & 1) * 2)) - zx;
+      const float dy = (0.5f + static_cast<float>(By + (q >> 1) * 2)) - zy;
+      const int p0 = (q >> 1) * 8 + (q & 1) * 2;
+      zp[p0] = z0 + ((dzx * dx) + (dzy * dy));
+      zp[p0 + 1] = zp[p0] + dzx;
+      zp[p0 + 4] = zp[p0] + dzy;
+      zp[p0 + 5] = zp[p0 + 1] + dzy;
+    }
+    float evp[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      evp[k] = e[k * 3 + 2] - ((static_cast<float>(Bx) * e[k * 3]) + (static_cast<float>(By) * e[k * 3 + 1]));
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      // the edge products every row and column of the block shares for this sample
+      float ax[3][4], ay[3][4];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          ax[k][c] = (P.sp[s][0] + static_cast<float>(c)) * e[k * 3];
+          ay[k][c] = (P.sp[s][1] + static_cast<float>(c)) * e[k * 3 + 1];
+        }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const bool covered = full | (!((ax[0][c] + ay[0][r]) < evp[0]) & !((ax[1][c] + ay[1][r]) < evp[1]) &
+                                       !((ax[2][c] + ay[2][r]) < evp[2]));
+          const float nd = (S > 1) ? (aa[s] + zp[r * 4 + c]) : zp[r * 4 + c];
+          const int li = lbase + (r * GT + c) * S + s;
+          unsigned long long k;
+          bool pass;
+          if (DEPTH) {
+            const uint32_t u = __float_as_uint(nd + 0.0f);  // -0 orders as +0
+            const uint32_t ord = u ^ (static_cast<uint32_t>(static_cast<int32_t>(u) >> 31) | 0x80000000u);
+            k = (static_cast<unsigned long long>(ord ^ flip) << 32) | lo;
+            pass = nd == nd;
+          } else {
+            k = lo;
+            pass = cmpf_by_table(table, nd, read_depth ? d0s[li] : 0.0f);
+          }
+          if (covered & pass & (c < bw) & (r < bh)) atomicMin(&keys[li], k);
+        }
+    }
+  }
 }
 
 // One 16x16 GPU tile per workgroup, in three phases:
@@ -2237,7 +2313,6 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
   const uint32_t tr0 = tstart[gtile], tr1 = tend[gtile];
   const uint32_t t0 = pass == 1 ? tr1 : min(tr1, tr0 + part * VIS_PART);
   const uint32_t t1 = (pass == 1 || entries == nullptr) ? tr1 : min(tr1, t0 + VIS_PART);
-  const uint32_t full_px = (S == 1) ? 1u : (S == 2 ? 3u : 15u);
   const int jx0 = (ox - t64x) >> 4, jy0 = (oy - t64y) >> 4;  // first stage-16 region of the tile
   // an 8x8 tile covers blocks [qx, qx+2) x [qy, qy+2) of its region; larger tiles whole regions
   const int qx = GT < 16 ? ((ox - t64x) & 15) >> 2 : 0, qy = GT < 16 ? ((oy - t64y) & 15) >> 2 : 0;
@@ -2381,8 +2456,12 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
       rs[2] = make_float4(e[8], d3.w, d3.y, d3.z);          // z0, zx, zy
       rs[3] = make_float4(d4.x, d4.y, __uint_as_float(id), 0.0f);  // dzx, dzy, id
     }
-    // phase 1b: work list of (record, block) pairs, spread evenly over the lanes
-    const uint32_t cnt = GT > 16 ? static_cast<uint32_t>(__popcll(static_cast<unsigned long long>(bm))) : __popc(static_cast<uint32_t>(bm));
+    // phase 1b: work list of (record, block) pairs spread evenly over the lanes: blocks tested
+    // sample by sample first, then blocks accepted whole, so each wave runs one kind
+    const bmask_t pm = bm & ~fm;
+    const uint32_t np = GT > 16 ? static_cast<uint32_t>(__popcll(static_cast<unsigned long long>(pm))) : __popc(static_cast<uint32_t>(pm));
+    const uint32_t nf = GT > 16 ? static_cast<uint32_t>(__popcll(static_cast<unsigned long long>(fm))) : __popc(static_cast<uint32_t>(fm));
+    const uint32_t cnt = np | (nf << 16);  // both sums stay below 2^16 (NT * NBS^2 <= 16384)
     const uint32_t incl = wave_incl_scan(cnt);
     if ((tid & 63) == 63) wsum[tid >> 6] = incl;
     __syncthreads();
@@ -2393,68 +2472,25 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
       if (w < (tid >> 6)) off += wsum[w];
       total += wsum[w];
     }
-    while (bm) {
-      const uint32_t b = static_cast<uint32_t>(__builtin_ctzll(static_cast<unsigned long long>(bm)));
-      bm &= bm - 1;
-      work[off++] = static_cast<uint16_t>((tid << 7) | (b << 1) | static_cast<uint32_t>((fm >> b) & 1u));
+    const uint32_t total_p = total & 0xFFFFu, total_all = total_p + (total >> 16);
+    uint32_t off_p = off & 0xFFFFu, off_f = total_p + (off >> 16);
+    bmask_t pr = pm, fr = fm;
+    while (pr) {
+      const uint32_t b = static_cast<uint32_t>(__builtin_ctzll(static_cast<unsigned long long>(pr)));
+      pr &= pr - 1;
+      work[off_p++] = static_cast<uint16_t>((tid << 7) | (b << 1));
+    }
+    while (fr) {
+      const uint32_t b = static_cast<uint32_t>(__builtin_ctzll(static_cast<unsigned long long>(fr)));
+      fr &= fr - 1;
+      work[off_f++] = static_cast<uint16_t>((tid << 7) | (b << 1) | 1u);
     }
     __syncthreads();
-    // phase 1c: one 4x4 block per lane: per-sample coverage (draw_partial_tile, rasterizer.cpp:298-439),
-    // quad depth (step_2d_unproj_pos_quad) and the key atomic-min
-    for (uint32_t wi = tid; wi < total; wi += NT) {
-      const uint32_t wk = work[wi];
-      const int j = static_cast<int>(wk >> 7), b = static_cast<int>((wk >> 1) & 63u);
-      const bool block_full = (wk & 1u) != 0;
-      const float4 r0 = recs[j * 4 + 0], r1 = recs[j * 4 + 1], r2 = recs[j * 4 + 2], r3 = recs[j * 4 + 3];
-      const float e[9] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x};
-      const float z0 = r2.y, zx = r2.z, zy = r2.w, dzx = r3.x, dzy = r3.y;
-      const uint32_t id = __float_as_uint(r3.z);
-      float aa[S];
-#pragma unroll
-      for (int s = 0; s < S; ++s) aa[s] = (S > 1) ? (P.sp[s][0] - 0.5f) * dzx + (P.sp[s][1] - 0.5f) * dzy : 0.0f;
-      const int Bx = ox + (b % NBS) * 4, By = oy + (b / NBS) * 4;
-      float evp[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        evp[k] = e[k * 3 + 2] - ((static_cast<float>(Bx) * e[k * 3]) + (static_cast<float>(By) * e[k * 3 + 1]));
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int qx0 = Bx + (q & 1) * 2, qy0 = By + (q >> 1) * 2;
-        const float dx = (0.5f + static_cast<float>(qx0)) - zx;
-        const float dy = (0.5f + static_cast<float>(qy0)) - zy;
-        float z[4];
-        z[0] = z0 + ((dzx * dx) + (dzy * dy));
-        z[1] = z[0] + dzx;
-        z[2] = z[0] + dzy;
-        z[3] = z[1] + dzy;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int px = qx0 + (i & 1), py = qy0 + (i >> 1);
-          if (px >= tw || py >= th) continue;
-          uint32_t m = full_px;
-          if (!block_full) {
-            m = 0;
-            const float fx = static_cast<float>(px - Bx), fy = static_cast<float>(py - By);
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-              float mx = P.sp[s][0] + fx, my = P.sp[s][1] + fy;
-              bool r = false;
-#pragma unroll
-              for (int k = 0; k < 3; ++k) r |= ((mx * e[k * 3]) + (my * e[k * 3 + 1])) < evp[k];
-              if (!r) m |= 1u << s;
-            }
-          }
-          const int li = ((py - oy) * GT + (px - ox)) * S;
-#pragma unroll
-          for (int s = 0; s < S; ++s) {
-            if (!(m & (1u << s))) continue;
-            float nd = (S > 1) ? (aa[s] + z[i]) : z[i];
-            unsigned long long k = sample_key(mode, P, nd, d0s[li + s], id);
-            if (k != ~0ull) atomicMin(&keys[li + s], k);
-          }
-        }
-      }
-    }
+    // phase 1c: one (record, block) item per lane: coverage, depth and the key atomic-min
+    if (depth_keys)
+      vis_items<S, GT, true>(P, mode, recs, work, total_p, total_all, keys, d0s, ox, oy, tw, th, tid, NT);
+    else if (mode == VIS_LATEST)
+      vis_items<S, GT, false>(P, mode, recs, work, total_p, total_all, keys, d0s, ox, oy, tw, th, tid, NT);
   }
   __syncthreads();
   if (pass == 0 && parts > 1) {  // one part of a split tile: leave the partial keys for the merge
