@@ -495,6 +495,7 @@ extern "C" {
 int32_t srx_batch_begin(srx_renderer rr) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);
   if (!r || r->recording) return SRX_INVALID_PARAMETER;
+  RCHK(run_pending_clears(r));  // clears issued before the batch run before it
   r->recording = new batch_obj();
   return SRX_OK;
 }
@@ -523,6 +524,7 @@ int32_t srx_batch_end(srx_renderer rr, srx_batch* out) {
 int32_t srx_batch_launch(srx_batch bb) {
   auto* b = as<batch_obj>(bb, K_BATCH);
   if (!b || !b->exec) return SRX_INVALID_PARAMETER;
+  RCHK(run_pending_clears(b->r));  // clears issued before the launch run before it
   HIPCHK(hipGraphLaunch(b->exec, b->r->stream));
   return SRX_OK;
 }
@@ -542,6 +544,7 @@ int32_t srx_batch_configure(srx_batch bb, int32_t tile, int32_t bin_items, int32
 int32_t srx_batch_autotune(srx_batch bb, uint32_t reps, srx_batch_tuning* results, size_t capacity, size_t* count) {
   auto* b = as<batch_obj>(bb, K_BATCH);
   if (!b || reps == 0) return SRX_INVALID_PARAMETER;
+  RCHK(run_pending_clears(b->r));
   hipStream_t s = b->r->stream;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIPCHK(hipEventCreate(&e0));

@@ -19,7 +19,9 @@ enum stat_slot {
 };
 // Setup counters
 // CT_WIDE: triangles whose records the clip launch counts by wave (their ids in wide_list)
-enum counter_slot { CT_CLIP_PRIMS = 0, CT_CLIP_SLOTS, CT_CLIP_OVERFLOW, CT_TOTAL, CT_WIDE, CT_COUNT };
+// CT_CLASS: GPU tiles per record-count class (16 slots), CT_CURSOR: their placement cursors
+enum counter_slot { CT_CLIP_PRIMS = 0, CT_CLIP_SLOTS, CT_CLIP_OVERFLOW, CT_TOTAL, CT_WIDE, CT_CLASS, CT_CURSOR = CT_CLASS + 16,
+                    CT_COUNT = CT_CURSOR + 16 };
 
 // Per-draw device buffers (owned by the renderer, grown on demand).
 struct draw_buffers {
@@ -117,13 +119,21 @@ void launch_clip(const draw_params& P, const draw_buffers& B, hipStream_t s);
 // In-place exclusive scan of n u32; writes the total to *total_dev.  tmp holds >= scan_tmp_size(n).
 size_t scan_tmp_size(uint32_t n);
 void launch_exclusive_scan(uint32_t* data, uint32_t n, uint32_t* tmp, uint32_t* total_dev, hipStream_t s);
+// One-launch exclusive scan of n u32 (decoupled look-back); writes the total to *total_dev when
+// given.  state holds scan_state_words(n) u32 words, zero at launch.
+size_t scan_state_words(uint32_t n);
+void launch_scan_onepass(uint32_t* data, uint32_t n, uint32_t* state, uint32_t* total_dev, hipStream_t s);
+// Zeroes the draw's counters and nstate words of scan state.
+void launch_draw_begin(uint32_t* counters, uint32_t* state, size_t nstate, hipStream_t s);
 void launch_emit(const draw_params& P, const draw_buffers& B, uint32_t capacity, hipStream_t s);
 // Stable LSD radix sort of (keys, vals) by the low `bits` bits of keys; returns the buffer index
 // (0 or 1) that holds the result.
 int launch_sort(const draw_buffers& B, uint32_t n, int bits, hipStream_t s);
 // Sorts the GPU tiles by record count, heaviest first, into O.vals[result]; O supplies keys/vals
 // scratch (>= tiles each) and shares hist/scan_tmp with B.
-void launch_tile_order(const draw_params& P, const draw_buffers& B, uint32_t* order, hipStream_t s);
+// GPU tile scheduling order, heaviest record-count class first.  `counted`: the bucket sort's range
+// pass has already counted the tiles per class.
+void launch_tile_order(const draw_params& P, const draw_buffers& B, uint32_t* order, bool counted, hipStream_t s);
 // One-pass (unstable within a tile) bucket sort of the n emitted records by GPU tile into vals[1],
 // with the tile ranges; false when the tile count exceeds its LDS histogram (then radix-sort).
 bool launch_bucket_sort(const draw_params& P, const draw_buffers& B, uint32_t n, hipStream_t s);

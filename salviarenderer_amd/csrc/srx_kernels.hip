@@ -982,6 +982,133 @@ __global__ void __launch_bounds__(256) scan_add_kernel(uint32_t* data, uint32_t 
   }
 }
 
+// ---- single-pass exclusive scan (decoupled look-back, Merrill & Garland 2016)
+// Each workgroup takes the next tile id from a counter, scans its SCAN_BLOCK elements and
+// publishes their sum, then adds its predecessors' published words, walking back 64 tiles at a
+// time, until one of them carries an inclusive prefix; it publishes its own inclusive prefix and
+// writes its elements.  Tile ids follow the order in which workgroups start, so a workgroup only
+// waits for tiles whose workgroups are already running.  state (u64): [0] the tile counter,
+// [1 + t] tile t's word, flag in the high half (1: sum of the tile, 2: inclusive prefix) and the
+// value in the low half; flag and value travel in one atomic word.  The state must be zero at
+// launch: scan_state_words(n) u32 words.
+constexpr unsigned long long SCAN_FLAG_SUM = 1ull << 32, SCAN_FLAG_INCL = 2ull << 32;
+
+size_t scan_state_words(uint32_t n) { return 2 * ((static_cast<size_t>(n) + SCAN_BLOCK - 1) / SCAN_BLOCK + 2); }
+
+__device__ __forceinline__ unsigned long long scan_state_load(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void scan_state_store(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void __launch_bounds__(256) scan_onepass_kernel(uint32_t* data, uint32_t n, unsigned long long* state,
+                                                          uint32_t* total) {
+  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t s_tile, s_prefix;
+  if (threadIdx.x == 0) s_tile = atomicAdd(reinterpret_cast<unsigned int*>(state), 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint64_t base = static_cast<uint64_t>(tile) * SCAN_BLOCK + threadIdx.x * SCAN_PER_THREAD;
+  const bool vec = (static_cast<uint64_t>(tile) + 1) * SCAN_BLOCK <= n && (reinterpret_cast<uintptr_t>(data) & 15) == 0;
+  uint32_t v[SCAN_PER_THREAD];
+  uint32_t t = 0;
+  if (vec) {
+    const uint4* d4 = reinterpret_cast<const uint4*>(data + base);
+#pragma unroll
+    for (int i = 0; i < SCAN_PER_THREAD / 4; ++i) {
+      const uint4 q = d4[i];
+      v[4 * i] = q.x;
+      v[4 * i + 1] = q.y;
+      v[4 * i + 2] = q.z;
+      v[4 * i + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < SCAN_PER_THREAD; ++i) v[i] = (base + i < n) ? data[base + i] : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < SCAN_PER_THREAD; ++i) t += v[i];
+  const uint32_t incl = wave_incl_scan(t);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  uint32_t wprefix = 0, tile_sum = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    if (w < wave) wprefix += wsum[w];
+    tile_sum += wsum[w];
+  }
+  if (wave == 0) {
+    uint32_t prefix = 0;
+    if (tile == 0) {
+      if (lane == 0) scan_state_store(state + 1, SCAN_FLAG_INCL | tile_sum);
+    } else {
+      if (lane == 0) scan_state_store(state + 1 + tile, SCAN_FLAG_SUM | tile_sum);
+      int64_t j = static_cast<int64_t>(tile) - 1 - lane;  // lane l reads tile - 1 - l
+      while (true) {
+        // before tile 0 the prefix is an inclusive zero
+        const unsigned long long w = j >= 0 ? scan_state_load(state + 1 + j) : SCAN_FLAG_INCL;
+        const uint32_t flag = static_cast<uint32_t>(w >> 32);
+        const unsigned long long incl_lanes = __ballot(flag == 2u);
+        const unsigned long long wait_lanes = __ballot(flag == 0u);
+        // lanes 0..stop are needed: up to the nearest inclusive prefix, else all 64
+        const int stop = incl_lanes ? __builtin_ctzll(incl_lanes) : 63;
+        const unsigned long long needed = stop == 63 ? ~0ull : ((2ull << stop) - 1ull);
+        if (wait_lanes & needed) {
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        uint32_t x = lane <= stop ? static_cast<uint32_t>(w) : 0u;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), off, 64));
+        prefix += x;
+        if (incl_lanes) break;
+        j -= 64;
+      }
+      if (lane == 0) scan_state_store(state + 1 + tile, SCAN_FLAG_INCL | (prefix + tile_sum));
+    }
+    if (lane == 0) s_prefix = prefix;
+  }
+  __syncthreads();
+  const uint32_t prefix = s_prefix;
+  uint32_t run = prefix + wprefix + incl - t;
+#pragma unroll
+  for (int i = 0; i < SCAN_PER_THREAD; ++i) {
+    const uint32_t x = v[i];
+    v[i] = run;
+    run += x;
+  }
+  if (vec) {
+    uint4* d4 = reinterpret_cast<uint4*>(data + base);
+#pragma unroll
+    for (int i = 0; i < SCAN_PER_THREAD / 4; ++i) d4[i] = make_uint4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < SCAN_PER_THREAD; ++i)
+      if (base + i < n) data[base + i] = v[i];
+  }
+  if (total && tile == gridDim.x - 1 && threadIdx.x == 0) *total = prefix + tile_sum;
+}
+
+void launch_scan_onepass(uint32_t* data, uint32_t n, uint32_t* state, uint32_t* total_dev, hipStream_t s) {
+  const uint32_t nb = std::max<uint32_t>(static_cast<uint32_t>((static_cast<uint64_t>(n) + SCAN_BLOCK - 1) / SCAN_BLOCK), 1u);
+  hipLaunchKernelGGL(scan_onepass_kernel, dim3(nb), dim3(256), 0, s, data, n,
+                     reinterpret_cast<unsigned long long*>(state), total_dev);
+}
+
+// Start of a draw: zero the counters and the state of the count scan.
+__global__ void draw_begin_kernel(uint32_t* counters, uint32_t ncounters, uint32_t* state, uint32_t nstate) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ncounters) counters[i] = 0u;
+  if (i < nstate) state[i] = 0u;
+}
+void launch_draw_begin(uint32_t* counters, uint32_t* state, size_t nstate, hipStream_t s) {
+  const uint32_t n = static_cast<uint32_t>(std::max<size_t>(nstate, CT_COUNT));
+  hipLaunchKernelGGL(draw_begin_kernel, dim3((n + 255) / 256), dim3(256), 0, s, counters, static_cast<uint32_t>(CT_COUNT),
+                     state, static_cast<uint32_t>(nstate));
+}
+
 size_t scan_tmp_size(uint32_t n) {
   size_t total = 0;
   uint64_t m = n;
@@ -1140,8 +1267,11 @@ __device__ __forceinline__ void bucket_hist_body(const uint32_t* __restrict__ ke
   const uint32_t col = bucket_column(bx, nblocks);
   for (uint32_t d = threadIdx.x; d < T; d += 256) hist[d * nblocks + col] = cnt[d];
 }
+// also zeroes the state of the one-pass scan of the histogram that follows
 __global__ void __launch_bounds__(256) bucket_hist_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t T,
-                                                          uint32_t nblocks, uint32_t* __restrict__ hist) {
+                                                          uint32_t nblocks, uint32_t* __restrict__ hist,
+                                                          uint32_t* __restrict__ state, uint32_t nstate) {
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < nstate; i += gridDim.x * 256) state[i] = 0u;
   bucket_hist_body<BUCKET_PER_THREAD>(keys, n, T, nblocks, hist, blockIdx.x);
 }
 
@@ -1184,6 +1314,32 @@ __global__ void __launch_bounds__(256) bucket_scatter_kernel(const uint32_t* __r
   bucket_scatter_body<BUCKET_PER_THREAD>(keys, vals, n, T, nblocks, hist_scanned, vout, n, blockIdx.x);
 }
 
+// atomicAdd(&counters[key], 1) for every active lane, one atomic per distinct key of the wave;
+// returns the lane's old value plus its rank among the lanes with the same key.
+__device__ __forceinline__ uint32_t wave_counted_add(uint32_t* counters, uint32_t key) {
+  const int lane = __lane_id();
+  uint32_t result = 0;
+  while (true) {
+    const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
+    const unsigned long long m = __ballot(key == k0);
+    if (key == k0) {
+      const uint32_t rank = static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+      uint32_t old = 0;
+      if (rank == 0) old = atomicAdd(&counters[k0], static_cast<uint32_t>(__popcll(m)));
+      result = static_cast<uint32_t>(__shfl(static_cast<int>(old), __builtin_ctzll(m), 64)) + rank;
+      break;
+    }
+  }
+  return result;
+}
+// Record-count class of a GPU tile for the scheduling order: class 11 - floor(log2 n) (heaviest
+// first, 2048 records and more in class 0), class 15 for an empty tile.
+__device__ __forceinline__ uint32_t tile_class(uint32_t n) {
+  if (n == 0) return 15u;
+  const int lg = 31 - __clz(n);
+  return static_cast<uint32_t>(min(max(11 - lg, 0), 14));
+}
+
 // tile t's records are [scanned[t * nblocks], scanned[(t + 1) * nblocks]) (`end` for the last tile)
 __device__ __forceinline__ void bucket_range_body(const uint32_t* __restrict__ hist_scanned, uint32_t end, uint32_t T,
                                                   uint32_t nblocks, uint32_t* __restrict__ ts, uint32_t* __restrict__ te,
@@ -1192,9 +1348,15 @@ __device__ __forceinline__ void bucket_range_body(const uint32_t* __restrict__ h
   ts[t] = hist_scanned[t * nblocks];
   te[t] = (t + 1 < T) ? hist_scanned[(t + 1) * nblocks] : end;
 }
+// also counts the tiles per record-count class (counters[CT_CLASS + class])
 __global__ void bucket_range_kernel(const uint32_t* __restrict__ hist_scanned, uint32_t n, uint32_t T, uint32_t nblocks,
-                                    uint32_t* __restrict__ ts, uint32_t* __restrict__ te) {
-  bucket_range_body(hist_scanned, n, T, nblocks, ts, te, blockIdx.x * blockDim.x + threadIdx.x);
+                                    uint32_t* __restrict__ ts, uint32_t* __restrict__ te, uint32_t* __restrict__ counters) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  bucket_range_body(hist_scanned, n, T, nblocks, ts, te, t);
+  if (t < T) {
+    const uint32_t a = hist_scanned[t * nblocks], b = (t + 1 < T) ? hist_scanned[(t + 1) * nblocks] : n;
+    (void)wave_counted_add(counters + CT_CLASS, tile_class(b - a));
+  }
 }
 
 uint32_t bucket_hist_size(uint32_t T, uint32_t n) {
@@ -1205,11 +1367,13 @@ bool launch_bucket_sort(const draw_params& P, const draw_buffers& B, uint32_t n,
   const uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
   if (T == 0 || T > BUCKET_MAX_TILES) return false;
   const uint32_t nblocks = std::max<uint32_t>((n + BUCKET_ITEMS - 1) / BUCKET_ITEMS, 1);
-  hipLaunchKernelGGL(bucket_hist_kernel, dim3(nblocks), dim3(256), 0, s, B.keys[0], n, T, nblocks, B.hist);
-  launch_exclusive_scan(B.hist, T * nblocks, B.scan_tmp, nullptr, s);
+  hipLaunchKernelGGL(bucket_hist_kernel, dim3(nblocks), dim3(256), 0, s, B.keys[0], n, T, nblocks, B.hist, B.scan_tmp,
+                     static_cast<uint32_t>(scan_state_words(T * nblocks)));
+  launch_scan_onepass(B.hist, T * nblocks, B.scan_tmp, nullptr, s);
   if (n) hipLaunchKernelGGL(bucket_scatter_kernel, dim3(nblocks), dim3(256), 0, s, B.keys[0], B.vals[0], n, T, nblocks, B.hist,
                             B.vals[1]);
-  hipLaunchKernelGGL(bucket_range_kernel, dim3((T + 255) / 256), dim3(256), 0, s, B.hist, n, T, nblocks, B.tstart, B.tend);
+  hipLaunchKernelGGL(bucket_range_kernel, dim3((T + 255) / 256), dim3(256), 0, s, B.hist, n, T, nblocks, B.tstart, B.tend,
+                     B.counters);
   return true;
 }
 
@@ -1239,29 +1403,54 @@ __device__ __forceinline__ uint32_t tile_bucket(const uint32_t* ts, const uint32
 __device__ __forceinline__ void tile_order_body(const uint32_t* __restrict__ ts, const uint32_t* __restrict__ te,
                                                 uint32_t T, uint32_t* __restrict__ order) {
   __shared__ uint32_t cnt[256];
-  if (threadIdx.x < 256) cnt[threadIdx.x] = 0;
+  __shared__ uint32_t wtot[4];
+  const uint32_t tid = threadIdx.x;
+  if (tid < 256) cnt[tid] = 0;
   __syncthreads();
-  for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) atomicAdd(&cnt[tile_bucket(ts, te, t)], 1u);
+  for (uint32_t t = tid; t < T; t += blockDim.x) (void)wave_counted_add(cnt, tile_bucket(ts, te, t));
   __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t sum = 0;
-    for (int b = 0; b < 256; ++b) {
-      uint32_t c = cnt[b];
-      cnt[b] = sum;
-      sum += c;
-    }
+  // exclusive scan of the 256 bucket counts by the first four waves
+  uint32_t c = 0, incl = 0;
+  if (tid < 256) {
+    c = cnt[tid];
+    incl = wave_incl_scan(c);
+    if ((tid & 63) == 63) wtot[tid >> 6] = incl;
   }
   __syncthreads();
-  for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) order[atomicAdd(&cnt[tile_bucket(ts, te, t)], 1u)] = t;
+  if (tid < 256) {
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < (tid >> 6); ++w) before += wtot[w];
+    cnt[tid] = before + incl - c;
+  }
+  __syncthreads();
+  for (uint32_t t = tid; t < T; t += blockDim.x) order[wave_counted_add(cnt, tile_bucket(ts, te, t))] = t;
 }
 __global__ void __launch_bounds__(1024) tile_order_kernel(const uint32_t* __restrict__ ts, const uint32_t* __restrict__ te,
                                                           uint32_t T, uint32_t* __restrict__ order) {
   tile_order_body(ts, te, T, order);
 }
 
-void launch_tile_order(const draw_params& P, const draw_buffers& B, uint32_t* order, hipStream_t s) {
-  uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
-  if (T) hipLaunchKernelGGL(tile_order_kernel, dim3(1), dim3(1024), 0, s, B.tstart, B.tend, T, order);
+__global__ void tile_class_count_kernel(const uint32_t* __restrict__ ts, const uint32_t* __restrict__ te, uint32_t T,
+                                        uint32_t* __restrict__ counters) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < T) (void)wave_counted_add(counters + CT_CLASS, tile_class(te[t] - ts[t]));
+}
+// Tiles of class c take the positions after every tile of the classes before c; within a class the
+// order is the atomics' (tiles are independent, so the order never changes a pixel).
+__global__ void tile_class_order_kernel(const uint32_t* __restrict__ ts, const uint32_t* __restrict__ te, uint32_t T,
+                                        uint32_t* __restrict__ counters, uint32_t* __restrict__ order) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  const uint32_t c = tile_class(te[t] - ts[t]);
+  uint32_t base = 0;
+  for (uint32_t k = 0; k < c; ++k) base += counters[CT_CLASS + k];
+  order[base + wave_counted_add(counters + CT_CURSOR, c)] = t;
+}
+void launch_tile_order(const draw_params& P, const draw_buffers& B, uint32_t* order, bool counted, hipStream_t s) {
+  const uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
+  if (!T) return;
+  if (!counted) hipLaunchKernelGGL(tile_class_count_kernel, dim3((T + 255) / 256), dim3(256), 0, s, B.tstart, B.tend, T, B.counters);
+  hipLaunchKernelGGL(tile_class_order_kernel, dim3((T + 255) / 256), dim3(256), 0, s, B.tstart, B.tend, T, B.counters, order);
 }
 
 // records per workgroup beyond which a tile of a depth-only draw is split

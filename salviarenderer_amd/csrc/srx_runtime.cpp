@@ -4,6 +4,8 @@
 // async renderer queue, async_renderer.cpp:17-110); srx_flush() waits for it.  Resources live
 // in HBM in the reference's linear layout ((y*w + x)*samples + s)*elem_size (surface.cpp:277-295),
 // so map/read-back bytes are identical.  State validation follows renderer_impl.cpp.
+#include <cstring>
+
 #include "srx_runtime_internal.h"
 
 namespace srx_rt {
@@ -147,7 +149,10 @@ int32_t build_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t primc
     sd.size[0] = t->size[0];
     sd.size[1] = t->size[1];
     sd.size[2] = t->size[2];
-    for (int l = 0; l < sd.nlevels; ++l) sd.levels[l] = t->surfs[l]->desc();
+    for (int l = 0; l < sd.nlevels; ++l) {
+      RCHK(run_pending_clear(t->surfs[l].get(), r->stream));
+      sd.levels[l] = t->surfs[l]->desc();
+    }
     bool rgba8 = sd.nlevels > 0;
     for (int l = 0; l < sd.nlevels; ++l) rgba8 &= sd.levels[l].pf == SRX_PF_RGBA8 && sd.levels[l].samples == 1;
     sd.fast_aniso = rgba8 && sd.mip_filter == SRX_FILTER_ANISOTROPIC && sd.max_aniso > 1 &&
@@ -195,7 +200,8 @@ static size_t record_scratch_bytes(uint32_t P3, uint32_t T, int samples, bool vi
   const uint32_t nblocks = (std::max(n, T) + 4095) / 4096;
   const size_t hist_n = std::max<size_t>(static_cast<size_t>(256) * std::max<uint32_t>(nblocks, 1), bucket_hist_size(T, n));
   size_t b = 2 * align256(rn * 4) + 2 * align256(rn * 4) + align256(hist_n * 4) +
-             align256(std::max(scan_tmp_size(P3), scan_tmp_size(static_cast<uint32_t>(hist_n))) * 4) +
+             align256(std::max({scan_tmp_size(P3), scan_tmp_size(static_cast<uint32_t>(hist_n)), scan_state_words(P3),
+                                scan_state_words(static_cast<uint32_t>(hist_n))}) * 4) +
              align256(std::max<uint32_t>(T, 1) * 4);
   if (vis_tables) {
     uint32_t ne, nm, nsl;
@@ -215,6 +221,8 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   if (skip) return SRX_OK;
   if (r->recording) {
     if (fixed) return batch_refuse(r, SRX_E_UNSUPPORTED, "a batch records allocating draws only");
+    for (auto& ct : r->st.color_targets) RCHK(run_pending_clear(ct.get(), r->stream));
+    RCHK(run_pending_clear(r->st.ds_target.get(), r->stream));
     return record_draw(r, P, SM);
   }
   const int nout = P.nattr;
@@ -245,7 +253,7 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   RCHK(X.take(r->clip_flag, primcnt, B.clip_flag));
   RCHK(X.take(r->clip_slot, primcnt, B.clip_slot));
   RCHK(X.take(r->wide, primcnt, B.wide_list));
-  RCHK(X.take(r->scan_tmp, scan_tmp_size(P3), B.scan_tmp));
+  RCHK(X.take(r->scan_tmp, std::max(scan_tmp_size(P3), scan_state_words(P3)), B.scan_tmp));
   RCHK(X.take(r->partial, std::max<size_t>(T, (primcnt + 255) / 256) + 1, B.partial));
   RCHK(X.take(r->tstart, std::max<uint32_t>(T, 1), B.tstart));
   RCHK(X.take(r->tend, std::max<uint32_t>(T, 1), B.tend));
@@ -297,21 +305,35 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   auto mark = [&](int i) {
     if (prof) (void)hipEventRecord(de.ev[i], s);
   };
-  // sampler table: staged in pinned memory; the stream syncs below before the staging is reused
-  *r->samplers_h = SM;
-  HIPCHK(hipMemcpyAsync(samplers_d, r->samplers_h, sizeof(ps_samplers), hipMemcpyHostToDevice, s));
-  launch_fill_u32(B.counters, CT_COUNT, 0u, s);
+  // sampler table: staged in pinned memory (the counter read-back below waits for the copy before the
+  // staging is reused); a renderer's own table is uploaded only when it changed
+  const bool same_table = !fixed && r->samplers_valid && samplers_d == r->samplers_last_d &&
+                          std::memcmp(&r->samplers_last, &SM, sizeof(ps_samplers)) == 0;
+  if (!same_table) {
+    *r->samplers_h = SM;
+    HIPCHK(hipMemcpyAsync(samplers_d, r->samplers_h, sizeof(ps_samplers), hipMemcpyHostToDevice, s));
+    r->samplers_last = SM;
+    r->samplers_last_d = samplers_d;
+    r->samplers_valid = !fixed;
+  }
+  launch_draw_begin(B.counters, B.scan_tmp, scan_state_words(P3), s);
   mark(0);
   launch_vs(P, B.vbuf, s);
   mark(1);
   launch_setup(P, B, s);
   launch_clip(P, B, s);  // returns at once on the device when setup flagged no primitive
   mark(2);
-  launch_exclusive_scan(B.cnt, P3, B.scan_tmp, B.counters + CT_TOTAL, s);
+  launch_scan_onepass(B.cnt, P3, B.scan_tmp, B.counters + CT_TOTAL, s);
   mark(3);
+  // the counters are final after the scan: their read-back overlaps the emission
+  HIPCHK(hipEventRecord(r->ev_counted, s));
   launch_emit(P, B, capacity, s);
+  // the targets' pending clears run here, while the record count travels to the host
+  for (auto& ct : r->st.color_targets) RCHK(run_pending_clear(ct.get(), s));
+  RCHK(run_pending_clear(r->st.ds_target.get(), s));
+  RCHK(start_counter_read(r));
   uint32_t ctr[CT_COUNT];
-  RCHK(read_counters(r, ctr));
+  RCHK(finish_counter_read(r, ctr));
   if (ctr[CT_CLIP_OVERFLOW]) return fail(SRX_OUT_OF_MEMORY, "clip buffer overflow");
   const uint32_t total = ctr[CT_TOTAL];
   if (fixed && total > capacity)
@@ -334,9 +356,12 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   RCHK(X.take(r->keys1, std::max<uint32_t>(sized, 1), B.keys[1]));
   RCHK(X.take(r->vals1, std::max<uint32_t>(sized, 1), B.vals[1]));
   RCHK(X.take(r->hist, hist_n, B.hist));
-  RCHK(X.take(r->scan_tmp, std::max(scan_tmp_size(P3), scan_tmp_size(static_cast<uint32_t>(hist_n))), B.scan_tmp));
+  RCHK(X.take(r->scan_tmp, std::max({scan_tmp_size(P3), scan_tmp_size(static_cast<uint32_t>(hist_n)), scan_state_words(P3),
+                                      scan_state_words(static_cast<uint32_t>(hist_n))}),
+              B.scan_tmp));
   int which = 1;
-  if (!(order_free && launch_bucket_sort(P, B, total, s))) {
+  const bool bucketed = order_free && launch_bucket_sort(P, B, total, s);
+  if (!bucketed) {
     int bits = 1;
     while ((1u << bits) < T) ++bits;
     which = launch_sort(B, total, bits, s);
@@ -344,7 +369,7 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   }
   uint32_t* order = nullptr;
   RCHK(X.take(r->order, std::max<uint32_t>(T, 1), order));
-  launch_tile_order(P, B, order, s);
+  launch_tile_order(P, B, order, bucketed, s);
   B.order = order;
   mark(4);
   if (vis_tables) {
@@ -368,7 +393,9 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
 
 renderer_obj::~renderer_obj() {
   drop_recording(this);
+  if (stream) (void)run_pending_clears(this);
   if (stream) (void)hipStreamSynchronize(stream);
+  if (side) (void)hipStreamSynchronize(side);
 
   auto fr = [](auto& a) {
     if (a.p) (void)hipFree(a.p);
@@ -384,6 +411,9 @@ renderer_obj::~renderer_obj() {
       if (e) (void)hipEventDestroy(e);
   for (auto e : event_pool) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
+  if (side) (void)hipStreamDestroy(side);
+  if (ev_counted) (void)hipEventDestroy(ev_counted);
+  if (ev_read) (void)hipEventDestroy(ev_read);
 }
 
 
@@ -419,6 +449,43 @@ int32_t resolve_query(renderer_obj* r, query_obj* q) {
   }
   q->draws.clear();
   q->resolved = true;
+  return SRX_OK;
+}
+
+// ---- pending clears
+static void forget_pending(renderer_obj* r, surface_obj* o) {
+  if (!r) return;
+  auto& v = r->pending;
+  for (size_t i = 0; i < v.size(); ++i)
+    if (v[i].get() == o) {
+      v.erase(v.begin() + static_cast<std::ptrdiff_t>(i));
+      return;
+    }
+}
+static void set_pending(renderer_obj* r, surface_obj* o, int kind) {
+  if (o->pend.owner != r) {
+    forget_pending(o->pend.owner, o);
+    r->pending.emplace_back(o);
+  }
+  o->pend.kind = kind;
+  o->pend.owner = r;
+}
+int32_t run_pending_clear(surface_obj* o, hipStream_t s, bool drop) {
+  if (!o || !o->pend.kind) return SRX_OK;
+  renderer_obj* owner = o->pend.owner;
+  const pending_clear c = o->pend;
+  o->pend = pending_clear{};
+  ref<surface_obj> keep(o);  // the owner's list may hold the last reference
+  forget_pending(owner, o);
+  if (drop) return SRX_OK;
+  if (!s) s = owner->stream;
+  if (c.kind == 1) launch_clear(o->desc(), c.color, s);
+  else launch_clear_ds(o->desc(), c.flags, c.depth, c.stencil, s);
+  HIPCHK(hipGetLastError());
+  return SRX_OK;
+}
+int32_t run_pending_clears(renderer_obj* r) {
+  while (!r->pending.empty()) RCHK(run_pending_clear(r->pending.back().get(), r->stream));
   return SRX_OK;
 }
 
@@ -472,11 +539,14 @@ int32_t srx_create_renderer(uint32_t kind, srx_renderer* out) {
   }
   hipError_t e = hipGetDevice(&r->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&r->stream, hipStreamDefault);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&r->ev_counted, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&r->ev_read, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&r->counters.p, CT_COUNT * sizeof(uint32_t));
   if (e == hipSuccess) r->counters.cap = CT_COUNT;
   if (e == hipSuccess) e = hipMalloc(&r->stats_d, ST_COUNT * sizeof(unsigned long long));
   if (e == hipSuccess) e = hipMemset(r->stats_d, 0, ST_COUNT * sizeof(unsigned long long));
-  if (e == hipSuccess) e = hipHostMalloc(&r->pinned, 64, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc(&r->pinned, CT_COUNT * sizeof(uint32_t), hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc(&r->samplers_h, sizeof(ps_samplers), hipHostMallocDefault);
   if (e != hipSuccess) {
     delete r;
@@ -561,6 +631,7 @@ int32_t srx_texture_gen_mipmap(srx_texture t, int32_t filter, int32_t auto_gen) 
     o->min_lod = n - 1;
   }
   o->surfs.resize(1);
+  RCHK(run_pending_clear(o->surfs[0].get(), nullptr));  // the mip kernels wait for the issuing stream
   for (int l = o->max_lod; l < o->min_lod; ++l) {
     surface_obj* src = o->surfs.back().get();
     surface_obj* dst = nullptr;
@@ -589,12 +660,14 @@ size_t srx_surface_size_bytes(srx_surface s) { auto* o = as<surface_obj>(s, K_SU
 int32_t srx_surface_write(srx_surface s, const void* d, size_t n) {
   auto* o = as<surface_obj>(s, K_SURFACE);
   if (!o || n != o->bytes || !d) return SRX_INVALID_PARAMETER;
+  RCHK(run_pending_clear(o, nullptr, true));  // overwritten whole
   HIPCHK(hipMemcpy(o->d, d, n, hipMemcpyHostToDevice));
   return SRX_OK;
 }
 int32_t srx_surface_read(srx_surface s, void* d, size_t n) {
   auto* o = as<surface_obj>(s, K_SURFACE);
   if (!o || n != o->bytes || !d) return SRX_INVALID_PARAMETER;
+  RCHK(run_pending_clear(o, nullptr));  // on the issuing renderer's stream, which the copy waits for
   HIPCHK(hipMemcpy(d, o->d, n, hipMemcpyDeviceToHost));
   return SRX_OK;
 }
@@ -603,6 +676,8 @@ int32_t srx_surface_resolve(srx_renderer rr, srx_surface src, srx_surface dst) {
   auto* a = as<surface_obj>(src, K_SURFACE);
   auto* b = as<surface_obj>(dst, K_SURFACE);
   if (!r || !a || !b || b->samples != 1 || a->w != b->w || a->h != b->h) return SRX_INVALID_PARAMETER;
+  RCHK(run_pending_clear(a, r->stream));
+  RCHK(run_pending_clear(b, r->stream));
   if (r->recording) return record_resolve(r, a, b);
   launch_resolve(a->desc(), b->desc(), r->stream);
   HIPCHK(hipGetLastError());
@@ -898,6 +973,7 @@ int32_t srx_map_surface(srx_renderer rr, srx_mapped_resource* m, srx_surface ss,
   auto* r = as<renderer_obj>(rr, K_RENDERER);
   auto* o = as<surface_obj>(ss, K_SURFACE);
   if (!r || !m || !o) return SRX_INVALID_PARAMETER;
+  RCHK(run_pending_clear(o, r->stream));
   const uint32_t pitch = static_cast<uint32_t>(static_cast<size_t>(o->w) * o->samples * pf_bytes(o->pf));
   RCHK(map_common(r, m, o->d, o->bytes, mode, pitch, pitch * static_cast<uint32_t>(o->h)));
   r->mapped_surf = ref<surface_obj>(o);
@@ -1000,9 +1076,14 @@ int32_t srx_clear_color(srx_renderer rr, srx_surface s, const float* c) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);
   auto* o = as<surface_obj>(s, K_SURFACE);
   if (!r || !o || !c) return SRX_INVALID_PARAMETER;
-  if (r->recording) return record_fill(r, o, make_float4(c[0], c[1], c[2], c[3]));
-  launch_clear(o->desc(), make_float4(c[0], c[1], c[2], c[3]), r->stream);
-  HIPCHK(hipGetLastError());
+  if (r->recording) {
+    RCHK(run_pending_clear(o, r->stream));
+    return record_fill(r, o, make_float4(c[0], c[1], c[2], c[3]));
+  }
+  // a full colour clear supersedes whatever clear is still pending
+  if (o->pend.kind == 2) RCHK(run_pending_clear(o, r->stream));
+  set_pending(r, o, 1);
+  o->pend.color = make_float4(c[0], c[1], c[2], c[3]);
   return SRX_OK;
 }
 int32_t srx_clear_depth_stencil(srx_renderer rr, srx_surface s, uint32_t flags, float d, uint32_t st) {
@@ -1016,15 +1097,22 @@ int32_t srx_clear_depth_stencil(srx_renderer rr, srx_surface s, uint32_t flags, 
       return batch_refuse(r, SRX_E_UNSUPPORTED, "batched depth/stencil clears set both");
     float sf;
     std::memcpy(&sf, &st, sizeof sf);
+    RCHK(run_pending_clear(o, r->stream));
     return record_fill(r, o, make_float4(d, sf, 0.0f, 0.0f));
   }
-  launch_clear_ds(o->desc(), flags, d, st, r->stream);
-  HIPCHK(hipGetLastError());
+  // merged with a pending depth/stencil clear: each channel keeps its latest clear value
+  if (o->pend.kind == 1) RCHK(run_pending_clear(o, r->stream));
+  const uint32_t had = o->pend.kind == 2 ? o->pend.flags : 0u;
+  set_pending(r, o, 2);
+  o->pend.flags = had | flags;
+  if (flags & SRX_CLEAR_DEPTH) o->pend.depth = d;
+  if (flags & SRX_CLEAR_STENCIL) o->pend.stencil = st;
   return SRX_OK;
 }
 int32_t srx_flush(srx_renderer rr) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);
   if (!r) return SRX_INVALID_PARAMETER;
+  RCHK(run_pending_clears(r));
   HIPCHK(hipStreamSynchronize(r->stream));
   return SRX_OK;
 }

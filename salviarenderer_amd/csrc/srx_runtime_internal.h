@@ -93,10 +93,24 @@ struct buffer_obj : object {
   }
 };
 
+struct renderer_obj;
+// A full clear issued by srx_clear_color / srx_clear_depth_stencil that has not run yet: it runs
+// right before the next command that reads or writes the surface (a draw queues its targets'
+// clears behind its record emission, where the GPU would otherwise wait for the record count).
+struct pending_clear {
+  int kind = 0;  // 0 none, 1 colour, 2 depth/stencil
+  float4 color{};
+  uint32_t flags = 0;  // depth/stencil: SRX_CLEAR_DEPTH | SRX_CLEAR_STENCIL
+  float depth = 0.0f;
+  uint32_t stencil = 0;
+  renderer_obj* owner = nullptr;  // the issuing renderer (its stream runs the clear without a requester)
+};
+
 struct surface_obj : object {
   int w, h, samples, pf;
   uint8_t* d = nullptr;
   size_t bytes = 0;
+  pending_clear pend;
   surface_obj(int w_, int h_, int s_, int pf_) : object(K_SURFACE), w(w_), h(h_), samples(s_), pf(pf_) {}
   ~surface_obj() override {
     if (d) (void)hipFree(d);
@@ -343,6 +357,10 @@ struct renderer_obj : object {
   dev_array<float4> vbuf, tri_rec, tri_attr, clip_verts;
   dev_array<ps_samplers> samplers_d;
   ps_samplers* samplers_h = nullptr;  // pinned staging of the per-draw sampler table
+  // the table last uploaded to samplers_d by an allocating draw (the no-allocation region is shared)
+  ps_samplers samplers_last{};
+  const ps_samplers* samplers_last_d = nullptr;
+  bool samplers_valid = false;
   dev_array<uint32_t> cnt, keys0, keys1, vals0, vals1, tstart, tend, hist, scan_tmp, clip_hdr, counters, vis;
   dev_array<uint32_t> order;  // GPU tile scheduling order
   dev_array<uint32_t> wide;   // triangles counted by wave
@@ -353,7 +371,11 @@ struct renderer_obj : object {
   dev_array<int32_t> clip_slot;
   dev_array<unsigned long long> partial;
   unsigned long long* stats_d = nullptr;
-  uint32_t* pinned = nullptr;  // host staging for counter read-back
+  uint32_t* pinned = nullptr;  // host staging for counter read-back (CT_COUNT words)
+  std::vector<ref<surface_obj>> pending;  // surfaces holding a pending clear issued here
+  // the counters are read back on `side` (default priority) while the main stream emits records
+  hipStream_t side = nullptr;
+  hipEvent_t ev_counted = nullptr, ev_read = nullptr;
   uint32_t last_records = 0;   // binned records of the previous draw (sizes the next emission)
   uint64_t host_stats[ST_COUNT] = {};
   // map/unmap (resource_manager.cpp:7-113): one mapped resource at a time, staged in pinned memory
@@ -413,11 +435,17 @@ inline int gpu_tile_size() {
   return g;
 }
 
-// Reads the draw's device counters (waits for the queued work).
-inline int32_t read_counters(renderer_obj* r, uint32_t* out) {
-  HIPCHK(hipMemcpyAsync(r->pinned, r->counters.p, CT_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, r->stream));
-  HIPCHK(hipStreamSynchronize(r->stream));
-  for (int i = 0; i < CT_COUNT; ++i) out[i] = r->pinned[i];
+// Starts the read-back of the draw's device counters as they stood at ev_counted (recorded on the
+// main stream), on the side stream, so that work queued after ev_counted overlaps it.
+inline int32_t start_counter_read(renderer_obj* r) {
+  HIPCHK(hipStreamWaitEvent(r->side, r->ev_counted, 0));
+  HIPCHK(hipMemcpyAsync(r->pinned, r->counters.p, CT_CLASS * sizeof(uint32_t), hipMemcpyDeviceToHost, r->side));
+  HIPCHK(hipEventRecord(r->ev_read, r->side));
+  return SRX_OK;
+}
+inline int32_t finish_counter_read(renderer_obj* r, uint32_t* out) {
+  HIPCHK(hipEventSynchronize(r->ev_read));
+  for (int i = 0; i < CT_COUNT; ++i) out[i] = i < CT_CLASS ? r->pinned[i] : 0u;
   return SRX_OK;
 }
 
@@ -482,6 +510,11 @@ int32_t build_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t primc
                    ps_samplers& SM, bool& skip);
 // srx_batch.cpp: commands recorded between srx_batch_begin and srx_batch_end
 int32_t record_draw(renderer_obj* r, const draw_params& P, const ps_samplers& SM);
+// Queues o's pending clear on stream s (its issuing renderer's stream when s is null); drop: the
+// surface is about to be overwritten whole, so the clear is discarded instead.
+int32_t run_pending_clear(surface_obj* o, hipStream_t s, bool drop = false);
+// Queues every pending clear renderer r issued, on r's stream.
+int32_t run_pending_clears(renderer_obj* r);
 int32_t record_fill(renderer_obj* r, surface_obj* o, float4 v);
 int32_t record_resolve(renderer_obj* r, surface_obj* a, surface_obj* d);
 int32_t batch_refuse(renderer_obj* r, int32_t code, const char* msg);  // the first recording error sticks
