@@ -2301,10 +2301,10 @@ __device__ __forceinline__ bool cmpf_by_table(uint32_t table, float a, float b) 
 
 // Phase 1c of vis_body over the chunk's work list: items [0, np) are (record, 4x4 block) pairs the
 // rasteriser tests sample by sample (draw_partial_tile, rasterizer.cpp:298-439), items [np, nt)
-// blocks it accepts whole (draw_full_tile, :285-296).  Sample by sample, the edge products of the
-// block's four columns and four rows are formed once and shared by its 16 pixels -- the same float
-// operations as the reference's per-sample test ((sx + fx) * ex) + ((sy + fy) * ey) < e_value --
-// and every covered sample's key goes to the LDS atomic-min.  Depth per sample is the quad chain of
+// blocks it accepts whole (draw_full_tile, :285-296).  A partial item tests, pixel by pixel, only
+// the pixels a conservative bound leaves (see below), each sample with the reference's expression
+// ((sx + fx) * ex) + ((sy + fy) * ey) < e_value, and every covered sample's key goes to the LDS
+// atomic-min.  Depth per sample is the quad chain of
 // step_2d_unproj_pos_quad (p00 = v0 + (ddx*dx + ddy*dy), +ddx to the right, +ddy below) plus the
 // sample offset term of rasterize_triangle (rasterizer.cpp:678-687).
 // Keys: DEPTH (the LESS, LEQUAL, GREATER, GEQUAL modes with depth writes) = the sample depth's
@@ -2323,6 +2323,15 @@ __device__ __forceinline__ void vis_items(const draw_params& P, int mode, const 
   const bool later_wins = mode == VIS_LEQUAL || mode == VIS_GEQUAL || mode == VIS_LATEST;
   const uint32_t table = P.depth_enable ? cmpf_table(P.depth_func) : 0xFu;
   const bool read_depth = P.read_depth != 0;
+  // extreme sample positions inside a pixel
+  float spx_lo = P.sp[0][0], spx_hi = P.sp[0][0], spy_lo = P.sp[0][1], spy_hi = P.sp[0][1];
+#pragma unroll
+  for (int s = 1; s < S; ++s) {
+    spx_lo = fminf(spx_lo, P.sp[s][0]);
+    spx_hi = fmaxf(spx_hi, P.sp[s][0]);
+    spy_lo = fminf(spy_lo, P.sp[s][1]);
+    spy_hi = fmaxf(spy_hi, P.sp[s][1]);
+  }
   for (uint32_t wi = tid; wi < nt; wi += NT) {
     const uint32_t wk = work[wi];
     const int j = static_cast<int>(wk >> 7), b = static_cast<int>((wk >> 1) & 63u);
@@ -2339,41 +2348,61 @@ __device__ __forceinline__ void vis_items(const draw_params& P, int mode, const 
     const int Bx = ox + bc * 4, By = oy + br * 4;
     const int lbase = (br * 4 * GT + bc * 4) * S;  // the block's first sample in `keys`
     const int bw = min(4, tw - Bx), bh = min(4, th - By);  // columns and rows inside the target
-    // pixel depths of the block (row r, column c at [r * 4 + c]), quad by quad
-    float zp[16];
+    // depth at the top-left pixel of each 2x2 quad of the block
+    float zq[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const float dx = (0.5f + static_cast<float>(Bx + (q & 1) * 2)) - zx;
       const float dy = (0.5f + static_cast<float>(By + (q >> 1) * 2)) - zy;
-      const int p0 = (q >> 1) * 8 + (q & 1) * 2;
-      zp[p0] = z0 + ((dzx * dx) + (dzy * dy));
-      zp[p0 + 1] = zp[p0] + dzx;
-      zp[p0 + 4] = zp[p0] + dzy;
-      zp[p0 + 5] = zp[p0 + 1] + dzy;
+      zq[q] = z0 + ((dzx * dx) + (dzy * dy));
     }
     float evp[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
       evp[k] = e[k * 3 + 2] - ((static_cast<float>(Bx) * e[k * 3]) + (static_cast<float>(By) * e[k * 3 + 1]));
+    if (!full) {
+      // Pixels that may hold a covered sample.  Float rounding is monotonic, so an edge's value at
+      // any sample of a pixel is at most its value at the pixel's extreme sample column and row
+      // for that edge's signs, computed with the same operations; a pixel whose extreme value
+      // fails an edge has no sample that passes it.  Only the remaining pixels are tested sample
+      // by sample.
+      uint32_t M = 0;
+      {
+        float cx[3][4], cy[3][4];
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-      // the edge products every row and column of the block shares for this sample
-      float ax[3][4], ay[3][4];
+        for (int k = 0; k < 3; ++k) {
+          const float sx = e[k * 3] >= 0.0f ? spx_hi : spx_lo, sy = e[k * 3 + 1] >= 0.0f ? spy_hi : spy_lo;
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          ax[k][c] = (P.sp[s][0] + static_cast<float>(c)) * e[k * 3];
-          ay[k][c] = (P.sp[s][1] + static_cast<float>(c)) * e[k * 3 + 1];
+          for (int c = 0; c < 4; ++c) {
+            cx[k][c] = (sx + static_cast<float>(c)) * e[k * 3];
+            cy[k][c] = (sy + static_cast<float>(c)) * e[k * 3 + 1];
+          }
         }
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
+        for (int r = 0; r < 4; ++r)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const bool covered = full | (!((ax[0][c] + ay[0][r]) < evp[0]) & !((ax[1][c] + ay[1][r]) < evp[1]) &
-                                       !((ax[2][c] + ay[2][r]) < evp[2]));
-          const float nd = (S > 1) ? (aa[s] + zp[r * 4 + c]) : zp[r * 4 + c];
-          const int li = lbase + (r * GT + c) * S + s;
+          for (int c = 0; c < 4; ++c) {
+            const bool maybe = !((cx[0][c] + cy[0][r]) < evp[0]) & !((cx[1][c] + cy[1][r]) < evp[1]) &
+                               !((cx[2][c] + cy[2][r]) < evp[2]) & (c < bw) & (r < bh);
+            if (maybe) M |= 1u << (r * 4 + c);
+          }
+      }
+      while (M) {
+        const int px = __builtin_ctz(M);
+        M &= M - 1u;
+        const int r = px >> 2, c = px & 3;
+        const int q = ((r >> 1) << 1) | (c >> 1);
+        float z = (q & 2) ? ((q & 1) ? zq[3] : zq[2]) : ((q & 1) ? zq[1] : zq[0]);
+        z = (c & 1) ? z + dzx : z;
+        z = (r & 1) ? z + dzy : z;
+        const float fx = static_cast<float>(c), fy = static_cast<float>(r);
+        const int lp = lbase + (r * GT + c) * S;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          const float mx = P.sp[s][0] + fx, my = P.sp[s][1] + fy;
+          const bool covered = !(((mx * e[0]) + (my * e[1])) < evp[0]) & !(((mx * e[3]) + (my * e[4])) < evp[1]) &
+                               !(((mx * e[6]) + (my * e[7])) < evp[2]);
+          const float nd = (S > 1) ? (aa[s] + z) : z;
           unsigned long long k;
           bool pass;
           if (DEPTH) {
@@ -2383,11 +2412,44 @@ __device__ __forceinline__ void vis_items(const draw_params& P, int mode, const 
             pass = nd == nd;
           } else {
             k = lo;
+            pass = cmpf_by_table(table, nd, read_depth ? d0s[lp + s] : 0.0f);
+          }
+          if (covered & pass) atomicMin(&keys[lp + s], k);
+        }
+      }
+      continue;
+    }
+    // a block the rasteriser accepts whole: every sample inside the target
+    float zp[16];  // pixel depths (row r, column c at [r * 4 + c]) by the quad chain
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int p0 = (q >> 1) * 8 + (q & 1) * 2;
+      zp[p0] = zq[q];
+      zp[p0 + 1] = zq[q] + dzx;
+      zp[p0 + 4] = zq[q] + dzy;
+      zp[p0 + 5] = zp[p0 + 1] + dzy;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          const float nd = (S > 1) ? (aa[s] + zp[r * 4 + c]) : zp[r * 4 + c];
+          const int li = lbase + (r * GT + c) * S + s;
+          unsigned long long k;
+          bool pass;
+          if (DEPTH) {
+            const uint32_t u = __float_as_uint(nd + 0.0f);
+            const uint32_t ord = u ^ (static_cast<uint32_t>(static_cast<int32_t>(u) >> 31) | 0x80000000u);
+            k = (static_cast<unsigned long long>(ord ^ flip) << 32) | lo;
+            pass = nd == nd;
+          } else {
+            k = lo;
             pass = cmpf_by_table(table, nd, read_depth ? d0s[li] : 0.0f);
           }
-          if (covered & pass & (c < bw) & (r < bh)) atomicMin(&keys[li], k);
+          if (pass & (c < bw) & (r < bh)) atomicMin(&keys[li], k);
         }
-    }
   }
 }
 
