@@ -45,6 +45,7 @@ struct draw_buffers {
   uint32_t* vis_counts;   // entries, merges, partial slots
   unsigned long long* vis_partial;  // partial per-sample keys of split tiles
   uint32_t* tstart;    // GPU tiles
+  uint32_t* tile_count;  // 2 * GPU tiles: records per tile, then the scatter's per-tile cursors
   uint32_t* tend;
   uint32_t* hist;      // radix histogram 256 * blocks
   uint32_t* scan_tmp;  // scan partial sums
@@ -124,7 +125,7 @@ void launch_exclusive_scan(uint32_t* data, uint32_t n, uint32_t* tmp, uint32_t* 
 size_t scan_state_words(uint32_t n);
 void launch_scan_onepass(uint32_t* data, uint32_t n, uint32_t* state, uint32_t* total_dev, hipStream_t s);
 // Zeroes the draw's counters and nstate words of scan state.
-void launch_draw_begin(uint32_t* counters, uint32_t* state, size_t nstate, hipStream_t s);
+void launch_draw_begin(uint32_t* counters, uint32_t* state, size_t nstate, uint32_t* tiles, size_t ntiles, hipStream_t s);
 void launch_emit(const draw_params& P, const draw_buffers& B, uint32_t capacity, hipStream_t s);
 // Stable LSD radix sort of (keys, vals) by the low `bits` bits of keys; returns the buffer index
 // (0 or 1) that holds the result.
@@ -136,7 +137,11 @@ int launch_sort(const draw_buffers& B, uint32_t n, int bits, hipStream_t s);
 void launch_tile_order(const draw_params& P, const draw_buffers& B, uint32_t* order, bool counted, hipStream_t s);
 // One-pass (unstable within a tile) bucket sort of the n emitted records by GPU tile into vals[1],
 // with the tile ranges; false when the tile count exceeds its LDS histogram (then radix-sort).
-bool launch_bucket_sort(const draw_params& P, const draw_buffers& B, uint32_t n, hipStream_t s);
+// Counting sort of the records by GPU tile (order-independent draws) into vals[1] with the tile ranges
+// in tstart/tend, sized for `cap` records; the kernels read the record count from
+// B.counters[CT_TOTAL] (at most cap are sorted).  B.tile_count must be zero.
+bool bucket_sort_applicable(const draw_params& P);
+bool launch_bucket_sort(const draw_params& P, const draw_buffers& B, uint32_t cap, hipStream_t s);
 uint32_t bucket_hist_size(uint32_t T, uint32_t n);
 // Whether launch_vis will take the draw (its visibility mode exists and the tiles are 16x16).
 bool vis_applicable(const draw_params& P);

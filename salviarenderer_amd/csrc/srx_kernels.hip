@@ -1097,16 +1097,19 @@ void launch_scan_onepass(uint32_t* data, uint32_t n, uint32_t* state, uint32_t* 
                      reinterpret_cast<unsigned long long*>(state), total_dev);
 }
 
-// Start of a draw: zero the counters and the state of the count scan.
-__global__ void draw_begin_kernel(uint32_t* counters, uint32_t ncounters, uint32_t* state, uint32_t nstate) {
+// Start of a draw: zero the counters, the state of the count scan and the per-tile record counts and
+// cursors of the binning.
+__global__ void draw_begin_kernel(uint32_t* counters, uint32_t ncounters, uint32_t* state, uint32_t nstate,
+                                  uint32_t* tiles, uint32_t ntiles) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < ncounters) counters[i] = 0u;
   if (i < nstate) state[i] = 0u;
+  if (i < ntiles) tiles[i] = 0u;
 }
-void launch_draw_begin(uint32_t* counters, uint32_t* state, size_t nstate, hipStream_t s) {
-  const uint32_t n = static_cast<uint32_t>(std::max<size_t>(nstate, CT_COUNT));
+void launch_draw_begin(uint32_t* counters, uint32_t* state, size_t nstate, uint32_t* tiles, size_t ntiles, hipStream_t s) {
+  const uint32_t n = static_cast<uint32_t>(std::max<size_t>(std::max<size_t>(nstate, ntiles), CT_COUNT));
   hipLaunchKernelGGL(draw_begin_kernel, dim3((n + 255) / 256), dim3(256), 0, s, counters, static_cast<uint32_t>(CT_COUNT),
-                     state, static_cast<uint32_t>(nstate));
+                     state, static_cast<uint32_t>(nstate), tiles, static_cast<uint32_t>(ntiles));
 }
 
 size_t scan_tmp_size(uint32_t n) {
@@ -1267,13 +1270,6 @@ __device__ __forceinline__ void bucket_hist_body(const uint32_t* __restrict__ ke
   const uint32_t col = bucket_column(bx, nblocks);
   for (uint32_t d = threadIdx.x; d < T; d += 256) hist[d * nblocks + col] = cnt[d];
 }
-// also zeroes the state of the one-pass scan of the histogram that follows
-__global__ void __launch_bounds__(256) bucket_hist_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t T,
-                                                          uint32_t nblocks, uint32_t* __restrict__ hist,
-                                                          uint32_t* __restrict__ state, uint32_t nstate) {
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < nstate; i += gridDim.x * 256) state[i] = 0u;
-  bucket_hist_body<BUCKET_PER_THREAD>(keys, n, T, nblocks, hist, blockIdx.x);
-}
 
 // Only the tiles present in the workgroup's records fetch their (strided) offsets.  Positions at
 // or beyond `limit` (the output's size) are dropped.
@@ -1306,12 +1302,6 @@ __device__ __forceinline__ void bucket_scatter_body(const uint32_t* __restrict__
       const uint32_t pos = atomicAdd(&off[k[j]], 1u);
       if (pos < limit) vout[pos] = v[j];
     }
-}
-__global__ void __launch_bounds__(256) bucket_scatter_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                                             uint32_t n, uint32_t T, uint32_t nblocks,
-                                                             const uint32_t* __restrict__ hist_scanned,
-                                                             uint32_t* __restrict__ vout) {
-  bucket_scatter_body<BUCKET_PER_THREAD>(keys, vals, n, T, nblocks, hist_scanned, vout, n, blockIdx.x);
 }
 
 // atomicAdd(&counters[key], 1) for every active lane, one atomic per distinct key of the wave;
@@ -1348,32 +1338,121 @@ __device__ __forceinline__ void bucket_range_body(const uint32_t* __restrict__ h
   ts[t] = hist_scanned[t * nblocks];
   te[t] = (t + 1 < T) ? hist_scanned[(t + 1) * nblocks] : end;
 }
-// also counts the tiles per record-count class (counters[CT_CLASS + class])
-__global__ void bucket_range_kernel(const uint32_t* __restrict__ hist_scanned, uint32_t n, uint32_t T, uint32_t nblocks,
-                                    uint32_t* __restrict__ ts, uint32_t* __restrict__ te, uint32_t* __restrict__ counters) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  bucket_range_body(hist_scanned, n, T, nblocks, ts, te, t);
-  if (t < T) {
-    const uint32_t a = hist_scanned[t * nblocks], b = (t + 1 < T) ? hist_scanned[(t + 1) * nblocks] : n;
-    (void)wave_counted_add(counters + CT_CLASS, tile_class(b - a));
-  }
-}
 
 uint32_t bucket_hist_size(uint32_t T, uint32_t n) {
   return T * std::max<uint32_t>((n + BUCKET_ITEMS - 1) / BUCKET_ITEMS, 1);
 }
 
-bool launch_bucket_sort(const draw_params& P, const draw_buffers& B, uint32_t n, hipStream_t s) {
+// Counting sort of one draw's records by GPU tile (order-independent draws): the order within a tile
+// is immaterial, so each workgroup of BUCKET_ITEMS records counts its tiles in LDS, noting the
+// tiles it touches, and adds the counts of those tiles to the per-tile totals; one workgroup turns
+// the totals into tile ranges; the scatter counts again, reserves its range in each touched tile
+// with one atomic and places its records at the LDS ranks.  tile_count and tile_cursor are zero
+// at the start of the draw.
+__device__ __forceinline__ void tile_count_lds(const uint32_t* __restrict__ keys, uint32_t n, uint32_t T, uint32_t b0,
+                                               uint32_t (&k)[BUCKET_PER_THREAD], uint32_t (&rank)[BUCKET_PER_THREAD],
+                                               uint32_t* cnt, uint16_t* touched, uint32_t* ntouched) {
+  for (uint32_t d = threadIdx.x; d < T; d += 256) cnt[d] = 0;
+  if (threadIdx.x == 0) *ntouched = 0;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < BUCKET_PER_THREAD; ++j) {
+    const uint32_t i = b0 + j * 256 + threadIdx.x;
+    k[j] = i < n ? keys[i] : 0xFFFFFFFFu;
+  }
+#pragma unroll
+  for (int j = 0; j < BUCKET_PER_THREAD; ++j)
+    if (k[j] != 0xFFFFFFFFu) {
+      rank[j] = atomicAdd(&cnt[k[j]], 1u);
+      if (rank[j] == 0) touched[atomicAdd(ntouched, 1u)] = static_cast<uint16_t>(k[j]);
+    }
+  __syncthreads();
+}
+__global__ void __launch_bounds__(256) tile_count_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ count,
+                                                         uint32_t cap, uint32_t T, uint32_t* __restrict__ tile_count) {
+  __shared__ uint32_t cnt[BUCKET_MAX_TILES];
+  __shared__ uint16_t touched[BUCKET_ITEMS];
+  __shared__ uint32_t ntouched;
+  const uint32_t n = min(*count, cap);
+  const uint32_t b0 = blockIdx.x * BUCKET_ITEMS;
+  if (b0 >= n) return;
+  uint32_t k[BUCKET_PER_THREAD], rank[BUCKET_PER_THREAD];
+  tile_count_lds(keys, n, T, b0, k, rank, cnt, touched, &ntouched);
+  for (uint32_t i = threadIdx.x; i < ntouched; i += 256) {
+    const uint32_t t = touched[i];
+    atomicAdd(&tile_count[t], cnt[t]);
+  }
+}
+// One workgroup: tile ranges from the totals (exclusive prefix), and the tiles per record-count class.
+__global__ void __launch_bounds__(1024) tile_prefix_kernel(const uint32_t* __restrict__ tile_count, uint32_t T,
+                                                          uint32_t* __restrict__ ts, uint32_t* __restrict__ te,
+                                                          uint32_t* __restrict__ counters) {
+  constexpr int PER = BUCKET_MAX_TILES / 1024;
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t classes[16];
+  const uint32_t tid = threadIdx.x, t0 = tid * PER;
+  if (tid < 16) classes[tid] = 0;
+  uint32_t c[PER], sum = 0;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    c[i] = t0 + i < T ? tile_count[t0 + i] : 0u;
+    sum += c[i];
+  }
+  const uint32_t incl = wave_incl_scan(sum);
+  if ((tid & 63) == 63) wsum[tid >> 6] = incl;
+  __syncthreads();
+  uint32_t run = incl - sum;
+  for (uint32_t w = 0; w < (tid >> 6); ++w) run += wsum[w];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    if (t0 + i < T) {
+      ts[t0 + i] = run;
+      te[t0 + i] = run + c[i];
+    }
+    run += c[i];
+  }
+#pragma unroll
+  for (int i = 0; i < PER; ++i)
+    if (t0 + i < T) (void)wave_counted_add(classes, tile_class(c[i]));
+  __syncthreads();
+  if (tid < 16) counters[CT_CLASS + tid] = classes[tid];
+}
+__global__ void __launch_bounds__(256) tile_scatter_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                           const uint32_t* __restrict__ count, uint32_t cap, uint32_t T,
+                                                           const uint32_t* __restrict__ ts, uint32_t* __restrict__ tile_cursor,
+                                                           uint32_t* __restrict__ vout) {
+  __shared__ uint32_t cnt[BUCKET_MAX_TILES];
+  __shared__ uint16_t touched[BUCKET_ITEMS];
+  __shared__ uint32_t ntouched;
+  const uint32_t n = min(*count, cap);
+  const uint32_t b0 = blockIdx.x * BUCKET_ITEMS;
+  if (b0 >= n) return;
+  uint32_t k[BUCKET_PER_THREAD], rank[BUCKET_PER_THREAD];
+  tile_count_lds(keys, n, T, b0, k, rank, cnt, touched, &ntouched);
+  // this workgroup's range in each touched tile; cnt[t] becomes its first position
+  for (uint32_t i = threadIdx.x; i < ntouched; i += 256) {
+    const uint32_t t = touched[i];
+    cnt[t] = ts[t] + atomicAdd(&tile_cursor[t], cnt[t]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < BUCKET_PER_THREAD; ++j)
+    if (k[j] != 0xFFFFFFFFu) vout[cnt[k[j]] + rank[j]] = vals[b0 + j * 256 + threadIdx.x];
+}
+
+bool bucket_sort_applicable(const draw_params& P) {
   const uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
-  if (T == 0 || T > BUCKET_MAX_TILES) return false;
-  const uint32_t nblocks = std::max<uint32_t>((n + BUCKET_ITEMS - 1) / BUCKET_ITEMS, 1);
-  hipLaunchKernelGGL(bucket_hist_kernel, dim3(nblocks), dim3(256), 0, s, B.keys[0], n, T, nblocks, B.hist, B.scan_tmp,
-                     static_cast<uint32_t>(scan_state_words(T * nblocks)));
-  launch_scan_onepass(B.hist, T * nblocks, B.scan_tmp, nullptr, s);
-  if (n) hipLaunchKernelGGL(bucket_scatter_kernel, dim3(nblocks), dim3(256), 0, s, B.keys[0], B.vals[0], n, T, nblocks, B.hist,
-                            B.vals[1]);
-  hipLaunchKernelGGL(bucket_range_kernel, dim3((T + 255) / 256), dim3(256), 0, s, B.hist, n, T, nblocks, B.tstart, B.tend,
-                     B.counters);
+  return T > 0 && T <= BUCKET_MAX_TILES;
+}
+bool launch_bucket_sort(const draw_params& P, const draw_buffers& B, uint32_t cap, hipStream_t s) {
+  const uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
+  if (!bucket_sort_applicable(P)) return false;
+  const uint32_t* count = B.counters + CT_TOTAL;
+  const uint32_t nblocks = std::max<uint32_t>((cap + BUCKET_ITEMS - 1) / BUCKET_ITEMS, 1);
+  hipLaunchKernelGGL(tile_count_kernel, dim3(nblocks), dim3(256), 0, s, B.keys[0], count, cap, T, B.tile_count);
+  hipLaunchKernelGGL(tile_prefix_kernel, dim3(1), dim3(1024), 0, s, B.tile_count, T, B.tstart, B.tend, B.counters);
+  hipLaunchKernelGGL(tile_scatter_kernel, dim3(nblocks), dim3(256), 0, s, B.keys[0], B.vals[0], count, cap, T, B.tstart,
+                     B.tile_count + T, B.vals[1]);
   return true;
 }
 

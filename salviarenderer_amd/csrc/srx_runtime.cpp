@@ -257,6 +257,7 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   RCHK(X.take(r->partial, std::max<size_t>(T, (primcnt + 255) / 256) + 1, B.partial));
   RCHK(X.take(r->tstart, std::max<uint32_t>(T, 1), B.tstart));
   RCHK(X.take(r->tend, std::max<uint32_t>(T, 1), B.tend));
+  RCHK(X.take(r->tile_count, 2 * static_cast<size_t>(std::max<uint32_t>(T, 1)), B.tile_count));
   // clipping capacity for every primitive, so no read-back decides whether clipping runs
   RCHK(X.take(r->clip_verts, static_cast<size_t>(primcnt) * 5 * (nout + 1), B.clip_verts));
   RCHK(X.take(r->clip_hdr, primcnt, B.clip_hdr));
@@ -271,9 +272,13 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   // Allocating draws emit into a grow-only buffer sized from the previous draw and repeat the
   // emission when it was too small; a fixed region gets the largest record capacity whose
   // record-sized buffers all fit, and a draw with more records is refused.
-  uint32_t capacity;
+  uint32_t capacity, sort_cap;  // records the buffer holds; records the early sort is sized for
   if (!fixed) {
-    const uint32_t guess = std::max<uint32_t>(std::max<uint32_t>(r->last_records + r->last_records / 4, P3 + T), 1u << 16);
+    uint32_t recent = 0;
+    for (uint32_t v : r->recent_records) recent = std::max(recent, v);
+    const uint64_t want = std::max<uint64_t>(std::max<uint64_t>(recent + static_cast<uint64_t>(recent) / 4, static_cast<uint64_t>(P3) + T),
+                                             1u << 16);
+    const uint32_t guess = static_cast<uint32_t>(std::min<uint64_t>(want, 0xFFFFFFF0u));
     if (r->keys0.cap < guess) {
       RCHK(ensure(r, r->keys0, guess));
       RCHK(ensure(r, r->vals0, guess));
@@ -281,6 +286,7 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
     B.keys[0] = r->keys0.p;
     B.vals[0] = r->vals0.p;
     capacity = static_cast<uint32_t>(std::min<size_t>(r->keys0.cap, r->vals0.cap));
+    sort_cap = std::min(capacity, guess);
   } else {
     const size_t room = fixed->bytes - std::min(fixed->bytes, align256(X.used));
     uint64_t lo = 0, hi = 0xFFFFFFFEu;  // largest capacity whose buffers fit
@@ -291,6 +297,7 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
     }
     if (lo == 0) return fail(SRX_OUT_OF_MEMORY, "the draw's scratch exceeds the no-allocation region");
     capacity = static_cast<uint32_t>(lo);
+    sort_cap = capacity;
     RCHK(X.take(r->keys0, capacity, B.keys[0]));
     RCHK(X.take(r->vals0, capacity, B.vals[0]));
   }
@@ -316,7 +323,7 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
     r->samplers_last_d = samplers_d;
     r->samplers_valid = !fixed;
   }
-  launch_draw_begin(B.counters, B.scan_tmp, scan_state_words(P3), s);
+  launch_draw_begin(B.counters, B.scan_tmp, scan_state_words(P3), B.tile_count, 2 * static_cast<size_t>(T), s);
   mark(0);
   launch_vs(P, B.vbuf, s);
   mark(1);
@@ -332,6 +339,33 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   for (auto& ct : r->st.color_targets) RCHK(run_pending_clear(ct.get(), s));
   RCHK(run_pending_clear(r->st.ds_target.get(), s));
   RCHK(start_counter_read(r));
+  const bool order_free = vis_tables && vis_applicable(P);  // tile lists need no primitive order
+  // record-sized buffers of the sort, for `sized` records
+  uint32_t sized = 0;
+  auto take_sort_buffers = [&](uint32_t n) -> int32_t {
+    sized = n;
+    const uint32_t nblocks = (std::max(n, T) + 4095) / 4096;
+    const size_t hist_n = std::max<size_t>(static_cast<size_t>(256) * std::max<uint32_t>(nblocks, 1), bucket_hist_size(T, n));
+    RCHK(X.take(r->keys1, std::max<uint32_t>(n, 1), B.keys[1]));
+    RCHK(X.take(r->vals1, std::max<uint32_t>(n, 1), B.vals[1]));
+    RCHK(X.take(r->hist, hist_n, B.hist));
+    RCHK(X.take(r->scan_tmp, std::max({scan_tmp_size(P3), scan_tmp_size(static_cast<uint32_t>(hist_n)), scan_state_words(P3),
+                                        scan_state_words(static_cast<uint32_t>(hist_n))}),
+                B.scan_tmp));
+    return SRX_OK;
+  };
+  uint32_t* order = nullptr;
+  // Order-free draws sort their records before the count reaches the host: the sort is sized for the
+  // record buffer's capacity and reads the count on the device, and only a count beyond the
+  // capacity makes the draw emit and sort again.
+  bool bucketed = false, early = false;
+  if (order_free && bucket_sort_applicable(P)) {
+    early = true;
+    RCHK(take_sort_buffers(sort_cap));
+    RCHK(X.take(r->order, std::max<uint32_t>(T, 1), order));
+    bucketed = launch_bucket_sort(P, B, sort_cap, s);
+    launch_tile_order(P, B, order, true, s);
+  }
   uint32_t ctr[CT_COUNT];
   RCHK(finish_counter_read(r, ctr));
   if (ctr[CT_CLIP_OVERFLOW]) return fail(SRX_OUT_OF_MEMORY, "clip buffer overflow");
@@ -339,7 +373,7 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   if (fixed && total > capacity)
     return fail(SRX_OUT_OF_MEMORY, "the draw's binned records exceed the no-allocation region");
   if (!fixed) {
-    r->last_records = total;
+    r->recent_records[r->recent_next++ % 8] = total;
     if (total > capacity) {
       RCHK(ensure(r, r->keys0, total));
       RCHK(ensure(r, r->vals0, total));
@@ -348,28 +382,26 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
       launch_emit(P, B, total, s);
     }
   }
-  const bool order_free = vis_tables && vis_applicable(P);  // tile lists need no primitive order
-  // sized for the capacity on a fixed region (the layout the capacity was chosen for), else for the count
-  const uint32_t sized = fixed ? capacity : total;
-  uint32_t nblocks = (std::max(sized, T) + 4095) / 4096;
-  const size_t hist_n = std::max<size_t>(static_cast<size_t>(256) * std::max<uint32_t>(nblocks, 1), bucket_hist_size(T, sized));
-  RCHK(X.take(r->keys1, std::max<uint32_t>(sized, 1), B.keys[1]));
-  RCHK(X.take(r->vals1, std::max<uint32_t>(sized, 1), B.vals[1]));
-  RCHK(X.take(r->hist, hist_n, B.hist));
-  RCHK(X.take(r->scan_tmp, std::max({scan_tmp_size(P3), scan_tmp_size(static_cast<uint32_t>(hist_n)), scan_state_words(P3),
-                                      scan_state_words(static_cast<uint32_t>(hist_n))}),
-              B.scan_tmp));
+  if (total > sort_cap) bucketed = false;  // the early sort saw only part of the records
   int which = 1;
-  const bool bucketed = order_free && launch_bucket_sort(P, B, total, s);
-  if (!bucketed) {
-    int bits = 1;
-    while ((1u << bits) < T) ++bits;
-    which = launch_sort(B, total, bits, s);
-    launch_tile_ranges(P, B, which, total, s);
+  if (!bucketed || !order) {
+    // the early sort's tile counts and classes are counted again below
+    if (early) {
+      launch_fill_u32(B.counters + CT_CLASS, CT_COUNT - CT_CLASS, 0u, s);
+      launch_fill_u32(B.tile_count, 2 * T, 0u, s);
+    }
+    // sized for the capacity on a fixed region (the layout the capacity was chosen for), else for the count
+    RCHK(take_sort_buffers(fixed ? capacity : total));
+    bucketed = order_free && launch_bucket_sort(P, B, std::max<uint32_t>(total, 1), s);
+    if (!bucketed) {
+      int bits = 1;
+      while ((1u << bits) < T) ++bits;
+      which = launch_sort(B, total, bits, s);
+      launch_tile_ranges(P, B, which, total, s);
+    }
+    RCHK(X.take(r->order, std::max<uint32_t>(T, 1), order));
+    launch_tile_order(P, B, order, bucketed, s);
   }
-  uint32_t* order = nullptr;
-  RCHK(X.take(r->order, std::max<uint32_t>(T, 1), order));
-  launch_tile_order(P, B, order, bucketed, s);
   B.order = order;
   mark(4);
   if (vis_tables) {
@@ -402,7 +434,7 @@ renderer_obj::~renderer_obj() {
   };
   fr(vbuf); fr(tri_rec); fr(tri_attr); fr(clip_verts); fr(samplers_d); fr(vis);
   if (samplers_h) (void)hipHostFree(samplers_h); fr(cnt); fr(keys0); fr(keys1); fr(vals0); fr(vals1);
-  fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot); fr(partial); fr(order); fr(wide); fr(vis_entries); fr(vis_merges); fr(vis_counts); fr(vis_partial);
+  fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot); fr(partial); fr(order); fr(tile_count); fr(wide); fr(vis_entries); fr(vis_merges); fr(vis_counts); fr(vis_partial);
   if (stats_d) (void)hipFree(stats_d);
   if (pinned) (void)hipHostFree(pinned);
   if (map_host) (void)hipHostFree(map_host);

@@ -363,6 +363,7 @@ struct renderer_obj : object {
   bool samplers_valid = false;
   dev_array<uint32_t> cnt, keys0, keys1, vals0, vals1, tstart, tend, hist, scan_tmp, clip_hdr, counters, vis;
   dev_array<uint32_t> order;  // GPU tile scheduling order
+  dev_array<uint32_t> tile_count;  // per-tile record counts and cursors of the counting sort
   dev_array<uint32_t> wide;   // triangles counted by wave
   dev_array<uint4> vis_entries, vis_merges;  // visibility work table
   dev_array<uint32_t> vis_counts;
@@ -376,7 +377,10 @@ struct renderer_obj : object {
   // the counters are read back on `side` (default priority) while the main stream emits records
   hipStream_t side = nullptr;
   hipEvent_t ev_counted = nullptr, ev_read = nullptr;
-  uint32_t last_records = 0;   // binned records of the previous draw (sizes the next emission)
+  // binned records of the last draws; their maximum sizes the next draw's record buffer, so that
+  // frames whose draws differ in size do not overflow it in turn
+  uint32_t recent_records[8] = {};
+  uint32_t recent_next = 0;
   uint64_t host_stats[ST_COUNT] = {};
   // map/unmap (resource_manager.cpp:7-113): one mapped resource at a time, staged in pinned memory
   int32_t map_mode = 0;
