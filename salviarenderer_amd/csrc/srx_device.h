@@ -80,6 +80,12 @@ struct draw_params {
   int32_t early_z, blend_id;
   int32_t count_stats;  // a statistics query is active: accumulate the device-side counters
   int32_t vis_stencil;  // stencil on with KEEP on pass for both faces: the stencil test is a fixed mask
+  // full clears of the targets that the order-independent visibility pass applies as it writes them
+  // (it then reads nothing from the depth/stencil target, and samples it leaves take the clear values)
+  int32_t clear_ds, clear_color;
+  float clear_depth;
+  uint32_t clear_stencil;
+  float clear_rgba[4];
   // pixel shader
   int32_t ps_id;
   float ps_consts[16];

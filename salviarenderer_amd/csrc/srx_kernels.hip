@@ -2339,17 +2339,14 @@ __device__ __forceinline__ uint32_t key_winner(int mode, unsigned long long k) {
 
 // Depth the reference's early-Z test wrote for winner w of sample smp of pixel (px, py).
 template <int S>
-__device__ __forceinline__ void write_winner_depth(const draw_params& P, int mode, unsigned long long key, uint32_t w,
-                                                   const float4* __restrict__ tri_rec, int px, int py, int smp) {
+__device__ __forceinline__ float winner_depth(const draw_params& P, int mode, unsigned long long key, uint32_t w,
+                                              const float4* __restrict__ tri_rec, int px, int py, int smp) {
   if (mode != VIS_LATEST) {
     // depth modes carry the winning sample depth in the key; only +-0 is ambiguous there
     uint32_t zk = static_cast<uint32_t>(key >> 32);
     if (mode == VIS_GREATER || mode == VIS_GEQUAL) zk = ~zk;
     const uint32_t u = (zk & 0x80000000u) ? (zk & 0x7FFFFFFFu) : ~zk;
-    if (u != 0u) {
-      *reinterpret_cast<float*>(sample_addr(P.ds, px, py, smp)) = __uint_as_float(u);
-      return;
-    }
+    if (u != 0u) return __uint_as_float(u);
   }
   // the winner's depth at this sample, as its quad computed it
   const float4* tr = tri_rec + tri_slot(P, w) * 5;
@@ -2365,7 +2362,7 @@ __device__ __forceinline__ void write_winner_depth(const draw_params& P, int mod
   else zp = (z00 + d4.x) + d4.y;
   float nd = zp;
   if (S > 1) nd = ((P.sp[smp][0] - 0.5f) * d4.x + (P.sp[smp][1] - 0.5f) * d4.y) + zp;
-  *reinterpret_cast<float*>(sample_addr(P.ds, px, py, smp)) = nd;
+  return nd;
 }
 
 // cmpf as a truth table over the comparison's outcome: bit 0 less, 1 equal, 2 greater, 3 unordered.
@@ -2625,7 +2622,8 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
       float d0 = 0.0f;
       bool blocked = false;
       if (has_ds && px < tw && py < th) {
-        const float2 ds = *reinterpret_cast<const float2*>(sample_addr(P.ds, px, py, smp));
+        const float2 ds = P.clear_ds ? make_float2(P.clear_depth, __uint_as_float(P.clear_stencil))
+                                     : *reinterpret_cast<const float2*>(sample_addr(P.ds, px, py, smp));
         d0 = ds.x;
         if (P.vis_stencil) {
           // framebuffer.cpp:445-479 (both faces share the function here); a failing sample takes
@@ -2838,9 +2836,20 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
     for (int smp = 0; smp < S; ++smp) {
       const unsigned long long key = keys[p * S + smp];
       w[smp] = key_winner(mode, key);
-      if (w[smp] == NO_WINNER) continue;
-      left |= 1u << smp;
-      if (has_ds && P.write_depth) write_winner_depth<S>(P, mode, key, w[smp], tri_rec, px, py, smp);
+      const bool won = w[smp] != NO_WINNER;
+      if (won) left |= 1u << smp;
+      if (has_ds && P.clear_ds) {
+        // a cleared target: every sample is written, the stencil keeps its clear value
+        const float d = (won && P.write_depth) ? winner_depth<S>(P, mode, key, w[smp], tri_rec, px, py, smp) : P.clear_depth;
+        *reinterpret_cast<float2*>(sample_addr(P.ds, px, py, smp)) = make_float2(d, __uint_as_float(P.clear_stencil));
+      } else if (won && has_ds && P.write_depth) {
+        *reinterpret_cast<float*>(sample_addr(P.ds, px, py, smp)) =
+            winner_depth<S>(P, mode, key, w[smp], tri_rec, px, py, smp);
+      }
+      if (SHADE && !won && P.clear_color) {
+        const float4 cc = make_float4(P.clear_rgba[0], P.clear_rgba[1], P.clear_rgba[2], P.clear_rgba[3]);
+        store_texel(P.color.pf, sample_addr(P.color, px, py, smp), cc);
+      }
     }
     if (SHADE) {
 #pragma unroll

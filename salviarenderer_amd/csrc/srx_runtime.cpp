@@ -335,9 +335,22 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   // the counters are final after the scan: their read-back overlaps the emission
   HIPCHK(hipEventRecord(r->ev_counted, s));
   launch_emit(P, B, capacity, s);
-  // the targets' pending clears run here, while the record count travels to the host
-  for (auto& ct : r->st.color_targets) RCHK(run_pending_clear(ct.get(), s));
-  RCHK(run_pending_clear(r->st.ds_target.get(), s));
+  // A target's pending full clear is applied by the order-independent visibility pass as it writes
+  // the target, when that pass covers the whole target (it then reads nothing back); the others run
+  // here, while the record count travels to the host.
+  const bool will_vis = vis_tables && vis_applicable(P);
+  surface_obj* const ct0 = r->st.color_targets.empty() ? nullptr : r->st.color_targets[0].get();
+  surface_obj* const dst = r->st.ds_target.get();
+  auto covers = [&](const surface_obj* o) {
+    return o && P.target_w == static_cast<float>(o->w) && P.target_h == static_cast<float>(o->h) &&
+           P.gtiles_x * P.gt >= o->w && P.gtiles_y * P.gt >= o->h;
+  };
+  const bool eat_color = will_vis && needs_shading(P) && ct0 && ct0->pend.kind == 1 && covers(ct0);
+  const bool eat_ds = will_vis && dst && dst->pend.kind == 2 && dst->pend.flags == (SRX_CLEAR_DEPTH | SRX_CLEAR_STENCIL) &&
+                      covers(dst);
+  for (auto& ct : r->st.color_targets)
+    if (!(eat_color && ct.get() == ct0)) RCHK(run_pending_clear(ct.get(), s));
+  if (!eat_ds) RCHK(run_pending_clear(dst, s));
   RCHK(start_counter_read(r));
   const bool order_free = vis_tables && vis_applicable(P);  // tile lists need no primitive order
   // record-sized buffers of the sort, for `sized` records
@@ -411,6 +424,17 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
     RCHK(X.take(r->vis_merges, nm, B.vis_merges));
     RCHK(X.take(r->vis_counts, 4, B.vis_counts));
     RCHK(X.take(r->vis_partial, static_cast<size_t>(nsl) * 256 * samples, B.vis_partial));
+  }
+  if (eat_color) {
+    P.clear_color = 1;
+    for (int k = 0; k < 4; ++k) P.clear_rgba[k] = (&ct0->pend.color.x)[k];
+    RCHK(run_pending_clear(ct0, s, true));
+  }
+  if (eat_ds) {
+    P.clear_ds = 1;
+    P.clear_depth = dst->pend.depth;
+    P.clear_stencil = dst->pend.stencil;
+    RCHK(run_pending_clear(dst, s, true));
   }
   const bool fused = vis_tables && launch_vis(P, B, which, needs_shading(P), total, s);
   if (!fused) launch_raster(P, B, which, s);
