@@ -35,12 +35,13 @@ def texture_chain_bytes(size=2048):
     return total
 
 
-# Algorithmic bytes of one launch of the dominant kernel, vis_kernel<2,4,true> (visibility + deferred
-# shading of the draw): per sample, depth read once (4 B), depth written once (4 B) and colour
-# written once (4 B); the 2048^2 texture chain read at most once.
-RASTER_BYTES = W * H * SAMPLES * (4 + 4 + 4) + texture_chain_bytes(2048)
+# Algorithmic bytes of one launch of the dominant kernel, vis_kernel<2,4,true> (visibility + shading of
+# the draw, with the frame's clears fused into it): per sample the RG32F depth/stencil written once
+# (8 B) and the RGBA8 colour written once (4 B); the 2048^2 texture chain read at most once.
+RASTER_BYTES = W * H * SAMPLES * (8 + 4) + texture_chain_bytes(2048)
 RASTER_KERNEL = "vis_kernel<2, 4, true>"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_vis_kernel_textured_mesh_1080p_msaa4.json")
+ISSUE_FILE = os.path.join(ROOT, "profiles", "pmc_vis_kernel_textured_mesh_1080p_msaa4.json")
 
 
 def measured_traffic():
@@ -51,6 +52,18 @@ def measured_traffic():
     with open(TRAFFIC_FILE) as fh:
         t = json.load(fh)
     return float(t["traffic_bytes"])
+
+
+def measured_issue():
+    """VALU and LDS issue of one vis_kernel launch from the committed SQ counter passes; None if absent."""
+    if not os.path.exists(ISSUE_FILE):
+        return None
+    with open(ISSUE_FILE) as fh:
+        t = json.load(fh)
+    return {"limiter": t["limiter"], "valu_issue_frac": t["valu_issue_frac"], "wait_frac": t["wait_frac"],
+            "salu_per_valu": t["salu_per_valu"],
+            "lds_bank_conflict_cycles_per_lds_inst": t["lds_bank_conflict_cycles_per_lds_inst"],
+            "source": os.path.relpath(ISSUE_FILE, ROOT)}
 
 
 def copy_calibration(mib, reps):
@@ -298,6 +311,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the secondary batched-jobs measurement")
     ap.add_argument("--batched-only", action="store_true", help="only the batched-jobs measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the secondary serving measurement")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline frames (no CPU baseline and no secondary workloads), for profiling")
     ap.add_argument("--serving-only", action="store_true", help="only the serving measurement")
     ap.add_argument("--serving-requests", type=int, default=10000)
     ap.add_argument("--batch-config", default="", metavar="TILE,BIN_ITEMS,THREADS",
@@ -351,12 +366,18 @@ def main():
         "stage_ms": {"vertex": prof[1] / draws / 1e6, "setup_clip": prof[2] / draws / 1e6,
                      "scan": prof[3] / draws / 1e6, "bin_sort": prof[5] / draws / 1e6,
                      "raster": vis_ns / 1e6},
+        # the path has no MFMA work, so HBM is the roofline priced here; the kernel's actual limiter
+        # (instruction issue or latency) is reported beside it from the committed counter passes
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": RASTER_KERNEL, "algorithmic_bytes": RASTER_BYTES, "kernel_ms": vis_ns / 1e6},
-        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(args.cpu_seconds),
+                     "kernel": RASTER_KERNEL, "algorithmic_bytes": RASTER_BYTES, "kernel_ms": vis_ns / 1e6,
+                     "issue": measured_issue()},
+        "cpu_baseline": None if (args.no_cpu_baseline or args.headline_only) else cpu_baseline(args.cpu_seconds),
         "secondary": {},
     }
+    if args.headline_only:
+        print(json.dumps(out), flush=True)
+        return
     out["secondary"]["close_camera"] = close_camera(lib, 20, 3)
     if not args.no_multipass:
         out["secondary"]["multipass"] = multipass_frames(lib, 20, 3)
