@@ -426,7 +426,7 @@ int32_t srx_draw_noalloc(srx_renderer r, size_t startpos, size_t primcnt);
  * executed one by one.  Requirements, checked while recording (SRX_E_UNSUPPORTED otherwise, also
  * returned by srx_batch_end): every draw is opaque (blend off or disabled) with an
  * order-independent depth/stencil test (early-Z, or a stencil that keeps its value), no query is
- * running, targets of at most 8192 16x16 tiles (e.g. 1920x1088), and no surface is a target of two
+ * running, and no surface is a target of two
  * draws, written after it is read or sampled while written.  Resources stay referenced by the
  * batch; their contents may change between launches (vertex positions included: a launch whose
  * records outgrow the prepared capacity stays in bounds and sets status bit 0). */

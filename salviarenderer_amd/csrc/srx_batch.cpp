@@ -122,7 +122,6 @@ int32_t record_draw(renderer_obj* r, const draw_params& P, const ps_samplers& SM
   if (!(P.early_z || P.vis_stencil) || !(P.blend_id == SRX_BLEND_OFF || P.blend_id == SRX_BLEND_DISABLED) ||
       !vis_applicable(P))
     return b->refuse(SRX_E_UNSUPPORTED, "a batched draw must be opaque with an order-independent depth test");
-  if (T > BATCH_MAX_TILES) return b->refuse(SRX_E_UNSUPPORTED, "batched draw target larger than 8192 16x16 tiles");
   if (P.nattr > 2) return b->refuse(SRX_E_UNSUPPORTED, "batched draws use vertex shaders with at most 2 attributes");
   if (3ull * P.prim_count >= 0x40000000ull) return b->refuse(SRX_E_UNSUPPORTED, "batched draw primitive count");
   surface_obj* ts[2] = {r->st.color_targets.empty() ? nullptr : r->st.color_targets[0].get(), r->st.ds_target.get()};
@@ -400,7 +399,7 @@ int32_t batch_configure(batch_obj* b, int tile, int bin_pt, int vis_threads) {
     int32_t gx, gy;
     gpu_tiles(d.P, tile, &gx, &gy);
     if (!vis_batch_supported(d.P.samples, needs_shading(d.P), vis_threads, tile) ||
-        static_cast<uint64_t>(gx) * gy > BATCH_MAX_TILES)
+        static_cast<uint64_t>(gx) * gy >= 0x40000000ull)
       return fail(SRX_E_UNSUPPORTED, "GPU tile size not available for every job of the batch");
   }
   renderer_obj* r = b->r;

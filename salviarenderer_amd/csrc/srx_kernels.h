@@ -109,7 +109,6 @@ void launch_batch_check(const batch_job* jobs, uint32_t njobs, uint32_t capacity
 // out[j]: records of job j; out[njobs + j]: its clip-overflow counter
 void launch_batch_counts(const batch_job* jobs, uint32_t njobs, uint32_t* out, hipStream_t s);
 int vis_mode(const draw_params& P);
-constexpr uint32_t BATCH_MAX_TILES = 8192;  // GPU tiles of one batched draw (its LDS histogram)
 
 hipError_t upload_ewa_table(const float* table256, hipStream_t s);
 

@@ -1250,12 +1250,13 @@ __device__ __forceinline__ uint32_t bucket_column(uint32_t b, uint32_t nblocks) 
   const uint32_t q = nblocks / 8, r = nblocks % 8, x = b % 8;
   return x * q + min(x, r) + b / 8;
 }
+// The per-workgroup tile counts live in LDS for BUCKET_MAX_TILES tiles at a time: a target with more
+// GPU tiles is counted range by range, each pass over the workgroup's keys taking the tiles of one
+// range (the keys stay in registers).
 template <int PT>
 __device__ __forceinline__ void bucket_hist_body(const uint32_t* __restrict__ keys, uint32_t n, uint32_t T,
                                                  uint32_t nblocks, uint32_t* __restrict__ hist, uint32_t bx) {
   __shared__ uint32_t cnt[BUCKET_MAX_TILES];
-  for (uint32_t d = threadIdx.x; d < T; d += 256) cnt[d] = 0;
-  __syncthreads();
   const uint32_t b0 = bx * (256 * PT);
   uint32_t k[PT];
 #pragma unroll
@@ -1263,12 +1264,18 @@ __device__ __forceinline__ void bucket_hist_body(const uint32_t* __restrict__ ke
     const uint32_t i = b0 + j * 256 + threadIdx.x;
     k[j] = i < n ? keys[i] : 0xFFFFFFFFu;
   }
-#pragma unroll
-  for (int j = 0; j < PT; ++j)
-    if (k[j] != 0xFFFFFFFFu) atomicAdd(&cnt[k[j]], 1u);
-  __syncthreads();
   const uint32_t col = bucket_column(bx, nblocks);
-  for (uint32_t d = threadIdx.x; d < T; d += 256) hist[d * nblocks + col] = cnt[d];
+  for (uint32_t r0 = 0; r0 < T; r0 += BUCKET_MAX_TILES) {
+    const uint32_t tn = min(T - r0, BUCKET_MAX_TILES);
+    if (r0) __syncthreads();  // the previous range's counts are written out
+    for (uint32_t d = threadIdx.x; d < tn; d += 256) cnt[d] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PT; ++j)
+      if (k[j] - r0 < tn) atomicAdd(&cnt[k[j] - r0], 1u);  // also skips the 0xFFFFFFFF padding
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < tn; d += 256) hist[(r0 + d) * nblocks + col] = cnt[d];
+  }
 }
 
 // Only the tiles present in the workgroup's records fetch their (strided) offsets.  Positions at
@@ -1279,8 +1286,6 @@ __device__ __forceinline__ void bucket_scatter_body(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ hist_scanned, uint32_t* __restrict__ vout,
                                                     uint32_t limit, uint32_t bx) {
   __shared__ uint32_t off[BUCKET_MAX_TILES];
-  for (uint32_t d = threadIdx.x; d < T; d += 256) off[d] = 0xFFFFFFFFu;
-  __syncthreads();
   const uint32_t b0 = bx * (256 * PT);
   uint32_t k[PT], v[PT];
 #pragma unroll
@@ -1289,19 +1294,25 @@ __device__ __forceinline__ void bucket_scatter_body(const uint32_t* __restrict__
     k[j] = i < n ? keys[i] : 0xFFFFFFFFu;
     v[j] = i < n ? vals[i] : 0u;
   }
+  for (uint32_t r0 = 0; r0 < T; r0 += BUCKET_MAX_TILES) {
+    const uint32_t tn = min(T - r0, BUCKET_MAX_TILES);
+    if (r0) __syncthreads();  // the previous range's offsets are consumed
+    for (uint32_t d = threadIdx.x; d < tn; d += 256) off[d] = 0xFFFFFFFFu;
+    __syncthreads();
 #pragma unroll
-  for (int j = 0; j < PT; ++j)
-    if (k[j] != 0xFFFFFFFFu) off[k[j]] = 0u;  // mark the tiles present
-  __syncthreads();
-  for (uint32_t d = threadIdx.x; d < T; d += 256)
-    if (off[d] == 0u) off[d] = hist_scanned[d * nblocks + bucket_column(bx, nblocks)];
-  __syncthreads();
+    for (int j = 0; j < PT; ++j)
+      if (k[j] - r0 < tn) off[k[j] - r0] = 0u;  // mark the tiles present
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < tn; d += 256)
+      if (off[d] == 0u) off[d] = hist_scanned[(r0 + d) * nblocks + bucket_column(bx, nblocks)];
+    __syncthreads();
 #pragma unroll
-  for (int j = 0; j < PT; ++j)
-    if (k[j] != 0xFFFFFFFFu) {
-      const uint32_t pos = atomicAdd(&off[k[j]], 1u);
-      if (pos < limit) vout[pos] = v[j];
-    }
+    for (int j = 0; j < PT; ++j)
+      if (k[j] - r0 < tn) {
+        const uint32_t pos = atomicAdd(&off[k[j] - r0], 1u);
+        if (pos < limit) vout[pos] = v[j];
+      }
+  }
 }
 
 // atomicAdd(&counters[key], 1) for every active lane, one atomic per distinct key of the wave;
@@ -1349,24 +1360,29 @@ uint32_t bucket_hist_size(uint32_t T, uint32_t n) {
 // the totals into tile ranges; the scatter counts again, reserves its range in each touched tile
 // with one atomic and places its records at the LDS ranks.  tile_count and tile_cursor are zero
 // at the start of the draw.
-__device__ __forceinline__ void tile_count_lds(const uint32_t* __restrict__ keys, uint32_t n, uint32_t T, uint32_t b0,
-                                               uint32_t (&k)[BUCKET_PER_THREAD], uint32_t (&rank)[BUCKET_PER_THREAD],
-                                               uint32_t* cnt, uint16_t* touched, uint32_t* ntouched) {
-  for (uint32_t d = threadIdx.x; d < T; d += 256) cnt[d] = 0;
+// Counts the workgroup's records of tiles [r0, r0 + tn) in LDS (rank[j]: the record's rank among the
+// workgroup's records of its tile) and lists the tiles it touches (relative to r0).
+__device__ __forceinline__ void tile_count_lds(const uint32_t (&k)[BUCKET_PER_THREAD], uint32_t (&rank)[BUCKET_PER_THREAD],
+                                               uint32_t r0, uint32_t tn, uint32_t* cnt, uint16_t* touched,
+                                               uint32_t* ntouched) {
+  for (uint32_t d = threadIdx.x; d < tn; d += 256) cnt[d] = 0;
   if (threadIdx.x == 0) *ntouched = 0;
   __syncthreads();
+#pragma unroll
+  for (int j = 0; j < BUCKET_PER_THREAD; ++j)
+    if (k[j] - r0 < tn) {  // also skips the 0xFFFFFFFF padding
+      rank[j] = atomicAdd(&cnt[k[j] - r0], 1u);
+      if (rank[j] == 0) touched[atomicAdd(ntouched, 1u)] = static_cast<uint16_t>(k[j] - r0);
+    }
+  __syncthreads();
+}
+__device__ __forceinline__ void tile_load_keys(const uint32_t* __restrict__ keys, uint32_t n, uint32_t b0,
+                                               uint32_t (&k)[BUCKET_PER_THREAD]) {
 #pragma unroll
   for (int j = 0; j < BUCKET_PER_THREAD; ++j) {
     const uint32_t i = b0 + j * 256 + threadIdx.x;
     k[j] = i < n ? keys[i] : 0xFFFFFFFFu;
   }
-#pragma unroll
-  for (int j = 0; j < BUCKET_PER_THREAD; ++j)
-    if (k[j] != 0xFFFFFFFFu) {
-      rank[j] = atomicAdd(&cnt[k[j]], 1u);
-      if (rank[j] == 0) touched[atomicAdd(ntouched, 1u)] = static_cast<uint16_t>(k[j]);
-    }
-  __syncthreads();
 }
 __global__ void __launch_bounds__(256) tile_count_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ count,
                                                          uint32_t cap, uint32_t T, uint32_t* __restrict__ tile_count) {
@@ -1377,43 +1393,58 @@ __global__ void __launch_bounds__(256) tile_count_kernel(const uint32_t* __restr
   const uint32_t b0 = blockIdx.x * BUCKET_ITEMS;
   if (b0 >= n) return;
   uint32_t k[BUCKET_PER_THREAD], rank[BUCKET_PER_THREAD];
-  tile_count_lds(keys, n, T, b0, k, rank, cnt, touched, &ntouched);
-  for (uint32_t i = threadIdx.x; i < ntouched; i += 256) {
-    const uint32_t t = touched[i];
-    atomicAdd(&tile_count[t], cnt[t]);
+  tile_load_keys(keys, n, b0, k);
+  for (uint32_t r0 = 0; r0 < T; r0 += BUCKET_MAX_TILES) {
+    const uint32_t tn = min(T - r0, BUCKET_MAX_TILES);
+    if (r0) __syncthreads();  // the previous range's touched tiles are consumed
+    tile_count_lds(k, rank, r0, tn, cnt, touched, &ntouched);
+    for (uint32_t i = threadIdx.x; i < ntouched; i += 256) {
+      const uint32_t t = touched[i];
+      atomicAdd(&tile_count[r0 + t], cnt[t]);
+    }
   }
 }
-// One workgroup: tile ranges from the totals (exclusive prefix), and the tiles per record-count class.
+// One workgroup: tile ranges from the totals (exclusive prefix, BUCKET_MAX_TILES tiles per step), and
+// the tiles per record-count class.
 __global__ void __launch_bounds__(1024) tile_prefix_kernel(const uint32_t* __restrict__ tile_count, uint32_t T,
                                                           uint32_t* __restrict__ ts, uint32_t* __restrict__ te,
                                                           uint32_t* __restrict__ counters) {
   constexpr int PER = BUCKET_MAX_TILES / 1024;
   __shared__ uint32_t wsum[16];
   __shared__ uint32_t classes[16];
-  const uint32_t tid = threadIdx.x, t0 = tid * PER;
+  const uint32_t tid = threadIdx.x;
   if (tid < 16) classes[tid] = 0;
-  uint32_t c[PER], sum = 0;
+  uint32_t carry = 0;  // records of the tiles before this step
+  for (uint32_t base = 0; base < T; base += BUCKET_MAX_TILES) {
+    const uint32_t t0 = base + tid * PER;
+    uint32_t c[PER], sum = 0;
 #pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    c[i] = t0 + i < T ? tile_count[t0 + i] : 0u;
-    sum += c[i];
-  }
-  const uint32_t incl = wave_incl_scan(sum);
-  if ((tid & 63) == 63) wsum[tid >> 6] = incl;
-  __syncthreads();
-  uint32_t run = incl - sum;
-  for (uint32_t w = 0; w < (tid >> 6); ++w) run += wsum[w];
-#pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    if (t0 + i < T) {
-      ts[t0 + i] = run;
-      te[t0 + i] = run + c[i];
+    for (int i = 0; i < PER; ++i) {
+      c[i] = t0 + i < T ? tile_count[t0 + i] : 0u;
+      sum += c[i];
     }
-    run += c[i];
-  }
+    const uint32_t incl = wave_incl_scan(sum);
+    if (base) __syncthreads();  // the previous step's wave sums are read
+    if ((tid & 63) == 63) wsum[tid >> 6] = incl;
+    __syncthreads();
+    uint32_t run = carry + incl - sum, step = 0;
+    for (uint32_t w = 0; w < 16; ++w) {
+      if (w < (tid >> 6)) run += wsum[w];
+      step += wsum[w];
+    }
 #pragma unroll
-  for (int i = 0; i < PER; ++i)
-    if (t0 + i < T) (void)wave_counted_add(classes, tile_class(c[i]));
+    for (int i = 0; i < PER; ++i) {
+      if (t0 + i < T) {
+        ts[t0 + i] = run;
+        te[t0 + i] = run + c[i];
+      }
+      run += c[i];
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i)
+      if (t0 + i < T) (void)wave_counted_add(classes, tile_class(c[i]));
+    carry += step;
+  }
   __syncthreads();
   if (tid < 16) counters[CT_CLASS + tid] = classes[tid];
 }
@@ -1428,22 +1459,24 @@ __global__ void __launch_bounds__(256) tile_scatter_kernel(const uint32_t* __res
   const uint32_t b0 = blockIdx.x * BUCKET_ITEMS;
   if (b0 >= n) return;
   uint32_t k[BUCKET_PER_THREAD], rank[BUCKET_PER_THREAD];
-  tile_count_lds(keys, n, T, b0, k, rank, cnt, touched, &ntouched);
-  // this workgroup's range in each touched tile; cnt[t] becomes its first position
-  for (uint32_t i = threadIdx.x; i < ntouched; i += 256) {
-    const uint32_t t = touched[i];
-    cnt[t] = ts[t] + atomicAdd(&tile_cursor[t], cnt[t]);
-  }
-  __syncthreads();
+  tile_load_keys(keys, n, b0, k);
+  for (uint32_t r0 = 0; r0 < T; r0 += BUCKET_MAX_TILES) {
+    const uint32_t tn = min(T - r0, BUCKET_MAX_TILES);
+    if (r0) __syncthreads();  // the previous range's positions are consumed
+    tile_count_lds(k, rank, r0, tn, cnt, touched, &ntouched);
+    // this workgroup's range in each touched tile; cnt[t] becomes its first position
+    for (uint32_t i = threadIdx.x; i < ntouched; i += 256) {
+      const uint32_t t = touched[i];
+      cnt[t] = ts[r0 + t] + atomicAdd(&tile_cursor[r0 + t], cnt[t]);
+    }
+    __syncthreads();
 #pragma unroll
-  for (int j = 0; j < BUCKET_PER_THREAD; ++j)
-    if (k[j] != 0xFFFFFFFFu) vout[cnt[k[j]] + rank[j]] = vals[b0 + j * 256 + threadIdx.x];
+    for (int j = 0; j < BUCKET_PER_THREAD; ++j)
+      if (k[j] - r0 < tn) vout[cnt[k[j] - r0] + rank[j]] = vals[b0 + j * 256 + threadIdx.x];
+  }
 }
 
-bool bucket_sort_applicable(const draw_params& P) {
-  const uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
-  return T > 0 && T <= BUCKET_MAX_TILES;
-}
+bool bucket_sort_applicable(const draw_params& P) { return P.gtiles_x > 0 && P.gtiles_y > 0; }
 bool launch_bucket_sort(const draw_params& P, const draw_buffers& B, uint32_t cap, hipStream_t s) {
   const uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
   if (!bucket_sort_applicable(P)) return false;

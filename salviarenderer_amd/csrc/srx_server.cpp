@@ -781,8 +781,7 @@ int32_t srx_server_submit(srx_server ss, const srx_request* q, uint64_t* ticket)
   if (!(q->samples == 1 || q->samples == 2 || q->samples == 4) || q->width == 0 || q->height == 0 || q->n_tris == 0 ||
       q->n_tris >= 0x15555555u)
     return SRX_INVALID_PARAMETER;
-  if (((q->width + 15) / 16) * ((q->height + 15) / 16) > BATCH_MAX_TILES)
-    return fail(SRX_E_UNSUPPORTED, "served target larger than 8192 16x16 tiles");
+  if (q->width > 32768 || q->height > 32768) return fail(SRX_E_UNSUPPORTED, "served target wider or taller than 32768");
   std::lock_guard<std::mutex> lk(S->mu);
   if (S->fatal != SRX_OK) return fail(S->fatal, S->fatal_msg.c_str());
   // each asset is held as soon as it is resident, so the next upload cannot evict it

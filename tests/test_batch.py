@@ -115,6 +115,29 @@ def test_batch_equals_one_by_one_and_oracle(hip_lib, cpu_lib):
 
 SINGLE = [s for s in SPECS if s.samples == 1]  # 8x8 and 32x32 tiles take single-sample jobs
 
+# Targets with more GPU tiles than one LDS range of tile counts (8192): a 2048x2048 job (16,384 16x16
+# tiles, 65,536 8x8 tiles) and a 1920x1080 job (8,160 16x16 tiles, 32,400 8x8 tiles).
+LARGE = [S.JobSpec(0x5A11B101, 2048, 2048, 30000, 64, 1), S.JobSpec(0x5A11B102, 1920, 1080, 20000, 64, 1),
+         S.JobSpec(0x5A11B103, 256, 256, 2000, 64, 1)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile", [16, 8])
+def test_batch_targets_beyond_8192_tiles(hip_lib, cpu_lib, tile):
+    r, jobs, ref = one_by_one(hip_lib, LARGE)
+    if tile == 16:  # the single-draw path's tile ranges against the oracle
+        _, _, cpu = one_by_one(cpu_lib, LARGE)
+        for j, (h, c) in enumerate(zip(ref, cpu)):
+            assert np.array_equal(h["ds"], c["ds"]), j
+            assert np.array_equal(h["color"], c["color"]), j
+    scramble(jobs)
+    b = jobs.record()
+    b.configure(tile, 4096, 64)
+    b.launch()
+    r.flush()
+    assert b.status() == 0
+    assert_same(outputs(jobs), ref)
+
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("tile,bin_items,threads", [(16, 2048, 64), (16, 4096, 128), (16, 8192, 256), (16, 2048, 256),

@@ -194,3 +194,20 @@ def test_oracle_server_release_before_poll(cpu_lib):
 def test_server_release_before_poll(hip_lib):
     _release_before_poll(hip_lib)
 
+
+
+@pytest.mark.gpu
+def test_served_target_beyond_8192_tiles(hip_lib):
+    """A 2048x2048 request (16,384 16x16 tiles: more than one LDS range of tile counts), 1x and 4x
+    MSAA, alone and beside a small one, equals the plain pipeline."""
+    big = S.JobSpec(0x5A11C101, 2048, 2048, 30000, 64, 1)
+    r = hip_lib.renderer()
+    srv = r.create_server()
+    for s in (1, 4):
+        ref = plain(hip_lib, big, s)
+        t = srv.submit(big.seed, big.n_tris, big.width, big.height, s, mesh=S.batch_triangles(big),
+                       texture_key=0, clear=CLEAR, want_depth=True)
+        t2 = submit(srv, 2, s)
+        got = drain(srv, 2)
+        assert got[t]["status"] == 0 and got[t2]["status"] == 0
+        assert np.array_equal(got[t]["color"], ref[0]) and np.array_equal(got[t]["depth"], ref[1]), s
