@@ -315,6 +315,11 @@ size_t srx_buffer_size(srx_buffer b);
 int32_t srx_buffer_write(srx_buffer b, size_t offset, const void* data, size_t size);
 /* map(read): waits for queued work that writes the buffer */
 int32_t srx_buffer_read(srx_buffer b, size_t offset, void* data, size_t size);
+/* buffer::transfer (buffer.cpp:42-62): `count` rows of `size` bytes from data (rows stride_src apart)
+   to the buffer at offset (rows stride_dest apart); SRX_INVALID_PARAMETER when the last row ends past
+   the buffer (the reference asserts and copies nothing). */
+int32_t srx_buffer_transfer(srx_buffer b, size_t offset, const void* data, size_t stride_dest, size_t stride_src,
+                            size_t size, size_t count);
 /* texture::subresource(level) (texture.h:47-52): retained surface handle */
 int32_t srx_texture_surface(srx_texture t, size_t level, srx_surface* out);
 /* texture_2d::gen_mipmap(filter, auto_gen) (texture2d.cpp:25-36) */

@@ -1706,6 +1706,17 @@ int32_t srx_buffer_write(srx_buffer b, size_t off, const void* data, size_t size
   std::memcpy(o->data.data() + off, data, size);
   return SRX_OK;
 }
+// buffer.cpp:42-62 (the contiguous case is the same row-by-row copy)
+int32_t srx_buffer_transfer(srx_buffer b, size_t off, const void* data, size_t stride_dest, size_t stride_src, size_t size,
+                            size_t count) {
+  auto* o = as<buffer_obj>(b, K_BUFFER);
+  if (!o || (!data && size && count)) return SRX_INVALID_PARAMETER;
+  if (count == 0 || size == 0) return SRX_OK;
+  if (off + stride_dest * (count - 1) + size > o->data.size()) return SRX_INVALID_PARAMETER;
+  const uint8_t* src = static_cast<const uint8_t*>(data);
+  for (size_t i = 0; i < count; ++i) std::memcpy(o->data.data() + off + i * stride_dest, src + i * stride_src, size);
+  return SRX_OK;
+}
 int32_t srx_buffer_read(srx_buffer b, size_t off, void* data, size_t size) {
   auto* o = as<buffer_obj>(b, K_BUFFER);
   if (!o || off + size > o->data.size()) return SRX_INVALID_PARAMETER;

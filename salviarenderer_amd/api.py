@@ -159,6 +159,7 @@ PROTOTYPES = {
     "srx_buffer_size": (_SZ, [_H]),
     "srx_buffer_write": (_I32, [_H, _SZ, _P, _SZ]),
     "srx_buffer_read": (_I32, [_H, _SZ, _P, _SZ]),
+    "srx_buffer_transfer": (_I32, [_H, _SZ, _P, _SZ, _SZ, _SZ, _SZ]),
     "srx_texture_surface": (_I32, [_H, _SZ, C.POINTER(_H)]),
     "srx_texture_gen_mipmap": (_I32, [_H, _I32, _I32]),
     "srx_texture_max_lod": (_SZ, [_H]),
@@ -274,6 +275,11 @@ class Buffer(_Obj):
     def write(self, data, offset=0):
         arr = np.ascontiguousarray(data)
         self.lib.call("srx_buffer_write", self.h, offset, arr.ctypes.data, arr.nbytes)
+
+    def transfer(self, data, offset, stride_dest, stride_src, size, count):
+        """buffer::transfer: count rows of size bytes, stride_src apart in data, stride_dest apart here."""
+        arr = np.ascontiguousarray(data)
+        self.lib.call("srx_buffer_transfer", self.h, offset, arr.ctypes.data, stride_dest, stride_src, size, count)
 
     def read(self, size=None, offset=0) -> bytes:
         n = self.lib.dll.srx_buffer_size(self.h) - offset if size is None else size

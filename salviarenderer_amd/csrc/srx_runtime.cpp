@@ -636,6 +636,17 @@ int32_t srx_buffer_write(srx_buffer b, size_t off, const void* data, size_t size
   if (size) HIPCHK(hipMemcpy(o->d + off, data, size, hipMemcpyHostToDevice));
   return SRX_OK;
 }
+// Rows land with one 2-D copy (host pitch stride_src, device pitch stride_dest).
+int32_t srx_buffer_transfer(srx_buffer b, size_t off, const void* data, size_t stride_dest, size_t stride_src, size_t size,
+                            size_t count) {
+  auto* o = as<buffer_obj>(b, K_BUFFER);
+  if (!o || (!data && size && count)) return SRX_INVALID_PARAMETER;
+  if (count == 0 || size == 0) return SRX_OK;
+  if (stride_dest < size || stride_src < size || off > o->size || (o->size - off - size) / stride_dest < count - 1)
+    return SRX_INVALID_PARAMETER;
+  HIPCHK(hipMemcpy2D(o->d + off, stride_dest, data, stride_src, size, count, hipMemcpyHostToDevice));
+  return SRX_OK;
+}
 int32_t srx_buffer_read(srx_buffer b, size_t off, void* data, size_t size) {
   auto* o = as<buffer_obj>(b, K_BUFFER);
   if (!o || off + size > o->size || (!data && size)) return SRX_INVALID_PARAMETER;

@@ -167,3 +167,36 @@ def test_end_does_not_wait_and_get_data_times_out(hip_lib):
     assert d.ia_primitives == 500
     st2, d2 = r.get_data(q)
     assert st2 == A.ASYNC_READY and bytes(d2) == bytes(d)
+
+
+def _transfer(lib):
+    r = lib.renderer()
+    b = r.create_buffer(256)
+    b.write(np.zeros(256, np.uint8))
+    src = np.arange(100, dtype=np.uint8)
+    # 4 rows of 6 bytes, 10 apart in the source, 32 apart in the buffer, from offset 8
+    b.transfer(src, 8, 32, 10, 6, 4)
+    got = np.frombuffer(b.read(), np.uint8)
+    want = np.zeros(256, np.uint8)
+    for i in range(4):
+        want[8 + 32 * i: 8 + 32 * i + 6] = src[10 * i: 10 * i + 6]
+    assert np.array_equal(got, want)
+    # contiguous rows are one copy
+    b.transfer(src, 0, 5, 5, 5, 20)
+    assert np.array_equal(np.frombuffer(b.read(100), np.uint8), src)
+    # the last row past the end copies nothing and fails (buffer.cpp:49)
+    before = b.read()
+    with pytest.raises(A.SrxError):
+        b.transfer(src, 200, 32, 10, 6, 3)
+    assert b.read() == before
+    b.transfer(src, 250, 32, 10, 6, 1)  # exactly to the end
+    assert np.array_equal(np.frombuffer(b.read(6, 250), np.uint8), src[:6])
+
+
+def test_buffer_transfer_oracle(cpu_lib):
+    _transfer(cpu_lib)
+
+
+@pytest.mark.gpu
+def test_buffer_transfer_hip(hip_lib):
+    _transfer(hip_lib)
