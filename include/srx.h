@@ -272,6 +272,7 @@ typedef struct srx_input_layout_t* srx_input_layout;
 typedef struct srx_raster_state_t* srx_raster_state;
 typedef struct srx_ds_state_t* srx_ds_state;
 typedef struct srx_query_t* srx_query;
+typedef struct srx_shader_code_t* srx_shader_code;
 
 /* ---- library ---- */
 uint32_t srx_abi_version(void);
@@ -389,6 +390,49 @@ typedef struct srx_mapped_resource {
 int32_t srx_map_buffer(srx_renderer r, srx_mapped_resource* mapped, srx_buffer b, int32_t mode);
 int32_t srx_map_surface(srx_renderer r, srx_mapped_resource* mapped, srx_surface s, int32_t mode);
 int32_t srx_unmap(srx_renderer r);
+
+/* ---- SASL pixel shaders (renderer.h:82-86, 136-147) ----
+ * srx_compile_shader is compile(code, lang) (renderer.cpp): it accepts the pixel-shader subset the
+ * reference's SIMD code generator implements (sasl/src/codegen/cg_simd.cpp): struct inputs/outputs
+ * or parameters with TEXCOORD(n) inputs and COLOR(n)/SV_Target(n) outputs, float/int uniforms and
+ * samplers, if/else, while, do-while, for, break, + - * / % (ints: + - *), comparisons, unary
+ * minus, swizzles and write masks, vector constructors, and the intrinsics abs acos asin atan
+ * ceil clamp cos cosh cross ddx ddy distance dot exp exp2 floor fmod frac ldexp length lerp log
+ * log10 log2 max min normalize pow rcp reflect round rsqrt saturate sin sinh sqrt tan tanh
+ * tex2D tex2Dgrad trunc.  Constructs the reference leaves unimplemented in pixel shaders (&&, ||,
+ * ?:, casts, continue, tex2Dlod/tex2Dbias/tex2Dproj) and matrices, arrays and helper functions
+ * return SRX_E_UNSUPPORTED; syntax and type errors SRX_INVALID_PARAMETER; srx_last_error_string()
+ * names the line.  Limits: 256 instructions, 32 registers, 32 literals and uniforms, 4 samplers,
+ * 8 inputs and outputs, nesting depth 8.  A quad stops after 65536 executed instructions.
+ *
+ * Pixel-shader input i (in TEXCOORD index order) is vertex-shader output attribute i
+ * (shader_unit.cpp:106-145 with a C++ vertex shader).  Colour target 0 receives COLOR(0) (zeros
+ * when the shader has none; components the output type lacks are zero).  A shader set with
+ * srx_set_pixel_shader_code shades instead of the srx_set_pixel_shader id (rasterizer.cpp:1402-1407)
+ * until it is cleared with NULL.  Variables (srx_set_ps_variable) and samplers
+ * (srx_set_ps_sampler_named) are kept by name on the renderer and applied at each draw
+ * (render_core.cpp:80-93); names the shader lacks are ignored, unset variables read zero, unbound
+ * samplers sample zero.  Draws whose colour such a shader writes run the in-order raster kernel. */
+enum srx_shader_language { SRX_LANG_VERTEX_SHADER = 2, SRX_LANG_PIXEL_SHADER = 3 };
+typedef struct srx_shader_info {
+  uint32_t input_count, output_count;
+  uint32_t input_components[8];
+  uint32_t output_targets[8], output_components[8]; /* outputs in COLOR index order */
+  uint32_t uniform_count, sampler_count, instruction_count, register_count;
+} srx_shader_info;
+int32_t srx_compile_shader(const char* code, size_t length, int32_t lang, srx_shader_code* out);
+/* shader_object::get_reflection (shader_reflection layouts) */
+int32_t srx_shader_reflect(srx_shader_code code, srx_shader_info* info);
+int32_t srx_set_pixel_shader_code(srx_renderer r, srx_shader_code code);
+/* get_pixel_shader_code: retained handle, NULL when none is set */
+int32_t srx_get_pixel_shader_code(srx_renderer r, srx_shader_code* out);
+int32_t srx_set_ps_variable(srx_renderer r, const char* name, const void* data, size_t size);
+/* set_ps_sampler(name, sampler) (renderer.h:83); NULL unbinds */
+int32_t srx_set_ps_sampler_named(srx_renderer r, const char* name, srx_sampler s);
+/* pixel_shader_unit::update + execute (shader_unit.cpp:106-176) on caller-supplied quads with the
+ * renderer's variables and named samplers: in holds nquads * 4 pixels (0 1 / 2 3) * input_count
+ * float4, out receives nquads * 4 * output_count float4.  Waits for the renderer's queued work. */
+int32_t srx_ps_execute_quads(srx_renderer r, srx_shader_code code, const float* in, float* out, size_t nquads);
 
 /* ---- render operations (renderer.h:117-129) ---- */
 /* begin/end do not wait for the GPU: they queue the counters' read-back on the renderer's stream. */

@@ -27,6 +27,8 @@
 #include "../include/srx.h"
 
 #include "oracle_sampler.h"
+#include "oracle_sasl.h"
+#include "../salviarenderer_amd/csrc/srx_sasl.h"
 
 #include <algorithm>
 #include <atomic>
@@ -168,7 +170,8 @@ pool& global_pool() {
 }
 
 // ---------------------------------------------------------------- objects
-enum obj_kind { K_RENDERER = 1, K_BUFFER, K_TEXTURE, K_SURFACE, K_SAMPLER, K_LAYOUT, K_RASTER, K_DS, K_QUERY, K_BATCH, K_SERVER };
+enum obj_kind { K_RENDERER = 1, K_BUFFER, K_TEXTURE, K_SURFACE, K_SAMPLER, K_LAYOUT, K_RASTER, K_DS, K_QUERY, K_BATCH, K_SERVER,
+                K_SHADER };
 
 struct object {
   std::atomic<int> refs{1};
@@ -500,7 +503,34 @@ struct ps_context {
   texture_view tex[SRX_MAX_SAMPLER_SLOTS];
   sampler_params sp[SRX_MAX_SAMPLER_SLOTS];
   bool has_tex[SRX_MAX_SAMPLER_SLOTS];
+  bool sasl = false;         // a SASL pixel shader shades instead of id
+  srx::sasl_program prog{};  // with its uniforms applied
 };
+
+// pixel_shader_unit::update + execute (shader_unit.cpp:106-176): input i of each pixel is vertex
+// output attribute prog.inputs[i].attr, the program runs the quad, colour target 0 takes COLOR(0)
+// (zero when the shader has none).
+void run_sasl_quad(const ps_context& c, const vso* q, vec4* color) {
+  using namespace srx;
+  float in[4][SASL_MAX_INPUTS][4] = {};
+  float out[4][SASL_MAX_OUTPUTS][4] = {};
+  for (int px = 0; px < 4; ++px)
+    for (uint32_t i = 0; i < c.prog.ninputs; ++i)
+      for (int k = 0; k < 4; ++k) in[px][i][k] = q[px].r[1 + c.prog.inputs[i].attr][k];
+  sasl_run_quad(c.prog, in, out, [&](int slot, int, float u, float v, float dxu, float dxv, float dyu, float dyv, float* r) {
+    if (!c.has_tex[slot]) {
+      r[0] = r[1] = r[2] = r[3] = 0.0f;
+      return;
+    }
+    const vec4 t = sample_2d_grad(c.tex[slot], c.sp[slot], u, v, dxu, dxv, dyu, dyv, 0.0f);
+    for (int k = 0; k < 4; ++k) r[k] = t[k];
+  });
+  for (int px = 0; px < 4; ++px) {
+    color[px] = v4(0, 0, 0, 0);
+    for (uint32_t o = 0; o < c.prog.noutputs; ++o)
+      if (c.prog.outputs[o].target == 0) color[px] = v4(out[px][o][0], out[px][o][1], out[px][o][2], out[px][o][3]);
+  }
+}
 
 // 3x3 percentage-closer shadow lookup (the tap layout of StandardShadowMap.cpp:86-106; each tap is a
 // tex2dlod at lod 0 and is lit when the light-space depth minus the bias does not exceed it).
@@ -651,6 +681,7 @@ void sasl_quad_derivatives(const float* p, float* ddx, float* ddy) {
 // Runs the pixel shader on the 4 pixels of a quad (cpp_pixel_shader::execute,
 // cpp_pixel_shader.cpp:62-74).  Returns the per-pixel pass mask (the sample shaders never discard).
 void run_ps_quad(const ps_context& c, const vso* q, vec4* color) {
+  if (c.sasl) return run_sasl_quad(c, q, color);
   switch (c.id) {
   case SRX_PS_GOURAUD:
     for (int i = 0; i < 4; ++i) color[i] = q[i].r[1];
@@ -729,6 +760,13 @@ struct vb_slot {
   size_t stride = 0, offset = 0;
 };
 
+// A compiled SASL pixel shader (the compiler is the product's front end; its programs are run by
+// sasl_run_quad here).
+struct shader_obj : object {
+  srx::sasl_module mod;
+  shader_obj() : object(K_SHADER) {}
+};
+
 struct render_state {
   vb_slot vbs[SRX_MAX_INPUT_SLOTS];
   ref<buffer_obj> ib;
@@ -749,6 +787,11 @@ struct render_state {
   srx_viewport target_vp{0, 0, 0, 0, 0, 0};
   size_t target_samples = 0;
   srx_viewport vp{0, 0, 0, 0, 0, 1};
+  // set_pixel_shader_code (renderer_impl.cpp:310-316) and the constant buffer's variables and
+  // samplers by name (shader_cbuffer.cpp, applied per draw by render_core.cpp:80-93)
+  ref<shader_obj> ps_code;
+  std::vector<std::pair<std::string, std::vector<uint8_t>>> ps_vars;
+  std::vector<std::pair<std::string, ref<sampler_obj>>> ps_named_samplers;
 };
 
 struct tri_info {
@@ -1359,6 +1402,10 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   if (nout <= 0) return fail(SRX_INVALID_PARAMETER, "no vertex shader bound");
   if (!st.layout) return fail(SRX_INVALID_PARAMETER, "no input layout");
   if (indexed && !st.ib) return fail(SRX_INVALID_PARAMETER, "no index buffer");
+  if (st.ps_code)
+    for (uint32_t i = 0; i < st.ps_code->mod.prog.ninputs; ++i)
+      if (st.ps_code->mod.prog.inputs[i].attr >= nout)
+        return fail(SRX_INVALID_PARAMETER, "the pixel shader reads more attributes than the vertex shader writes");
 
   // stream assembler register map (stream_assembler.cpp:57-86)
   vs_binding binds[8];
@@ -1410,11 +1457,24 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   c.stencil_ref = c.stencil ? (static_cast<uint32_t>(st.stencil_ref) & c.read_mask) : 0;
   c.ps.id = st.ps_id;
   c.ps.consts = st.ps_consts.data();
+  sampler_obj* bound[SRX_MAX_SAMPLER_SLOTS] = {};
+  if (st.ps_code) {
+    // SASL: its samplers are bound by name, slot k being its k-th sampler variable
+    const srx::sasl_module& m = st.ps_code->mod;
+    c.ps.sasl = true;
+    c.ps.prog = m.prog;
+    for (auto& v : st.ps_vars) srx::sasl_set_uniform(m, c.ps.prog, v.first, v.second.data(), v.second.size());
+    for (size_t k = 0; k < m.samplers.size() && k < SRX_MAX_SAMPLER_SLOTS; ++k)
+      for (auto& ns : st.ps_named_samplers)
+        if (ns.first == m.samplers[k]) bound[k] = ns.second.get();
+  } else {
+    for (int i = 0; i < SRX_MAX_SAMPLER_SLOTS; ++i) bound[i] = st.samplers[i].get();
+  }
   for (int i = 0; i < SRX_MAX_SAMPLER_SLOTS; ++i) {
-    c.ps.has_tex[i] = st.samplers[i] && st.samplers[i]->tex;
+    c.ps.has_tex[i] = bound[i] && bound[i]->tex;
     if (c.ps.has_tex[i]) {
-      c.ps.tex[i] = st.samplers[i]->tex->view();
-      c.ps.sp[i] = st.samplers[i]->params();
+      c.ps.tex[i] = bound[i]->tex->view();
+      c.ps.sp[i] = bound[i]->params();
     }
   }
 
@@ -1914,6 +1974,117 @@ int32_t srx_set_pixel_shader(srx_renderer rr, int32_t id, const void* consts, si
   r->st.ps_id = id;
   r->st.ps_consts.assign(16, 0.0f);
   if (consts && size) std::memcpy(r->st.ps_consts.data(), consts, std::min<size_t>(size, 64));
+  return SRX_OK;
+}
+int32_t srx_compile_shader(const char* code, size_t length, int32_t lang, srx_shader_code* out) {
+  if (!code || !out) return SRX_INVALID_PARAMETER;
+  *out = nullptr;
+  if (lang != SRX_LANG_PIXEL_SHADER)
+    return fail(SRX_E_UNSUPPORTED, "only pixel shaders are compiled; vertex shaders are the built-in ids");
+  auto* o = new shader_obj();
+  std::string msg;
+  const int st = srx::sasl_compile_ps(code, length, o->mod, msg);
+  if (st != srx::SASL_OK) {
+    delete o;
+    return fail(st == srx::SASL_UNSUPPORTED ? SRX_E_UNSUPPORTED : SRX_INVALID_PARAMETER, msg.c_str());
+  }
+  *out = reinterpret_cast<srx_shader_code>(static_cast<object*>(o));
+  return SRX_OK;
+}
+int32_t srx_shader_reflect(srx_shader_code c, srx_shader_info* info) {
+  auto* o = as<shader_obj>(c, K_SHADER);
+  if (!o || !info) return SRX_INVALID_PARAMETER;
+  std::memset(info, 0, sizeof *info);
+  const srx::sasl_program& p = o->mod.prog;
+  info->input_count = p.ninputs;
+  info->output_count = p.noutputs;
+  for (uint32_t i = 0; i < p.ninputs; ++i) info->input_components[i] = p.inputs[i].ncomp;
+  for (uint32_t i = 0; i < p.noutputs; ++i) {
+    info->output_targets[i] = p.outputs[i].target;
+    info->output_components[i] = p.outputs[i].ncomp;
+  }
+  info->uniform_count = static_cast<uint32_t>(o->mod.uniforms.size());
+  info->sampler_count = static_cast<uint32_t>(o->mod.samplers.size());
+  info->instruction_count = p.ninst;
+  info->register_count = p.nregs;
+  return SRX_OK;
+}
+int32_t srx_set_pixel_shader_code(srx_renderer rr, srx_shader_code c) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r) return SRX_INVALID_PARAMETER;
+  auto* o = c ? as<shader_obj>(c, K_SHADER) : nullptr;
+  if (c && !o) return SRX_INVALID_PARAMETER;
+  r->st.ps_code = ref<shader_obj>(o);
+  return SRX_OK;
+}
+int32_t srx_get_pixel_shader_code(srx_renderer rr, srx_shader_code* out) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !out) return SRX_INVALID_PARAMETER;
+  retain(r->st.ps_code.get());
+  *out = reinterpret_cast<srx_shader_code>(static_cast<object*>(r->st.ps_code.get()));
+  return SRX_OK;
+}
+int32_t srx_set_ps_variable(srx_renderer rr, const char* name, const void* data, size_t size) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !name || (size && !data)) return SRX_INVALID_PARAMETER;
+  std::vector<uint8_t> bytes(static_cast<const uint8_t*>(data), static_cast<const uint8_t*>(data) + size);
+  for (auto& v : r->st.ps_vars)
+    if (v.first == name) {
+      v.second = std::move(bytes);
+      return SRX_OK;
+    }
+  r->st.ps_vars.emplace_back(name, std::move(bytes));
+  return SRX_OK;
+}
+int32_t srx_set_ps_sampler_named(srx_renderer rr, const char* name, srx_sampler s) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !name) return SRX_INVALID_PARAMETER;
+  auto* so = s ? as<sampler_obj>(s, K_SAMPLER) : nullptr;
+  if (s && !so) return SRX_INVALID_PARAMETER;
+  for (auto& v : r->st.ps_named_samplers)
+    if (v.first == name) {
+      v.second = ref<sampler_obj>(so);
+      return SRX_OK;
+    }
+  r->st.ps_named_samplers.emplace_back(name, ref<sampler_obj>(so));
+  return SRX_OK;
+}
+int32_t srx_ps_execute_quads(srx_renderer rr, srx_shader_code c, const float* in, float* out, size_t nquads) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  auto* o = as<shader_obj>(c, K_SHADER);
+  if (!r || !o || (nquads && (!out || (!in && o->mod.prog.ninputs)))) return SRX_INVALID_PARAMETER;
+  if (nquads > (1u << 24)) return fail(SRX_E_UNSUPPORTED, "more than 2^24 quads in one call");
+  const srx::sasl_module& m = o->mod;
+  ps_context pc{};
+  pc.sasl = true;
+  pc.prog = m.prog;
+  for (auto& v : r->st.ps_vars) srx::sasl_set_uniform(m, pc.prog, v.first, v.second.data(), v.second.size());
+  for (size_t k = 0; k < m.samplers.size() && k < SRX_MAX_SAMPLER_SLOTS; ++k)
+    for (auto& ns : r->st.ps_named_samplers)
+      if (ns.first == m.samplers[k] && ns.second && ns.second->tex) {
+        pc.has_tex[k] = true;
+        pc.tex[k] = ns.second->tex->view();
+        pc.sp[k] = ns.second->params();
+      }
+  const uint32_t nin = pc.prog.ninputs, nout = pc.prog.noutputs;
+  for (size_t qi = 0; qi < nquads; ++qi) {
+    float qin[4][srx::SASL_MAX_INPUTS][4] = {};
+    float qout[4][srx::SASL_MAX_OUTPUTS][4] = {};
+    for (int px = 0; px < 4; ++px)
+      for (uint32_t i = 0; i < nin; ++i)
+        for (int k = 0; k < 4; ++k) qin[px][i][k] = in[((qi * 4 + px) * nin + i) * 4 + k];
+    sasl_run_quad(pc.prog, qin, qout, [&](int slot, int, float u, float v, float dxu, float dxv, float dyu, float dyv, float* res) {
+      if (!pc.has_tex[slot]) {
+        res[0] = res[1] = res[2] = res[3] = 0.0f;
+        return;
+      }
+      const vec4 t = sample_2d_grad(pc.tex[slot], pc.sp[slot], u, v, dxu, dxv, dyu, dyv, 0.0f);
+      for (int k = 0; k < 4; ++k) res[k] = t[k];
+    });
+    for (int px = 0; px < 4; ++px)
+      for (uint32_t oo = 0; oo < nout; ++oo)
+        for (int k = 0; k < 4; ++k) out[((qi * 4 + px) * nout + oo) * 4 + k] = qout[px][oo][k];
+  }
   return SRX_OK;
 }
 int32_t srx_set_ps_sampler(srx_renderer rr, uint32_t slot, srx_sampler s) {

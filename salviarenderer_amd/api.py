@@ -14,6 +14,7 @@ import numpy as np
 
 # ---- enums (include/srx.h) -------------------------------------------------------------
 OK, FAILED, OUT_OF_MEMORY, INVALID_PARAMETER, E_HIP, E_UNSUPPORTED = 0, 1, 2, 3, 16, 17
+LANG_VERTEX_SHADER, LANG_PIXEL_SHADER = 2, 3
 PF_RGBA32F, PF_RGB32F, PF_BGRA8, PF_RGBA8, PF_R32F, PF_RG32F, PF_R32I = range(7)
 PF_SIZE = {PF_RGBA32F: 16, PF_RGB32F: 12, PF_BGRA8: 4, PF_RGBA8: 4, PF_R32F: 4, PF_RG32F: 8, PF_R32I: 4}
 FMT_R32G32B32A32_FLOAT, FMT_R32G32B32_FLOAT, FMT_R32G32_FLOAT, FMT_R32_FLOAT = 2, 6, 16, 41
@@ -131,6 +132,13 @@ class ServerStats(C.Structure):
                                           "queued", "running")]
 
 
+class ShaderInfo(C.Structure):
+    _fields_ = [("input_count", C.c_uint32), ("output_count", C.c_uint32), ("input_components", C.c_uint32 * 8),
+                ("output_targets", C.c_uint32 * 8), ("output_components", C.c_uint32 * 8),
+                ("uniform_count", C.c_uint32), ("sampler_count", C.c_uint32), ("instruction_count", C.c_uint32),
+                ("register_count", C.c_uint32)]
+
+
 _H = C.c_void_p
 _SZ = C.c_size_t
 _I32 = C.c_int32
@@ -180,6 +188,13 @@ PROTOTYPES = {
     "srx_set_pixel_shader": (_I32, [_H, _I32, _P, _SZ]),
     "srx_set_ps_sampler": (_I32, [_H, _U32, _H]),
     "srx_set_blend_shader": (_I32, [_H, _I32]),
+    "srx_compile_shader": (_I32, [C.c_char_p, _SZ, _I32, C.POINTER(_H)]),
+    "srx_shader_reflect": (_I32, [_H, C.POINTER(ShaderInfo)]),
+    "srx_set_pixel_shader_code": (_I32, [_H, _H]),
+    "srx_get_pixel_shader_code": (_I32, [_H, C.POINTER(_H)]),
+    "srx_set_ps_variable": (_I32, [_H, C.c_char_p, _P, _SZ]),
+    "srx_set_ps_sampler_named": (_I32, [_H, C.c_char_p, _H]),
+    "srx_ps_execute_quads": (_I32, [_H, _H, _P, _P, _SZ]),
     "srx_set_rasterizer_state": (_I32, [_H, _H]),
     "srx_set_depth_stencil_state": (_I32, [_H, _H, _I32]),
     "srx_set_render_targets": (_I32, [_H, _SZ, C.POINTER(_H), _H]),
@@ -253,6 +268,13 @@ class Lib:
     def renderer(self, kind=RENDERER_BENCHMARK) -> "Renderer":
         return Renderer(self, kind)
 
+    def compile_shader(self, code: str, lang=LANG_PIXEL_SHADER) -> "ShaderCode":
+        """compile(code, lang) (salvia/include/salvia/core/renderer.h:136-147)."""
+        src = code.encode()
+        h = _H()
+        self.call("srx_compile_shader", src, len(src), lang, C.byref(h))
+        return ShaderCode(self, h.value)
+
 
 class _Obj:
     def __init__(self, lib: Lib, handle):
@@ -269,6 +291,15 @@ class _Obj:
             self.release()
         except Exception:
             pass
+
+
+class ShaderCode(_Obj):
+    """A compiled SASL pixel shader (shader_object)."""
+
+    def reflect(self) -> ShaderInfo:
+        info = ShaderInfo()
+        self.lib.call("srx_shader_reflect", self.h, C.byref(info))
+        return info
 
 
 class Buffer(_Obj):
@@ -549,6 +580,35 @@ class Renderer(_Obj):
     def set_ps_sampler(self, slot, samp):
         self._keep[f"samp{slot}"] = samp
         self.lib.call("srx_set_ps_sampler", self.h, slot, samp.h if samp is not None else None)
+
+    def set_pixel_shader_code(self, code):
+        self._keep["ps_code"] = code
+        self.lib.call("srx_set_pixel_shader_code", self.h, code.h if code is not None else None)
+
+    def get_pixel_shader_code(self):
+        h = _H()
+        self.lib.call("srx_get_pixel_shader_code", self.h, C.byref(h))
+        return ShaderCode(self.lib, h.value) if h.value else None
+
+    def set_ps_variable(self, name, value, dtype=np.float32):
+        a = np.ascontiguousarray(np.atleast_1d(np.asarray(value, dtype=dtype)))
+        self.lib.call("srx_set_ps_variable", self.h, name.encode(), a.ctypes.data, a.nbytes)
+
+    def set_ps_sampler_named(self, name, samp):
+        self._keep[f"psnamed:{name}"] = samp
+        self.lib.call("srx_set_ps_sampler_named", self.h, name.encode(), samp.h if samp is not None else None)
+
+    def ps_execute_quads(self, code, inputs):
+        """Runs `code` on quads: inputs (nquads, 4, input_count, 4) float32 -> (nquads, 4, output_count, 4)."""
+        info = code.reflect()
+        x = np.ascontiguousarray(inputs, np.float32)
+        nq = x.shape[0] if x.ndim else 0
+        if x.size and x.shape[1:] != (4, info.input_count, 4):
+            raise ValueError(f"inputs must be (nquads, 4, {info.input_count}, 4)")
+        out = np.zeros((nq, 4, info.output_count, 4), np.float32)
+        self.lib.call("srx_ps_execute_quads", self.h, code.h, x.ctypes.data if x.size else None,
+                      out.ctypes.data if out.size else None, nq)
+        return out
 
     def set_blend_shader(self, bid):
         self.lib.call("srx_set_blend_shader", self.h, bid)

@@ -13,7 +13,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 OUT_DIR = os.path.join(PKG, "lib")
 OUT = os.path.join(OUT_DIR, "libsrx_hip.so")
-SOURCES = ["srx_runtime.cpp", "srx_batch.cpp", "srx_server.cpp", "srx_noalloc.hip", "srx_kernels.hip"]
+SOURCES = ["srx_runtime.cpp", "srx_batch.cpp", "srx_server.cpp", "srx_sasl.cpp", "srx_noalloc.hip", "srx_kernels.hip"]
 ARCH = "gfx950"
 
 FLAGS = [

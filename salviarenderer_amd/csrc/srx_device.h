@@ -48,6 +48,10 @@ struct input_elem_dev {
   float default_w;
 };
 
+struct sasl_program;
+// ps_id of a draw whose pixel shader is a compiled SASL program (set_pixel_shader_code)
+constexpr int32_t PS_SASL = 0x100;
+
 // Everything a draw's kernels read; passed by value as a kernel argument.
 struct draw_params {
   // vertex stage
@@ -90,6 +94,7 @@ struct draw_params {
   int32_t ps_id;
   float ps_consts[16];
   int32_t has_sampler[MAX_SAMPLERS];
+  const sasl_program* sasl;  // ps_id == PS_SASL: the compiled SASL program (device memory)
 };
 
 // ------------------------------------------------------------------ scalar helpers

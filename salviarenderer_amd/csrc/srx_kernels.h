@@ -154,6 +154,10 @@ bool launch_vis(const draw_params& P, const draw_buffers& B, int which, bool sha
 // (first launch), merges (second launch) and partial-key slots (256 * samples keys each).
 void vis_table_bounds(uint32_t T, uint32_t records, uint32_t* entries, uint32_t* merges, uint32_t* slots);
 bool needs_shading(const draw_params& P);
+// A SASL pixel shader whose colour is written: such draws run the in-order raster kernel.
+bool sasl_shades(const draw_params& P);
+void launch_sasl_quads(const sasl_program* prog, const ps_samplers* SM, uint32_t has_mask, const float4* in, float4* out,
+                       uint32_t nquads, hipStream_t s);
 void launch_shade(const draw_params& P, const draw_buffers& B, hipStream_t s);
 
 void launch_clear(const surface_desc& s, float4 value, hipStream_t st);

@@ -16,6 +16,7 @@
 //                   the final per-sample winners.
 #include "srx_kernels.h"
 #include "srx_sampler.h"
+#include "srx_sasl_exec.h"
 
 #include <type_traits>
 
@@ -2013,12 +2014,47 @@ constexpr uint32_t NO_WINNER = 0xFFFFFFFFu;
 // Pixel shader and backend for one triangle on a 2x2 quad: `pm` holds the samples to store, bits
 // 4i..4i+S-1 for pixel i.  Interpolants are formed per pixel where read (quad_lazy), so only one
 // pixel's registers are live at a time; the colour of each pixel is the quad shader's.
-template <int NA, int S>
+template <int NA, int S, bool SASL = false>
 __device__ __forceinline__ void shade_store(const draw_params& P, const ps_samplers* __restrict__ SM,
                                          const float4* __restrict__ tri_attr, uint32_t id, int px0, int py0, uint32_t pm) {
   attr_setup<NA> at;
   load_attr<NA>(P, tri_attr, id, at);
   const quad_lazy<NA> q(at, P, px0, py0);
+  if constexpr (SASL) {
+    // pixel_shader_unit::update + execute (shader_unit.cpp:106-176) on the whole quad: input i is
+    // attribute inputs[i].attr of each pixel, the program runs the four pixels together
+    const sasl_program* __restrict__ prog = P.sasl;
+    sasl_regs R;
+    const uint32_t nregs = prog->nregs;
+    for (uint32_t r = 0; r < nregs; ++r)
+      for (int px = 0; px < 4; ++px) R.r[r][px] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint32_t nin = prog->ninputs;
+    for (int px = 0; px < 4; ++px) {
+      const float iw = q.inv_w(px);
+      for (uint32_t k = 0; k < nin; ++k) {
+        const sasl_input inp = prog->inputs[k];
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+          if (a == inp.attr) v = q.reg(a + 1, px, iw);
+        R.r[inp.reg][px] = make_float4(v.x, inp.ncomp > 1 ? v.y : 0.0f, inp.ncomp > 2 ? v.z : 0.0f, inp.ncomp > 3 ? v.w : 0.0f);
+      }
+    }
+    sasl_exec(prog, R, [&](int slot, int, float u, float v, float dxu, float dxv, float dyu, float dyv) -> float4 {
+      if (!P.has_sampler[slot]) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      const sampler_desc_dev& sp = SM->s[slot];
+      return sp.fast_aniso ? sample_2d_grad_fast(sp, u, v, dxu, dxv, dyu, dyv, 0.0f)
+                           : sample_2d_grad(sp, u, v, dxu, dxv, dyu, dyv, 0.0f);
+    });
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t m = (pm >> (4 * i)) & 15u;
+      if (!m) continue;
+      const float4 c = sasl_color0(prog, R, i);
+      for (int s = 0; s < S; ++s)
+        if (m & (1u << s)) blend_store(P, px0 + (i & 1), py0 + (i >> 1), s, c);
+    }
+    return;
+  }
 #pragma unroll 1
   for (int i = 0; i < 4; ++i) {
     const uint32_t m = (pm >> (4 * i)) & 15u;
@@ -2032,7 +2068,7 @@ __device__ __forceinline__ void shade_store(const draw_params& P, const ps_sampl
 // One workgroup per GPU tile, one thread per 2x2 quad.  VIS: opaque early-Z draws -- record for every
 // sample the last triangle that passed (the colour it ends with) into `vis`; the pixel shader runs
 // later in shade_kernel.  Otherwise every quad runs the pixel shader and the backend in primitive order.
-template <int NA, int S, bool VIS>
+template <int NA, int S, bool VIS, bool SASL = false>
 __global__ void __launch_bounds__(256) raster_kernel(draw_params P, const ps_samplers* __restrict__ SM,
                                                       const uint32_t* __restrict__ order, const uint32_t* __restrict__ vals,
                                                       const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ tend,
@@ -2263,8 +2299,8 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, const ps_sam
               for (int s = 0; s < S; ++s)
                 if (tested[i] & (1u << s)) winner[i][s] = id;
           } else {
-            shade_store<NA, S>(P, SM, tri_attr, id, px0, py0,
-                               tested[0] | (tested[1] << 4) | (tested[2] << 8) | (tested[3] << 12));
+            shade_store<NA, S, SASL>(P, SM, tri_attr, id, px0, py0,
+                                     tested[0] | (tested[1] << 4) | (tested[2] << 8) | (tested[3] << 12));
           }
         } else if constexpr (!VIS) {
           // late depth/stencil (render_sample, framebuffer.cpp:445-479); writes only when both pass.
@@ -2287,8 +2323,8 @@ __global__ void __launch_bounds__(256) raster_kernel(draw_params P, const ps_sam
               if (dp && sp) passed[i] |= 1u << s;
             }
           if ((passed[0] | passed[1] | passed[2] | passed[3]) == 0) continue;
-          shade_store<NA, S>(P, SM, tri_attr, id, px0, py0,
-                             passed[0] | (passed[1] << 4) | (passed[2] << 8) | (passed[3] << 12));
+          shade_store<NA, S, SASL>(P, SM, tri_attr, id, px0, py0,
+                                   passed[0] | (passed[1] << 4) | (passed[2] << 8) | (passed[3] << 12));
 #pragma unroll
           for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -3032,12 +3068,16 @@ void launch_raster(const draw_params& P, const draw_buffers& B, int which, hipSt
   if (!T) return;
   int threads = (P.gt / 2) * (P.gt / 2);
   size_t lds = sizeof(prim_lds) * threads;
-  bool vis = P.early_z && (P.blend_id == BLEND_OFF || P.blend_id == BLEND_DISABLED);
+  const bool sasl = P.ps_id == PS_SASL && sasl_shades(P);
+  bool vis = P.early_z && (P.blend_id == BLEND_OFF || P.blend_id == BLEND_DISABLED) && !sasl;
 #define SRX_RASTER_S(NA, S)                                                                                          \
   do {                                                                                                               \
     if (vis)                                                                                                         \
       hipLaunchKernelGGL((raster_kernel<NA, S, true>), dim3(T), dim3(threads), lds, s, P, B.samplers, B.order, B.vals[which], \
                          B.tstart, B.tend, B.tri_rec, B.tri_attr, B.vis, B.partial);                                 \
+    else if (sasl)                                                                                                   \
+      hipLaunchKernelGGL((raster_kernel<NA, S, false, true>), dim3(T), dim3(threads), lds, s, P, B.samplers, B.order, \
+                         B.vals[which], B.tstart, B.tend, B.tri_rec, B.tri_attr, B.vis, B.partial);                  \
     else                                                                                                             \
       hipLaunchKernelGGL((raster_kernel<NA, S, false>), dim3(T), dim3(threads), lds, s, P, B.samplers, B.order,      \
                          B.vals[which], B.tstart, B.tend, B.tri_rec, B.tri_attr, B.vis, B.partial);                  \
@@ -3060,6 +3100,7 @@ void launch_raster(const draw_params& P, const draw_buffers& B, int which, hipSt
 // Visibility mode of an opaque early-Z draw for vis_kernel, or -1 when only the in-order kernel
 // reproduces the reference (NOT_EQUAL with depth writes).
 int vis_mode(const draw_params& P) {
+  if (sasl_shades(P)) return -1;  // SASL pixel shaders run in the in-order kernel only
   if (P.vis_stencil && !(P.depth_enable && P.write_depth && (P.depth_func == 1 || P.depth_func == 3 ||
                                                             P.depth_func == 4 || P.depth_func == 6)))
     return -1;  // a stencil mask is expressed through the depth keys only
@@ -3122,6 +3163,49 @@ bool launch_vis(const draw_params& P, const draw_buffers& B, int which, bool sha
 }
 
 bool vis_applicable(const draw_params& P) { return vis_mode(P) >= 0 && P.gt == 16; }
+
+// One thread per quad: pixel_shader_unit::update/execute on caller-supplied inputs (the quad
+// evaluation behind srx_ps_execute_quads).  in: [quad][pixel][input] float4, out: [quad][pixel][output].
+__global__ void __launch_bounds__(64) sasl_quads_kernel(const sasl_program* __restrict__ prog,
+                                                        const ps_samplers* __restrict__ SM, uint32_t has_mask,
+                                                        const float4* __restrict__ in, float4* __restrict__ out,
+                                                        uint32_t nquads) {
+  const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (qi >= nquads) return;
+  const uint32_t nin = prog->ninputs, nout = prog->noutputs, nregs = prog->nregs;
+  sasl_regs R;
+  for (uint32_t r = 0; r < nregs; ++r)
+    for (int px = 0; px < 4; ++px) R.r[r][px] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (int px = 0; px < 4; ++px)
+    for (uint32_t k = 0; k < nin; ++k) {
+      const sasl_input inp = prog->inputs[k];
+      const float4 v = in[(static_cast<size_t>(qi) * 4 + px) * nin + k];
+      R.r[inp.reg][px] = make_float4(v.x, inp.ncomp > 1 ? v.y : 0.0f, inp.ncomp > 2 ? v.z : 0.0f, inp.ncomp > 3 ? v.w : 0.0f);
+    }
+  sasl_exec(prog, R, [&](int slot, int, float u, float v, float dxu, float dxv, float dyu, float dyv) -> float4 {
+    if (!((has_mask >> slot) & 1u)) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const sampler_desc_dev& sp = SM->s[slot];
+    return sp.fast_aniso ? sample_2d_grad_fast(sp, u, v, dxu, dxv, dyu, dyv, 0.0f)
+                         : sample_2d_grad(sp, u, v, dxu, dxv, dyu, dyv, 0.0f);
+  });
+  for (int px = 0; px < 4; ++px)
+    for (uint32_t o = 0; o < nout; ++o) {
+      const float4 v = R.r[prog->outputs[o].reg][px];
+      const int nc = prog->outputs[o].ncomp;
+      out[(static_cast<size_t>(qi) * 4 + px) * nout + o] =
+          make_float4(v.x, nc > 1 ? v.y : 0.0f, nc > 2 ? v.z : 0.0f, nc > 3 ? v.w : 0.0f);
+    }
+}
+
+void launch_sasl_quads(const sasl_program* prog, const ps_samplers* SM, uint32_t has_mask, const float4* in, float4* out,
+                       uint32_t nquads, hipStream_t s) {
+  if (!nquads) return;
+  hipLaunchKernelGGL(sasl_quads_kernel, dim3((nquads + 63) / 64), dim3(64), 0, s, prog, SM, has_mask, in, out, nquads);
+}
+
+bool sasl_shades(const draw_params& P) {
+  return P.ps_id == PS_SASL && P.color.data != nullptr && P.blend_id != BLEND_DISABLED;
+}
 
 bool needs_shading(const draw_params& P) {
   return (P.early_z || P.vis_stencil) && P.blend_id == BLEND_OFF && P.color.data != nullptr;

@@ -11,6 +11,37 @@
 namespace srx_rt {
 
 
+// The device descriptor of a sampler and its texture (sampler.cpp's state); queues pending clears
+// of the texture's levels.
+int32_t fill_sampler(renderer_obj* r, sampler_obj* so, sampler_desc_dev& sd) {
+  texture_obj* t = so->tex.get();
+  sd.min_filter = so->desc.min_filter;
+  sd.mag_filter = so->desc.mag_filter;
+  sd.mip_filter = so->desc.mip_filter;
+  sd.mip_qual = so->desc.mip_qual;
+  sd.addr_u = so->desc.addr_mode_u;
+  sd.addr_v = so->desc.addr_mode_v;
+  sd.lod_bias = so->desc.mip_lod_bias;
+  sd.max_aniso = so->desc.max_anisotropy;
+  for (int k = 0; k < 4; ++k) sd.border[k] = so->desc.border_color[k];
+  sd.nlevels = static_cast<int>(std::min<size_t>(t->surfs.size(), MAX_LEVELS));
+  sd.max_lod = t->max_lod;
+  sd.min_lod = std::min(t->min_lod, sd.nlevels - 1);
+  sd.size[0] = t->size[0];
+  sd.size[1] = t->size[1];
+  sd.size[2] = t->size[2];
+  for (int l = 0; l < sd.nlevels; ++l) {
+    RCHK(run_pending_clear(t->surfs[l].get(), r->stream));
+    sd.levels[l] = t->surfs[l]->desc();
+  }
+  bool rgba8 = sd.nlevels > 0;
+  for (int l = 0; l < sd.nlevels; ++l) rgba8 &= sd.levels[l].pf == SRX_PF_RGBA8 && sd.levels[l].samples == 1;
+  sd.fast_aniso = rgba8 && sd.mip_filter == SRX_FILTER_ANISOTROPIC && sd.max_aniso > 1 &&
+                  sd.min_filter == SRX_FILTER_LINEAR && sd.mag_filter == SRX_FILTER_LINEAR &&
+                  sd.addr_u == SRX_ADDRESS_WRAP && sd.addr_v == SRX_ADDRESS_WRAP;
+  return SRX_OK;
+}
+
 // The draw's parameters and sampler table from the renderer's state (renderer_impl.cpp:159-238
 // validation); skip = nothing to draw into.
 int32_t build_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t primcnt, int32_t basevert, draw_params& P,
@@ -129,35 +160,26 @@ int32_t build_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t primc
   std::memcpy(P.ps_consts, st.ps_consts, sizeof P.ps_consts);
 
   std::memset(&SM, 0, sizeof SM);
+  // SASL pixel shader: its samplers are bound by name, slot k being its k-th sampler variable
+  sampler_obj* bound[MAX_SAMPLERS] = {};
+  if (st.ps_code) {
+    const sasl_module& m = st.ps_code->mod;
+    for (uint32_t i = 0; i < m.prog.ninputs; ++i)
+      if (m.prog.inputs[i].attr >= nout)
+        return fail(SRX_INVALID_PARAMETER, "the pixel shader reads more attributes than the vertex shader writes");
+    P.ps_id = PS_SASL;
+    r->sasl_cur = m.prog;
+    for (auto& v : st.ps_vars) sasl_set_uniform(m, r->sasl_cur, v.first, v.second.data(), v.second.size());
+    for (size_t k = 0; k < m.samplers.size() && k < MAX_SAMPLERS; ++k)
+      for (auto& ns : st.ps_named_samplers)
+        if (ns.first == m.samplers[k]) bound[k] = ns.second.get();
+  } else {
+    for (int i = 0; i < MAX_SAMPLERS; ++i) bound[i] = st.samplers[i].get();
+  }
   for (int i = 0; i < MAX_SAMPLERS; ++i) {
-    sampler_obj* so = st.samplers[i].get();
+    sampler_obj* so = bound[i];
     if (!so || !so->tex) continue;
-    texture_obj* t = so->tex.get();
-    sampler_desc_dev& sd = SM.s[i];
-    sd.min_filter = so->desc.min_filter;
-    sd.mag_filter = so->desc.mag_filter;
-    sd.mip_filter = so->desc.mip_filter;
-    sd.mip_qual = so->desc.mip_qual;
-    sd.addr_u = so->desc.addr_mode_u;
-    sd.addr_v = so->desc.addr_mode_v;
-    sd.lod_bias = so->desc.mip_lod_bias;
-    sd.max_aniso = so->desc.max_anisotropy;
-    for (int k = 0; k < 4; ++k) sd.border[k] = so->desc.border_color[k];
-    sd.nlevels = static_cast<int>(std::min<size_t>(t->surfs.size(), MAX_LEVELS));
-    sd.max_lod = t->max_lod;
-    sd.min_lod = std::min(t->min_lod, sd.nlevels - 1);
-    sd.size[0] = t->size[0];
-    sd.size[1] = t->size[1];
-    sd.size[2] = t->size[2];
-    for (int l = 0; l < sd.nlevels; ++l) {
-      RCHK(run_pending_clear(t->surfs[l].get(), r->stream));
-      sd.levels[l] = t->surfs[l]->desc();
-    }
-    bool rgba8 = sd.nlevels > 0;
-    for (int l = 0; l < sd.nlevels; ++l) rgba8 &= sd.levels[l].pf == SRX_PF_RGBA8 && sd.levels[l].samples == 1;
-    sd.fast_aniso = rgba8 && sd.mip_filter == SRX_FILTER_ANISOTROPIC && sd.max_aniso > 1 &&
-                    sd.min_filter == SRX_FILTER_LINEAR && sd.mag_filter == SRX_FILTER_LINEAR &&
-                    sd.addr_u == SRX_ADDRESS_WRAP && sd.addr_v == SRX_ADDRESS_WRAP;
+    RCHK(fill_sampler(r, so, SM.s[i]));
     P.has_sampler[i] = 1;
   }
   // Opaque draws whose samples keep or lose a colour in one depth reduction: early-Z draws, and
@@ -181,6 +203,11 @@ struct draw_scratch {
       p = a.p;
       return SRX_OK;
     }
+    return carve(n, p);
+  }
+  // n elements of the fixed region
+  template <class T>
+  int32_t carve(size_t n, T*& p) {
     const size_t off = (used + 255) & ~static_cast<size_t>(255);
     if (off > fixed->bytes || n > (fixed->bytes - off) / sizeof(T))
       return fail(SRX_OUT_OF_MEMORY, "the draw's scratch exceeds the no-allocation region");
@@ -239,7 +266,8 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   draw_scratch X{r, fixed};
   draw_buffers B{};
   uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
-  bool vis_pass = (P.early_z || P.vis_stencil) && (P.blend_id == SRX_BLEND_OFF || P.blend_id == SRX_BLEND_DISABLED);
+  bool vis_pass = (P.early_z || P.vis_stencil) && (P.blend_id == SRX_BLEND_OFF || P.blend_id == SRX_BLEND_DISABLED) &&
+                  !sasl_shades(P);
   RCHK(X.take(r->vbuf, (static_cast<size_t>(P.nverts) + 1) * (nout + 1), B.vbuf));
   RCHK(X.take(r->tri_rec, static_cast<size_t>(P3) * 5, B.tri_rec));
   RCHK(X.take(r->tri_attr, static_cast<size_t>(P3) * 3 * (nout + 1), B.tri_attr));
@@ -249,6 +277,16 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   ps_samplers* samplers_d = nullptr;
   RCHK(X.take(r->samplers_d, 1, samplers_d));
   B.samplers = samplers_d;
+  if (P.ps_id == PS_SASL) {
+    sasl_program* prog_d = nullptr;
+    if (fixed) {
+      RCHK(X.carve(1, prog_d));
+    } else {
+      if (!r->sasl_d) HIPCHK(hipMalloc(&r->sasl_d, sizeof(sasl_program)));
+      prog_d = r->sasl_d;
+    }
+    P.sasl = prog_d;
+  }
   RCHK(X.take(r->cnt, P3, B.cnt));
   RCHK(X.take(r->clip_flag, primcnt, B.clip_flag));
   RCHK(X.take(r->clip_slot, primcnt, B.clip_slot));
@@ -322,6 +360,16 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
     r->samplers_last = SM;
     r->samplers_last_d = samplers_d;
     r->samplers_valid = !fixed;
+  }
+  // the SASL program with its uniforms, staged like the sampler table
+  if (P.ps_id == PS_SASL) {
+    const bool same_prog = !fixed && r->sasl_valid && std::memcmp(&r->sasl_last, &r->sasl_cur, sizeof(sasl_program)) == 0;
+    if (!same_prog) {
+      *r->sasl_h = r->sasl_cur;
+      HIPCHK(hipMemcpyAsync(const_cast<sasl_program*>(P.sasl), r->sasl_h, sizeof(sasl_program), hipMemcpyHostToDevice, s));
+      r->sasl_last = r->sasl_cur;
+      r->sasl_valid = !fixed;
+    }
   }
   launch_draw_begin(B.counters, B.scan_tmp, scan_state_words(P3), B.tile_count, 2 * static_cast<size_t>(T), s);
   mark(0);
@@ -457,7 +505,9 @@ renderer_obj::~renderer_obj() {
     if (a.p) (void)hipFree(a.p);
   };
   fr(vbuf); fr(tri_rec); fr(tri_attr); fr(clip_verts); fr(samplers_d); fr(vis);
-  if (samplers_h) (void)hipHostFree(samplers_h); fr(cnt); fr(keys0); fr(keys1); fr(vals0); fr(vals1);
+  if (samplers_h) (void)hipHostFree(samplers_h);
+  if (sasl_h) (void)hipHostFree(sasl_h);
+  if (sasl_d) (void)hipFree(sasl_d); fr(cnt); fr(keys0); fr(keys1); fr(vals0); fr(vals1);
   fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot); fr(partial); fr(order); fr(tile_count); fr(wide); fr(vis_entries); fr(vis_merges); fr(vis_counts); fr(vis_partial);
   if (stats_d) (void)hipFree(stats_d);
   if (pinned) (void)hipHostFree(pinned);
@@ -604,6 +654,7 @@ int32_t srx_create_renderer(uint32_t kind, srx_renderer* out) {
   if (e == hipSuccess) e = hipMemset(r->stats_d, 0, ST_COUNT * sizeof(unsigned long long));
   if (e == hipSuccess) e = hipHostMalloc(&r->pinned, CT_COUNT * sizeof(uint32_t), hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc(&r->samplers_h, sizeof(ps_samplers), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc(&r->sasl_h, sizeof(sasl_program), hipHostMallocDefault);
   if (e != hipSuccess) {
     delete r;
     return hip_fail(e, "srx_create_renderer");
@@ -854,6 +905,121 @@ int32_t srx_set_ps_sampler(srx_renderer rr, uint32_t slot, srx_sampler s) {
   auto* r = as<renderer_obj>(rr, K_RENDERER);
   if (!r || slot >= SRX_MAX_SAMPLER_SLOTS) return SRX_INVALID_PARAMETER;
   r->st.samplers[slot] = ref<sampler_obj>(as<sampler_obj>(s, K_SAMPLER));
+  return SRX_OK;
+}
+int32_t srx_compile_shader(const char* code, size_t length, int32_t lang, srx_shader_code* out) {
+  if (!code || !out) return SRX_INVALID_PARAMETER;
+  *out = nullptr;
+  if (lang != SRX_LANG_PIXEL_SHADER)
+    return fail(SRX_E_UNSUPPORTED, "only pixel shaders are compiled; vertex shaders are the built-in ids");
+  auto* o = new shader_obj();
+  std::string msg;
+  const int st = sasl_compile_ps(code, length, o->mod, msg);
+  if (st != SASL_OK) {
+    delete o;
+    return fail(st == SASL_UNSUPPORTED ? SRX_E_UNSUPPORTED : SRX_INVALID_PARAMETER, msg.c_str());
+  }
+  *out = reinterpret_cast<srx_shader_code>(static_cast<object*>(o));
+  return SRX_OK;
+}
+int32_t srx_shader_reflect(srx_shader_code c, srx_shader_info* info) {
+  auto* o = as<shader_obj>(c, K_SHADER);
+  if (!o || !info) return SRX_INVALID_PARAMETER;
+  std::memset(info, 0, sizeof *info);
+  const sasl_program& p = o->mod.prog;
+  info->input_count = p.ninputs;
+  info->output_count = p.noutputs;
+  for (uint32_t i = 0; i < p.ninputs; ++i) info->input_components[i] = p.inputs[i].ncomp;
+  for (uint32_t i = 0; i < p.noutputs; ++i) {
+    info->output_targets[i] = p.outputs[i].target;
+    info->output_components[i] = p.outputs[i].ncomp;
+  }
+  info->uniform_count = static_cast<uint32_t>(o->mod.uniforms.size());
+  info->sampler_count = static_cast<uint32_t>(o->mod.samplers.size());
+  info->instruction_count = p.ninst;
+  info->register_count = p.nregs;
+  return SRX_OK;
+}
+int32_t srx_set_pixel_shader_code(srx_renderer rr, srx_shader_code c) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r) return SRX_INVALID_PARAMETER;
+  auto* o = c ? as<shader_obj>(c, K_SHADER) : nullptr;
+  if (c && !o) return SRX_INVALID_PARAMETER;
+  r->st.ps_code = ref<shader_obj>(o);
+  return SRX_OK;
+}
+int32_t srx_get_pixel_shader_code(srx_renderer rr, srx_shader_code* out) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !out) return SRX_INVALID_PARAMETER;
+  retain(r->st.ps_code.get());
+  *out = reinterpret_cast<srx_shader_code>(static_cast<object*>(r->st.ps_code.get()));
+  return SRX_OK;
+}
+int32_t srx_set_ps_variable(srx_renderer rr, const char* name, const void* data, size_t size) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !name || (size && !data)) return SRX_INVALID_PARAMETER;
+  std::vector<uint8_t> bytes(static_cast<const uint8_t*>(data), static_cast<const uint8_t*>(data) + size);
+  for (auto& v : r->st.ps_vars)
+    if (v.first == name) {
+      v.second = std::move(bytes);
+      return SRX_OK;
+    }
+  r->st.ps_vars.emplace_back(name, std::move(bytes));
+  return SRX_OK;
+}
+int32_t srx_set_ps_sampler_named(srx_renderer rr, const char* name, srx_sampler s) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !name) return SRX_INVALID_PARAMETER;
+  auto* so = s ? as<sampler_obj>(s, K_SAMPLER) : nullptr;
+  if (s && !so) return SRX_INVALID_PARAMETER;
+  for (auto& v : r->st.ps_named_samplers)
+    if (v.first == name) {
+      v.second = ref<sampler_obj>(so);
+      return SRX_OK;
+    }
+  r->st.ps_named_samplers.emplace_back(name, ref<sampler_obj>(so));
+  return SRX_OK;
+}
+int32_t srx_ps_execute_quads(srx_renderer rr, srx_shader_code c, const float* in, float* out, size_t nquads) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  auto* o = as<shader_obj>(c, K_SHADER);
+  if (!r || !o || (nquads && (!out || (!in && o->mod.prog.ninputs)))) return SRX_INVALID_PARAMETER;
+  if (nquads > (1u << 24)) return fail(SRX_E_UNSUPPORTED, "more than 2^24 quads in one call");
+  if (!nquads) return SRX_OK;
+  if (r->recording) return batch_refuse(r, SRX_E_UNSUPPORTED, "a batch does not record shader evaluations");
+  const sasl_module& m = o->mod;
+  sasl_program prog = m.prog;
+  for (auto& v : r->st.ps_vars) sasl_set_uniform(m, prog, v.first, v.second.data(), v.second.size());
+  // sampler table of the named samplers, as a draw builds it
+  ps_samplers SM;
+  std::memset(&SM, 0, sizeof SM);
+  uint32_t has = 0;
+  for (size_t k = 0; k < m.samplers.size() && k < MAX_SAMPLERS; ++k)
+    for (auto& ns : r->st.ps_named_samplers)
+      if (ns.first == m.samplers[k] && ns.second && ns.second->tex) {
+        RCHK(fill_sampler(r, ns.second.get(), SM.s[k]));
+        has |= 1u << k;
+      }
+  const size_t in_n = nquads * 4 * m.prog.ninputs, out_n = nquads * 4 * m.prog.noutputs;
+  void* dev = nullptr;
+  const size_t bytes = sizeof(sasl_program) + sizeof(ps_samplers) + (in_n + out_n) * sizeof(float4);
+  HIPCHK(hipStreamSynchronize(r->stream));
+  HIPCHK(hipMalloc(&dev, bytes));
+  auto* prog_d = static_cast<sasl_program*>(dev);
+  auto* sm_d = reinterpret_cast<ps_samplers*>(prog_d + 1);
+  auto* in_d = reinterpret_cast<float4*>(sm_d + 1);
+  float4* out_d = in_d + in_n;
+  hipError_t he = hipMemcpyAsync(prog_d, &prog, sizeof prog, hipMemcpyHostToDevice, r->stream);
+  if (he == hipSuccess) he = hipMemcpyAsync(sm_d, &SM, sizeof SM, hipMemcpyHostToDevice, r->stream);
+  if (he == hipSuccess && in_n) he = hipMemcpyAsync(in_d, in, in_n * sizeof(float4), hipMemcpyHostToDevice, r->stream);
+  if (he == hipSuccess) {
+    launch_sasl_quads(prog_d, sm_d, has, in_d, out_d, static_cast<uint32_t>(nquads), r->stream);
+    he = hipGetLastError();
+  }
+  if (he == hipSuccess && out_n) he = hipMemcpyAsync(out, out_d, out_n * sizeof(float4), hipMemcpyDeviceToHost, r->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(r->stream);
+  (void)hipFree(dev);
+  if (he != hipSuccess) return hip_fail(he, "srx_ps_execute_quads");
   return SRX_OK;
 }
 int32_t srx_set_blend_shader(srx_renderer rr, int32_t id) {

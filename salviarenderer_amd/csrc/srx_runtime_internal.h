@@ -4,6 +4,7 @@
 
 #include "../../include/srx.h"
 #include "srx_kernels.h"
+#include "srx_sasl.h"
 
 #include <hip/hip_runtime.h>
 
@@ -41,7 +42,8 @@ inline int32_t hip_fail(hipError_t e, const char* where) {
   } while (0)
 
 // ---------------------------------------------------------------- objects
-enum obj_kind { K_RENDERER = 1, K_BUFFER, K_TEXTURE, K_SURFACE, K_SAMPLER, K_LAYOUT, K_RASTER, K_DS, K_QUERY, K_BATCH, K_SERVER };
+enum obj_kind { K_RENDERER = 1, K_BUFFER, K_TEXTURE, K_SURFACE, K_SAMPLER, K_LAYOUT, K_RASTER, K_DS, K_QUERY, K_BATCH, K_SERVER,
+                K_SHADER };
 
 struct object {
   std::atomic<int> refs{1};
@@ -147,6 +149,12 @@ struct sampler_obj : object {
   srx_sampler_desc desc;
   ref<texture_obj> tex;
   sampler_obj() : object(K_SAMPLER) {}
+};
+
+// A compiled SASL pixel shader (shader_object, salvia/include/salvia/shader/shader_object.h).
+struct shader_obj : object {
+  sasl_module mod;
+  shader_obj() : object(K_SHADER) {}
 };
 
 struct layout_elem {
@@ -337,6 +345,12 @@ struct render_state {
   srx_viewport target_vp{0, 0, 0, 0, 0, 0};
   size_t target_samples = 0;
   srx_viewport vp{0, 0, 0, 0, 0, 1};
+  // set_pixel_shader_code (renderer_impl.cpp:310-316): when set it shades instead of ps_id
+  // (rasterizer.cpp:1402-1407); its variables and samplers are the renderer's constant buffer
+  // (shader_cbuffer, set by name and kept across shaders, render_core.cpp:80-93)
+  ref<shader_obj> ps_code;
+  std::vector<std::pair<std::string, std::vector<uint8_t>>> ps_vars;
+  std::vector<std::pair<std::string, ref<sampler_obj>>> ps_named_samplers;
 };
 
 template <class T>
@@ -361,6 +375,13 @@ struct renderer_obj : object {
   ps_samplers samplers_last{};
   const ps_samplers* samplers_last_d = nullptr;
   bool samplers_valid = false;
+  // the SASL program of the draw being built (uniforms applied), its device copy, pinned staging
+  // and the program last uploaded by an allocating draw
+  sasl_program sasl_cur{};
+  sasl_program* sasl_d = nullptr;
+  sasl_program* sasl_h = nullptr;
+  sasl_program sasl_last{};
+  bool sasl_valid = false;
   dev_array<uint32_t> cnt, keys0, keys1, vals0, vals1, tstart, tend, hist, scan_tmp, clip_hdr, counters, vis;
   dev_array<uint32_t> order;  // GPU tile scheduling order
   dev_array<uint32_t> tile_count;  // per-tile record counts and cursors of the counting sort

@@ -706,3 +706,153 @@ class SampleShaderFrame:
         r.draw_index(0, self.n_tris, 0)
         if resolve and self.resolved is not None:
             r.resolve(self.color, self.resolved)
+
+
+# ---------------------------------------------------------------- SASL pixel shaders
+# Pixel shaders written in SASL for draws through set_pixel_shader_code, modelled on the
+# reference's SASL sample shaders: TextureAndBlending/AnisotropicFilter's box and plane shaders
+# (tex2D with a fixed alpha), Triangles' constant colour, VertexTextureFetch's grey from one
+# attribute; plus a textured Lambert shader, a procedural shader with loops and library math, and
+# a shader writing its derivatives.
+SASL_SHADERS = {
+    "box": """
+sampler samp;
+float4 box_main(float4 p : TEXCOORD(0), float2 uv : TEXCOORD(1)) : COLOR {
+  float4 c = tex2D(samp, uv);
+  c.a = 0.5f;
+  return c;
+}
+""",
+    "plane": """
+sampler samp;
+float4 plane_main(float2 uv : TEXCOORD1) : COLOR {
+  float4 c = tex2D(samp, uv);
+  c.w = 1.0f;
+  return c;
+}
+""",
+    "constant": """
+float4 color;
+float4 const_main() : COLOR { return color; }
+""",
+    "grey": """
+float4 grey_main(float h : TEXCOORD0) : COLOR { return float4(h, h, h, 1.0f); }
+""",
+    "lambert": """
+sampler diffuse;
+float4 light_dir;
+struct VOut { float3 n : TEXCOORD(0); float2 uv : TEXCOORD(1); };
+float4 lambert_main(VOut v) : COLOR {
+  float3 n = normalize(v.n);
+  float k = saturate(dot(n, light_dir.xyz));
+  float4 t = tex2D(diffuse, v.uv);
+  return float4(t.rgb * k, 1.0);
+}
+""",
+    "procedural": """
+float4 procedural_main(float3 n : TEXCOORD(0), float2 uv : TEXCOORD(1)) : COLOR {
+  float2 p = uv * 8.0;
+  float checker = fmod(floor(p.x) + floor(p.y), 2.0);
+  float x = frac(p.x);
+  float acc = 0.0;
+  int i = 0;
+  while (i < 6) {
+    acc = acc + pow(x, 2.0) * 0.15;
+    x = x * 1.5;
+    if (x > 1.0) { x = x - 1.0; }
+    i = i + 1;
+  }
+  float e = exp(-length(n) * 0.5) + sin(p.y) * 0.1;
+  float band = 0.0;
+  do { band = band + 0.25; } while (band < frac(p.y));
+  return float4(checker * 0.8 + 0.1, saturate(acc), e, band);
+}
+""",
+    "derivatives": """
+float4 dd_main(float3 n : TEXCOORD(0), float2 uv : TEXCOORD(1)) : COLOR {
+  float2 dx = ddx(uv);
+  float2 dy = ddy(uv);
+  if (uv.x > 0.5) { dx = ddx(uv * uv); }
+  return float4(dx * 64.0, dy * 64.0);
+}
+""",
+}
+# the vertex shader each one runs behind (its attributes are the pixel shader's inputs)
+SASL_VS = {"box": "box", "plane": "box", "constant": "uv", "grey": "uv", "lambert": "uv", "procedural": "uv",
+           "derivatives": "uv"}
+
+
+class SaslFrame:
+    """One frame of the grid mesh drawn with a SASL pixel shader: clear, draw, (MSAA: resolve)."""
+
+    def __init__(self, r: A.Renderer, name, width=320, height=240, samples=1, fmt=A.PF_RGBA8, grid=48, tex_size=64,
+                 aniso=False, blend=A.BLEND_OFF):
+        lib = r.lib
+        self.r, self.name, self.width, self.height, self.samples = r, name, width, height, samples
+        ct = r.create_tex2d(width, height, samples, fmt)
+        dt = r.create_tex2d(width, height, samples, A.PF_RG32F)
+        self.color, self.ds = ct.surface(0), dt.surface(0)
+        self.keep = [ct, dt]
+        self.resolved = None
+        if samples > 1:
+            rt = r.create_tex2d(width, height, 1, fmt)
+            self.resolved = rt.surface(0)
+            self.keep.append(rt)
+        verts, idx = grid_mesh(grid)
+        self.n_tris = idx.size // 3
+        self.vb = r.create_buffer_from(verts)
+        self.ib = r.create_buffer_from(idx)
+        self.tex = r.create_tex2d(tex_size, tex_size, 1, A.PF_RGBA8)
+        self.tex.surface(0).write(noise_texture(tex_size, SEED_TEXTURE + 11))
+        self.tex.gen_mipmap(A.FILTER_LINEAR, True)
+        sd = A.default_sampler_desc(lib)
+        sd.min_filter = sd.mag_filter = sd.mip_filter = A.FILTER_LINEAR
+        if aniso:
+            sd.mip_filter = A.FILTER_ANISOTROPIC
+            sd.max_anisotropy = 16
+        sd.addr_mode_u = sd.addr_mode_v = sd.addr_mode_w = A.ADDRESS_WRAP
+        self.sampler = r.create_sampler(sd, self.tex)
+        rd = A.default_raster_desc(lib)
+        rd.cull_mode = A.CULL_NONE
+        self.rs = r.create_raster_state(rd)
+        self.dss = r.create_depth_stencil_state(A.default_depth_stencil_desc(lib))
+        eye, target = MESH_CAMERAS["close"]
+        view = look_at_lh(eye, target, (0.0, 1.0, 0.0))
+        proj = perspective_fov_lh(MESH_FOVY, width / height, 0.1, 100.0)
+        self.wvp = (view @ proj).astype(np.float32)
+        if SASL_VS[name] == "box":
+            self.vs = (A.VS_BOX, self.wvp.ravel())
+            self.layout = r.create_input_layout([("POSITION", 0, A.FMT_R32G32B32_FLOAT, 0, 0),
+                                                 ("TEXCOORD", 0, A.FMT_R32G32_FLOAT, 0, 24)], A.VS_BOX)
+        else:
+            self.vs = (A.VS_WVP_NORMAL_UV, self.wvp.ravel())
+            self.layout = _grid_layout(r, A.VS_WVP_NORMAL_UV)
+        self.code = lib.compile_shader(SASL_SHADERS[name])
+        self.blend = blend
+        self.clear_rgba = (0.2, 0.2, 0.5, 1.0)
+
+    def render(self, clear=True, resolve=True):
+        r = self.r
+        r.set_render_targets([self.color], self.ds)
+        r.set_viewport(0, 0, self.width, self.height, 0.0, 1.0)
+        if clear:
+            r.clear_color(self.color, self.clear_rgba)
+            r.clear_depth_stencil(self.ds, A.CLEAR_DEPTH | A.CLEAR_STENCIL, 1.0, 0)
+        r.set_input_layout(self.layout)
+        r.set_vertex_buffers(0, [self.vb], [32])
+        r.set_index_buffer(self.ib, A.FMT_R32_UINT)
+        r.set_primitive_topology(A.TOPOLOGY_TRIANGLE_LIST)
+        r.set_vertex_shader(*self.vs)
+        r.set_pixel_shader(A.PS_CONSTANT, np.array([1.0, 0.0, 1.0, 1.0], np.float32))  # overridden by the code
+        r.set_pixel_shader_code(self.code)
+        for name in ("samp", "diffuse"):
+            r.set_ps_sampler_named(name, self.sampler)
+        r.set_ps_variable("color", [0.25, 0.5, 0.75, 1.0])
+        ld = np.array([0.3, 0.8, -0.5], np.float64)
+        r.set_ps_variable("light_dir", [*(ld / np.linalg.norm(ld)), 0.0])
+        r.set_rasterizer_state(self.rs)
+        r.set_depth_stencil_state(self.dss, 0)
+        r.set_blend_shader(self.blend)
+        r.draw_index(0, self.n_tris, 0)
+        if resolve and self.resolved is not None:
+            r.resolve(self.color, self.resolved)
