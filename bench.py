@@ -83,9 +83,9 @@ def copy_calibration(mib, reps):
                       "launches": reps}), flush=True)
 
 
-def run_frames(lib, steps, warmup, profile=False, camera="oblique"):
+def run_frames(lib, steps, warmup, profile=False, camera="oblique", sasl=False):
     r = lib.renderer(A.RENDERER_BENCHMARK)
-    f = scenes.setup_mesh(r, W, H, SAMPLES, camera=camera)
+    f = scenes.setup_mesh(r, W, H, SAMPLES, camera=camera, sasl=sasl)
     for _ in range(warmup):
         f.render()
         r.flush()
@@ -271,6 +271,17 @@ def close_camera(lib, steps, warmup):
             "bin_sort_ms": prof[5] / draws / 1e6, "setup_clip_ms": prof[2] / draws / 1e6}
 
 
+def sasl_frames(lib, steps, warmup):
+    """Secondary: the headline frame shaded by the SASL Lambert shader (set_pixel_shader_code), which
+    runs the quad interpreter."""
+    total, lat, prof, n_tris = run_frames(lib, steps, warmup, profile=True, sasl=True)
+    draws = max(prof[7], 1)
+    return {"workload": "textured mesh as the headline, shaded by a SASL pixel shader (tex2D, normalize, saturate, "
+                        "dot)", "ms_per_frame": total / steps * 1e3, "mpix_per_s": W * H * steps / total / 1e6,
+            "raster_kernel_ms": prof[6] / draws / 1e6, "shade_kernel_ms": (prof[9] if len(prof) > 9 else 0) / draws / 1e6,
+            "bin_sort_ms": prof[5] / draws / 1e6, "setup_clip_ms": prof[2] / draws / 1e6}
+
+
 def cpu_baseline(seconds):
     oracle = os.path.join(ROOT, "oracle", "libsrx_cpu.so")
     if not os.path.exists(oracle):
@@ -379,6 +390,7 @@ def main():
         print(json.dumps(out), flush=True)
         return
     out["secondary"]["close_camera"] = close_camera(lib, 20, 3)
+    out["secondary"]["sasl"] = sasl_frames(lib, 10, 2)
     if not args.no_multipass:
         out["secondary"]["multipass"] = multipass_frames(lib, 20, 3)
     if not args.no_batched:

@@ -119,6 +119,7 @@ int32_t record_draw(renderer_obj* r, const draw_params& P, const ps_samplers& SM
   const uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
   if (T == 0) return SRX_OK;
   if (!r->active.empty()) return b->refuse(SRX_E_UNSUPPORTED, "queries do not observe batched draws");
+  if (sasl_shades(P)) return b->refuse(SRX_E_UNSUPPORTED, "batched draws do not run SASL pixel shaders");
   if (!(P.early_z || P.vis_stencil) || !(P.blend_id == SRX_BLEND_OFF || P.blend_id == SRX_BLEND_DISABLED) ||
       !vis_applicable(P))
     return b->refuse(SRX_E_UNSUPPORTED, "a batched draw must be opaque with an order-independent depth test");

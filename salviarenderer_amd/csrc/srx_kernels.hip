@@ -2629,7 +2629,7 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
                                          const uint32_t* __restrict__ vals, const uint32_t* __restrict__ tstart,
                                          const uint32_t* __restrict__ tend, const float4* __restrict__ tri_rec,
                                          const float4* __restrict__ tri_attr, const ps_samplers* __restrict__ SM,
-                                         uint32_t bx) {
+                                         uint32_t bx, uint32_t* __restrict__ wout = nullptr) {
   static_assert(GT == 8 || GT == 16 || GT == 32, "GPU tile side");
   constexpr int PX = GT * GT;       // pixels per GPU tile
   constexpr int NBS = GT / 4;       // 4x4 blocks per side
@@ -2919,6 +2919,8 @@ __device__ __forceinline__ void vis_body(const draw_params& P, int mode, int pas
         const float4 cc = make_float4(P.clear_rgba[0], P.clear_rgba[1], P.clear_rgba[2], P.clear_rgba[3]);
         store_texel(P.color.pf, sample_addr(P.color, px, py, smp), cc);
       }
+      // winners for a separate shading pass (SASL pixel shaders), laid out like the colour samples
+      if (!SHADE && wout) wout[(static_cast<size_t>(py) * tw + px) * S + smp] = w[smp];
     }
     if (SHADE) {
 #pragma unroll
@@ -2966,9 +2968,9 @@ __global__ void __launch_bounds__(SRX_VIS_THREADS, 4) vis_kernel(draw_params P, 
                                                    const uint32_t* __restrict__ vals,
                                                    const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ tend,
                                                    const float4* __restrict__ tri_rec, const float4* __restrict__ tri_attr,
-                                                   const ps_samplers* __restrict__ SM) {
+                                                   const ps_samplers* __restrict__ SM, uint32_t* __restrict__ wout) {
   vis_body<NA, S, SHADE, SRX_VIS_THREADS, 16>(P, mode, pass, order, entries, merges, counts, partial, vals, tstart, tend,
-                                              tri_rec, tri_attr, SM, blockIdx.x);
+                                              tri_rec, tri_attr, SM, blockIdx.x, wout);
 }
 
 
@@ -3005,6 +3007,76 @@ __global__ void __launch_bounds__(256) shade_kernel(draw_params P, const ps_samp
         store_texel(P.color.pf, sample_addr(P.color, x, y, s), c);
         w[s] = NO_WINNER;
       }
+  }
+}
+
+// Deferred SASL shading of an opaque early-Z draw: one thread per 2x2 quad of a 16x16 GPU tile
+// (the tiles the visibility pass wrote).  For each distinct winning triangle among the quad's samples
+// the program runs once on the whole quad (pixel_shader_unit::execute, shader_unit.cpp:147-176) and
+// the samples that triangle won take their pixel's colour: a SASL pixel's colour depends only on the
+// triangle and the quad, so this equals shading every early-Z pass in order.
+template <int NA, int S>
+__global__ void __launch_bounds__(64) sasl_shade_kernel(draw_params P, const ps_samplers* __restrict__ SM,
+                                                        const float4* __restrict__ tri_attr, const uint32_t* __restrict__ vis) {
+  const int px0 = blockIdx.x * 16 + (threadIdx.x & 7) * 2, py0 = blockIdx.y * 16 + (threadIdx.x >> 3) * 2;
+  const int tw = static_cast<int>(P.target_w), th = static_cast<int>(P.target_h);
+  if (px0 >= tw || py0 >= th) return;
+  uint32_t w[4][S];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int x = px0 + (i & 1), y = py0 + (i >> 1);
+    const bool in = x < tw && y < th;
+#pragma unroll
+    for (int s = 0; s < S; ++s) w[i][s] = in ? vis[(static_cast<size_t>(y) * tw + x) * S + s] : NO_WINNER;
+  }
+  const sasl_program* __restrict__ prog = P.sasl;
+  for (int it = 0; it < 4 * S; ++it) {
+    uint32_t cur = NO_WINNER;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int s = 0; s < S; ++s)
+        if (cur == NO_WINNER) cur = w[i][s];
+    if (cur >= 3u * P.prim_count) break;  // NO_WINNER (every id is below 3 * prim_count)
+    attr_setup<NA> at;
+    load_attr<NA>(P, tri_attr, cur, at);
+    const quad_lazy<NA> q(at, P, px0, py0);
+    sasl_regs R;
+    const uint32_t nregs = prog->nregs, nin = prog->ninputs;
+    for (uint32_t r = 0; r < nregs; ++r)
+      for (int px = 0; px < 4; ++px) R.r[r][px] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int px = 0; px < 4; ++px) {
+      const float iw = q.inv_w(px);
+      for (uint32_t k = 0; k < nin; ++k) {
+        const sasl_input inp = prog->inputs[k];
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+          if (a == inp.attr) v = q.reg(a + 1, px, iw);
+        R.r[inp.reg][px] = make_float4(v.x, inp.ncomp > 1 ? v.y : 0.0f, inp.ncomp > 2 ? v.z : 0.0f, inp.ncomp > 3 ? v.w : 0.0f);
+      }
+    }
+    sasl_exec(prog, R, [&](int slot, int, float u, float v, float dxu, float dxv, float dyu, float dyv) -> float4 {
+      if (!P.has_sampler[slot]) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      const sampler_desc_dev& sp = SM->s[slot];
+      return sp.fast_aniso ? sample_2d_grad_fast(sp, u, v, dxu, dxv, dyu, dyv, 0.0f)
+                           : sample_2d_grad(sp, u, v, dxu, dxv, dyu, dyv, 0.0f);
+    });
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      bool any = false;
+#pragma unroll
+      for (int s = 0; s < S; ++s) any |= w[i][s] == cur;
+      if (!any) continue;
+      const float4 c = sasl_color0(prog, R, i);
+      const int x = px0 + (i & 1), y = py0 + (i >> 1);
+#pragma unroll
+      for (int s = 0; s < S; ++s)
+        if (w[i][s] == cur) {
+          store_texel(P.color.pf, sample_addr(P.color, x, y, s), c);
+          w[i][s] = NO_WINNER;
+        }
+    }
   }
 }
 
@@ -3068,8 +3140,8 @@ void launch_raster(const draw_params& P, const draw_buffers& B, int which, hipSt
   if (!T) return;
   int threads = (P.gt / 2) * (P.gt / 2);
   size_t lds = sizeof(prim_lds) * threads;
-  const bool sasl = P.ps_id == PS_SASL && sasl_shades(P);
-  bool vis = P.early_z && (P.blend_id == BLEND_OFF || P.blend_id == BLEND_DISABLED) && !sasl;
+  bool vis = P.early_z && (P.blend_id == BLEND_OFF || P.blend_id == BLEND_DISABLED);
+  const bool sasl = !vis && sasl_shades(P);  // in-order shading by the SASL program
 #define SRX_RASTER_S(NA, S)                                                                                          \
   do {                                                                                                               \
     if (vis)                                                                                                         \
@@ -3100,7 +3172,6 @@ void launch_raster(const draw_params& P, const draw_buffers& B, int which, hipSt
 // Visibility mode of an opaque early-Z draw for vis_kernel, or -1 when only the in-order kernel
 // reproduces the reference (NOT_EQUAL with depth writes).
 int vis_mode(const draw_params& P) {
-  if (sasl_shades(P)) return -1;  // SASL pixel shaders run in the in-order kernel only
   if (P.vis_stencil && !(P.depth_enable && P.write_depth && (P.depth_func == 1 || P.depth_func == 3 ||
                                                             P.depth_func == 4 || P.depth_func == 6)))
     return -1;  // a stencil mask is expressed through the depth keys only
@@ -3127,6 +3198,9 @@ void vis_table_bounds(uint32_t T, uint32_t records, uint32_t* entries, uint32_t*
 
 bool launch_vis(const draw_params& P, const draw_buffers& B, int which, bool shade, uint32_t records, hipStream_t s) {
   int mode = vis_mode(P);
+  // a SASL pixel shader runs in its own pass (launch_shade) over the winners this pass writes
+  uint32_t* const wout = sasl_shades(P) ? B.vis : nullptr;
+  if (wout) shade = false;
   uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
   if (mode < 0 || P.gt != 16) return false;
   if (!T) return true;
@@ -3141,7 +3215,7 @@ bool launch_vis(const draw_params& P, const draw_buffers& B, int which, bool sha
   for (int pass = 0; pass < (SH ? 1 : 2); ++pass)                                                                 \
     hipLaunchKernelGGL((vis_kernel<NA, S, SH>), dim3(pass == 0 ? ne : nm), dim3(SRX_VIS_THREADS), 0, s, P, mode, pass, \
                        B.order, entries, B.vis_merges, B.vis_counts, B.vis_partial, B.vals[which], B.tstart,      \
-                       B.tend, B.tri_rec, B.tri_attr, B.samplers)
+                       B.tend, B.tri_rec, B.tri_attr, B.samplers, wout)
 #define SRX_VIS_SHADE(NA)                   \
   switch (P.samples) {                      \
   case 1: SRX_VIS_LAUNCH(NA, 1, true); break; \
@@ -3216,6 +3290,20 @@ bool needs_shading(const draw_params& P) {
 void launch_shade(const draw_params& P, const draw_buffers& B, hipStream_t s) {
   if (P.gtiles_x <= 0 || P.gtiles_y <= 0) return;
   dim3 grid(P.gtiles_x, P.gtiles_y);
+  if (P.ps_id == PS_SASL) {
+#define SRX_SASL_SHADE_S(NA, S) \
+  hipLaunchKernelGGL((sasl_shade_kernel<NA, S>), grid, dim3(64), 0, s, P, B.samplers, B.tri_attr, B.vis)
+#define SRX_SASL_SHADE(NA)                    \
+  switch (P.samples) {                        \
+  case 1: SRX_SASL_SHADE_S(NA, 1); break;     \
+  case 2: SRX_SASL_SHADE_S(NA, 2); break;     \
+  default: SRX_SASL_SHADE_S(NA, 4); break;    \
+  }
+    SRX_NA_DISPATCH(P.nattr, SRX_SASL_SHADE)
+#undef SRX_SASL_SHADE
+#undef SRX_SASL_SHADE_S
+    return;
+  }
 #define SRX_SHADE_S(NA, S) \
   hipLaunchKernelGGL((shade_kernel<NA, S>), grid, dim3(256), 0, s, P, B.samplers, B.tri_attr, B.vis)
 #define SRX_SHADE(NA)                    \

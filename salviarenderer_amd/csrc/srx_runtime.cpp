@@ -266,8 +266,7 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   draw_scratch X{r, fixed};
   draw_buffers B{};
   uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
-  bool vis_pass = (P.early_z || P.vis_stencil) && (P.blend_id == SRX_BLEND_OFF || P.blend_id == SRX_BLEND_DISABLED) &&
-                  !sasl_shades(P);
+  bool vis_pass = (P.early_z || P.vis_stencil) && (P.blend_id == SRX_BLEND_OFF || P.blend_id == SRX_BLEND_DISABLED);
   RCHK(X.take(r->vbuf, (static_cast<size_t>(P.nverts) + 1) * (nout + 1), B.vbuf));
   RCHK(X.take(r->tri_rec, static_cast<size_t>(P3) * 5, B.tri_rec));
   RCHK(X.take(r->tri_attr, static_cast<size_t>(P3) * 3 * (nout + 1), B.tri_attr));
@@ -393,7 +392,7 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
     return o && P.target_w == static_cast<float>(o->w) && P.target_h == static_cast<float>(o->h) &&
            P.gtiles_x * P.gt >= o->w && P.gtiles_y * P.gt >= o->h;
   };
-  const bool eat_color = will_vis && needs_shading(P) && ct0 && ct0->pend.kind == 1 && covers(ct0);
+  const bool eat_color = will_vis && needs_shading(P) && P.ps_id != PS_SASL && ct0 && ct0->pend.kind == 1 && covers(ct0);
   const bool eat_ds = will_vis && dst && dst->pend.kind == 2 && dst->pend.flags == (SRX_CLEAR_DEPTH | SRX_CLEAR_STENCIL) &&
                       covers(dst);
   for (auto& ct : r->st.color_targets)
@@ -488,7 +487,8 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   if (!fused) launch_raster(P, B, which, s);
   mark(5);
   // the in-order kernel defers shading (to shade_kernel) only for early-Z draws
-  if (!fused && vis_pass && P.early_z && needs_shading(P)) launch_shade(P, B, s);
+  // deferred shading: the in-order kernel's early-Z winners, or a SASL shader over vis_kernel's winners
+  if (vis_pass && needs_shading(P) && ((!fused && P.early_z) || (fused && P.ps_id == PS_SASL))) launch_shade(P, B, s);
   mark(6);
   if (prof) r->pending_prof.push_back(de);
   HIPCHK(hipGetLastError());

@@ -203,7 +203,10 @@ def setup_plain(r: A.Renderer, width=512, height=512, n_tris=2000, samples=1, se
 
 
 def setup_mesh(r: A.Renderer, width=1920, height=1080, samples=4, grid=512, tex_size=2048,
-               max_aniso=16, mip_filter=A.FILTER_ANISOTROPIC, camera="oblique") -> Frame:
+               max_aniso=16, mip_filter=A.FILTER_ANISOTROPIC, camera="oblique", sasl=False) -> Frame:
+    """The textured-mesh frame; sasl=True shades it with SASL_SHADERS["lambert"] instead of
+    PS_LAMBERT_TEX (the same lighting with SASL's normalize, v / sqrt(dot(v, v)), in place of
+    eflib's normalize3)."""
     lib = r.lib
     color, ds, resolved, keep = _targets(r, width, height, samples)
     verts, idx = grid_mesh(grid)
@@ -228,6 +231,12 @@ def setup_mesh(r: A.Renderer, width=1920, height=1080, samples=4, grid=512, tex_
     sd.max_anisotropy = max_aniso
     samp = r.create_sampler(sd, tex)
     r.set_ps_sampler(0, samp)
+    if sasl:
+        code = lib.compile_shader(SASL_SHADERS["lambert"])
+        r.set_pixel_shader_code(code)
+        r.set_ps_sampler_named("diffuse", samp)
+        r.set_ps_variable("light_dir", [*LIGHT_DIR, 0.0])
+        keep.append(code)
     r.set_blend_shader(A.BLEND_OFF)
     rd = A.default_raster_desc(lib)
     rd.cull_mode = A.CULL_NONE

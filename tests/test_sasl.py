@@ -749,3 +749,44 @@ def test_sasl_lambert_aniso_hip(hip_lib, cpu_lib):
     got = render_sasl(hip_lib, "lambert", aniso=True, width=480, height=270, grid=96, tex_size=256)
     assert np.array_equal(got["ds"], want["ds"])
     assert np.array_equal(got["color"], want["color"])
+
+
+@pytest.mark.gpu
+def test_sasl_textured_mesh_small_hip(hip_lib, cpu_lib):
+    """The textured-mesh frame (320x176, 4x MSAA, 16x anisotropic) shaded by the SASL Lambert shader."""
+    outs = []
+    for lib in (cpu_lib, hip_lib):
+        r = lib.renderer()
+        f = S.setup_mesh(r, 320, 176, 4, grid=128, tex_size=256, sasl=True)
+        f.render()
+        r.flush()
+        outs.append((f.color.read().copy(), f.ds.read().copy(), f.resolved.read().copy()))
+    for w, g in zip(*outs):
+        assert np.array_equal(w, g)
+
+
+@pytest.mark.gpu
+def test_sasl_draw_refused_in_a_batch(hip_lib):
+    r = hip_lib.renderer()
+    f = S.SaslFrame(r, "lambert")
+    r.batch_begin()
+    with pytest.raises(A.SrxError) as e:
+        f.render(resolve=False)
+    assert e.value.code == A.E_UNSUPPORTED
+    with pytest.raises(A.SrxError):
+        r.batch_end()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["lambert", "procedural"])
+def test_sasl_in_order_paths_hip(hip_lib, cpu_lib, name):
+    """Blended SASL draws and SASL draws under a statistics query run the in-order kernel's quad
+    shading; both against the oracle."""
+    for blend, stats in ((A.BLEND_ON, False), (A.BLEND_OFF, True)):
+        want = render_sasl(cpu_lib, name, stats=stats, samples=4, blend=blend)
+        got = render_sasl(hip_lib, name, stats=stats, samples=4, blend=blend)
+        assert np.array_equal(got["ds"], want["ds"])
+        d = np.abs(got["color"].astype(np.int32) - want["color"].astype(np.int32))
+        assert d.max() <= (1 if name in LIBM_SHADERS else 0), (name, blend, stats)
+        if stats:
+            assert got["ps_invocations"] == want["ps_invocations"]
