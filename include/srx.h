@@ -403,7 +403,7 @@ int32_t srx_unmap(srx_renderer r);
  * ?:, casts, continue, tex2Dlod/tex2Dbias/tex2Dproj) and matrices, arrays and helper functions
  * return SRX_E_UNSUPPORTED; syntax and type errors SRX_INVALID_PARAMETER; srx_last_error_string()
  * names the line.  Limits: 256 instructions, 32 registers, 32 literals and uniforms, 4 samplers,
- * 8 inputs and outputs, nesting depth 8.  A quad stops after 65536 executed instructions.
+ * 8 inputs and outputs, nesting depth 8.  A quad stops after 65536 loop iterations.
  *
  * Pixel-shader input i (in TEXCOORD index order) is vertex-shader output attribute i
  * (shader_unit.cpp:106-145 with a C++ vertex shader).  Colour target 0 receives COLOR(0) (zeros

@@ -101,10 +101,10 @@ void sasl_run_quad(const srx::sasl_program& p, const float (*in)[srx::SASL_MAX_I
     return m;
   };
 
-  uint32_t steps = 0;
+  uint32_t steps = 0;  // loop iterations: taken backward jumps
   uint32_t pc = 0;
-  while (pc < p.ninst && steps < SASL_STEP_LIMIT) {
-    ++steps;
+  bool stopped = false;
+  while (pc < p.ninst && !stopped) {
     const sasl_inst& in = p.code[pc];
     uint32_t next = pc + 1;
     float res[4][4] = {};
@@ -148,7 +148,10 @@ void sasl_run_quad(const srx::sasl_program& p, const float (*in)[srx::SASL_MAX_I
       exec = f.mv & cond_bits(in);
       f.brk = 0;
       f.mv = exec;
-      if (exec) next = in.target;
+      if (exec) {
+        if (++steps > SASL_STEP_LIMIT) stopped = true;
+        next = in.target;
+      }
       writes = false;
       break;
     }
@@ -161,6 +164,7 @@ void sasl_run_quad(const srx::sasl_program& p, const float (*in)[srx::SASL_MAX_I
       writes = false;
       break;
     case SOP_JMP:
+      if (++steps > SASL_STEP_LIMIT) stopped = true;
       next = in.target;
       writes = false;
       break;

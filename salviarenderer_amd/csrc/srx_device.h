@@ -6,8 +6,13 @@
 // rounds exactly like the reference's SSE code.
 #pragma once
 
+#ifndef __HIPCC_RTC__  // run-time compilation provides the HIP builtins and integer types itself
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
+#ifndef INT32_MIN
+#define INT32_MIN (-2147483647 - 1)
+#endif
 
 namespace srx {
 
@@ -46,6 +51,14 @@ struct input_elem_dev {
   uint32_t fmt;
   int32_t reg;
   float default_w;
+};
+
+// vs_output attribute modifiers (shader_regs.h)
+enum { AM_NOINTERPOLATION = 4, AM_NOPERSPECTIVE = 8 };
+
+// the sampler table of a draw (one descriptor per slot), in device memory
+struct ps_samplers {
+  sampler_desc_dev s[MAX_SAMPLERS];
 };
 
 struct sasl_program;

@@ -8,9 +8,6 @@
 
 namespace srx {
 
-struct ps_samplers {
-  sampler_desc_dev s[MAX_SAMPLERS];
-};
 
 // Device statistics slots (pipeline_statistics / internal_statistics, async_object.h:90-175)
 enum stat_slot {

@@ -16,6 +16,7 @@
 //                   the final per-sample winners.
 #include "srx_kernels.h"
 #include "srx_sampler.h"
+#include "srx_shade_common.h"
 #include "srx_sasl_exec.h"
 
 #include <type_traits>
@@ -29,7 +30,6 @@ enum { VS_PASSTHROUGH_COLOR = 1, VS_WVP_NORMAL_UV = 2, VS_WVP_COLOR = 3, VS_WVP_
 enum { PS_GOURAUD = 1, PS_LAMBERT_TEX = 2, PS_TEX_CPP = 3, PS_CONSTANT = 4, PS_SHADOW_PCF = 5, PS_COLORIZED = 6,
        PS_MESH = 7, PS_SSM = 8, PS_CUP = 9, PS_BOX = 10 };
 enum { BLEND_OFF = 0, BLEND_ON = 1, BLEND_DISABLED = 2 };
-enum { AM_NOINTERPOLATION = 4, AM_NOPERSPECTIVE = 8 };
 enum { TOPO_TRIANGLE_LIST = 2, TOPO_TRIANGLE_STRIP = 4 };
 enum { FMT_RGBA32F = 2, FMT_RGBA32I = 4, FMT_RGB32F = 6, FMT_RG32F = 16, FMT_R32F = 41 };
 enum { CULL_NONE = 0, CULL_FRONT = 1, CULL_BACK = 2 };
@@ -163,13 +163,6 @@ __device__ __forceinline__ void vs_body(const draw_params& P, float4* __restrict
 __global__ void vs_kernel(draw_params P, float4* __restrict__ vbuf) { vs_body(P, vbuf, blockIdx.x); }
 
 // ============================================================== primitive assembly helpers
-// Setup records of clipped triangle id = 3 * primitive + fan index live at slot
-// fan index * prim_count + primitive: the fan-0 records of consecutive primitives (all of them when
-// nothing is clipped) are contiguous, so setup writes them densely.
-__device__ __forceinline__ size_t tri_slot(const draw_params& P, uint32_t id) {
-  const uint32_t p = id / 3u;
-  return static_cast<size_t>(id - 3u * p) * P.prim_count + p;
-}
 __device__ __forceinline__ uint32_t fetch_index(const draw_params& P, uint32_t prim, int k) {
   uint32_t id;
   if (P.topo == TOPO_TRIANGLE_LIST) {
@@ -1652,25 +1645,6 @@ __device__ __forceinline__ void normalize3(float x, float y, float z, float& ox,
   oz = z * inv;
 }
 
-// Per-triangle interpolation data: v0 (the reordered first vertex) and the screen-space
-// derivatives of every register (compute_derivative_n, shader.cpp:414-450).
-template <int NA>
-struct attr_setup {
-  float4 v0[NA + 1], dx[NA + 1], dy[NA + 1];
-};
-
-template <int NA>
-__device__ __forceinline__ void load_attr(const draw_params& P, const float4* __restrict__ tri_attr, uint32_t id,
-                                          attr_setup<NA>& a) {
-  const float4* src = tri_attr + tri_slot(P, id) * 3 * (NA + 1);
-#pragma unroll
-  for (int r = 0; r <= NA; ++r) {
-    a.v0[r] = src[r];
-    a.dx[r] = src[NA + 1 + r];
-    a.dy[r] = src[2 * (NA + 1) + r];
-  }
-}
-
 // 3x3 percentage-closer shadow lookup of one pixel (attr0 colour, attr1 light clip position): the
 // tap layout of StandardShadowMap.cpp:86-106, each tap a tex2dlod at lod 0, lit when the light-space
 // depth minus the bias does not exceed it.
@@ -1691,43 +1665,6 @@ __device__ __forceinline__ float4 shadow_pcf(const draw_params& P, const ps_samp
   const float k = P.ps_consts[0] + (1.0f - P.ps_consts[0]) * lit;
   return make_float4(a.x * k, a.y * k, a.z * k, a.w);
 }
-
-// Interpolated registers of single pixels of a 2x2 quad, computed only where read: pixel i's value
-// follows the quad's chain (p00 = v0 + (ddx*dx + ddy*dy), +ddx for the right column, then +ddy for
-// the bottom row), perspective-divided by its own 1/w, so every pixel gets the value a whole-quad
-// interpolation would give it.
-template <int NA>
-struct quad_lazy {
-  const attr_setup<NA>& t;
-  const draw_params& P;
-  float dx, dy;
-  __device__ __forceinline__ quad_lazy(const attr_setup<NA>& at, const draw_params& p, int px0, int py0)
-      : t(at), P(p), dx((0.5f + static_cast<float>(px0)) - at.v0[0].x), dy((0.5f + static_cast<float>(py0)) - at.v0[0].y) {}
-  __device__ __forceinline__ static float chain(float v, float a, float b, float dx, float dy, int i) {
-    float c = v + ((a * dx) + (b * dy));
-    c = (i & 1) ? c + a : c;
-    return (i & 2) ? c + b : c;
-  }
-  __device__ __forceinline__ float inv_w(int i) const {
-    return 1.0f / chain(t.v0[0].w, t.dx[0].w, t.dy[0].w, dx, dy, i);
-  }
-  // register r (1..NA) of pixel i, components [0, nc)
-  __device__ __forceinline__ float4 reg(int r, int i, float iw, int nc = 4) const {
-    const float4 v = t.v0[r], a = t.dx[r], b = t.dy[r];
-    const uint32_t mod = P.attr_mod[r - 1];
-    float4 o;
-    if (mod & AM_NOINTERPOLATION) {
-      o = v;
-    } else {
-      o.x = chain(v.x, a.x, b.x, dx, dy, i);
-      o.y = nc > 1 ? chain(v.y, a.y, b.y, dx, dy, i) : 0.0f;
-      o.z = nc > 2 ? chain(v.z, a.z, b.z, dx, dy, i) : 0.0f;
-      o.w = nc > 3 ? chain(v.w, a.w, b.w, dx, dy, i) : 0.0f;
-    }
-    if (!(mod & AM_NOPERSPECTIVE)) o = make_float4(o.x * iw, o.y * iw, o.z * iw, o.w * iw);
-    return o;
-  }
-};
 
 // cpp_pixel_shader::tex2d(s, reg) (cpp_pixel_shader.cpp:13-31): one LOD per quad from the register's
 // p1 - p0 and p2 - p0 (calc_lod_2d, sampler.cpp:834-852), then sample_2d_lod at pixel pi
@@ -1995,9 +1932,6 @@ __device__ __forceinline__ uint32_t stencil_op(int op, uint32_t ref, uint32_t cu
   }
 }
 
-__device__ __forceinline__ uint8_t* sample_addr(const surface_desc& s, int x, int y, int smp) {
-  return s.data + ((static_cast<size_t>(y) * s.w + x) * s.samples + smp) * pf_size(s.pf);
-}
 
 __device__ __forceinline__ void blend_store(const draw_params& P, int x, int y, int smp, float4 src) {
   if (!P.color.data) return;
@@ -2009,7 +1943,6 @@ __device__ __forceinline__ void blend_store(const draw_params& P, int x, int y, 
   }
 }
 
-constexpr uint32_t NO_WINNER = 0xFFFFFFFFu;
 
 // Pixel shader and backend for one triangle on a 2x2 quad: `pm` holds the samples to store, bits
 // 4i..4i+S-1 for pixel i.  Interpolants are formed per pixel where read (quad_lazy), so only one

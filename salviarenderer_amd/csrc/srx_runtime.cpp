@@ -488,7 +488,11 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   mark(5);
   // the in-order kernel defers shading (to shade_kernel) only for early-Z draws
   // deferred shading: the in-order kernel's early-Z winners, or a SASL shader over vis_kernel's winners
-  if (vis_pass && needs_shading(P) && ((!fused && P.early_z) || (fused && P.ps_id == PS_SASL))) launch_shade(P, B, s);
+  if (vis_pass && needs_shading(P) && ((!fused && P.early_z) || (fused && P.ps_id == PS_SASL))) {
+    bool jit = false;
+    if (P.ps_id == PS_SASL) RCHK(sasl_jit_launch(r->sasl_cur, P, B, r->device, s, !fixed, &jit));
+    if (!jit) launch_shade(P, B, s);
+  }
   mark(6);
   if (prof) r->pending_prof.push_back(de);
   HIPCHK(hipGetLastError());

@@ -538,6 +538,12 @@ int32_t record_draw(renderer_obj* r, const draw_params& P, const ps_samplers& SM
 // Queues o's pending clear on stream s (its issuing renderer's stream when s is null); drop: the
 // surface is about to be overwritten whole, so the clear is discarded instead.
 int32_t run_pending_clear(surface_obj* o, hipStream_t s, bool drop = false);
+// Deferred SASL shading with the program compiled to native code (srx_sasl_jit.cpp); *launched is
+// false when it did not launch (run-time compilation off with SRX_SASL_JIT=0, a failed compilation,
+// or a module not yet compiled when allow_compile is false): the interpreter kernel shades then.
+int32_t sasl_jit_launch(const sasl_program& prog, const draw_params& P, const draw_buffers& B, int device, hipStream_t s,
+                        bool allow_compile, bool* launched);
+std::string sasl_jit_source(const sasl_program& p, int NA, int S);
 // Queues every pending clear renderer r issued, on r's stream.
 int32_t run_pending_clears(renderer_obj* r);
 int32_t record_fill(renderer_obj* r, surface_obj* o, float4 v);

@@ -4,9 +4,8 @@
 // thread's private memory; the program and its constants are read from device memory.
 #pragma once
 
-#include "srx_kernels.h"
-#include "srx_sampler.h"
 #include "srx_sasl_program.h"
+#include "srx_shade_common.h"
 
 namespace srx {
 
@@ -121,8 +120,7 @@ __device__ void sasl_exec(const sasl_program* __restrict__ p, sasl_regs& R, TEX&
   };
   const uint32_t n = p->ninst;
   uint32_t pc = 0, steps = 0;
-  while (pc < n && steps < SASL_STEP_LIMIT) {
-    ++steps;
+  while (pc < n) {
     const sasl_inst in = p->code[pc];
     uint32_t next = pc + 1;
     float4 res[4];
@@ -161,12 +159,18 @@ __device__ void sasl_exec(const sasl_program* __restrict__ p, sasl_regs& R, TEX&
       exec = f.mv & cond_bits(in.a, in.sa);
       f.brk = 0;
       f.mv = static_cast<uint8_t>(exec);
-      if (exec) next = in.target;
+      if (exec) {
+        if (++steps > SASL_STEP_LIMIT) return;
+        next = in.target;
+      }
       break;
     }
     case SOP_DBODY: exec = st[depth - 1].mv; break;
     case SOP_LSAVE: st[depth - 1].mv = static_cast<uint8_t>(exec); break;
-    case SOP_JMP: next = in.target; break;
+    case SOP_JMP:
+      if (++steps > SASL_STEP_LIMIT) return;
+      next = in.target;
+      break;
     case SOP_LEND:
       --depth;
       exec = st[depth].saved;

@@ -21,7 +21,9 @@
 //    sampled only for pixels in the execution mask (tex2Dgrad_ps, sampler_api.cpp:13-18).
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <cstdint>
+#endif
 
 namespace srx {
 
@@ -32,8 +34,8 @@ constexpr int SASL_MAX_INPUTS = 8;
 constexpr int SASL_MAX_OUTPUTS = 8;
 constexpr int SASL_MAX_SAMPLERS = 4;
 constexpr int SASL_MAX_DEPTH = 8;     // nesting of if and loops
-// Instructions one quad may execute; a quad that reaches it stops with its outputs as they are
-// (a loop that never ends in the reference never returns).
+// Loop iterations (taken backward jumps) one quad may run; a quad that reaches the limit stops with
+// its registers as they are (a loop that never ends in the reference never returns).
 constexpr uint32_t SASL_STEP_LIMIT = 1u << 16;
 
 constexpr uint8_t SASL_CONST = 32;    // operand >= 32: constant operand - 32

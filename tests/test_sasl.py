@@ -790,3 +790,24 @@ def test_sasl_in_order_paths_hip(hip_lib, cpu_lib, name):
         assert d.max() <= (1 if name in LIBM_SHADERS else 0), (name, blend, stats)
         if stats:
             assert got["ps_invocations"] == want["ps_invocations"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["lambert", "procedural", "derivatives"])
+def test_sasl_interpreter_and_native_agree_hip(hip_lib, cpu_lib, name, monkeypatch):
+    """Opaque SASL draws shade with the program compiled to native code at run time; with
+    SRX_SASL_JIT=0 the quad interpreter shades them instead.  Both against the oracle."""
+    kw = dict(samples=4, fmt=A.PF_RGBA32F if name == "derivatives" else A.PF_RGBA8)
+    want = render_sasl(cpu_lib, name, **kw)
+    got = {}
+    for jit in ("1", "0"):
+        monkeypatch.setenv("SRX_SASL_JIT", jit)
+        got[jit] = render_sasl(hip_lib, name, **kw)
+    for k in ("ds", "color", "resolved"):
+        if name in LIBM_SHADERS and k != "ds":
+            for jit in ("1", "0"):
+                assert np.abs(got[jit][k].astype(np.int32) - want[k].astype(np.int32)).max() <= 1
+            assert np.array_equal(got["1"][k], got["0"][k])  # the same device math either way
+        else:
+            assert np.array_equal(got["1"][k], want[k]), k
+            assert np.array_equal(got["0"][k], want[k]), k
