@@ -325,6 +325,8 @@ def main():
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline frames (no CPU baseline and no secondary workloads), for profiling")
     ap.add_argument("--serving-only", action="store_true", help="only the serving measurement")
+    ap.add_argument("--sasl-only", action="store_true",
+                    help="only the textured mesh shaded by the SASL pixel shader (steps/warmup as given), for profiling")
     ap.add_argument("--serving-requests", type=int, default=10000)
     ap.add_argument("--batch-config", default="", metavar="TILE,BIN_ITEMS,THREADS",
                     help="configure the batch instead of autotuning it")
@@ -346,6 +348,9 @@ def main():
         return
     if args.serving_only:
         print(json.dumps(serving(lib, args.serving_requests)), flush=True)
+        return
+    if args.sasl_only:
+        print(json.dumps(sasl_frames(lib, args.steps, args.warmup)), flush=True)
         return
     total, lat, prof, n_tris = run_frames(lib, args.steps, args.warmup, profile=True)
     fps = args.steps / total
