@@ -273,6 +273,7 @@ typedef struct srx_raster_state_t* srx_raster_state;
 typedef struct srx_ds_state_t* srx_ds_state;
 typedef struct srx_query_t* srx_query;
 typedef struct srx_shader_code_t* srx_shader_code;
+typedef struct srx_swap_chain_t* srx_swap_chain;
 
 /* ---- library ---- */
 uint32_t srx_abi_version(void);
@@ -451,6 +452,20 @@ int32_t srx_clear_color(srx_renderer r, srx_surface color_target, const float rg
 int32_t srx_clear_depth_stencil(srx_renderer r, srx_surface ds_target, uint32_t flags, float depth,
                                 uint32_t stencil);
 int32_t srx_flush(srx_renderer r);
+
+/* ---- swap chain without a window (salvia/include/salvia/ext/swap_chain/swap_chain.h:16-21) ----
+ * salviax_create_swap_chain_and_renderer's swap chain for an existing renderer, with no display
+ * (present_impl's copy to a D3D11/GL window is out of scope): a back buffer of width x height x
+ * samples in fmt, plus a single-sample surface of the same size when samples > 1
+ * (swap_chain_impl.cpp:14-35).  srx_swap_chain_present is present() (:41-49): it flushes the
+ * renderer and resolves the back buffer into the presented surface, and returns when that image is
+ * complete; srx_swap_chain_presented returns that surface (the back buffer itself for 1 sample).
+ * Both getters return retained handles. */
+int32_t srx_create_swap_chain(srx_renderer r, size_t width, size_t height, size_t samples, int32_t fmt,
+                              srx_swap_chain* out);
+int32_t srx_swap_chain_surface(srx_swap_chain sc, srx_surface* out);
+int32_t srx_swap_chain_present(srx_swap_chain sc);
+int32_t srx_swap_chain_presented(srx_swap_chain sc, srx_surface* out);
 
 /* ---- a draw that allocates nothing ----
  * No counterpart in the reference.  For callers that cannot allocate device memory (code under HIP

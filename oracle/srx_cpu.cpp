@@ -171,7 +171,7 @@ pool& global_pool() {
 
 // ---------------------------------------------------------------- objects
 enum obj_kind { K_RENDERER = 1, K_BUFFER, K_TEXTURE, K_SURFACE, K_SAMPLER, K_LAYOUT, K_RASTER, K_DS, K_QUERY, K_BATCH, K_SERVER,
-                K_SHADER };
+                K_SHADER, K_SWAP_CHAIN };
 
 struct object {
   std::atomic<int> refs{1};
@@ -1392,6 +1392,13 @@ vec4 fetch_elem(const uint8_t* base, size_t size, size_t off, uint32_t fmt, floa
   }
 }
 
+// A swap chain without a window (swap_chain_impl.cpp): back buffer and presented surface.
+struct swap_obj : object {
+  ref<renderer_obj> r;
+  ref<texture_obj> back, resolved;
+  swap_obj() : object(K_SWAP_CHAIN) {}
+};
+
 int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t primcnt, int32_t basevert) {
   render_state& st = r->st;
   if (st.color_targets.empty() && !st.ds_target) return SRX_OK;
@@ -2400,6 +2407,54 @@ int32_t srx_clear_depth_stencil(srx_renderer rr, srx_surface s, uint32_t flags, 
   return SRX_OK;
 }
 int32_t srx_flush(srx_renderer rr) { return as<renderer_obj>(rr, K_RENDERER) ? SRX_OK : SRX_INVALID_PARAMETER; }
+
+int32_t srx_create_swap_chain(srx_renderer rr, size_t w, size_t h, size_t ns, int32_t fmt, srx_swap_chain* out) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !out) return SRX_INVALID_PARAMETER;
+  srx_texture back = nullptr, res = nullptr;
+  int32_t e = srx_create_tex2d(rr, w, h, ns, fmt, &back);
+  if (e != SRX_OK) return e;
+  if (ns > 1) {
+    e = srx_create_tex2d(rr, w, h, 1, fmt, &res);
+    if (e != SRX_OK) {
+      srx_release(back);
+      return e;
+    }
+  }
+  auto* sc = new swap_obj();
+  sc->r = ref<renderer_obj>(r);
+  sc->back = ref<texture_obj>(as<texture_obj>(back, K_TEXTURE));
+  sc->resolved = ref<texture_obj>(as<texture_obj>(res ? res : back, K_TEXTURE));
+  srx_release(back);
+  if (res) srx_release(res);
+  *out = reinterpret_cast<srx_swap_chain>(static_cast<object*>(sc));
+  return SRX_OK;
+}
+int32_t srx_swap_chain_surface(srx_swap_chain s, srx_surface* out) {
+  auto* sc = as<swap_obj>(s, K_SWAP_CHAIN);
+  if (!sc || !out) return SRX_INVALID_PARAMETER;
+  return srx_texture_surface(reinterpret_cast<srx_texture>(static_cast<object*>(sc->back.get())), 0, out);
+}
+int32_t srx_swap_chain_presented(srx_swap_chain s, srx_surface* out) {
+  auto* sc = as<swap_obj>(s, K_SWAP_CHAIN);
+  if (!sc || !out) return SRX_INVALID_PARAMETER;
+  return srx_texture_surface(reinterpret_cast<srx_texture>(static_cast<object*>(sc->resolved.get())), 0, out);
+}
+// swap_chain_impl::present (swap_chain_impl.cpp:41-49) without present_impl's window copy
+int32_t srx_swap_chain_present(srx_swap_chain s) {
+  auto* sc = as<swap_obj>(s, K_SWAP_CHAIN);
+  if (!sc) return SRX_INVALID_PARAMETER;
+  auto* rr = reinterpret_cast<srx_renderer>(static_cast<object*>(sc->r.get()));
+  int32_t e = srx_flush(rr);
+  if (e != SRX_OK || sc->resolved.get() == sc->back.get()) return e;
+  srx_surface a = nullptr, b = nullptr;
+  e = srx_swap_chain_surface(s, &a);
+  if (e == SRX_OK) e = srx_swap_chain_presented(s, &b);
+  if (e == SRX_OK) e = srx_surface_resolve(rr, a, b);
+  if (a) srx_release(a);
+  if (b) srx_release(b);
+  return e;
+}
 
 void srx_default_server_desc(srx_server_desc* d) {
   *d = srx_server_desc{200ull << 30, 64ull << 30, 2ull << 30, 64, 3, 8192, 64};

@@ -195,6 +195,10 @@ PROTOTYPES = {
     "srx_set_ps_variable": (_I32, [_H, C.c_char_p, _P, _SZ]),
     "srx_set_ps_sampler_named": (_I32, [_H, C.c_char_p, _H]),
     "srx_ps_execute_quads": (_I32, [_H, _H, _P, _P, _SZ]),
+    "srx_create_swap_chain": (_I32, [_H, _SZ, _SZ, _SZ, _I32, C.POINTER(_H)]),
+    "srx_swap_chain_surface": (_I32, [_H, C.POINTER(_H)]),
+    "srx_swap_chain_present": (_I32, [_H]),
+    "srx_swap_chain_presented": (_I32, [_H, C.POINTER(_H)]),
     "srx_set_rasterizer_state": (_I32, [_H, _H]),
     "srx_set_depth_stencil_state": (_I32, [_H, _H, _I32]),
     "srx_set_render_targets": (_I32, [_H, _SZ, C.POINTER(_H), _H]),
@@ -291,6 +295,23 @@ class _Obj:
             self.release()
         except Exception:
             pass
+
+
+class SwapChain(_Obj):
+    """A swap chain without a window: back buffer, present() = flush + resolve."""
+
+    def surface(self) -> "Surface":
+        h = _H()
+        self.lib.call("srx_swap_chain_surface", self.h, C.byref(h))
+        return Surface(self.lib, h.value)
+
+    def present(self):
+        self.lib.call("srx_swap_chain_present", self.h)
+
+    def presented(self) -> "Surface":
+        h = _H()
+        self.lib.call("srx_swap_chain_presented", self.h, C.byref(h))
+        return Surface(self.lib, h.value)
 
 
 class ShaderCode(_Obj):
@@ -580,6 +601,11 @@ class Renderer(_Obj):
     def set_ps_sampler(self, slot, samp):
         self._keep[f"samp{slot}"] = samp
         self.lib.call("srx_set_ps_sampler", self.h, slot, samp.h if samp is not None else None)
+
+    def create_swap_chain(self, w, h, samples=1, fmt=PF_RGBA8) -> SwapChain:
+        out = _H()
+        self.lib.call("srx_create_swap_chain", self.h, w, h, samples, fmt, C.byref(out))
+        return SwapChain(self.lib, out.value)
 
     def set_pixel_shader_code(self, code):
         self._keep["ps_code"] = code

@@ -10,6 +10,13 @@
 
 namespace srx_rt {
 
+// A swap chain without a window (swap_chain_impl.cpp): back buffer and presented surface.
+struct swap_obj : object {
+  ref<renderer_obj> r;
+  ref<texture_obj> back, resolved;
+  swap_obj() : object(K_SWAP_CHAIN) {}
+};
+
 
 // The device descriptor of a sampler and its texture (sampler.cpp's state); queues pending clears
 // of the texture's levels.
@@ -1351,6 +1358,55 @@ int32_t srx_flush(srx_renderer rr) {
   if (!r) return SRX_INVALID_PARAMETER;
   RCHK(run_pending_clears(r));
   HIPCHK(hipStreamSynchronize(r->stream));
+  return SRX_OK;
+}
+
+int32_t srx_create_swap_chain(srx_renderer rr, size_t w, size_t h, size_t ns, int32_t fmt, srx_swap_chain* out) {
+  auto* r = as<renderer_obj>(rr, K_RENDERER);
+  if (!r || !out) return SRX_INVALID_PARAMETER;
+  srx_texture back = nullptr, res = nullptr;
+  RCHK(srx_create_tex2d(rr, w, h, ns, fmt, &back));
+  if (ns > 1) {
+    const int32_t e = srx_create_tex2d(rr, w, h, 1, fmt, &res);
+    if (e != SRX_OK) {
+      srx_release(back);
+      return e;
+    }
+  }
+  auto* sc = new swap_obj();
+  sc->r = ref<renderer_obj>(r);
+  sc->back = ref<texture_obj>(as<texture_obj>(back, K_TEXTURE));
+  sc->resolved = ref<texture_obj>(as<texture_obj>(res ? res : back, K_TEXTURE));
+  srx_release(back);
+  if (res) srx_release(res);
+  *out = reinterpret_cast<srx_swap_chain>(static_cast<object*>(sc));
+  return SRX_OK;
+}
+int32_t srx_swap_chain_surface(srx_swap_chain s, srx_surface* out) {
+  auto* sc = as<swap_obj>(s, K_SWAP_CHAIN);
+  if (!sc || !out) return SRX_INVALID_PARAMETER;
+  return srx_texture_surface(reinterpret_cast<srx_texture>(static_cast<object*>(sc->back.get())), 0, out);
+}
+int32_t srx_swap_chain_presented(srx_swap_chain s, srx_surface* out) {
+  auto* sc = as<swap_obj>(s, K_SWAP_CHAIN);
+  if (!sc || !out) return SRX_INVALID_PARAMETER;
+  return srx_texture_surface(reinterpret_cast<srx_texture>(static_cast<object*>(sc->resolved.get())), 0, out);
+}
+int32_t srx_swap_chain_present(srx_swap_chain s) {
+  auto* sc = as<swap_obj>(s, K_SWAP_CHAIN);
+  if (!sc) return SRX_INVALID_PARAMETER;
+  auto* rr = reinterpret_cast<srx_renderer>(static_cast<object*>(sc->r.get()));
+  RCHK(srx_flush(rr));
+  if (sc->resolved.get() != sc->back.get()) {
+    srx_surface a = nullptr, b = nullptr;
+    RCHK(srx_swap_chain_surface(s, &a));
+    int32_t e = srx_swap_chain_presented(s, &b);
+    if (e == SRX_OK) e = srx_surface_resolve(rr, a, b);
+    srx_release(a);
+    if (b) srx_release(b);
+    RCHK(e);
+    RCHK(srx_flush(rr));
+  }
   return SRX_OK;
 }
 

@@ -43,7 +43,7 @@ inline int32_t hip_fail(hipError_t e, const char* where) {
 
 // ---------------------------------------------------------------- objects
 enum obj_kind { K_RENDERER = 1, K_BUFFER, K_TEXTURE, K_SURFACE, K_SAMPLER, K_LAYOUT, K_RASTER, K_DS, K_QUERY, K_BATCH, K_SERVER,
-                K_SHADER };
+                K_SHADER, K_SWAP_CHAIN };
 
 struct object {
   std::atomic<int> refs{1};

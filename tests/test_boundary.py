@@ -200,3 +200,36 @@ def test_buffer_transfer_oracle(cpu_lib):
 @pytest.mark.gpu
 def test_buffer_transfer_hip(hip_lib):
     _transfer(hip_lib)
+
+
+def _swap_chain_frame(lib, samples):
+    r = lib.renderer()
+    f = S.setup_plain(r, 320, 200, 300, samples)
+    sc = r.create_swap_chain(320, 200, samples)
+    back = sc.surface()
+    r.set_render_targets([back], f.ds)
+    r.clear_color(back, (0.1, 0.2, 0.3, 1.0))
+    r.clear_depth_stencil(f.ds, A.CLEAR_DEPTH | A.CLEAR_STENCIL, 1.0, 0)
+    f.draw()
+    sc.present()
+    shown = sc.presented().read().copy()
+    # present() = flush + resolve of the back buffer (swap_chain_impl.cpp:41-49)
+    if samples > 1:
+        r.resolve(back, f.resolved)
+        r.flush()
+        assert np.array_equal(shown, f.resolved.read())
+    else:
+        assert np.array_equal(shown, back.read())
+    return shown
+
+
+@pytest.mark.parametrize("samples", [1, 4])
+def test_swap_chain_present_oracle(cpu_lib, samples):
+    shown = _swap_chain_frame(cpu_lib, samples)
+    assert len(np.unique(shown.reshape(-1, 4), axis=0)) > 10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("samples", [1, 4])
+def test_swap_chain_present_hip(hip_lib, cpu_lib, samples):
+    assert np.array_equal(_swap_chain_frame(hip_lib, samples), _swap_chain_frame(cpu_lib, samples))

@@ -33,3 +33,25 @@ def test_c_program_on_oracle(cpu_lib, tmp_path):
 @pytest.mark.gpu
 def test_c_program_on_hip(hip_lib, tmp_path):
     assert build_and_run(hip_lib.path, tmp_path) == GOLD["scenes"]["plain_small_512"]["color_ds"]
+
+
+def build_and_run_sasl(lib_path, tmp_path):
+    exe = str(tmp_path / "sasl_quad")
+    lib_dir, lib_name = os.path.split(lib_path)
+    cmd = ["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "c", "sasl_quad.c"), "-o", exe, "-L", lib_dir, f"-l:{lib_name}",
+           f"-Wl,-rpath,{lib_dir}", "-Wl,-rpath,/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib"]
+    subprocess.run(cmd, check=True)
+    res = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(res.stdout, res.stderr, flush=True)
+    assert res.returncode == 0, (res.returncode, res.stderr)
+
+
+def test_c_sasl_program_on_oracle(cpu_lib, tmp_path):
+    """A SASL pixel shader compiled and run on one quad through the C ABI (oracle)."""
+    build_and_run_sasl(cpu_lib.path, tmp_path)
+
+
+@pytest.mark.gpu
+def test_c_sasl_program_on_hip(hip_lib, tmp_path):
+    build_and_run_sasl(hip_lib.path, tmp_path)

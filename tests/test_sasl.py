@@ -811,3 +811,38 @@ def test_sasl_interpreter_and_native_agree_hip(hip_lib, cpu_lib, name, monkeypat
         else:
             assert np.array_equal(got["1"][k], want[k]), k
             assert np.array_equal(got["0"][k], want[k]), k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["lambert", "procedural"])
+def test_sasl_noalloc_draw_hip(hip_lib, cpu_lib, name, monkeypatch):
+    """srx_draw_noalloc with a SASL shader: a program not yet compiled shades with the interpreter
+    (compiling would allocate), a compiled one with its native code; both as srx_draw_index."""
+    want = render_sasl(cpu_lib, name, samples=2)
+    monkeypatch.setenv("SRX_SASL_JIT", "1")
+    for warm in (False, True):
+        r = hip_lib.renderer()
+        f = S.SaslFrame(r, name, samples=2)
+        # binds the frame's state; with warm, an allocating draw compiles the program first (no
+        # other test draws these programs at 2 samples, so the first pass finds them uncompiled)
+        monkeypatch.setenv("SRX_SASL_JIT", "1" if warm else "0")
+        f.render()
+        r.flush()
+        monkeypatch.setenv("SRX_SASL_JIT", "1")
+        n = f.n_tris
+        r2 = r
+        r2.set_render_targets([f.color], f.ds)
+        r2.set_viewport(0, 0, f.width, f.height, 0.0, 1.0)
+        r2.clear_color(f.color, f.clear_rgba)
+        r2.clear_depth_stencil(f.ds, A.CLEAR_DEPTH | A.CLEAR_STENCIL, 1.0, 0)
+        # a non-indexed draw of the same triangles: expand the index buffer into a vertex buffer
+        verts, idx = S.grid_mesh(48)
+        vb = r2.create_buffer_from(np.ascontiguousarray(verts[idx.ravel()]))
+        r2.set_vertex_buffers(0, [vb], [32])
+        r2.draw_noalloc(0, n)
+        r2.resolve(f.color, f.resolved)
+        r2.flush()
+        got_ds = f.ds.read()
+        assert np.array_equal(got_ds, want["ds"]), (name, warm)
+        d = np.abs(f.color.read().astype(np.int32) - want["color"].astype(np.int32))
+        assert d.max() <= (1 if name in LIBM_SHADERS else 0), (name, warm)
