@@ -522,23 +522,68 @@ __device__ __forceinline__ uint32_t count_gtiles(const draw_params& P, const flo
   return n;
 }
 
+// Coalesced stores of a wave's per-lane blocks of W float4 (lane i's block at dst + i*W): the lanes
+// in `mask` put their blocks in the wave's LDS staging area, then the wave stores the span with
+// consecutive lanes on consecutive float4, skipping the blocks of lanes not in `mask` (their memory
+// is left as it was).  Blocks wider than the staging area are staged a group of lanes at a time.
+constexpr int WAVE_STAGE = 320;  // float4 of LDS staging per wave
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <int W>
+__device__ __forceinline__ void wave_store_blocks(float4* __restrict__ dst, const float4 (&blk)[W], uint64_t mask,
+                                                  float4* __restrict__ stage) {
+  constexpr int LG = WAVE_STAGE / W < 64 ? WAVE_STAGE / W : 64;  // lanes per pass
+  const int lane = __lane_id();
+  for (int g = 0; g < 64; g += LG) {
+    const uint64_t pm = (mask >> g) & (LG >= 64 ? ~0ull : ((1ull << LG) - 1ull));
+    if (!pm) continue;  // wave-uniform
+    if (lane >= g && lane < g + LG && ((mask >> lane) & 1ull)) {
+#pragma unroll
+      for (int k = 0; k < W; ++k) stage[(lane - g) * W + k] = blk[k];
+    }
+    wave_lds_sync();
+    const int n = min(LG, 64 - g) * W;
+    for (int j = lane; j < n; j += 64)
+      if ((pm >> (j / W)) & 1ull) dst[static_cast<size_t>(g) * W + j] = stage[j];
+    wave_lds_sync();  // the stores have read the staging area before the next group writes it
+  }
+}
+
 template <int NA>
-__device__ __forceinline__ void write_tri_rec(float4* rec, const tri_setup<NA>& t) {
+__device__ __forceinline__ void tri_rec_words(const tri_setup<NA>& t, float4 (&rec)[5]) {
   rec[0] = make_float4(t.e[0][0], t.e[0][1], t.e[0][2], t.e[1][0]);
   rec[1] = make_float4(t.e[1][1], t.e[1][2], t.e[2][0], t.e[2][1]);
   rec[2] = make_float4(t.e[2][2], t.bb[0], t.bb[1], t.bb[2]);
   rec[3] = make_float4(t.bb[3], t.v0.r[0].x, t.v0.r[0].y, t.v0.r[0].z);
   rec[4] = make_float4(t.ddx.r[0].z, t.ddy.r[0].z, t.front ? 1.0f : 0.0f, 0.0f);
 }
-
 template <int NA>
-__device__ __forceinline__ void write_tri_attr(float4* a, const tri_setup<NA>& t) {
+__device__ __forceinline__ void tri_attr_words(const tri_setup<NA>& t, float4 (&a)[3 * (NA + 1)]) {
 #pragma unroll
   for (int r = 0; r <= NA; ++r) {
     a[r] = t.v0.r[r];
     a[NA + 1 + r] = t.ddx.r[r];
     a[2 * (NA + 1) + r] = t.ddy.r[r];
   }
+}
+
+template <int NA>
+__device__ __forceinline__ void write_tri_rec(float4* rec, const tri_setup<NA>& t) {
+  float4 w[5];
+  tri_rec_words<NA>(t, w);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) rec[k] = w[k];
+}
+
+template <int NA>
+__device__ __forceinline__ void write_tri_attr(float4* a, const tri_setup<NA>& t) {
+  float4 w[3 * (NA + 1)];
+  tri_attr_words<NA>(t, w);
+#pragma unroll
+  for (int k = 0; k < 3 * (NA + 1); ++k) a[k] = w[k];
 }
 
 __device__ __forceinline__ void read_tri_rec(const float4* rec, float* e, float* bb) {
@@ -587,10 +632,12 @@ __device__ __forceinline__ void setup_body(const draw_params& P, const float4* _
                                            uint32_t bx) {
   uint32_t p = bx * blockDim.x + threadIdx.x;
   unsigned long long cprims = 0;
+  float4 rec[5], attr[3 * (NA + 1)];  // this lane's setup words, stored by the whole wave below
+  bool have = false;
   if (p < P.prim_count) {
-    cnt[3 * p + 0] = 0;
     cnt[3 * p + 1] = 0;
     cnt[3 * p + 2] = 0;
+    uint32_t c0 = 0;
     vtx<NA> v[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) load_vertex<NA>(P, vbuf, fetch_index(P, p, i), v[i]);
@@ -616,8 +663,9 @@ __device__ __forceinline__ void setup_body(const draw_params& P, const float4* _
         tri_setup<NA> t;
         triangle_info<NA>(o[0], o[1], o[2], t);
         if (t.valid) {
-          write_tri_rec<NA>(tri_rec + static_cast<size_t>(p) * 5, t);
-          write_tri_attr<NA>(tri_attr + static_cast<size_t>(p) * 3 * (NA + 1), t);
+          have = true;
+          tri_rec_words<NA>(t, rec);
+          tri_attr_words<NA>(t, attr);
           float e[9], bb[4];
           for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) e[i * 3 + j] = t.e[i][j];
@@ -625,9 +673,22 @@ __device__ __forceinline__ void setup_body(const draw_params& P, const float4* _
           if (bin_span(P, bb) > COUNT_WIDE)
             wide_list[atomicAdd(&counters[CT_WIDE], 1u)] = p;  // counted in the clip launch
           else
-            cnt[3 * p] = count_gtiles(P, e, bb);
+            c0 = count_gtiles(P, e, bb);
         }
       }
+    }
+    cnt[3 * p] = c0;
+  }
+  // The fan-0 slots of the wave's 64 consecutive primitives are consecutive: store the set-up
+  // triangles' words coalesced (blockDim.x is 256).
+  {
+    __shared__ float4 stage[4 * WAVE_STAGE];
+    const uint64_t mask = __ballot(have);
+    if (mask) {
+      const size_t p0 = static_cast<size_t>(bx) * blockDim.x + (threadIdx.x & ~63u);
+      float4* ws = stage + (threadIdx.x >> 6) * WAVE_STAGE;
+      wave_store_blocks<5>(tri_rec + p0 * 5, rec, mask, ws);
+      wave_store_blocks<3 * (NA + 1)>(tri_attr + p0 * 3 * (NA + 1), attr, mask, ws);
     }
   }
   if (P.count_stats) block_partial(partial, cprims);  // uniform over the launch
