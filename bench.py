@@ -84,14 +84,15 @@ def copy_calibration(mib, reps):
 
 
 def run_frames(lib, steps, warmup, profile=False, camera="oblique", sasl=False):
+    """Times `steps` frames with no query active (the renderer as an application runs it); with
+    `profile`, the same number of frames is then rendered again under the pipeline-profiles query,
+    whose per-stage HIP events (recorded on the draw's stream between its kernels) give the stage
+    and kernel times without adding their own gaps to the timed frames."""
     r = lib.renderer(A.RENDERER_BENCHMARK)
     f = scenes.setup_mesh(r, W, H, SAMPLES, camera=camera, sasl=sasl)
     for _ in range(warmup):
         f.render()
         r.flush()
-    q = r.create_query(A.QUERY_PIPELINE_PROFILES) if profile else None
-    if q is not None:
-        r.begin(q)
     lat = []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -101,7 +102,12 @@ def run_frames(lib, steps, warmup, profile=False, camera="oblique", sasl=False):
         lat.append(time.perf_counter() - t)
     total = time.perf_counter() - t0
     prof = None
-    if q is not None:
+    if profile:
+        q = r.create_query(A.QUERY_PIPELINE_PROFILES)
+        r.begin(q)
+        for _ in range(steps):
+            f.render()
+            r.flush()
         r.end(q)
         st, data = r.get_data(q)
         prof = list(data)
@@ -116,13 +122,16 @@ def multipass_frames(lib, steps, warmup):
     for _ in range(warmup):
         f.render()
         r.flush()
-    q = r.create_query(A.QUERY_PIPELINE_PROFILES)
-    r.begin(q)
     t0 = time.perf_counter()
     for _ in range(steps):
         f.render()
         r.flush()
     total = time.perf_counter() - t0
+    q = r.create_query(A.QUERY_PIPELINE_PROFILES)  # stage times from a second, profiled pass
+    r.begin(q)
+    for _ in range(steps):
+        f.render()
+        r.flush()
     r.end(q)
     prof = list(r.get_data(q)[1])
     return {"workload": "multipass 1920x1080 4xMSAA: 2048^2 shadow map of all 200k triangles, 140k opaque "

@@ -18,7 +18,7 @@ enum stat_slot {
 // CT_WIDE: triangles whose records the clip launch counts by wave (their ids in wide_list)
 // CT_CLASS: GPU tiles per record-count class (16 slots), CT_CURSOR: their placement cursors
 enum counter_slot { CT_CLIP_PRIMS = 0, CT_CLIP_SLOTS, CT_CLIP_OVERFLOW, CT_TOTAL, CT_WIDE, CT_CLASS, CT_CURSOR = CT_CLASS + 16,
-                    CT_COUNT = CT_CURSOR + 16 };
+                    CT_CLIP_DONE = CT_CURSOR + 16, CT_COUNT };
 
 // Per-draw device buffers (owned by the renderer, grown on demand).
 struct draw_buffers {
@@ -27,7 +27,8 @@ struct draw_buffers {
   float4* tri_attr;    // 3*prims * 3*(nattr+1): v0, ddx, ddy of every register
   uint32_t* vis;       // target w*h*samples: winning triangle per sample (opaque early-Z draws)
   const ps_samplers* samplers;  // device copy of the bound samplers
-  uint32_t* cnt;       // 3*prims: GPU-tile records per clipped triangle (then its exclusive scan)
+  uint32_t* cnt;       // 3*prims: GPU-tile records per clipped triangle (batches: then its exclusive scan)
+  uint32_t* bsum;      // single draws: records of each 256-primitive block's triangles (null in batches)
   uint8_t* clip_flag;  // prims
   int32_t* clip_slot;  // prims
   float4* clip_verts;  // clip_capacity * 5 * (nattr+1), screen space
@@ -116,12 +117,8 @@ void launch_clip(const draw_params& P, const draw_buffers& B, hipStream_t s);
 // In-place exclusive scan of n u32; writes the total to *total_dev.  tmp holds >= scan_tmp_size(n).
 size_t scan_tmp_size(uint32_t n);
 void launch_exclusive_scan(uint32_t* data, uint32_t n, uint32_t* tmp, uint32_t* total_dev, hipStream_t s);
-// One-launch exclusive scan of n u32 (decoupled look-back); writes the total to *total_dev when
-// given.  state holds scan_state_words(n) u32 words, zero at launch.
-size_t scan_state_words(uint32_t n);
-void launch_scan_onepass(uint32_t* data, uint32_t n, uint32_t* state, uint32_t* total_dev, hipStream_t s);
-// Zeroes the draw's counters and nstate words of scan state.
-void launch_draw_begin(uint32_t* counters, uint32_t* state, size_t nstate, uint32_t* tiles, size_t ntiles, hipStream_t s);
+// Zeroes the draw's counters and ntiles words of per-tile counts.
+void launch_draw_begin(uint32_t* counters, uint32_t* tiles, size_t ntiles, hipStream_t s);
 void launch_emit(const draw_params& P, const draw_buffers& B, uint32_t capacity, hipStream_t s);
 // Stable LSD radix sort of (keys, vals) by the low `bits` bits of keys; returns the buffer index
 // (0 or 1) that holds the result.
@@ -135,9 +132,10 @@ void launch_tile_order(const draw_params& P, const draw_buffers& B, uint32_t* or
 // with the tile ranges; false when the tile count exceeds its LDS histogram (then radix-sort).
 // Counting sort of the records by GPU tile (order-independent draws) into vals[1] with the tile ranges
 // in tstart/tend, sized for `cap` records; the kernels read the record count from
-// B.counters[CT_TOTAL] (at most cap are sorted).  B.tile_count must be zero.
+// B.counters[CT_TOTAL] (at most cap are sorted).  B.tile_count must be zero.  With `order`, the
+// tiles' scheduling order (as launch_tile_order writes it) is written too.
 bool bucket_sort_applicable(const draw_params& P);
-bool launch_bucket_sort(const draw_params& P, const draw_buffers& B, uint32_t cap, hipStream_t s);
+bool launch_bucket_sort(const draw_params& P, const draw_buffers& B, uint32_t cap, uint32_t* order, hipStream_t s);
 uint32_t bucket_hist_size(uint32_t T, uint32_t n);
 // Whether launch_vis will take the draw (its visibility mode exists and the tiles are 16x16).
 bool vis_applicable(const draw_params& P);

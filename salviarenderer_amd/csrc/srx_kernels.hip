@@ -5,10 +5,13 @@
 //                   default_vertex_cache.cpp: the shader runs once per vertex instead of per cache miss)
 //   setup_kernel    index fetch, near/far trivial accept, cull, winding, viewport transform and
 //                   triangle setup, one thread per primitive                (clipper.cpp, rasterizer.cpp:864-945)
-//   clip_kernel     Sutherland-Hodgman near/far clipping for the few primitives that need it
-//   scan + emit     per-triangle GPU-tile records written in clipped-primitive order
-//   sort            stable 8-bit LSD radix sort by GPU tile -> per-tile lists in primitive order
-//                   (the reference's per-tile std::sort, rasterizer.cpp:947-994)
+//   clip_kernel     Sutherland-Hodgman near/far clipping for the few primitives that need it; the
+//                   draw's record total from setup's per-block totals
+//   emit_kernel     per-triangle GPU-tile records written in clipped-primitive order (offsets from
+//                   the per-block totals and a workgroup scan)
+//   sort            by GPU tile: a counting sort (order-independent draws) or a stable 8-bit LSD
+//                   radix sort -> per-tile lists in primitive order (the reference's per-tile
+//                   std::sort, rasterizer.cpp:947-994)
 //   raster_kernel   one workgroup per GPU tile, one thread per 2x2 quad; walks its tile's list in
 //                   order and evaluates the reference's 64->16->4->sample decision chain per quad
 //                   (rasterizer.cpp:298-773), early/late depth-stencil, pixel shader and blending
@@ -81,11 +84,69 @@ __device__ __forceinline__ float4 fetch_elem(const input_elem_dev& e, uint32_t v
   }
 }
 
+// a[i] for i < n, else 0, loaded without a branch (the index is clamped): guarded loads in an
+// unrolled loop would each sit in a branch of its own and wait for its data before the next issues.
+// n must be at least 1.
+__device__ __forceinline__ uint32_t load_or_zero(const uint32_t* __restrict__ a, uint32_t i, uint32_t n) {
+  const uint32_t x = a[min(i, n - 1u)];
+  return i < n ? x : 0u;
+}
+
+// Coalesced stores of a wave's per-lane blocks of W float4 (lane i's block at dst + i*W): the lanes
+// in `mask` put their blocks in the wave's LDS staging area, then the wave stores the span with
+// consecutive lanes on consecutive float4, skipping the blocks of lanes not in `mask` (their memory
+// is left as it was).  Blocks wider than the staging area are staged a group of lanes at a time.
+constexpr int WAVE_STAGE = 320;  // float4 of LDS staging per wave
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <int W>
+__device__ __forceinline__ void wave_store_blocks(float4* __restrict__ dst, const float4 (&blk)[W], uint64_t mask,
+                                                  float4* __restrict__ stage) {
+  constexpr int LG = WAVE_STAGE / W < 64 ? WAVE_STAGE / W : 64;  // lanes per pass
+  const int lane = __lane_id();
+  for (int g = 0; g < 64; g += LG) {
+    const uint64_t pm = (mask >> g) & (LG >= 64 ? ~0ull : ((1ull << LG) - 1ull));
+    if (!pm) continue;  // wave-uniform
+    if (lane >= g && lane < g + LG && ((mask >> lane) & 1ull)) {
+#pragma unroll
+      for (int k = 0; k < W; ++k) stage[(lane - g) * W + k] = blk[k];
+    }
+    wave_lds_sync();
+    const int n = min(LG, 64 - g) * W;
+    for (int j = lane; j < n; j += 64)
+      if ((pm >> (j / W)) & 1ull) dst[static_cast<size_t>(g) * W + j] = stage[j];
+    wave_lds_sync();  // the stores have read the staging area before the next group writes it
+  }
+}
+// The same with a run-time block width w <= MAX_REGS (lane i's block: blk[0..w)).
+__device__ __forceinline__ void wave_store_blocks_n(float4* __restrict__ dst, const float4 (&blk)[MAX_REGS], int w,
+                                                    uint64_t mask, float4* __restrict__ stage) {
+  const int lg = min(WAVE_STAGE / w, 64);
+  const int lane = __lane_id();
+  for (int g = 0; g < 64; g += lg) {
+    const uint64_t pm = (mask >> g) & (lg >= 64 ? ~0ull : ((1ull << lg) - 1ull));
+    if (!pm) continue;  // wave-uniform
+    if (lane >= g && lane < g + lg && ((mask >> lane) & 1ull)) {
+#pragma unroll
+      for (int k = 0; k < MAX_REGS; ++k)
+        if (k < w) stage[(lane - g) * w + k] = blk[k];
+    }
+    wave_lds_sync();
+    const int n = min(lg, 64 - g) * w;
+    for (int j = lane; j < n; j += 64)
+      if ((pm >> (j / w)) & 1ull) dst[static_cast<size_t>(g) * w + j] = stage[j];
+    wave_lds_sync();
+  }
+}
+
 // Kernel bodies take their workgroup index `bx` (and grid size where they stride) as arguments:
 // the single-draw kernels pass blockIdx.x, the batched kernels a job's local workgroup index.
 __device__ __forceinline__ void vs_body(const draw_params& P, float4* __restrict__ vbuf, uint32_t bx) {
   uint32_t v = bx * blockDim.x + threadIdx.x;
-  if (v > P.nverts) return;  // slot nverts holds the output for an all-out-of-bounds fetch
+  const bool live = v <= P.nverts;  // slot nverts holds the output for an all-out-of-bounds fetch
   float4 in[MAX_INPUTS];
 #pragma unroll
   for (int i = 0; i < MAX_INPUTS; ++i) in[i] = make_float4(0, 0, 0, 0);
@@ -155,10 +216,14 @@ __device__ __forceinline__ void vs_body(const draw_params& P, float4* __restrict
     out[2] = in[2];
     break;
   }
-  size_t base = static_cast<size_t>(v) * (P.nattr + 1);
-#pragma unroll
-  for (int r = 0; r < MAX_REGS; ++r)
-    if (r <= P.nattr) vbuf[base + r] = out[r];
+  // the wave's 64 consecutive vertices' outputs are stored coalesced (blockDim.x is 256)
+  __shared__ float4 stage[4 * WAVE_STAGE];
+  const uint64_t mask = __ballot(live);
+  if (mask) {
+    const int w = P.nattr + 1;
+    const size_t v0 = static_cast<size_t>(bx) * blockDim.x + (threadIdx.x & ~63u);
+    wave_store_blocks_n(vbuf + v0 * w, out, w, mask, stage + (threadIdx.x >> 6) * WAVE_STAGE);
+  }
 }
 __global__ void vs_kernel(draw_params P, float4* __restrict__ vbuf) { vs_body(P, vbuf, blockIdx.x); }
 
@@ -522,36 +587,6 @@ __device__ __forceinline__ uint32_t count_gtiles(const draw_params& P, const flo
   return n;
 }
 
-// Coalesced stores of a wave's per-lane blocks of W float4 (lane i's block at dst + i*W): the lanes
-// in `mask` put their blocks in the wave's LDS staging area, then the wave stores the span with
-// consecutive lanes on consecutive float4, skipping the blocks of lanes not in `mask` (their memory
-// is left as it was).  Blocks wider than the staging area are staged a group of lanes at a time.
-constexpr int WAVE_STAGE = 320;  // float4 of LDS staging per wave
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-template <int W>
-__device__ __forceinline__ void wave_store_blocks(float4* __restrict__ dst, const float4 (&blk)[W], uint64_t mask,
-                                                  float4* __restrict__ stage) {
-  constexpr int LG = WAVE_STAGE / W < 64 ? WAVE_STAGE / W : 64;  // lanes per pass
-  const int lane = __lane_id();
-  for (int g = 0; g < 64; g += LG) {
-    const uint64_t pm = (mask >> g) & (LG >= 64 ? ~0ull : ((1ull << LG) - 1ull));
-    if (!pm) continue;  // wave-uniform
-    if (lane >= g && lane < g + LG && ((mask >> lane) & 1ull)) {
-#pragma unroll
-      for (int k = 0; k < W; ++k) stage[(lane - g) * W + k] = blk[k];
-    }
-    wave_lds_sync();
-    const int n = min(LG, 64 - g) * W;
-    for (int j = lane; j < n; j += 64)
-      if ((pm >> (j / W)) & 1ull) dst[static_cast<size_t>(g) * W + j] = stage[j];
-    wave_lds_sync();  // the stores have read the staging area before the next group writes it
-  }
-}
-
 template <int NA>
 __device__ __forceinline__ void tri_rec_words(const tri_setup<NA>& t, float4 (&rec)[5]) {
   rec[0] = make_float4(t.e[0][0], t.e[0][1], t.e[0][2], t.e[1][0]);
@@ -610,6 +645,20 @@ __device__ __forceinline__ void block_partial(unsigned long long* __restrict__ p
   }
 }
 
+// Sum of v over the workgroup, returned to thread 0.  All threads must call it.
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v) {
+  __shared__ uint32_t wsum[16];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), off, 64));
+  const int wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) wsum[wave] = v;
+  __syncthreads();
+  uint32_t t = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < nw; ++w) t += wsum[w];
+  return t;
+}
+
 // stats[slot] += sum(partial[0..n))
 __global__ void __launch_bounds__(256) reduce_partials_kernel(const unsigned long long* __restrict__ partial, uint32_t n,
                                                                unsigned long long* __restrict__ stats, int slot) {
@@ -629,15 +678,15 @@ __device__ __forceinline__ void setup_body(const draw_params& P, const float4* _
                                            float4* __restrict__ tri_attr, uint32_t* __restrict__ cnt,
                                            uint8_t* __restrict__ clip_flag, uint32_t* __restrict__ counters,
                                            uint32_t* __restrict__ wide_list, unsigned long long* __restrict__ partial,
-                                           uint32_t bx) {
+                                           uint32_t* __restrict__ bsum, uint32_t bx) {
   uint32_t p = bx * blockDim.x + threadIdx.x;
   unsigned long long cprims = 0;
   float4 rec[5], attr[3 * (NA + 1)];  // this lane's setup words, stored by the whole wave below
   bool have = false;
+  uint32_t c0 = 0;  // records of fan triangle 0 counted here
   if (p < P.prim_count) {
     cnt[3 * p + 1] = 0;
     cnt[3 * p + 2] = 0;
-    uint32_t c0 = 0;
     vtx<NA> v[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) load_vertex<NA>(P, vbuf, fetch_index(P, p, i), v[i]);
@@ -691,6 +740,10 @@ __device__ __forceinline__ void setup_body(const draw_params& P, const float4* _
       wave_store_blocks<3 * (NA + 1)>(tri_attr + p0 * 3 * (NA + 1), attr, mask, ws);
     }
   }
+  if (bsum) {  // uniform over the launch: the block's total; the clip launch adds the counts it makes
+    const uint32_t t = block_sum_u32(c0);
+    if (threadIdx.x == 0) bsum[bx] = t;
+  }
   if (P.count_stats) block_partial(partial, cprims);  // uniform over the launch
 }
 template <int NA>
@@ -698,8 +751,8 @@ __global__ void __launch_bounds__(256) setup_kernel(draw_params P, const float4*
                                                      float4* __restrict__ tri_attr,
                                                      uint32_t* __restrict__ cnt, uint8_t* __restrict__ clip_flag,
                                                      uint32_t* __restrict__ counters, uint32_t* __restrict__ wide_list,
-                                                     unsigned long long* __restrict__ partial) {
-  setup_body<NA>(P, vbuf, tri_rec, tri_attr, cnt, clip_flag, counters, wide_list, partial, blockIdx.x);
+                                                     unsigned long long* __restrict__ partial, uint32_t* __restrict__ bsum) {
+  setup_body<NA>(P, vbuf, tri_rec, tri_attr, cnt, clip_flag, counters, wide_list, partial, bsum, blockIdx.x);
 }
 
 // ============================================================== clipping (slow path)
@@ -789,16 +842,16 @@ __device__ uint32_t clip_primitive(const draw_params& P, uint32_t p, const float
   return fans;
 }
 
-// Clipping launch: a bounded grid strides over the primitives setup flagged (none, usually).
+// Clipping launch: a bounded grid strides over the primitives setup flagged (none, usually).  With
+// `bsum` (single draws) the counts it makes are added to their 256-primitive block's total.
 template <int NA>
-__device__ __forceinline__ void clip_body(const draw_params& P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
+__device__ __forceinline__ void clip_work(const draw_params& P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
                                           float4* __restrict__ tri_attr, uint32_t* __restrict__ cnt,
                                           const uint8_t* __restrict__ clip_flag, int32_t* __restrict__ clip_slot,
                                           float4* __restrict__ clip_verts, uint32_t* __restrict__ clip_hdr, uint32_t capacity,
                                           uint32_t* __restrict__ counters, unsigned long long* __restrict__ stats,
-                                          const uint32_t* __restrict__ wide_list, uint32_t bx, uint32_t gx) {
-  const uint32_t nclip = counters[CT_CLIP_PRIMS], nwide = counters[CT_WIDE];
-  if (nclip == 0 && nwide == 0) return;  // setup flagged nothing
+                                          const uint32_t* __restrict__ wide_list, uint32_t* __restrict__ bsum,
+                                          uint32_t nclip, uint32_t nwide, uint32_t bx, uint32_t gx) {
   // triangles over many 64x64 tiles: one wave counts each one's records
   const uint32_t wave = (bx * blockDim.x + threadIdx.x) >> 6, waves = (gx * blockDim.x) >> 6;
   for (uint32_t w = wave; w < nwide; w += waves) {
@@ -806,7 +859,10 @@ __device__ __forceinline__ void clip_body(const draw_params& P, const float4* __
     float e[9], bb[4];
     read_tri_rec(tri_rec + static_cast<size_t>(p) * 5, e, bb);  // fan 0 of primitive p
     const uint32_t n = count_gtiles_wave(P, e, bb);
-    if (__lane_id() == 0) cnt[3 * p] = n;
+    if (__lane_id() == 0) {
+      cnt[3 * p] = n;
+      if (bsum) atomicAdd(&bsum[p / 256], n);
+    }
   }
   if (nclip == 0) return;
   // lanes clip the flagged primitives among the wave's 64; the wave then counts the records of the
@@ -828,19 +884,72 @@ __device__ __forceinline__ void clip_body(const draw_params& P, const float4* __
         float e[9], bb[4];
         read_tri_rec(tri_rec + tri_slot(P, id) * 5, e, bb);
         const uint32_t n = count_gtiles_wave(P, e, bb);
-        if (lane == 0) cnt[id] = n;
+        if (lane == 0) {
+          cnt[id] = n;
+          if (bsum) atomicAdd(&bsum[ps / 256], n);
+        }
       }
     }
   }
+}
+
+// counters[CT_TOTAL] = the sum of the block totals (every thread of the workgroup calls it).
+__device__ __forceinline__ void record_total(const draw_params& P, const uint32_t* __restrict__ bsum,
+                                             uint32_t* __restrict__ counters) {
+  const uint32_t nb = (P.prim_count + 255) / 256;
+  uint32_t t = 0;
+  for (uint32_t i0 = 0; i0 < nb; i0 += 8 * blockDim.x) {
+    uint32_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t i = i0 + k * blockDim.x + threadIdx.x;
+      const uint32_t x = __hip_atomic_load(bsum + min(i, nb - 1u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      v[k] = i < nb ? x : 0u;  // no branch per load (see load_or_zero)
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t += v[k];
+  }
+  t = block_sum_u32(t);
+  if (threadIdx.x == 0) counters[CT_TOTAL] = t;
+}
+
+template <int NA>
+__device__ __forceinline__ void clip_body(const draw_params& P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
+                                          float4* __restrict__ tri_attr, uint32_t* __restrict__ cnt,
+                                          const uint8_t* __restrict__ clip_flag, int32_t* __restrict__ clip_slot,
+                                          float4* __restrict__ clip_verts, uint32_t* __restrict__ clip_hdr, uint32_t capacity,
+                                          uint32_t* __restrict__ counters, unsigned long long* __restrict__ stats,
+                                          const uint32_t* __restrict__ wide_list, uint32_t* __restrict__ bsum, uint32_t bx,
+                                          uint32_t gx) {
+  const uint32_t nclip = counters[CT_CLIP_PRIMS], nwide = counters[CT_WIDE];
+  const bool flagged = nclip != 0 || nwide != 0;  // uniform over the launch
+  if (flagged)
+    clip_work<NA>(P, vbuf, tri_rec, tri_attr, cnt, clip_flag, clip_slot, clip_verts, clip_hdr, capacity, counters, stats,
+                  wide_list, bsum, nclip, nwide, bx, gx);
+  if (!bsum) return;
+  // single draws: the record total, once every workgroup's counts are in the block totals (the last
+  // workgroup to finish sums them; with nothing flagged, setup's totals are final and workgroup 0 does)
+  if (flagged) {
+    __shared__ uint32_t s_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(&counters[CT_CLIP_DONE], 1u) == gx - 1 ? 1u : 0u;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+  } else if (bx != 0) {
+    return;
+  }
+  record_total(P, bsum, counters);
 }
 template <int NA>
 __global__ void clip_kernel(draw_params P, const float4* __restrict__ vbuf, float4* __restrict__ tri_rec,
                             float4* __restrict__ tri_attr, uint32_t* __restrict__ cnt, const uint8_t* __restrict__ clip_flag, int32_t* __restrict__ clip_slot,
                             float4* __restrict__ clip_verts, uint32_t* __restrict__ clip_hdr, uint32_t capacity,
                             uint32_t* __restrict__ counters, unsigned long long* __restrict__ stats,
-                            const uint32_t* __restrict__ wide_list) {
+                            const uint32_t* __restrict__ wide_list, uint32_t* __restrict__ bsum) {
   clip_body<NA>(P, vbuf, tri_rec, tri_attr, cnt, clip_flag, clip_slot, clip_verts, clip_hdr, capacity, counters, stats,
-                wide_list, blockIdx.x, gridDim.x);
+                wide_list, bsum, blockIdx.x, gridDim.x);
 }
 
 // ============================================================== record emission
@@ -900,20 +1009,14 @@ __device__ __forceinline__ void emit_wide(const draw_params& P, const float4* __
 // COUNT_WIDE 64x64 tiles).
 constexpr uint32_t EMIT_WIDE = 64;
 
-__device__ __forceinline__ void emit_body(const draw_params& P, const float4* __restrict__ tri_rec,
-                                          const uint32_t* __restrict__ off, const uint32_t* __restrict__ total,
-                                          uint32_t capacity, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                          uint32_t bx) {
-  const uint32_t p = bx * blockDim.x + threadIdx.x;
-  const uint32_t n = 3 * P.prim_count;
-  uint32_t o[4] = {0, 0, 0, 0};
+// Records of primitive p's fan triangles: triangle 3p+k owns [o[k], o[k+1]).  Every lane of the wave
+// calls it (p >= prim_count: nothing of its own).
+__device__ __forceinline__ void emit_records(const draw_params& P, const float4* __restrict__ tri_rec, uint32_t p,
+                                             const uint32_t (&o)[4], uint32_t capacity, uint32_t* __restrict__ keys,
+                                             uint32_t* __restrict__ vals) {
   uint32_t wide = 0;  // fan triangles left to the wave
   if (p < P.prim_count) {
     const uint32_t id3 = 3 * p;
-    o[0] = off[id3];
-    o[1] = off[id3 + 1];
-    o[2] = off[id3 + 2];
-    o[3] = (id3 + 3 < n) ? off[id3 + 3] : *total;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const uint32_t id = id3 + k;
@@ -955,11 +1058,67 @@ __device__ __forceinline__ void emit_body(const draw_params& P, const float4* __
     }
   }
 }
+// off: exclusive scan of the counts, followed by the total (batches)
+__device__ __forceinline__ void emit_body(const draw_params& P, const float4* __restrict__ tri_rec,
+                                          const uint32_t* __restrict__ off, const uint32_t* __restrict__ total,
+                                          uint32_t capacity, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                          uint32_t bx) {
+  const uint32_t p = bx * blockDim.x + threadIdx.x;
+  const uint32_t n = 3 * P.prim_count;
+  uint32_t o[4] = {0, 0, 0, 0};
+  if (p < P.prim_count) {
+    const uint32_t id3 = 3 * p;
+    o[0] = off[id3];
+    o[1] = off[id3 + 1];
+    o[2] = off[id3 + 2];
+    o[3] = (id3 + 3 < n) ? off[id3 + 3] : *total;
+  }
+  emit_records(P, tri_rec, p, o, capacity, keys, vals);
+}
+// Single draws: the offsets come from the counts of the workgroup's 256 primitives (a workgroup
+// scan) and the totals of the blocks before it, which setup and the clip launch left in `bsum`, so
+// no scan of the counts runs: triangle id's first record is the sum of the counts of the triangles
+// before it, as an exclusive scan of the counts gives it.
 __global__ void __launch_bounds__(256) emit_kernel(draw_params P, const float4* __restrict__ tri_rec,
-                                                   const uint32_t* __restrict__ off, const uint32_t* __restrict__ total,
+                                                   const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ bsum,
                                                    uint32_t capacity, uint32_t* __restrict__ keys,
                                                    uint32_t* __restrict__ vals) {
-  emit_body(P, tri_rec, off, total, capacity, keys, vals, blockIdx.x);
+  __shared__ uint32_t wtot[4], wbefore[4];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t p = blockIdx.x * 256 + tid;
+  uint32_t c[3] = {0, 0, 0};
+  if (p < P.prim_count) {
+    c[0] = cnt[3 * p];
+    c[1] = cnt[3 * p + 1];
+    c[2] = cnt[3 * p + 2];
+  }
+  uint32_t before = 0;  // records of the blocks before this one
+  for (uint32_t i0 = 0; i0 < blockIdx.x; i0 += 4 * 256) {
+    uint32_t v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t i = i0 + k * 256 + tid;
+      v[k] = load_or_zero(bsum, i, blockIdx.x);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) before += v[k];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) before += static_cast<uint32_t>(__shfl_xor(static_cast<int>(before), off, 64));
+  const uint32_t t = c[0] + c[1] + c[2];
+  const uint32_t incl = wave_incl_scan(t);
+  if (lane == 63) {
+    wtot[wave] = incl;
+    wbefore[wave] = before;
+  }
+  __syncthreads();
+  uint32_t o[4];
+  o[0] = wbefore[0] + wbefore[1] + wbefore[2] + wbefore[3] + incl - t;
+  for (uint32_t w = 0; w < wave; ++w) o[0] += wtot[w];
+  o[1] = o[0] + c[0];
+  o[2] = o[1] + c[1];
+  o[3] = o[2] + c[2];
+  emit_records(P, tri_rec, p, o, capacity, keys, vals);
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
@@ -1037,134 +1196,16 @@ __global__ void __launch_bounds__(256) scan_add_kernel(uint32_t* data, uint32_t 
   }
 }
 
-// ---- single-pass exclusive scan (decoupled look-back, Merrill & Garland 2016)
-// Each workgroup takes the next tile id from a counter, scans its SCAN_BLOCK elements and
-// publishes their sum, then adds its predecessors' published words, walking back 64 tiles at a
-// time, until one of them carries an inclusive prefix; it publishes its own inclusive prefix and
-// writes its elements.  Tile ids follow the order in which workgroups start, so a workgroup only
-// waits for tiles whose workgroups are already running.  state (u64): [0] the tile counter,
-// [1 + t] tile t's word, flag in the high half (1: sum of the tile, 2: inclusive prefix) and the
-// value in the low half; flag and value travel in one atomic word.  The state must be zero at
-// launch: scan_state_words(n) u32 words.
-constexpr unsigned long long SCAN_FLAG_SUM = 1ull << 32, SCAN_FLAG_INCL = 2ull << 32;
-
-size_t scan_state_words(uint32_t n) { return 2 * ((static_cast<size_t>(n) + SCAN_BLOCK - 1) / SCAN_BLOCK + 2); }
-
-__device__ __forceinline__ unsigned long long scan_state_load(const unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void scan_state_store(unsigned long long* p, unsigned long long v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__global__ void __launch_bounds__(256) scan_onepass_kernel(uint32_t* data, uint32_t n, unsigned long long* state,
-                                                          uint32_t* total) {
-  __shared__ uint32_t wsum[4];
-  __shared__ uint32_t s_tile, s_prefix;
-  if (threadIdx.x == 0) s_tile = atomicAdd(reinterpret_cast<unsigned int*>(state), 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  const uint64_t base = static_cast<uint64_t>(tile) * SCAN_BLOCK + threadIdx.x * SCAN_PER_THREAD;
-  const bool vec = (static_cast<uint64_t>(tile) + 1) * SCAN_BLOCK <= n && (reinterpret_cast<uintptr_t>(data) & 15) == 0;
-  uint32_t v[SCAN_PER_THREAD];
-  uint32_t t = 0;
-  if (vec) {
-    const uint4* d4 = reinterpret_cast<const uint4*>(data + base);
-#pragma unroll
-    for (int i = 0; i < SCAN_PER_THREAD / 4; ++i) {
-      const uint4 q = d4[i];
-      v[4 * i] = q.x;
-      v[4 * i + 1] = q.y;
-      v[4 * i + 2] = q.z;
-      v[4 * i + 3] = q.w;
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < SCAN_PER_THREAD; ++i) v[i] = (base + i < n) ? data[base + i] : 0u;
-  }
-#pragma unroll
-  for (int i = 0; i < SCAN_PER_THREAD; ++i) t += v[i];
-  const uint32_t incl = wave_incl_scan(t);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (lane == 63) wsum[wave] = incl;
-  __syncthreads();
-  uint32_t wprefix = 0, tile_sum = 0;
-#pragma unroll
-  for (int w = 0; w < 4; ++w) {
-    if (w < wave) wprefix += wsum[w];
-    tile_sum += wsum[w];
-  }
-  if (wave == 0) {
-    uint32_t prefix = 0;
-    if (tile == 0) {
-      if (lane == 0) scan_state_store(state + 1, SCAN_FLAG_INCL | tile_sum);
-    } else {
-      if (lane == 0) scan_state_store(state + 1 + tile, SCAN_FLAG_SUM | tile_sum);
-      int64_t j = static_cast<int64_t>(tile) - 1 - lane;  // lane l reads tile - 1 - l
-      while (true) {
-        // before tile 0 the prefix is an inclusive zero
-        const unsigned long long w = j >= 0 ? scan_state_load(state + 1 + j) : SCAN_FLAG_INCL;
-        const uint32_t flag = static_cast<uint32_t>(w >> 32);
-        const unsigned long long incl_lanes = __ballot(flag == 2u);
-        const unsigned long long wait_lanes = __ballot(flag == 0u);
-        // lanes 0..stop are needed: up to the nearest inclusive prefix, else all 64
-        const int stop = incl_lanes ? __builtin_ctzll(incl_lanes) : 63;
-        const unsigned long long needed = stop == 63 ? ~0ull : ((2ull << stop) - 1ull);
-        if (wait_lanes & needed) {
-          __builtin_amdgcn_s_sleep(1);
-          continue;
-        }
-        uint32_t x = lane <= stop ? static_cast<uint32_t>(w) : 0u;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) x += static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), off, 64));
-        prefix += x;
-        if (incl_lanes) break;
-        j -= 64;
-      }
-      if (lane == 0) scan_state_store(state + 1 + tile, SCAN_FLAG_INCL | (prefix + tile_sum));
-    }
-    if (lane == 0) s_prefix = prefix;
-  }
-  __syncthreads();
-  const uint32_t prefix = s_prefix;
-  uint32_t run = prefix + wprefix + incl - t;
-#pragma unroll
-  for (int i = 0; i < SCAN_PER_THREAD; ++i) {
-    const uint32_t x = v[i];
-    v[i] = run;
-    run += x;
-  }
-  if (vec) {
-    uint4* d4 = reinterpret_cast<uint4*>(data + base);
-#pragma unroll
-    for (int i = 0; i < SCAN_PER_THREAD / 4; ++i) d4[i] = make_uint4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
-  } else {
-#pragma unroll
-    for (int i = 0; i < SCAN_PER_THREAD; ++i)
-      if (base + i < n) data[base + i] = v[i];
-  }
-  if (total && tile == gridDim.x - 1 && threadIdx.x == 0) *total = prefix + tile_sum;
-}
-
-void launch_scan_onepass(uint32_t* data, uint32_t n, uint32_t* state, uint32_t* total_dev, hipStream_t s) {
-  const uint32_t nb = std::max<uint32_t>(static_cast<uint32_t>((static_cast<uint64_t>(n) + SCAN_BLOCK - 1) / SCAN_BLOCK), 1u);
-  hipLaunchKernelGGL(scan_onepass_kernel, dim3(nb), dim3(256), 0, s, data, n,
-                     reinterpret_cast<unsigned long long*>(state), total_dev);
-}
-
-// Start of a draw: zero the counters, the state of the count scan and the per-tile record counts and
-// cursors of the binning.
-__global__ void draw_begin_kernel(uint32_t* counters, uint32_t ncounters, uint32_t* state, uint32_t nstate,
-                                  uint32_t* tiles, uint32_t ntiles) {
+// Start of a draw: zero the counters and the per-tile record counts and cursors of the binning.
+__global__ void draw_begin_kernel(uint32_t* counters, uint32_t ncounters, uint32_t* tiles, uint32_t ntiles) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < ncounters) counters[i] = 0u;
-  if (i < nstate) state[i] = 0u;
   if (i < ntiles) tiles[i] = 0u;
 }
-void launch_draw_begin(uint32_t* counters, uint32_t* state, size_t nstate, uint32_t* tiles, size_t ntiles, hipStream_t s) {
-  const uint32_t n = static_cast<uint32_t>(std::max<size_t>(std::max<size_t>(nstate, ntiles), CT_COUNT));
+void launch_draw_begin(uint32_t* counters, uint32_t* tiles, size_t ntiles, hipStream_t s) {
+  const uint32_t n = static_cast<uint32_t>(std::max<size_t>(ntiles, CT_COUNT));
   hipLaunchKernelGGL(draw_begin_kernel, dim3((n + 255) / 256), dim3(256), 0, s, counters, static_cast<uint32_t>(CT_COUNT),
-                     state, static_cast<uint32_t>(nstate), tiles, static_cast<uint32_t>(ntiles));
+                     tiles, static_cast<uint32_t>(ntiles));
 }
 
 size_t scan_tmp_size(uint32_t n) {
@@ -1459,11 +1500,33 @@ __global__ void __launch_bounds__(256) tile_count_kernel(const uint32_t* __restr
     }
   }
 }
-// One workgroup: tile ranges from the totals (exclusive prefix, BUCKET_MAX_TILES tiles per step), and
-// the tiles per record-count class.
+// counters[key] += the number of valid lanes with this lane's key (< 16), one LDS atomic per distinct
+// key; returns the counter's old value plus the lane's rank among them.  Lanes with equal keys are
+// matched by four ballots, one per key bit.  All lanes of the wave call it.
+__device__ __forceinline__ uint32_t wave_counted_add16(uint32_t* counters, uint32_t key, bool valid) {
+  const int lane = __lane_id();
+  uint64_t m = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const bool bit = ((key >> b) & 1u) != 0;
+    const uint64_t bal = __ballot(bit);
+    m &= bit ? bal : ~bal;
+  }
+  const uint64_t lt = (1ull << lane) - 1ull;
+  uint32_t old = 0;
+  if (valid && (m & lt) == 0) old = atomicAdd(&counters[key], static_cast<uint32_t>(__popcll(m)));
+  const int leader = valid ? __builtin_ctzll(m) : lane;
+  old = static_cast<uint32_t>(__shfl(static_cast<int>(old), leader, 64));
+  return old + static_cast<uint32_t>(__popcll(m & lt));
+}
+
+// One workgroup: tile ranges from the totals (exclusive prefix, BUCKET_MAX_TILES tiles per step), the
+// tiles per record-count class, and, when `order` is given, the tiles' scheduling order: tiles of
+// class c after every tile of the classes before c, within a class in the atomics' order (tiles are
+// independent, so the order never changes a pixel).
 __global__ void __launch_bounds__(1024) tile_prefix_kernel(const uint32_t* __restrict__ tile_count, uint32_t T,
                                                           uint32_t* __restrict__ ts, uint32_t* __restrict__ te,
-                                                          uint32_t* __restrict__ counters) {
+                                                          uint32_t* __restrict__ counters, uint32_t* __restrict__ order) {
   constexpr int PER = BUCKET_MAX_TILES / 1024;
   __shared__ uint32_t wsum[16];
   __shared__ uint32_t classes[16];
@@ -1475,7 +1538,7 @@ __global__ void __launch_bounds__(1024) tile_prefix_kernel(const uint32_t* __res
     uint32_t c[PER], sum = 0;
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-      c[i] = t0 + i < T ? tile_count[t0 + i] : 0u;
+      c[i] = load_or_zero(tile_count, t0 + i, T);
       sum += c[i];
     }
     const uint32_t incl = wave_incl_scan(sum);
@@ -1496,12 +1559,151 @@ __global__ void __launch_bounds__(1024) tile_prefix_kernel(const uint32_t* __res
       run += c[i];
     }
 #pragma unroll
-    for (int i = 0; i < PER; ++i)
-      if (t0 + i < T) (void)wave_counted_add(classes, tile_class(c[i]));
+    for (int i = 0; i < PER; ++i) (void)wave_counted_add16(classes, tile_class(c[i]), t0 + i < T);
     carry += step;
   }
   __syncthreads();
   if (tid < 16) counters[CT_CLASS + tid] = classes[tid];
+  if (!order) return;
+  // classes[] becomes each class's next position
+  if (tid == 0) {
+    uint32_t b = 0;
+    for (int k = 0; k < 16; ++k) {
+      const uint32_t n = classes[k];
+      classes[k] = b;
+      b += n;
+    }
+  }
+  __syncthreads();
+  for (uint32_t base = 0; base < T; base += BUCKET_MAX_TILES) {
+    const uint32_t t0 = base + tid * PER;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const bool valid = t0 + i < T;
+      const uint32_t pos = wave_counted_add16(classes, tile_class(load_or_zero(tile_count, t0 + i, T)), valid);
+      if (valid) order[pos] = t0 + i;
+    }
+  }
+}
+// The same for targets of at most PREFIX_MAX_TILES tiles, one workgroup per PREFIX_TILES tiles and
+// no communication between them: every workgroup reads all T tile totals (8 per thread at 1080p),
+// from which it takes the records of the tiles before its own, the tiles of each record-count class
+// in the whole target and before its own tiles, and writes its tiles' ranges and scheduling
+// positions.  The order is class by class (heaviest first), by tile within a class.
+constexpr uint32_t PREFIX_TILES = 1024;
+constexpr uint32_t PREFIX_MAX_TILES = 65536;
+__global__ void __launch_bounds__(1024) tile_prefix_par_kernel(const uint32_t* __restrict__ tile_count, uint32_t T,
+                                                              uint32_t* __restrict__ ts, uint32_t* __restrict__ te,
+                                                              uint32_t* __restrict__ counters, uint32_t* __restrict__ order) {
+  __shared__ uint32_t w_all[16][16], w_before[16][16], w_mine[16][16];  // [wave][class]
+  __shared__ uint32_t w_rec[16], w_scan[16], c_base[16], s_start[16], s_pos[16][16];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, bx = blockIdx.x;
+  const uint32_t nj = (T + PREFIX_TILES - 1) / PREFIX_TILES;
+  uint32_t all[16], before[16];  // this wave's tiles per class: all of them, and those before the workgroup's
+#pragma unroll
+  for (int k = 0; k < 16; ++k) all[k] = before[k] = 0;
+  uint32_t rec_before = 0, mine = 0;
+  for (uint32_t j0 = 0; j0 < nj; j0 += 8) {
+    uint32_t v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const uint32_t i = (j0 + q) * PREFIX_TILES + tid;
+      v[q] = load_or_zero(tile_count, i, T);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const uint32_t j = j0 + q;  // uniform
+      if (j >= nj) break;
+      const bool valid = j * PREFIX_TILES + tid < T;
+      const uint32_t c = tile_class(v[q]);
+      if (j < bx) rec_before += v[q];
+      if (j == bx) mine = v[q];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const uint32_t m = static_cast<uint32_t>(__popcll(__ballot(valid && c == static_cast<uint32_t>(k))));
+        all[k] += m;
+        if (j < bx) before[k] += m;
+      }
+    }
+  }
+  const uint32_t t = bx * PREFIX_TILES + tid;
+  const bool mine_valid = t < T;
+  const uint32_t cm = tile_class(mine);
+  // rank among the wave's own tiles of the same class (four ballots, one per class bit)
+  uint64_t m = __ballot(mine_valid);
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const bool bit = ((cm >> b) & 1u) != 0;
+    const uint64_t bal = __ballot(bit);
+    m &= bit ? bal : ~bal;
+  }
+  const uint32_t rank = static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+  uint32_t mine_k[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) mine_k[k] = static_cast<uint32_t>(__popcll(__ballot(mine_valid && cm == static_cast<uint32_t>(k))));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) rec_before += static_cast<uint32_t>(__shfl_xor(static_cast<int>(rec_before), off, 64));
+  const uint32_t incl = wave_incl_scan(mine);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      w_all[wave][k] = all[k];
+      w_before[wave][k] = before[k];
+      w_mine[wave][k] = mine_k[k];
+    }
+    w_rec[wave] = rec_before;
+  }
+  if (lane == 63) w_scan[wave] = incl;
+  __syncthreads();
+  // Per-wave and per-class bases, each a fixed-length sum over the 16 waves (unrolled, the loads
+  // issued together):
+  //   c_base[k]     tiles of classes < k in the target + tiles of class k before the workgroup's
+  //   s_start[w]    records of the tiles before the workgroup's + of waves < w's tiles
+  //   s_pos[w][k]   c_base[k] + this workgroup's class-k tiles in waves < w
+  if (tid < 64) {
+    const uint32_t k = tid & 15;
+    uint32_t tot = 0, bef = 0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+      tot += w_all[w][k];
+      bef += w_before[w][k];
+    }
+    uint32_t below = tot;  // inclusive scan over the 16 classes, then exclusive
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(below), off, 16));
+      if (k >= static_cast<uint32_t>(off)) below += y;
+    }
+    below -= tot;
+    if (tid < 16) {
+      c_base[k] = below + bef;
+      if (bx == 0) counters[CT_CLASS + k] = tot;
+    }
+  } else if (tid < 80) {
+    const uint32_t w = tid - 64;
+    uint32_t rec = 0, pre = 0;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      rec += w_rec[v];
+      pre += static_cast<uint32_t>(v) < w ? w_scan[v] : 0u;
+    }
+    s_start[w] = rec + pre;
+  }
+  __syncthreads();
+  if (tid < 256) {
+    const uint32_t w = tid >> 4, k = tid & 15;
+    uint32_t pre = 0;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) pre += static_cast<uint32_t>(v) < w ? w_mine[v][k] : 0u;
+    s_pos[w][k] = c_base[k] + pre;
+  }
+  __syncthreads();
+  if (mine_valid) {
+    const uint32_t start = s_start[wave] + incl - mine;
+    ts[t] = start;
+    te[t] = start + mine;
+    order[s_pos[wave][cm] + rank] = t;
+  }
 }
 __global__ void __launch_bounds__(256) tile_scatter_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                                                            const uint32_t* __restrict__ count, uint32_t cap, uint32_t T,
@@ -1532,13 +1734,17 @@ __global__ void __launch_bounds__(256) tile_scatter_kernel(const uint32_t* __res
 }
 
 bool bucket_sort_applicable(const draw_params& P) { return P.gtiles_x > 0 && P.gtiles_y > 0; }
-bool launch_bucket_sort(const draw_params& P, const draw_buffers& B, uint32_t cap, hipStream_t s) {
+bool launch_bucket_sort(const draw_params& P, const draw_buffers& B, uint32_t cap, uint32_t* order, hipStream_t s) {
   const uint32_t T = static_cast<uint32_t>(P.gtiles_x * P.gtiles_y);
   if (!bucket_sort_applicable(P)) return false;
   const uint32_t* count = B.counters + CT_TOTAL;
   const uint32_t nblocks = std::max<uint32_t>((cap + BUCKET_ITEMS - 1) / BUCKET_ITEMS, 1);
   hipLaunchKernelGGL(tile_count_kernel, dim3(nblocks), dim3(256), 0, s, B.keys[0], count, cap, T, B.tile_count);
-  hipLaunchKernelGGL(tile_prefix_kernel, dim3(1), dim3(1024), 0, s, B.tile_count, T, B.tstart, B.tend, B.counters);
+  if (order && T <= PREFIX_MAX_TILES)
+    hipLaunchKernelGGL(tile_prefix_par_kernel, dim3((T + PREFIX_TILES - 1) / PREFIX_TILES), dim3(1024), 0, s, B.tile_count, T,
+                       B.tstart, B.tend, B.counters, order);
+  else
+    hipLaunchKernelGGL(tile_prefix_kernel, dim3(1), dim3(1024), 0, s, B.tile_count, T, B.tstart, B.tend, B.counters, order);
   hipLaunchKernelGGL(tile_scatter_kernel, dim3(nblocks), dim3(256), 0, s, B.keys[0], B.vals[0], count, cap, T, B.tstart,
                      B.tile_count + T, B.vals[1]);
   return true;
@@ -3105,7 +3311,7 @@ void launch_setup(const draw_params& P, const draw_buffers& B, hipStream_t s) {
   dim3 grid((P.prim_count + 255) / 256);
 #define SRX_SETUP(NA) \
   hipLaunchKernelGGL(setup_kernel<NA>, grid, dim3(256), 0, s, P, B.vbuf, B.tri_rec, B.tri_attr, B.cnt, B.clip_flag, B.counters, \
-                     B.wide_list, B.partial)
+                     B.wide_list, B.partial, B.bsum)
   SRX_NA_DISPATCH(P.nattr, SRX_SETUP)
 #undef SRX_SETUP
   if (P.count_stats)
@@ -3117,7 +3323,7 @@ void launch_clip(const draw_params& P, const draw_buffers& B, hipStream_t s) {
   dim3 grid(std::min<uint32_t>((P.prim_count + 63) / 64, 1024));
 #define SRX_CLIP(NA)                                                                                                     \
   hipLaunchKernelGGL(clip_kernel<NA>, grid, dim3(64), 0, s, P, B.vbuf, B.tri_rec, B.tri_attr, B.cnt, B.clip_flag, B.clip_slot, \
-                     B.clip_verts, B.clip_hdr, B.clip_capacity, B.counters, B.stats, B.wide_list)
+                     B.clip_verts, B.clip_hdr, B.clip_capacity, B.counters, B.stats, B.wide_list, B.bsum)
   SRX_NA_DISPATCH(P.nattr, SRX_CLIP)
 #undef SRX_CLIP
 }
@@ -3125,7 +3331,7 @@ void launch_clip(const draw_params& P, const draw_buffers& B, hipStream_t s) {
 void launch_emit(const draw_params& P, const draw_buffers& B, uint32_t capacity, hipStream_t s) {
   const uint32_t n = P.prim_count;
   if (!n) return;
-  hipLaunchKernelGGL(emit_kernel, dim3((n + 255) / 256), dim3(256), 0, s, P, B.tri_rec, B.cnt, B.counters + CT_TOTAL,
+  hipLaunchKernelGGL(emit_kernel, dim3((n + 255) / 256), dim3(256), 0, s, P, B.tri_rec, B.cnt, B.bsum,
                      capacity, B.keys[0], B.vals[0]);
 }
 
@@ -3466,7 +3672,7 @@ __global__ void __launch_bounds__(256) setup_batch_kernel(const batch_job* __res
   uint32_t lb, nb;
   const batch_job& J = jobs[stage_job(m, lb, nb)];
   setup_body<NA>(J.P, J.B.vbuf, J.B.tri_rec, J.B.tri_attr, J.B.cnt, J.B.clip_flag, J.B.counters, J.B.wide_list, J.B.partial,
-                 lb);
+                 nullptr, lb);
 }
 
 template <int NA>
@@ -3474,7 +3680,7 @@ __global__ void clip_batch_kernel(const batch_job* __restrict__ jobs, stage_map 
   uint32_t lb, nb;
   const batch_job& J = jobs[stage_job(m, lb, nb)];
   clip_body<NA>(J.P, J.B.vbuf, J.B.tri_rec, J.B.tri_attr, J.B.cnt, J.B.clip_flag, J.B.clip_slot, J.B.clip_verts,
-                J.B.clip_hdr, J.B.clip_capacity, J.B.counters, J.B.stats, J.B.wide_list, lb, nb);
+                J.B.clip_hdr, J.B.clip_capacity, J.B.counters, J.B.stats, J.B.wide_list, nullptr, lb, nb);
 }
 
 // The job's counts are followed by the next job's first scanned count (or the batch total): the

@@ -234,8 +234,7 @@ static size_t record_scratch_bytes(uint32_t P3, uint32_t T, int samples, bool vi
   const uint32_t nblocks = (std::max(n, T) + 4095) / 4096;
   const size_t hist_n = std::max<size_t>(static_cast<size_t>(256) * std::max<uint32_t>(nblocks, 1), bucket_hist_size(T, n));
   size_t b = 2 * align256(rn * 4) + 2 * align256(rn * 4) + align256(hist_n * 4) +
-             align256(std::max({scan_tmp_size(P3), scan_tmp_size(static_cast<uint32_t>(hist_n)), scan_state_words(P3),
-                                scan_state_words(static_cast<uint32_t>(hist_n))}) * 4) +
+             align256(std::max(scan_tmp_size(P3), scan_tmp_size(static_cast<uint32_t>(hist_n))) * 4) +
              align256(std::max<uint32_t>(T, 1) * 4);
   if (vis_tables) {
     uint32_t ne, nm, nsl;
@@ -297,7 +296,8 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   RCHK(X.take(r->clip_flag, primcnt, B.clip_flag));
   RCHK(X.take(r->clip_slot, primcnt, B.clip_slot));
   RCHK(X.take(r->wide, primcnt, B.wide_list));
-  RCHK(X.take(r->scan_tmp, std::max(scan_tmp_size(P3), scan_state_words(P3)), B.scan_tmp));
+  RCHK(X.take(r->bsum, (static_cast<size_t>(primcnt) + 255) / 256, B.bsum));
+  RCHK(X.take(r->scan_tmp, scan_tmp_size(P3), B.scan_tmp));
   RCHK(X.take(r->partial, std::max<size_t>(T, (primcnt + 255) / 256) + 1, B.partial));
   RCHK(X.take(r->tstart, std::max<uint32_t>(T, 1), B.tstart));
   RCHK(X.take(r->tend, std::max<uint32_t>(T, 1), B.tend));
@@ -377,16 +377,18 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
       r->sasl_valid = !fixed;
     }
   }
-  launch_draw_begin(B.counters, B.scan_tmp, scan_state_words(P3), B.tile_count, 2 * static_cast<size_t>(T), s);
+  launch_draw_begin(B.counters, B.tile_count, 2 * static_cast<size_t>(T), s);
   mark(0);
   launch_vs(P, B.vbuf, s);
   mark(1);
+  // setup leaves each 256-primitive block's record total in bsum; the clip launch (which returns at
+  // once on the device when setup flagged no primitive) adds the counts it makes and writes the
+  // draw's total, from which the emission takes every triangle's first record: no scan runs
   launch_setup(P, B, s);
-  launch_clip(P, B, s);  // returns at once on the device when setup flagged no primitive
+  launch_clip(P, B, s);
   mark(2);
-  launch_scan_onepass(B.cnt, P3, B.scan_tmp, B.counters + CT_TOTAL, s);
   mark(3);
-  // the counters are final after the scan: their read-back overlaps the emission
+  // the counters are final after the clip launch: their read-back overlaps the emission
   HIPCHK(hipEventRecord(r->ev_counted, s));
   launch_emit(P, B, capacity, s);
   // A target's pending full clear is applied by the order-independent visibility pass as it writes
@@ -416,9 +418,7 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
     RCHK(X.take(r->keys1, std::max<uint32_t>(n, 1), B.keys[1]));
     RCHK(X.take(r->vals1, std::max<uint32_t>(n, 1), B.vals[1]));
     RCHK(X.take(r->hist, hist_n, B.hist));
-    RCHK(X.take(r->scan_tmp, std::max({scan_tmp_size(P3), scan_tmp_size(static_cast<uint32_t>(hist_n)), scan_state_words(P3),
-                                        scan_state_words(static_cast<uint32_t>(hist_n))}),
-                B.scan_tmp));
+    RCHK(X.take(r->scan_tmp, std::max(scan_tmp_size(P3), scan_tmp_size(static_cast<uint32_t>(hist_n))), B.scan_tmp));
     return SRX_OK;
   };
   uint32_t* order = nullptr;
@@ -430,8 +430,7 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
     early = true;
     RCHK(take_sort_buffers(sort_cap));
     RCHK(X.take(r->order, std::max<uint32_t>(T, 1), order));
-    bucketed = launch_bucket_sort(P, B, sort_cap, s);
-    launch_tile_order(P, B, order, true, s);
+    bucketed = launch_bucket_sort(P, B, sort_cap, order, s);
   }
   uint32_t ctr[CT_COUNT];
   RCHK(finish_counter_read(r, ctr));
@@ -459,15 +458,15 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
     }
     // sized for the capacity on a fixed region (the layout the capacity was chosen for), else for the count
     RCHK(take_sort_buffers(fixed ? capacity : total));
-    bucketed = order_free && launch_bucket_sort(P, B, std::max<uint32_t>(total, 1), s);
+    RCHK(X.take(r->order, std::max<uint32_t>(T, 1), order));
+    bucketed = order_free && launch_bucket_sort(P, B, std::max<uint32_t>(total, 1), order, s);
     if (!bucketed) {
       int bits = 1;
       while ((1u << bits) < T) ++bits;
       which = launch_sort(B, total, bits, s);
       launch_tile_ranges(P, B, which, total, s);
+      launch_tile_order(P, B, order, false, s);
     }
-    RCHK(X.take(r->order, std::max<uint32_t>(T, 1), order));
-    launch_tile_order(P, B, order, bucketed, s);
   }
   B.order = order;
   mark(4);
@@ -519,7 +518,7 @@ renderer_obj::~renderer_obj() {
   if (samplers_h) (void)hipHostFree(samplers_h);
   if (sasl_h) (void)hipHostFree(sasl_h);
   if (sasl_d) (void)hipFree(sasl_d); fr(cnt); fr(keys0); fr(keys1); fr(vals0); fr(vals1);
-  fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot); fr(partial); fr(order); fr(tile_count); fr(wide); fr(vis_entries); fr(vis_merges); fr(vis_counts); fr(vis_partial);
+  fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot); fr(partial); fr(order); fr(tile_count); fr(wide); fr(bsum); fr(vis_entries); fr(vis_merges); fr(vis_counts); fr(vis_partial);
   if (stats_d) (void)hipFree(stats_d);
   if (pinned) (void)hipHostFree(pinned);
   if (map_host) (void)hipHostFree(map_host);
