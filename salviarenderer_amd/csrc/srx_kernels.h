@@ -111,14 +111,13 @@ int vis_mode(const draw_params& P);
 hipError_t upload_ewa_table(const float* table256, hipStream_t s);
 
 void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
-void launch_vs(const draw_params& P, float4* vbuf, hipStream_t s);
+// The vertex launch of a single draw; it also zeroes the draw's counters and ntiles words of tiles.
+void launch_vs(const draw_params& P, float4* vbuf, uint32_t* counters, uint32_t* tiles, size_t ntiles, hipStream_t s);
 void launch_setup(const draw_params& P, const draw_buffers& B, hipStream_t s);
 void launch_clip(const draw_params& P, const draw_buffers& B, hipStream_t s);
 // In-place exclusive scan of n u32; writes the total to *total_dev.  tmp holds >= scan_tmp_size(n).
 size_t scan_tmp_size(uint32_t n);
 void launch_exclusive_scan(uint32_t* data, uint32_t n, uint32_t* tmp, uint32_t* total_dev, hipStream_t s);
-// Zeroes the draw's counters and ntiles words of per-tile counts.
-void launch_draw_begin(uint32_t* counters, uint32_t* tiles, size_t ntiles, hipStream_t s);
 void launch_emit(const draw_params& P, const draw_buffers& B, uint32_t capacity, hipStream_t s);
 // Stable LSD radix sort of (keys, vals) by the low `bits` bits of keys; returns the buffer index
 // (0 or 1) that holds the result.

@@ -225,7 +225,17 @@ __device__ __forceinline__ void vs_body(const draw_params& P, float4* __restrict
     wave_store_blocks_n(vbuf + v0 * w, out, w, mask, stage + (threadIdx.x >> 6) * WAVE_STAGE);
   }
 }
-__global__ void vs_kernel(draw_params P, float4* __restrict__ vbuf) { vs_body(P, vbuf, blockIdx.x); }
+// A single draw's first launch: it also zeroes the draw's counters and ntiles per-tile record counts
+// and cursors, which the launches after it add to.
+__global__ void __launch_bounds__(256) vs_kernel(draw_params P, float4* __restrict__ vbuf, uint32_t* __restrict__ counters,
+                                                 uint32_t* __restrict__ tiles, uint32_t ntiles) {
+  const uint32_t nz = max(ntiles, static_cast<uint32_t>(CT_COUNT));
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nz; i += gridDim.x * blockDim.x) {
+    if (i < static_cast<uint32_t>(CT_COUNT)) counters[i] = 0u;
+    if (i < ntiles) tiles[i] = 0u;
+  }
+  vs_body(P, vbuf, blockIdx.x);
+}
 
 // ============================================================== primitive assembly helpers
 __device__ __forceinline__ uint32_t fetch_index(const draw_params& P, uint32_t prim, int k) {
@@ -1194,18 +1204,6 @@ __global__ void __launch_bounds__(256) scan_add_kernel(uint32_t* data, uint32_t 
     const uint64_t i = b0 + k * 256 + threadIdx.x;
     if (i < n) data[i] += add;
   }
-}
-
-// Start of a draw: zero the counters and the per-tile record counts and cursors of the binning.
-__global__ void draw_begin_kernel(uint32_t* counters, uint32_t ncounters, uint32_t* tiles, uint32_t ntiles) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < ncounters) counters[i] = 0u;
-  if (i < ntiles) tiles[i] = 0u;
-}
-void launch_draw_begin(uint32_t* counters, uint32_t* tiles, size_t ntiles, hipStream_t s) {
-  const uint32_t n = static_cast<uint32_t>(std::max<size_t>(ntiles, CT_COUNT));
-  hipLaunchKernelGGL(draw_begin_kernel, dim3((n + 255) / 256), dim3(256), 0, s, counters, static_cast<uint32_t>(CT_COUNT),
-                     tiles, static_cast<uint32_t>(ntiles));
 }
 
 size_t scan_tmp_size(uint32_t n) {
@@ -3285,9 +3283,10 @@ hipError_t upload_ewa_table(const float* table256, hipStream_t s) {
   return hipMemcpyToSymbolAsync(HIP_SYMBOL(c_ewa), table256, 256 * sizeof(float), 0, hipMemcpyHostToDevice, s);
 }
 
-void launch_vs(const draw_params& P, float4* vbuf, hipStream_t s) {
+void launch_vs(const draw_params& P, float4* vbuf, uint32_t* counters, uint32_t* tiles, size_t ntiles, hipStream_t s) {
   uint32_t n = P.nverts + 1;
-  hipLaunchKernelGGL(vs_kernel, dim3((n + 255) / 256), dim3(256), 0, s, P, vbuf);
+  hipLaunchKernelGGL(vs_kernel, dim3((n + 255) / 256), dim3(256), 0, s, P, vbuf, counters, tiles,
+                     static_cast<uint32_t>(ntiles));
 }
 
 // Kernels are instantiated per attribute count: the vertex and setup buffers are strided by it.

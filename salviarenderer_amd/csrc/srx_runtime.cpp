@@ -377,9 +377,8 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
       r->sasl_valid = !fixed;
     }
   }
-  launch_draw_begin(B.counters, B.tile_count, 2 * static_cast<size_t>(T), s);
   mark(0);
-  launch_vs(P, B.vbuf, s);
+  launch_vs(P, B.vbuf, B.counters, B.tile_count, 2 * static_cast<size_t>(T), s);
   mark(1);
   // setup leaves each 256-primitive block's record total in bsum; the clip launch (which returns at
   // once on the device when setup flagged no primitive) adds the counts it makes and writes the

@@ -97,3 +97,24 @@ def test_slivers_reach_many_tiles():
     span = (np.floor(x.max(1) / 64) - np.floor(x.min(1) / 64) + 1) * (np.floor(y.max(1) / 64) - np.floor(y.min(1) / 64) + 1)
     assert (span > 32).sum() > 100
     assert ((t[..., 2] < 0) | (t[..., 2] > 1)).any(1).sum() > 50
+
+
+def _large_target_frame(lib):
+    """60 plain triangles (most over more than 32 64x64 tiles) into a 4160x4128 target:
+    67,080 16x16 GPU tiles, more than the multi-workgroup tile-range kernel takes (65,536), so the
+    counting sort's ranges and scheduling order come from its one-workgroup form."""
+    from salviarenderer_amd import scenes as S
+    r = lib.renderer()
+    f = S.setup_plain(r, 4160, 4128, 60, 1)
+    f.render()
+    r.flush()
+    return f.color.read().copy(), f.ds.read().copy()
+
+
+@pytest.mark.gpu
+def test_target_beyond_65536_tiles(hip_lib, cpu_lib):
+    hc, hd = _large_target_frame(hip_lib)
+    cc, cd = _large_target_frame(cpu_lib)
+    assert np.array_equal(hd, cd)
+    assert np.array_equal(hc, cc), f"{np.count_nonzero(hc != cc)} colour bytes differ"
+    assert np.count_nonzero(cd[..., 0] < 1.0) > 1_000_000  # the triangles cover a large part
