@@ -1596,10 +1596,10 @@ __global__ void __launch_bounds__(1024) tile_prefix_par_kernel(const uint32_t* _
   __shared__ uint32_t w_all[16][16], w_before[16][16], w_mine[16][16];  // [wave][class]
   __shared__ uint32_t w_rec[16], w_scan[16], c_base[16], s_start[16], s_pos[16][16];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, bx = blockIdx.x;
-  const uint32_t nj = (T + PREFIX_TILES - 1) / PREFIX_TILES;
-  uint32_t all[16], before[16];  // this wave's tiles per class: all of them, and those before the workgroup's
-#pragma unroll
-  for (int k = 0; k < 16; ++k) all[k] = before[k] = 0;
+  const uint32_t nj = (T + PREFIX_TILES - 1) / PREFIX_TILES;  // at most 64
+  // this lane's tiles per class, one 8-bit field per class (classes 0-7 in [0], 8-15 in [1]): all of
+  // them, and those before the workgroup's
+  unsigned long long all[2] = {0, 0}, before[2] = {0, 0};
   uint32_t rec_before = 0, mine = 0;
   for (uint32_t j0 = 0; j0 < nj; j0 += 8) {
     uint32_t v[8];
@@ -1616,12 +1616,9 @@ __global__ void __launch_bounds__(1024) tile_prefix_par_kernel(const uint32_t* _
       const uint32_t c = tile_class(v[q]);
       if (j < bx) rec_before += v[q];
       if (j == bx) mine = v[q];
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        const uint32_t m = static_cast<uint32_t>(__popcll(__ballot(valid && c == static_cast<uint32_t>(k))));
-        all[k] += m;
-        if (j < bx) before[k] += m;
-      }
+      const unsigned long long one = valid ? 1ull << (8 * (c & 7)) : 0ull;
+      all[c >> 3] += one;
+      if (j < bx) before[c >> 3] += one;
     }
   }
   const uint32_t t = bx * PREFIX_TILES + tid;
@@ -1636,21 +1633,44 @@ __global__ void __launch_bounds__(1024) tile_prefix_par_kernel(const uint32_t* _
     m &= bit ? bal : ~bal;
   }
   const uint32_t rank = static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
-  uint32_t mine_k[16];
+  unsigned long long mine_f[2] = {0, 0};
+  mine_f[cm >> 3] = mine_valid ? 1ull << (8 * (cm & 7)) : 0ull;
+  // the wave's per-class sums: the 8-bit fields widened to 16 bits (4 classes per word, at most
+  // 64 lanes x 64 chunks per class), added over the wave
+  unsigned long long wa[4], wb[4], wm[4];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) mine_k[k] = static_cast<uint32_t>(__popcll(__ballot(mine_valid && cm == static_cast<uint32_t>(k))));
+  for (int h = 0; h < 2; ++h) {
+    const unsigned long long lo = 0x00FF00FF00FF00FFull;
+    wa[2 * h] = all[h] & lo;
+    wa[2 * h + 1] = (all[h] >> 8) & lo;
+    wb[2 * h] = before[h] & lo;
+    wb[2 * h + 1] = (before[h] >> 8) & lo;
+    wm[2 * h] = mine_f[h] & lo;
+    wm[2 * h + 1] = (mine_f[h] >> 8) & lo;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      wa[w] += __shfl_xor(wa[w], off, 64);
+      wb[w] += __shfl_xor(wb[w], off, 64);
+      wm[w] += __shfl_xor(wm[w], off, 64);
+    }
+  // class k = 8h + 2f + e lives in word 2h + e, 16-bit field f (fields of a word: classes of one parity)
+  auto field = [](const unsigned long long (&w)[4], uint32_t k) -> uint32_t {
+    const bool h = (k >> 3) != 0, e = (k & 1) != 0;
+    const unsigned long long x = h ? (e ? w[3] : w[2]) : (e ? w[1] : w[0]);  // selects, no indexing
+    return static_cast<uint32_t>((x >> (16 * ((k & 7) >> 1))) & 0xFFFFull);
+  };
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) rec_before += static_cast<uint32_t>(__shfl_xor(static_cast<int>(rec_before), off, 64));
   const uint32_t incl = wave_incl_scan(mine);
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      w_all[wave][k] = all[k];
-      w_before[wave][k] = before[k];
-      w_mine[wave][k] = mine_k[k];
-    }
-    w_rec[wave] = rec_before;
+  if (lane < 16) {
+    w_all[wave][lane] = field(wa, lane);
+    w_before[wave][lane] = field(wb, lane);
+    w_mine[wave][lane] = field(wm, lane);
   }
+  if (lane == 0) w_rec[wave] = rec_before;
   if (lane == 63) w_scan[wave] = incl;
   __syncthreads();
   // Per-wave and per-class bases, each a fixed-length sum over the 16 waves (unrolled, the loads
