@@ -297,6 +297,7 @@ int32_t execute_draw(renderer_obj* r, bool indexed, uint32_t start, uint32_t pri
   RCHK(X.take(r->clip_slot, primcnt, B.clip_slot));
   RCHK(X.take(r->wide, primcnt, B.wide_list));
   RCHK(X.take(r->bsum, (static_cast<size_t>(primcnt) + 255) / 256, B.bsum));
+  RCHK(X.take(r->pairs, primcnt, B.pairs));
   RCHK(X.take(r->scan_tmp, scan_tmp_size(P3), B.scan_tmp));
   RCHK(X.take(r->partial, std::max<size_t>(T, (primcnt + 255) / 256) + 1, B.partial));
   RCHK(X.take(r->tstart, std::max<uint32_t>(T, 1), B.tstart));
@@ -517,7 +518,7 @@ renderer_obj::~renderer_obj() {
   if (samplers_h) (void)hipHostFree(samplers_h);
   if (sasl_h) (void)hipHostFree(sasl_h);
   if (sasl_d) (void)hipFree(sasl_d); fr(cnt); fr(keys0); fr(keys1); fr(vals0); fr(vals1);
-  fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot); fr(partial); fr(order); fr(tile_count); fr(wide); fr(bsum); fr(vis_entries); fr(vis_merges); fr(vis_counts); fr(vis_partial);
+  fr(tstart); fr(tend); fr(hist); fr(scan_tmp); fr(clip_hdr); fr(counters); fr(clip_flag); fr(clip_slot); fr(partial); fr(order); fr(tile_count); fr(wide); fr(bsum); fr(pairs); fr(vis_entries); fr(vis_merges); fr(vis_counts); fr(vis_partial);
   if (stats_d) (void)hipFree(stats_d);
   if (pinned) (void)hipHostFree(pinned);
   if (map_host) (void)hipHostFree(map_host);
