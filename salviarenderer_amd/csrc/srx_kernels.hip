@@ -3387,9 +3387,9 @@ void launch_setup(const draw_params& P, const draw_buffers& B, hipStream_t s) {
 
 void launch_clip(const draw_params& P, const draw_buffers& B, hipStream_t s) {
   if (P.prim_count == 0) return;
-  dim3 grid(std::min<uint32_t>((P.prim_count + 63) / 64, 1024));
+  dim3 grid(std::min<uint32_t>((P.prim_count + 255) / 256, 256));  // up to 1024 waves in 256-lane workgroups
 #define SRX_CLIP(NA)                                                                                                     \
-  hipLaunchKernelGGL(clip_kernel<NA>, grid, dim3(64), 0, s, P, B.vbuf, B.tri_rec, B.tri_attr, B.cnt, B.clip_flag, B.clip_slot, \
+  hipLaunchKernelGGL(clip_kernel<NA>, grid, dim3(256), 0, s, P, B.vbuf, B.tri_rec, B.tri_attr, B.cnt, B.clip_flag, B.clip_slot, \
                      B.clip_verts, B.clip_hdr, B.clip_capacity, B.counters, B.stats, B.wide_list, B.bsum)
   SRX_NA_DISPATCH(P.nattr, SRX_CLIP)
 #undef SRX_CLIP
