@@ -29,7 +29,7 @@ struct draw_buffers {
   const ps_samplers* samplers;  // device copy of the bound samplers
   uint32_t* cnt;       // 3*prims: GPU-tile records per clipped triangle (batches: then its exclusive scan)
   uint32_t* bsum;      // single draws: records of each 256-primitive block's triangles (null in batches)
-  uint2* pairs;        // single draws, per primitive: fan triangle 0's first two (tile << 1 | full)
+  uint4* pairs;        // single draws, per primitive: fan triangle 0's first four (tile << 1 | full)
                        // records as setup found them, x = 0xFFFFFFFF when not kept (null in batches)
   uint8_t* clip_flag;  // prims
   int32_t* clip_slot;  // prims

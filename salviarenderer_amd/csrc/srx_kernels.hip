@@ -597,28 +597,29 @@ __device__ __forceinline__ uint32_t count_gtiles(const draw_params& P, const flo
   return n;
 }
 
-// count_gtiles that also keeps the first two GPU tiles for_each_gtile visits, packed as
-// tile << 1 | full (setup's pairs, which spare the emission the walk of triangles with at most two
+// count_gtiles that also keeps the first four GPU tiles for_each_gtile visits, packed as
+// tile << 1 | full (setup's pairs, which spare the emission the walk of triangles with at most four
 // records).
 constexpr uint32_t NO_PAIRS = 0xFFFFFFFFu;  // no tile is that large
-__device__ __forceinline__ uint32_t count_gtiles_pairs(const draw_params& P, const float* e, const float* bb, uint32_t& k0,
-                                                       uint32_t& k1) {
+__device__ __forceinline__ uint32_t count_gtiles_pairs(const draw_params& P, const float* e, const float* bb, uint4& k) {
   tri_bin t;
   bin_setup(P, e, bb, t);
   const int sub = TILE / P.gt;
   uint32_t n = 0;
-  k0 = k1 = NO_PAIRS;
+  k = make_uint4(NO_PAIRS, NO_PAIRS, NO_PAIRS, NO_PAIRS);
   for (int y = t.sy; y < t.ey; ++y)
     for (int x = t.sx; x < t.ex; ++x) {
       bool full;
       uint64_t m = tile_gmask(P, e, bb, t, x, y, full);
       const uint32_t c = static_cast<uint32_t>(__popcll(m));
-      for (uint32_t i = n; m && i < 2; m &= m - 1, ++i) {
+      for (uint32_t i = n; m && i < 4; m &= m - 1, ++i) {
         const int b = __builtin_ctzll(m);
         const uint32_t g = static_cast<uint32_t>((y * sub + b / sub) * P.gtiles_x + x * sub + b % sub);
         const uint32_t v = (g << 1) | (full ? 1u : 0u);
-        if (i == 0) k0 = v;
-        else k1 = v;
+        if (i == 0) k.x = v;
+        else if (i == 1) k.y = v;
+        else if (i == 2) k.z = v;
+        else k.w = v;
       }
       n += c;
     }
@@ -716,13 +717,13 @@ __device__ __forceinline__ void setup_body(const draw_params& P, const float4* _
                                            float4* __restrict__ tri_attr, uint32_t* __restrict__ cnt,
                                            uint8_t* __restrict__ clip_flag, uint32_t* __restrict__ counters,
                                            uint32_t* __restrict__ wide_list, unsigned long long* __restrict__ partial,
-                                           uint32_t* __restrict__ bsum, uint2* __restrict__ pairs, uint32_t bx) {
+                                           uint32_t* __restrict__ bsum, uint4* __restrict__ pairs, uint32_t bx) {
   uint32_t p = bx * blockDim.x + threadIdx.x;
   unsigned long long cprims = 0;
   float4 rec[5], attr[3 * (NA + 1)];  // this lane's setup words, stored by the whole wave below
   bool have = false;
   uint32_t c0 = 0;  // records of fan triangle 0 counted here
-  uint32_t k0 = NO_PAIRS, k1 = NO_PAIRS;  // its first two (tile, full) pairs
+  uint4 kp = make_uint4(NO_PAIRS, NO_PAIRS, NO_PAIRS, NO_PAIRS);  // its first four (tile, full) pairs
   if (p < P.prim_count) {
     cnt[3 * p + 1] = 0;
     cnt[3 * p + 2] = 0;
@@ -761,14 +762,17 @@ __device__ __forceinline__ void setup_body(const draw_params& P, const float4* _
           if (bin_span(P, bb) > COUNT_WIDE)
             wide_list[atomicAdd(&counters[CT_WIDE], 1u)] = p;  // counted in the clip launch
           else if (pairs)
-            c0 = count_gtiles_pairs(P, e, bb, k0, k1);
+            c0 = count_gtiles_pairs(P, e, bb, kp);
           else
             c0 = count_gtiles(P, e, bb);
         }
       }
     }
     cnt[3 * p] = c0;
-    if (pairs) pairs[p] = make_uint2(c0 <= 2u ? k0 : NO_PAIRS, k1);
+    if (pairs) {
+      if (c0 > 4u) kp.x = NO_PAIRS;
+      pairs[p] = kp;
+    }
   }
   // The fan-0 slots of the wave's 64 consecutive primitives are consecutive: store the set-up
   // triangles' words coalesced (blockDim.x is 256).
@@ -794,7 +798,7 @@ __global__ void __launch_bounds__(256) setup_kernel(draw_params P, const float4*
                                                      uint32_t* __restrict__ cnt, uint8_t* __restrict__ clip_flag,
                                                      uint32_t* __restrict__ counters, uint32_t* __restrict__ wide_list,
                                                      unsigned long long* __restrict__ partial, uint32_t* __restrict__ bsum,
-                                                     uint2* __restrict__ pairs) {
+                                                     uint4* __restrict__ pairs) {
   setup_body<NA>(P, vbuf, tri_rec, tri_attr, cnt, clip_flag, counters, wide_list, partial, bsum, pairs, blockIdx.x);
 }
 
@@ -1054,10 +1058,10 @@ constexpr uint32_t EMIT_WIDE = 64;
 
 // Records of primitive p's fan triangles: triangle 3p+k owns [o[k], o[k+1]).  Every lane of the wave
 // calls it (p >= prim_count: nothing of its own).
-// pr: setup's first two (tile << 1 | full) pairs of fan triangle 0, pr.x == NO_PAIRS when it kept
+// pr: setup's first four (tile << 1 | full) pairs of fan triangle 0, pr.x == NO_PAIRS when it kept
 // none (then, and for the other fan triangles, the tiles are walked again).
 __device__ __forceinline__ void emit_records(const draw_params& P, const float4* __restrict__ tri_rec, uint32_t p,
-                                             const uint32_t (&o)[4], uint2 pr, uint32_t capacity,
+                                             const uint32_t (&o)[4], uint4 pr, uint32_t capacity,
                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   uint32_t wide = 0;  // fan triangles left to the wave
   if (p < P.prim_count) {
@@ -1067,15 +1071,14 @@ __device__ __forceinline__ void emit_records(const draw_params& P, const float4*
       const uint32_t id = id3 + k;
       const uint32_t first = o[k], next = o[k + 1];
       if (next == first) continue;
-      if (k == 0 && pr.x != NO_PAIRS) {  // at most two records, as setup's walk found them
-        if (first < capacity) {
-          keys[first] = pr.x >> 1;
-          vals[first] = (id << 1) | (pr.x & 1u);
-        }
-        if (next - first == 2 && first + 1 < capacity) {
-          keys[first + 1] = pr.y >> 1;
-          vals[first + 1] = (id << 1) | (pr.y & 1u);
-        }
+      if (k == 0 && pr.x != NO_PAIRS) {  // at most four records, as setup's walk found them
+        const uint32_t v[4] = {pr.x, pr.y, pr.z, pr.w};
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i)
+          if (first + i < next && first + i < capacity) {
+            keys[first + i] = v[i] >> 1;
+            vals[first + i] = (id << 1) | (v[i] & 1u);
+          }
         continue;
       }
       if (next - first > EMIT_WIDE) {
@@ -1129,7 +1132,7 @@ __device__ __forceinline__ void emit_body(const draw_params& P, const float4* __
     o[2] = off[id3 + 2];
     o[3] = (id3 + 3 < n) ? off[id3 + 3] : *total;
   }
-  emit_records(P, tri_rec, p, o, make_uint2(NO_PAIRS, NO_PAIRS), capacity, keys, vals);
+  emit_records(P, tri_rec, p, o, make_uint4(NO_PAIRS, NO_PAIRS, NO_PAIRS, NO_PAIRS), capacity, keys, vals);
 }
 // Single draws: the offsets come from the counts of the workgroup's 256 primitives (a workgroup
 // scan) and the totals of the blocks before it, which setup and the clip launch left in `bsum`, so
@@ -1137,13 +1140,13 @@ __device__ __forceinline__ void emit_body(const draw_params& P, const float4* __
 // before it, as an exclusive scan of the counts gives it.
 __global__ void __launch_bounds__(256) emit_kernel(draw_params P, const float4* __restrict__ tri_rec,
                                                    const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ bsum,
-                                                   const uint2* __restrict__ pairs, uint32_t capacity,
+                                                   const uint4* __restrict__ pairs, uint32_t capacity,
                                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   __shared__ uint32_t wtot[4], wbefore[4];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t p = blockIdx.x * 256 + tid;
   uint32_t c[3] = {0, 0, 0};
-  uint2 pr = make_uint2(NO_PAIRS, NO_PAIRS);
+  uint4 pr = make_uint4(NO_PAIRS, NO_PAIRS, NO_PAIRS, NO_PAIRS);
   if (p < P.prim_count) {
     c[0] = cnt[3 * p];
     c[1] = cnt[3 * p + 1];

@@ -387,7 +387,7 @@ struct renderer_obj : object {
   dev_array<uint32_t> tile_count;  // per-tile record counts and cursors of the counting sort
   dev_array<uint32_t> wide;   // triangles counted by wave
   dev_array<uint32_t> bsum;   // records per 256-primitive block
-  dev_array<uint2> pairs;     // setup's first two records of each primitive
+  dev_array<uint4> pairs;     // setup's first four records of each primitive
   dev_array<uint4> vis_entries, vis_merges;  // visibility work table
   dev_array<uint32_t> vis_counts;
   dev_array<unsigned long long> vis_partial;
