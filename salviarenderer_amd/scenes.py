@@ -785,10 +785,21 @@ float4 dd_main(float3 n : TEXCOORD(0), float2 uv : TEXCOORD(1)) : COLOR {
   return float4(dx * 64.0, dy * 64.0);
 }
 """,
+    # derivatives of texture samples and a branch on one: every pixel of a quad must sample
+    "tex_derivatives": """
+sampler diffuse;
+float4 td_main(float3 n : TEXCOORD(0), float2 uv : TEXCOORD(1)) : COLOR {
+  float4 t = tex2D(diffuse, uv);
+  float a = ddx(t.x) * 4.0 + 0.5;
+  float b = ddy(t.y) * 4.0 + 0.5;
+  if (t.z > 0.5) { a = 1.0 - a; }
+  return float4(a, b, t.z, 1.0);
+}
+""",
 }
 # the vertex shader each one runs behind (its attributes are the pixel shader's inputs)
 SASL_VS = {"box": "box", "plane": "box", "constant": "uv", "grey": "uv", "lambert": "uv", "procedural": "uv",
-           "derivatives": "uv"}
+           "derivatives": "uv", "tex_derivatives": "uv"}
 
 
 class SaslFrame:

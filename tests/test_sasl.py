@@ -793,11 +793,13 @@ def test_sasl_in_order_paths_hip(hip_lib, cpu_lib, name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["lambert", "procedural", "derivatives"])
+@pytest.mark.parametrize("name", ["lambert", "procedural", "derivatives", "tex_derivatives"])
 def test_sasl_interpreter_and_native_agree_hip(hip_lib, cpu_lib, name, monkeypatch):
     """Opaque SASL draws shade with the program compiled to native code at run time; with
-    SRX_SASL_JIT=0 the quad interpreter shades them instead.  Both against the oracle."""
-    kw = dict(samples=4, fmt=A.PF_RGBA32F if name == "derivatives" else A.PF_RGBA8)
+    SRX_SASL_JIT=0 the quad interpreter shades them instead.  Both against the oracle.
+    "tex_derivatives" takes derivatives of texture samples and branches on one, so the samples of
+    pixels that store nothing reach the stored colours."""
+    kw = dict(samples=4, fmt=A.PF_RGBA32F if name in ("derivatives", "tex_derivatives") else A.PF_RGBA8)
     want = render_sasl(cpu_lib, name, **kw)
     got = {}
     for jit in ("1", "0"):
