@@ -352,6 +352,8 @@ def main():
 
     lib = hip_lib()
     config = tuple(int(x) for x in args.batch_config.split(",")) if args.batch_config else None
+    if config is not None and len(config) != 3:
+        ap.error("--batch-config takes TILE,BIN_ITEMS,THREADS")
     if args.batched_only:
         print(json.dumps(batched_jobs(lib, 3, config=config)), flush=True)
         return
